@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain-model time stepper on one MI355X.
+
+Workload: the plain curved-interface case at 256^3 (q0 = 1, nw = 8,
+eps = -0.7, alpha = 1, beta = 2, gamma = 1, dt = 0.1; psi0 = 0.6 cos(q0 r)
+about a seeded centre inside the slab, noise 1e-3, seed 1), stepped in FP64
+(`value`) and in FP32-mixed.  A "step" is one loop-body iteration of
+no-adv/cosNoAdvTrackMult.cpp:729-1147.  Inputs are resident in HBM before
+the timed region; timing uses HIP events on the stepper's own stream.
+
+Also reported: FP32-mixed steps/s and its relative L2 error against FP64
+after 1000 steps, the per-kernel roofline of the dominant kernel, and the
+FP64 oracle timed on the host cores as a CPU baseline.
+
+Prints ONE JSON line.
+"""
+import argparse
+import glob
+import importlib
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("steps/s @256³: FP64 plain vs FP32-mixed (rel err≤1e-5); adjoint/fwd time ratio")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(kernel_key, grid, prec):
+    """HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE runs,
+    recorded in profiles/*.md as
+    `traffic: <kernel_key> <grid> <prec> <bytes>`; None if not recorded."""
+    pat = re.compile(rf"^traffic:\s+{re.escape(kernel_key)}\s+{grid}\s+{prec}\s+([0-9.eE+]+)", re.M)
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        m = pat.search(open(f).read())
+        if m:
+            return float(m.group(1))
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--acc-steps", type=int, default=1000)
+    ap.add_argument("--prof-steps", type=int, default=50)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-fp", choices=["64", "32"], default=None,
+                    help="time one precision only (for profiler runs)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on exactly one GPU (the grid is resident on one "
+              "device; multi-GPU sharding is design-only). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    smadft = importlib.import_module("sma-dft_amd")
+    n = args.grid
+    p64 = smadft.Params.default(smadft.MODEL_PLAIN, n)
+    psi0 = smadft.ic_curved(p64, 0.6, 1e-3, seed=1)
+    res = {}
+
+    def run(prec):
+        p = smadft.Params.default(smadft.MODEL_PLAIN, n)
+        p.precision = prec
+        s = smadft.Stepper(p, psi0)
+        s.step(args.warmup)
+        s.sync()
+        ms = s.time_steps(args.steps)
+        log(f"fp{prec}: {args.steps} steps in {ms:.3f} ms -> {1e3 * args.steps / ms:.1f} steps/s")
+        return s, ms
+
+    precs = [64, 32] if args.only_fp is None else [int(args.only_fp)]
+    steppers = {}
+    for prec in precs:
+        s, ms = run(prec)
+        steppers[prec] = s
+        res[prec] = ms
+
+    # per-kernel profile of the FP64 path (dominant kernel -> roofline)
+    prec_main = 64 if 64 in steppers else precs[0]
+    prof = steppers[prec_main].profile_steps(args.prof_steps)
+    bytes_t = 8 if prec_main == 64 else 4
+    F = n ** 3 * bytes_t
+    alg_bytes = {"plain_z": 2 * F, "axis_y": 2 * F, "plain_x": 6 * F}
+    dom = max(prof, key=lambda r: r[1])
+    key = dom[0].split(":")[0]
+    avg_ms = dom[1] / dom[2]
+    achieved = alg_bytes[key] / (avg_ms * 1e-3) / 1e9
+    kernels = [{"kernel": k, "avg_us": round(1e3 * ms / c, 2), "launches": c,
+                "alg_GBps": round(alg_bytes[k.split(':')[0]] / (ms / c * 1e-3) / 1e9, 1)}
+               for k, ms, c in prof]
+    for kd in kernels:
+        log(f"kernel {kd}")
+    traffic = traffic_from_profiles(key, n, f"fp{prec_main}")
+
+    # FP32-mixed error against FP64 after acc_steps, on the workload IC and on
+    # the reference-amplitude IC (1.328, where the smectic persists)
+    acc = {}
+    if args.only_fp is None and args.acc_steps > 0:
+        for label, amp in (("workload_ic_amp0.6", 0.6), ("amp1.328", 1.328)):
+            ic = smadft.ic_curved(p64, amp, 1e-3, seed=1)
+            out = {}
+            for prec in (64, 32):
+                p = smadft.Params.default(smadft.MODEL_PLAIN, n)
+                p.precision = prec
+                with smadft.Stepper(p, ic) as s:
+                    s.step(args.acc_steps)
+                    out[prec] = s.fetch()
+            err = float(np.linalg.norm(out[32] - out[64]) / np.linalg.norm(out[64]))
+            acc[label] = {"rel_l2_fp32_vs_fp64": err, "norm_fp64": float(np.linalg.norm(out[64]))}
+            log(f"accuracy {label}: rel L2 = {err:.3e}")
+
+    # CPU baseline: the FP64 oracle on the host cores, bounded sample
+    cpu = None
+    if not args.no_cpu and args.cpu_steps > 0:
+        import oracle
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        o = oracle.PlainOracle(p64, psi0)
+        o.step(1)
+        t0 = time.perf_counter()
+        o.step(args.cpu_steps)
+        dt = time.perf_counter() - t0
+        cpu = {"value": args.cpu_steps / dt, "unit": "steps/s", "cores": cores, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the FP64 oracle plain step at {n}^3 "
+                         f"(3 r2r transforms/step, OpenMP over lines) after 1 warm step"}
+        log(f"cpu baseline: {cpu}")
+
+    ms64 = res.get(64)
+    ms32 = res.get(32)
+    line = {
+        "metric": METRIC,
+        "value": (1e3 * args.steps / ms64) if ms64 else None,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": (ms64 / args.steps) if ms64 else None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain curved-interface relaxation {n}^3, FP64 (and FP32-mixed)",
+                   "grid": [n, n, n], "dt": p64.dt, "eps": p64.eps, "nw": p64.nw,
+                   "ic": "0.6 cos(q0 r), seeded centre, noise 1e-3 seed 1"},
+        "fp32_mixed": None if ms32 is None else {
+            "steps_per_s": 1e3 * args.steps / ms32,
+            "ms_per_step": ms32 / args.steps,
+            "speedup_vs_fp64": (ms64 / ms32) if ms64 else None,
+            "accuracy_after_steps": args.acc_steps,
+            "accuracy": acc,
+        },
+        "adjoint_fwd_ratio": None,
+        "kernels": kernels,
+        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "alg_bytes_per_launch": alg_bytes[key],
+                     "avg_launch_us": 1e3 * avg_ms},
+        "step_alg_GBps": (18 * F) / (ms64 / args.steps * 1e-3) / 1e9 if ms64 else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

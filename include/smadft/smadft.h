@@ -1,0 +1,183 @@
+/*
+ * smadft.h -- C ABI of the MI355X (gfx950) time stepper for the sma-dft
+ * smectic-A / isotropic phase-field models.
+ *
+ * The reference programs have no library interface: the time step is the
+ * inline body of `while (L1 > limL1)` in each main()
+ *   plain      no-adv/cosNoAdvTrackMult.cpp:729-1147 (cosNoAdvTrackSing.cpp:681-1072)
+ *   advective  adv/cosAdvTrackMult.cpp:827-1742
+ * and below it the FFTW-MPI calls fftw_mpi_plan_r2r_3d / fftw_execute.  This
+ * header cuts at that seam: everything above the loop (parameters, initial
+ * conditions, L1 cadence, output writers) stays host code, and the loop body
+ * becomes smad_step().  Plain pointers and sizes only; no C++ or torch types.
+ *
+ * Conventions
+ *   - Fields are row-major (i, j, k), k fastest, index (i*ny + j)*nz + k,
+ *     exactly the reference's slab index with the slabs concatenated.
+ *   - Transforms are unnormalised (FFTW r2r); scale = 1/(8 nx ny nz) as in
+ *     no-adv/cosNoAdvTrackSing.cpp:172.
+ *   - Host buffers belong to the caller, device state to the context.  Each
+ *     context owns one HIP stream; calls on one context are not thread-safe,
+ *     distinct contexts may be used concurrently.
+ *   - Return codes: 0 OK, negative on error (see SMAD_E*); the message is in
+ *     smad_last_error().  No C++ exception crosses this boundary.
+ */
+#ifndef SMADFT_SMADFT_H
+#define SMADFT_SMADFT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct smad_ctx smad_ctx;
+
+enum {
+    SMAD_MODEL_PLAIN = 0, /* non-conserved SH relaxation, CN/AB2 */
+    SMAD_MODEL_ADV = 1,   /* + uniform-density Stokes advection */
+    SMAD_MODEL_QUASI = 2  /* quasi-incompressible, varying density, BDF2/AB2 */
+};
+
+enum { SMAD_FP64 = 64, SMAD_FP32_MIXED = 32 };
+
+/* Parameters.  Names follow the reference programs' constants
+ * (no-adv/cosNoAdvTrackMult.cpp:130-172, adv/cosAdvTrackMult.cpp:149-198,
+ * quasi/quasi.cpp:204-267). */
+typedef struct {
+    int model;              /* SMAD_MODEL_* */
+    int nx, ny, nz;         /* grid (each 16, 32, 64, 128, 256, 384 or 512) */
+    double nw;              /* points per wavelength: L = N 2 pi/(q0 nw) */
+    double q0, eps, alpha, beta, gamma, dt;
+    double nu;              /* viscosity (advective, quasi) */
+    double lambda;          /* bulk viscosity (quasi), reference -2 nu/3 */
+    double rho0, kp, rho_m; /* density law rho = kp sqrt(q0^2 psi^2 + |grad psi|^2) + rho0 (quasi) */
+    double filter_sigma;    /* Gaussian filter width of rho (quasi), reference 1.57 */
+    int div_switch;         /* quasi: step after which divv and grad^2(rho f) are used */
+    int precision;          /* SMAD_FP64 or SMAD_FP32_MIXED */
+} smad_params;
+
+enum {
+    SMAD_OK = 0,
+    SMAD_EINVAL = -1,     /* bad argument / unsupported size */
+    SMAD_EHIP = -2,       /* HIP runtime error */
+    SMAD_ENOMEM = -3,     /* device allocation failed */
+    SMAD_ENONFINITE = -4, /* NaN or Inf found in the state (reduce / fetch) */
+    SMAD_ESTATE = -5      /* call not valid in the current state */
+};
+
+/* Fields for smad_fetch / smad_set_field (FP64 on the host side). */
+enum {
+    SMAD_FIELD_PSI = 0,     /* real psi after the last step */
+    SMAD_FIELD_PSI_HAT = 1, /* spectral psi, scaled: psi = REDFT01^3(psi_hat) */
+    SMAD_FIELD_N_HAT = 2,   /* REDFT10^3 of the last nonlinear term (AB2 history) */
+    SMAD_FIELD_GRAD_X = 3,  /* real grad psi of the last step (advective) */
+    SMAD_FIELD_GRAD_Y = 4,
+    SMAD_FIELD_GRAD_Z = 5,
+    SMAD_FIELD_VEL_X = 6,   /* real velocity of the last step (advective) */
+    SMAD_FIELD_VEL_Y = 7,
+    SMAD_FIELD_VEL_Z = 8
+};
+
+/* Reductions (FP64 on the device, deterministic order). */
+enum {
+    SMAD_RED_L1 = 0 /* out[0] = sum|psi_n+1 - psi_n| over the last step taken,
+                       out[1] = sum|psi_n+1|, out[2] = out[0]/(dt out[1]);
+                       reference L1 check, no-adv/cosNoAdvTrackMult.cpp:814-842 */
+};
+
+/* Create a context and run the reference pre-loop on psi0 (host, FP64,
+ * nx*ny*nz values): N_hat^0 = REDFT10^3(beta psi0^3 - gamma psi0^5), so the
+ * first step is forward Euler in N (no-adv/cosNoAdvTrackMult.cpp:630-678,
+ * adv/cosAdvTrackMult.cpp:760-777).  Replaces the reference's plan creation
+ * (fftw_mpi_plan_r2r_3d, no-adv/cosNoAdvTrackMult.cpp:350-380). */
+int smad_init(const smad_params *params, const double *psi0, smad_ctx **out);
+
+/* Enqueue nsteps time steps on the context stream (the loop body).  Returns
+ * after enqueueing; errors of earlier asynchronous work surface here or at
+ * the next synchronising call. */
+int smad_step(smad_ctx *ctx, int nsteps);
+
+/* Copy a field to the host as FP64 (synchronises).  SMAD_ENONFINITE if the
+ * field holds NaN or Inf (the values are still copied). */
+int smad_fetch(smad_ctx *ctx, int field, double *host_out);
+
+/* Overwrite a state field from the host (restart, fault injection).
+ * SMAD_FIELD_PSI re-runs the pre-loop on the new psi, as the reference's
+ * load == 1 path does (no-adv/cosNoAdvTrackMult.cpp:517-537). */
+int smad_set_field(smad_ctx *ctx, int field, const double *host_in);
+
+/* Global FP64 reduction (synchronises).  Returns SMAD_ENONFINITE when psi
+ * holds NaN/Inf -- the reference would silently stop its loop instead
+ * (no-adv/cosNoAdvTrackSing.cpp:681). */
+int smad_reduce(smad_ctx *ctx, int what, double *out);
+
+/* Wait for all work on the context stream. */
+int smad_sync(smad_ctx *ctx);
+
+/* Enqueue nsteps steps between two HIP events recorded on the context
+ * stream and wait; *ms = elapsed device time in milliseconds.  The
+ * reference's only timers are std::clock / chrono around the whole loop
+ * (no-adv/cosNoAdvTrackSing.cpp:661-671, :1088-1091). */
+int smad_time_steps(smad_ctx *ctx, int nsteps, double *ms);
+
+/* As smad_time_steps, with an event after every kernel launch.  For each
+ * kernel class i < nmax: kernel_ms[i] = summed duration, launches[i] =
+ * number of launches; *nclasses = classes used by the model. */
+int smad_profile_steps(smad_ctx *ctx, int nsteps, double *kernel_ms, int *launches, int nmax,
+                       int *nclasses);
+
+/* Human-readable name of kernel class cls ("" if out of range). */
+const char *smad_kernel_class_name(const smad_ctx *ctx, int cls);
+
+/* The context's hipStream_t (as void*), for event timing by the caller. */
+void *smad_stream(smad_ctx *ctx);
+
+/* Number of steps taken since smad_init. */
+long smad_steps_taken(const smad_ctx *ctx);
+
+void smad_destroy(smad_ctx *ctx);
+
+/* Last error message of the context, or of the last failed smad_init when
+ * ctx is NULL.  Never NULL. */
+const char *smad_last_error(const smad_ctx *ctx);
+
+/* Standalone separable 3D real-to-real transform on the device, unnormalised,
+ * kinds per axis (x, y, z) from SMAD_REDFT10 .. SMAD_RODFT01.  Replaces one
+ * fftw_mpi_plan_r2r_3d + fftw_execute pair of the reference (e.g.
+ * adv/cosAdvTrackMult.cpp:419-467 and :961-975).  Host buffers, FP64 on
+ * both sides; precision selects the device arithmetic.  in may equal out. */
+enum { SMAD_REDFT10 = 0, SMAD_REDFT01 = 1, SMAD_RODFT10 = 2, SMAD_RODFT01 = 3 };
+int smad_r2r_3d(int nx, int ny, int nz, const int kinds[3], int precision, const double *in,
+                double *out);
+
+/* ----------------------------------------------------------------------
+ * Initial-condition builders (host only; no GPU needed).  All fill
+ * nx*ny*nz doubles in the field layout above.  Noise is SplitMix64 of
+ * (seed, linear index) mapped to U[-amp, amp) with 53-bit resolution, so it
+ * does not depend on any decomposition.  The reference has no RNG.
+ */
+
+/* Focal-conic / hyperboloid defect of no-adv/cosNoAdvTrackSing.cpp:389-449
+ * (amp 1.328 there; a_e = b_e set the conic size in grid units). */
+void smad_ic_focal_conic(const smad_params *p, double amp, double a_e, double b_e, double *psi);
+
+/* Planar smectic slab: amp cos(q0 k dz) for |i dx - Lx/2| < Lx/4, else 0,
+ * plus noise. */
+void smad_ic_slab(const smad_params *p, double amp, double noise, unsigned long long seed,
+                  double *psi);
+
+/* Curved layers: amp cos(q0 r), r = distance to a seeded centre below the
+ * slab centre inside the slab |i dx - Lx/2| < Lx/4, plus noise. */
+void smad_ic_curved(const smad_params *p, double amp, double noise, unsigned long long seed,
+                    double *psi);
+
+/* Adds SplitMix64 noise U[-amp, amp) to psi. */
+void smad_ic_add_noise(const smad_params *p, double amp, unsigned long long seed, double *psi);
+
+/* Reference parameter set for the given model with the reference's
+ * constants (q0 = 1, alpha = 1, beta = 2, gamma = 1, ...). */
+void smad_default_params(int model, int n, smad_params *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,120 @@
+"""ctypes binding of the CPU FP64 oracle -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+as the checker.  The product (sma-dft_amd/) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REDFT10, REDFT01, RODFT10, RODFT01 = 0, 1, 2, 3
+
+_LIB = None
+
+
+class OParams(ctypes.Structure):
+    _fields_ = [
+        ("model", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("nw", ctypes.c_double), ("q0", ctypes.c_double), ("eps", ctypes.c_double),
+        ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("gamma", ctypes.c_double),
+        ("dt", ctypes.c_double), ("nu", ctypes.c_double), ("lambda_", ctypes.c_double),
+        ("rho0", ctypes.c_double), ("kp", ctypes.c_double), ("rho_m", ctypes.c_double),
+        ("filter_sigma", ctypes.c_double), ("div_switch", ctypes.c_int),
+        ("precision", ctypes.c_int),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        pp = ctypes.POINTER(OParams)
+        L.orc_r2r1d_naive.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int]
+        L.orc_r2r1d.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int]
+        L.orc_r2r3d.argtypes = [dp, dp] + [ctypes.c_int] * 6
+        L.orc_plain_preloop.argtypes = [pp, dp, dp]
+        L.orc_plain_step.argtypes = [pp, dp, dp, dp, dp, ctypes.c_int]
+        L.orc_l1_sums.argtypes = [dp, dp, ctypes.c_long, dp]
+        L.orc_adv_preloop.argtypes = [pp, dp, dp]
+        L.orc_adv_step.argtypes = [pp, dp, dp, dp, dp, dp, ctypes.c_int]
+        for f in ("orc_r2r1d_naive", "orc_r2r1d", "orc_r2r3d", "orc_plain_preloop",
+                  "orc_plain_step", "orc_l1_sums", "orc_adv_preloop", "orc_adv_step"):
+            getattr(L, f).restype = None
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def oparams(p):
+    """Oracle parameter struct from any object with the smad_params fields."""
+    return OParams(**{name: getattr(p, name) for name, _ in OParams._fields_})
+
+
+def r2r1d(x, kind, naive=False):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    (lib().orc_r2r1d_naive if naive else lib().orc_r2r1d)(_p(x), _p(y), len(x), kind)
+    return y
+
+
+def r2r3d(x, kinds):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    lib().orc_r2r3d(_p(x), _p(y), *x.shape, *kinds)
+    return y
+
+
+class PlainOracle:
+    """Reference-order plain stepper (no-adv/cosNoAdvTrackMult.cpp:630-805)."""
+
+    def __init__(self, p, psi0):
+        self.op = oparams(p)
+        self.psi = np.ascontiguousarray(psi0, dtype=np.float64).copy()
+        self.psiq = np.zeros_like(self.psi)
+        self.nq = np.zeros_like(self.psi)
+        self.nq_past = np.zeros_like(self.psi)
+        lib().orc_plain_preloop(ctypes.byref(self.op), _p(self.psi), _p(self.nq))
+
+    def step(self, n=1):
+        lib().orc_plain_step(ctypes.byref(self.op), _p(self.psi), _p(self.psiq), _p(self.nq),
+                             _p(self.nq_past), int(n))
+
+
+class AdvOracle:
+    """Reference-order advective stepper (adv/cosAdvTrackMult.cpp:760-1369)."""
+
+    def __init__(self, p, psi0):
+        self.op = oparams(p)
+        self.psi = np.ascontiguousarray(psi0, dtype=np.float64).copy()
+        self.nq = np.zeros_like(self.psi)
+        self.nq_past = np.zeros_like(self.psi)
+        n = self.psi.size
+        self.grad = np.zeros((3,) + self.psi.shape)
+        self.vel = np.zeros((3,) + self.psi.shape)
+        lib().orc_adv_preloop(ctypes.byref(self.op), _p(self.psi), _p(self.nq))
+
+    def step(self, n=1):
+        lib().orc_adv_step(ctypes.byref(self.op), _p(self.psi), _p(self.nq), _p(self.nq_past),
+                           _p(self.grad), _p(self.vel), int(n))
+
+
+def l1_sums(new, old):
+    out = np.zeros(2)
+    a = np.ascontiguousarray(new, dtype=np.float64)
+    b = np.ascontiguousarray(old, dtype=np.float64)
+    lib().orc_l1_sums(_p(a), _p(b), a.size, _p(out))
+    return out

@@ -1,0 +1,60 @@
+/*
+ * CPU FP64 oracle -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the sma-dft time steppers used as the checker for
+ * the HIP path.  Only the tests, the smoke check and bench.py's CPU-baseline
+ * leg load it.  The product library never links or calls it.
+ *
+ * Parity pinning (what each function is checked against) is listed in the
+ * header comment of each source file.
+ */
+#ifndef SMADFT_ORACLE_H
+#define SMADFT_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_REDFT10 = 0, ORC_REDFT01 = 1, ORC_RODFT10 = 2, ORC_RODFT01 = 3 };
+
+void orc_r2r1d_naive(const double *x, double *y, int n, int kind);
+void orc_r2r1d(const double *x, double *y, int n, int kind);
+/* separable 3D transform, row-major (i,j,k) with k fastest; in may equal out */
+void orc_r2r3d(const double *in, double *out, int nx, int ny, int nz, int kx, int ky, int kz);
+
+/* Model parameters.  Field order and meaning mirror smad_params of the product
+ * header so that one ctypes structure serves both. */
+typedef struct {
+    int model; /* 0 plain, 1 advective, 2 quasi-incompressible */
+    int nx, ny, nz;
+    double nw, q0, eps, alpha, beta, gamma, dt;
+    double nu, lambda, rho0, kp, rho_m, filter_sigma;
+    int div_switch;
+    int precision;
+} orc_params;
+
+/* ------------------------------------------------------------ plain model -- */
+/* State: psi (real), psiq (spectral, last update), nq / nq_past (spectral N).
+ * orc_plain_preloop: nq = REDFT10^3(beta psi^3 - gamma psi^5)
+ *   (no-adv/cosNoAdvTrackMult.cpp:630-678).
+ * orc_plain_step: nsteps iterations of the loop body
+ *   (no-adv/cosNoAdvTrackMult.cpp:729-805 and :1147).  */
+void orc_plain_preloop(const orc_params *p, const double *psi, double *nq);
+void orc_plain_step(const orc_params *p, double *psi, double *psiq, double *nq, double *nq_past,
+                    int nsteps);
+
+/* L1 sums of no-adv/cosNoAdvTrackMult.cpp:814-842: out[0] = sum|a-b|, out[1] = sum|a| */
+void orc_l1_sums(const double *a_new, const double *b_old, long n, double *out);
+
+/* ------------------------------------------------------- advective model -- */
+/* adv/cosAdvTrackMult.cpp:824-1369.  Pre-loop N without advection (:760-777).
+ * State: psi (real), nq, nq_past.  Diagnostic fields (may be NULL):
+ * grad[3*n] (real grad psi), vel[3*n] (real u). */
+void orc_adv_preloop(const orc_params *p, const double *psi, double *nq);
+void orc_adv_step(const orc_params *p, double *psi, double *nq, double *nq_past, double *grad,
+                  double *vel, int nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,262 @@
+/*
+ * CPU FP64 oracle -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Restatement of the reference time steppers in reference order, with the
+ * reference's index shifts, truncations and scale placement.  The product
+ * never links this file; tests compare the HIP path against it.
+ *
+ * Pinning (tests/test_oracle.py):
+ *   - plain update: linear single mode decays as G(q)^n with
+ *     G = (1 + dt L/2)/(1 - dt L/2) when beta = gamma = 0; uniform psi follows
+ *     the scalar AB2/CN recurrence (first step Euler in N).
+ *   - advective gradient: single cosine modes give the analytic derivative;
+ *     planar layers psi(z) give u == 0; single-mode Stokes solve matches
+ *     (I - q q^T/|q|^2) F / (nu |q|^2) for an x-only force.
+ *   - the cross-index reads of the velocity projection
+ *     (adv/cosAdvTrackMult.cpp:1103-1274) are pinned only by the single-mode
+ *     and divergence checks (parity partially unpinned: no compiled reference
+ *     can be built here, FFTW and MPI are absent).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+typedef struct {
+    int nx, ny, nz;
+    long n;
+    double dx, dy, dz, scale;
+    double *qx, *qy, *qz; /* DCT wavenumbers pi i / L */
+    double *sx, *sy, *sz; /* DST wavenumbers pi (i+1) / L */
+} grid_t;
+
+/* no-adv/cosNoAdvTrackSing.cpp:154-172 and :309-324; adv/cosAdvTrackMult.cpp:386-401 */
+static void grid_make(const orc_params *p, grid_t *g)
+{
+    g->nx = p->nx; g->ny = p->ny; g->nz = p->nz;
+    g->n = (long)p->nx * p->ny * p->nz;
+    double lx = p->nx * 2.0 * M_PI / (p->q0 * p->nw);
+    double ly = p->ny * 2.0 * M_PI / (p->q0 * p->nw);
+    double lz = p->nz * 2.0 * M_PI / (p->q0 * p->nw);
+    g->dx = lx / p->nx; g->dy = ly / p->ny; g->dz = lz / p->nz;
+    g->scale = 0.125 / ((double)p->nx * p->ny * p->nz);
+    g->qx = malloc(sizeof(double) * p->nx); g->sx = malloc(sizeof(double) * p->nx);
+    g->qy = malloc(sizeof(double) * p->ny); g->sy = malloc(sizeof(double) * p->ny);
+    g->qz = malloc(sizeof(double) * p->nz); g->sz = malloc(sizeof(double) * p->nz);
+    for (int i = 0; i < p->nx; ++i) { g->qx[i] = 1.0 * M_PI * i / (g->dx * p->nx); g->sx[i] = 1.0 * M_PI * (i + 1) / (g->dx * p->nx); }
+    for (int j = 0; j < p->ny; ++j) { g->qy[j] = M_PI * j / (g->dy * p->ny); g->sy[j] = M_PI * (j + 1) / (g->dy * p->ny); }
+    for (int k = 0; k < p->nz; ++k) { g->qz[k] = M_PI * k / (g->dz * p->nz); g->sz[k] = M_PI * (k + 1) / (g->dz * p->nz); }
+}
+
+static void grid_free(grid_t *g)
+{
+    free(g->qx); free(g->qy); free(g->qz); free(g->sx); free(g->sy); free(g->sz);
+}
+
+#define IDX(i, j, k) (((long)(i) * g.ny + (j)) * g.nz + (k))
+
+static void ct(const grid_t *g, const double *in, double *out)
+{
+    orc_r2r3d(in, out, g->nx, g->ny, g->nz, ORC_REDFT10, ORC_REDFT10, ORC_REDFT10);
+}
+static void ict(const grid_t *g, const double *in, double *out)
+{
+    orc_r2r3d(in, out, g->nx, g->ny, g->nz, ORC_REDFT01, ORC_REDFT01, ORC_REDFT01);
+}
+
+/* aLin = ep - alpha (|q|^2 - q0^2)^2 (no-adv/cosNoAdvTrackMult.cpp:639-645) */
+static double alin_at(const orc_params *p, const grid_t *gp, int i, int j, int k)
+{
+    double mq2 = pow(gp->qx[i], 2) + pow(gp->qy[j], 2) + pow(gp->qz[k], 2);
+    double op = p->alpha * pow(mq2 - p->q0 * p->q0, 2);
+    return p->eps - op;
+}
+
+/* N = beta psi^3 - gamma psi^5 (no-adv/cosNoAdvTrackMult.cpp:760-766) */
+static void nonlin(const orc_params *p, const double *psi, double *nr, long n)
+{
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) nr[i] = p->beta * pow(psi[i], 3) - p->gamma * pow(psi[i], 5);
+}
+
+void orc_plain_preloop(const orc_params *p, const double *psi, double *nq)
+{
+    grid_t g;
+    grid_make(p, &g);
+    double *nr = malloc(sizeof(double) * g.n);
+    nonlin(p, psi, nr, g.n);
+    ct(&g, nr, nq);
+    free(nr);
+    grid_free(&g);
+}
+
+void orc_plain_step(const orc_params *p, double *psi, double *psiq, double *nq, double *nq_past,
+                    int nsteps)
+{
+    grid_t g;
+    grid_make(p, &g);
+    double *nr = malloc(sizeof(double) * g.n);
+    const double dtd2 = p->dt / 2;
+    for (int s = 0; s < nsteps; ++s) {
+        memcpy(nq_past, nq, sizeof(double) * g.n);          /* :731 */
+        nonlin(p, psi, nr, g.n);                              /* :760-766 */
+        ct(&g, nr, nq);                                       /* :770 */
+        ct(&g, psi, psiq);                                    /* :774 */
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int i = 0; i < g.nx; ++i)                        /* :778-800 */
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    double al = alin_at(p, &g, i, j, k);
+                    double c1 = 1.0 + dtd2 * al, c2 = 1.0 - dtd2 * al;
+                    psiq[id] = g.scale * (c1 * psiq[id] + dtd2 * (3.0 * nq[id] - nq_past[id])) / c2;
+                }
+        ict(&g, psiq, psi);                                   /* :805, :1147 */
+    }
+    free(nr);
+    grid_free(&g);
+}
+
+void orc_l1_sums(const double *a_new, const double *b_old, long n, double *out)
+{
+    double sa = 0.0, sb = 0.0;
+    for (long i = 0; i < n; ++i) {
+        sa += fabs(a_new[i] - b_old[i]);
+        sb += fabs(a_new[i]);
+    }
+    out[0] = sa;
+    out[1] = sb;
+}
+
+/* -------------------------------------------------------- advective model -- */
+
+void orc_adv_preloop(const orc_params *p, const double *psi, double *nq)
+{
+    /* adv/cosAdvTrackMult.cpp:760-777: N without advection */
+    orc_plain_preloop(p, psi, nq);
+}
+
+void orc_adv_step(const orc_params *p, double *psi, double *nq, double *nq_past, double *grad,
+                  double *vel, int nsteps)
+{
+    grid_t g;
+    grid_make(p, &g);
+    const long n = g.n;
+    const double dtd2 = p->dt / 2, nu = p->nu, sc = g.scale;
+    double *psiq = malloc(sizeof(double) * n);
+    double *gx = calloc(n, sizeof(double)), *gy = calloc(n, sizeof(double)), *gz = calloc(n, sizeof(double));
+    double *vx = calloc(n, sizeof(double)), *vy = calloc(n, sizeof(double)), *vz = calloc(n, sizeof(double));
+    double *sp2 = malloc(sizeof(double) * n);
+    double *sx = malloc(sizeof(double) * n), *sy = malloc(sizeof(double) * n), *sz = malloc(sizeof(double) * n);
+    double *nr = malloc(sizeof(double) * n);
+
+    for (int s = 0; s < nsteps; ++s) {
+        /* :831-837 zero fill */
+        memset(gx, 0, sizeof(double) * n); memset(gy, 0, sizeof(double) * n); memset(gz, 0, sizeof(double) * n);
+        memset(vx, 0, sizeof(double) * n); memset(vy, 0, sizeof(double) * n); memset(vz, 0, sizeof(double) * n);
+        memcpy(nq_past, nq, sizeof(double) * n);             /* :842 */
+        ct(&g, psi, psiq);                                   /* :847-860 */
+        for (long i = 0; i < n; ++i) psiq[i] = sc * psiq[i];
+
+        /* DST-shift gradient :866-959 (x: i <= Nx-2, the last plane stays 0) */
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    if (i < g.nx - 1) gx[id] = -g.sx[i] * psiq[IDX(i + 1, j, k)];
+                    if (j < g.ny - 1) gy[id] = -g.sy[j] * psiq[IDX(i, j + 1, k)];
+                    if (k < g.nz - 1) gz[id] = -g.sz[k] * psiq[IDX(i, j, k + 1)];
+                }
+        ict(&g, psiq, psi);                                  /* :961-963 */
+        orc_r2r3d(gx, gx, g.nx, g.ny, g.nz, ORC_RODFT01, ORC_REDFT01, ORC_REDFT01); /* :965-967 */
+        orc_r2r3d(gy, gy, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_RODFT01, ORC_REDFT01); /* :969-971 */
+        orc_r2r3d(gz, gz, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_REDFT01, ORC_RODFT01); /* :973-975 */
+
+        /* Sp2 = iCT(-aLin psiq) :980-991 */
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    sp2[id] = -alin_at(p, &g, i, j, k) * psiq[id];
+                }
+        ict(&g, sp2, sp2);
+
+        /* S = mu grad psi :996-1015 */
+#pragma omp parallel for schedule(static)
+        for (long id = 0; id < n; ++id) {
+            double mu = sp2[id] - p->beta * pow(psi[id], 3) + p->gamma * pow(psi[id], 5);
+            sx[id] = mu * gx[id];
+            sy[id] = mu * gy[id];
+            sz[id] = mu * gz[id];
+        }
+        orc_r2r3d(sx, sx, g.nx, g.ny, g.nz, ORC_RODFT10, ORC_REDFT10, ORC_REDFT10); /* :1017-1027 */
+        orc_r2r3d(sy, sy, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_RODFT10, ORC_REDFT10);
+        orc_r2r3d(sz, sz, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_REDFT10, ORC_RODFT10);
+
+        /* Stokes projection :1103-1274, global (single-slab) indexing */
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    if (i < g.nx - 1) {
+                        double syv = (j != 0) ? sy[IDX(i + 1, j - 1, k)] : 0.0;
+                        double szv = (k != 0) ? sz[IDX(i + 1, j, k - 1)] : 0.0;
+                        double mq2 = pow(g.sx[i], 2) + pow(g.qy[j], 2) + pow(g.qz[k], 2);
+                        double cm1 = sc / (nu * mq2), cm2 = g.sx[i] / mq2;
+                        double dot = g.sx[i] * sx[id] + g.qy[j] * syv + g.qz[k] * szv;
+                        vx[id] = cm1 * (sx[id] - cm2 * dot);
+                    }
+                    if (j < g.ny - 1 && k < g.nz - 1) {
+                        double sxv = (i != 0) ? sx[IDX(i - 1, j + 1, k)] : 0.0;
+                        double szv = (k != 0) ? sz[IDX(i, j + 1, k - 1)] : 0.0;
+                        double mq2 = pow(g.qx[i], 2) + pow(g.sy[j], 2) + pow(g.qz[k], 2);
+                        double cm1 = sc / (nu * mq2), cm2 = g.sy[j] / mq2;
+                        double dot = g.qx[i] * sxv + g.sy[j] * sy[id] + g.qz[k] * szv;
+                        vy[id] = cm1 * (sy[id] - cm2 * dot);
+
+                        sxv = (i != 0) ? sx[IDX(i - 1, j, k + 1)] : 0.0;
+                        double syv = (j != 0) ? sy[IDX(i, j - 1, k + 1)] : 0.0;
+                        mq2 = pow(g.qx[i], 2) + pow(g.qy[j], 2) + pow(g.sz[k], 2);
+                        cm1 = sc / (nu * mq2);
+                        cm2 = g.sz[k] / mq2;
+                        dot = g.qx[i] * sxv + g.qy[j] * syv + g.sz[k] * sz[id];
+                        vz[id] = cm1 * (sz[id] - cm2 * dot);
+                    }
+                }
+        orc_r2r3d(vx, vx, g.nx, g.ny, g.nz, ORC_RODFT01, ORC_REDFT01, ORC_REDFT01); /* :1276-1286 */
+        orc_r2r3d(vy, vy, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_RODFT01, ORC_REDFT01);
+        orc_r2r3d(vz, vz, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_REDFT01, ORC_RODFT01);
+
+        /* Nr :1290-1303, Nq :1307 */
+#pragma omp parallel for schedule(static)
+        for (long id = 0; id < n; ++id)
+            nr[id] = p->beta * pow(psi[id], 3) - p->gamma * pow(psi[id], 5) - vx[id] * gx[id] -
+                     vy[id] * gy[id] - vz[id] * gz[id];
+        ct(&g, nr, nq);
+
+        /* CN/AB2 :1355-1365 with pre-scaled psiq */
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    double al = alin_at(p, &g, i, j, k);
+                    double c1 = 1.0 + dtd2 * al, c2 = 1.0 - dtd2 * al;
+                    psiq[id] = (c1 * psiq[id] + dtd2 * sc * (3.0 * nq[id] - nq_past[id])) / c2;
+                }
+        ict(&g, psiq, psi);                                  /* :1369 */
+    }
+    if (grad) { memcpy(grad, gx, sizeof(double) * n); memcpy(grad + n, gy, sizeof(double) * n); memcpy(grad + 2 * n, gz, sizeof(double) * n); }
+    if (vel) { memcpy(vel, vx, sizeof(double) * n); memcpy(vel + n, vy, sizeof(double) * n); memcpy(vel + 2 * n, vz, sizeof(double) * n); }
+    free(psiq); free(gx); free(gy); free(gz); free(vx); free(vy); free(vz);
+    free(sp2); free(sx); free(sy); free(sz); free(nr);
+    grid_free(&g);
+}

@@ -1,0 +1,3 @@
+// FP32 instantiation of the stepper kernels.
+#include "launch.hpp"
+template struct smad::Launch<float>;

@@ -1,0 +1,612 @@
+// C ABI of the MI355X time stepper: context, device state, step loop.
+//
+// The reference's loop body (no-adv/cosNoAdvTrackMult.cpp:729-1147) runs
+// 3 FFTW-MPI r2r transforms per step on FP64 slabs.  Here the whole grid is
+// resident in HBM and one plain step is four fused axis passes:
+//
+//   Kz : psi = REDFT01_z(w);  N = beta psi^3 - gamma psi^5;  w = REDFT10_z(N)
+//   Ky : w = REDFT10_y(w)
+//   Kx : Nq = REDFT10_x(w);  CN/AB2 update of psi_hat (FP64 arithmetic);
+//        w = REDFT01_x(psi_hat')
+//   Ky': w = REDFT01_y(w)
+//
+// so w carries psi_hat' with the x and y inverse transforms already applied
+// between steps, and the real field is only materialised on demand.
+#include "../../include/smadft/smadft.h"
+#include "kernels.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace smad;
+
+namespace {
+
+thread_local std::string g_init_error;
+
+struct HipError {
+    std::string msg;
+};
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw HipError{std::string(#expr) + ": " + hipGetErrorString(e_)};             \
+    } while (0)
+
+#define LAUNCHCHK()                                                                        \
+    do {                                                                                   \
+        hipError_t e_ = hipGetLastError();                                                 \
+        if (e_ != hipSuccess) throw HipError{std::string("kernel launch: ") + hipGetErrorString(e_)}; \
+    } while (0)
+
+struct ApiError {
+    int code;
+    std::string msg;
+};
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        n = count;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw ApiError{SMAD_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                            " bytes failed: " + hipGetErrorString(e)};
+        }
+    }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// Twiddle tables for one line length, computed in long double on the host.
+template <typename T>
+struct TwTable {
+    DevBuf<cpx<T>> buf;
+    Tw<T> tw{};
+    void make(int n, hipStream_t s)
+    {
+        const int m = n / 2;
+        std::vector<cpx<T>> h(m + n + m);
+        const long double pi = 3.141592653589793238462643383279502884L;
+        for (int j = 0; j < m; ++j) {
+            long double a = 2.0L * pi * j / m;
+            h[j] = cpx<T>{T(cosl(a)), T(-sinl(a))};
+        }
+        for (int k = 0; k < n; ++k) {
+            long double a = pi * k / (2.0L * n);
+            h[m + k] = cpx<T>{T(cosl(a)), T(-sinl(a))};
+        }
+        for (int k = 0; k < m; ++k) {
+            long double a = 2.0L * pi * k / n;
+            h[m + n + k] = cpx<T>{T(cosl(a)), T(-sinl(a))};
+        }
+        buf.alloc(h.size());
+        HIPCHK(hipMemcpyAsync(buf.p, h.data(), h.size() * sizeof(cpx<T>), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        tw.fft = buf.p;
+        tw.h = buf.p + m;
+        tw.s = buf.p + m + n;
+    }
+};
+
+struct Prof;
+
+struct EngineBase {
+    virtual ~EngineBase() = default;
+    virtual void init(const double* psi0) = 0;
+    virtual void step(int n, Prof* prof) = 0;
+    virtual const char* kernel_class(int c) const = 0;
+    virtual int fetch(int field, double* out) = 0;
+    virtual void set_field(int field, const double* in) = 0;
+    virtual int reduce(int what, double* out) = 0;
+};
+
+template <typename T>
+int download(hipStream_t s, const T* dev, size_t n, double* out)
+{
+    std::vector<T> h(n);
+    HIPCHK(hipMemcpyAsync(h.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int bad = 0;
+    for (size_t i = 0; i < n; ++i) {
+        double v = (double)h[i];
+        if (!std::isfinite(v)) bad = 1;
+        out[i] = v;
+    }
+    return bad ? SMAD_ENONFINITE : SMAD_OK;
+}
+
+template <typename T>
+void upload(hipStream_t s, T* dev, size_t n, const double* in)
+{
+    std::vector<T> h(n);
+    for (size_t i = 0; i < n; ++i) h[i] = T(in[i]);
+    HIPCHK(hipMemcpyAsync(dev, h.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+// Event markers on the context stream: mark(c) after each launch of kernel
+// class c; consecutive events bracket exactly one launch.
+struct Prof {
+    hipStream_t s;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> cls;
+    explicit Prof(hipStream_t st) : s(st) { mark(-1); }
+    void mark(int c)
+    {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(e, s));
+        ev.push_back(e);
+        cls.push_back(c);
+    }
+    // per-class summed milliseconds and launch counts
+    void collect(std::vector<double>& ms, std::vector<int>& cnt)
+    {
+        HIPCHK(hipEventSynchronize(ev.back()));
+        for (size_t i = 1; i < ev.size(); ++i) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, ev[i - 1], ev[i]));
+            int c = cls[i];
+            if (c < 0) continue;
+            if ((size_t)c >= ms.size()) {
+                ms.resize(c + 1, 0.0);
+                cnt.resize(c + 1, 0);
+            }
+            ms[c] += t;
+            cnt[c] += 1;
+        }
+    }
+    ~Prof()
+    {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+// ---------------------------------------------------------------- plain --
+template <typename T>
+struct PlainEngine : EngineBase {
+    smad_params p;
+    hipStream_t s;
+    Dims d;
+    size_t n;
+    PlainCoef cf{};
+    TwTable<T> twx, twy, twz;
+    DevBuf<T> w, psihat, nhat, psi, psi_prev;
+    DevBuf<double> part, red;
+    int nparts = 1024;
+    bool psi_valid = false, prev_valid = false;
+
+    PlainEngine(const smad_params& pp, hipStream_t st) : p(pp), s(st)
+    {
+        d = Dims{p.nx, p.ny, p.nz};
+        n = (size_t)d.total();
+        const double pi = M_PI;
+        double lx = p.nx * 2.0 * pi / (p.q0 * p.nw), ly = p.ny * 2.0 * pi / (p.q0 * p.nw),
+               lz = p.nz * 2.0 * pi / (p.q0 * p.nw);
+        double dx = lx / p.nx, dy = ly / p.ny, dz = lz / p.nz;
+        cf.qfx = pi / (dx * p.nx);
+        cf.qfy = pi / (dy * p.ny);
+        cf.qfz = pi / (dz * p.nz);
+        cf.eps = p.eps;
+        cf.alpha = p.alpha;
+        cf.q0sq = p.q0 * p.q0;
+        cf.beta = p.beta;
+        cf.gamma = p.gamma;
+        cf.dtd2 = p.dt / 2;
+        cf.scale = 0.125 / ((double)p.nx * p.ny * p.nz);
+        twx.make(p.nx, s);
+        twy.make(p.ny, s);
+        twz.make(p.nz, s);
+        w.alloc(n);
+        psihat.alloc(n);
+        nhat.alloc(n);
+        psi.alloc(n);
+        psi_prev.alloc(n);
+        part.alloc(3 * nparts);
+        red.alloc(8);
+    }
+
+    void ct3(const T* in, T* out, T scale)  // out = scale * REDFT10^3(in)
+    {
+        Launch<T>::axis(s, in, out, d, 2, REDFT10, T(1), twz.tw);
+        Launch<T>::axis(s, out, out, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(s, out, out, d, 0, REDFT10, scale, twx.tw);
+        LAUNCHCHK();
+    }
+
+    // pre-loop: N_hat^0, psi_hat^0 = s CT(psi0), w = iCT_y iCT_x psi_hat^0
+    void preloop()
+    {
+        Launch<T>::plain_z(s, psi.p, w.p, nullptr, d, cf, 1, twz.tw);
+        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(s, w.p, nhat.p, d, 0, REDFT10, T(1), twx.tw);
+        ct3(psi.p, psihat.p, T(cf.scale));
+        Launch<T>::axis(s, psihat.p, w.p, d, 0, REDFT01, T(1), twx.tw);
+        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
+        LAUNCHCHK();
+        psi_valid = true;
+        prev_valid = false;
+    }
+
+    void init(const double* psi0) override
+    {
+        upload(s, psi.p, n, psi0);
+        preloop();
+    }
+
+    const char* kernel_class(int c) const override
+    {
+        static const char* names[] = {"plain_z: REDFT01_z, psi^3/psi^5, REDFT10_z",
+                                      "axis_y: REDFT10_y",
+                                      "plain_x: REDFT10_x, CN/AB2 update, REDFT01_x",
+                                      "axis_y: REDFT01_y"};
+        return (c >= 0 && c < 4) ? names[c] : nullptr;
+    }
+
+    void step(int nsteps, Prof* prof) override
+    {
+        for (int it = 0; it < nsteps; ++it) {
+            bool last = it == nsteps - 1;
+            Launch<T>::plain_z(s, w.p, w.p, last ? psi_prev.p : nullptr, d, cf, 0, twz.tw);
+            if (prof) prof->mark(0);
+            Launch<T>::axis(s, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
+            if (prof) prof->mark(1);
+            Launch<T>::plain_x(s, w.p, psihat.p, nhat.p, d, cf, twx.tw);
+            if (prof) prof->mark(2);
+            Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
+            if (prof) prof->mark(3);
+        }
+        LAUNCHCHK();
+        if (nsteps > 0) {
+            psi_valid = false;
+            prev_valid = true;
+        }
+    }
+
+    void materialize()
+    {
+        if (!psi_valid) {
+            Launch<T>::axis(s, w.p, psi.p, d, 2, REDFT01, T(1), twz.tw);
+            LAUNCHCHK();
+            psi_valid = true;
+        }
+    }
+
+    int fetch(int field, double* out) override
+    {
+        switch (field) {
+        case SMAD_FIELD_PSI: materialize(); return download(s, psi.p, n, out);
+        case SMAD_FIELD_PSI_HAT: return download(s, psihat.p, n, out);
+        case SMAD_FIELD_N_HAT: return download(s, nhat.p, n, out);
+        default: throw ApiError{SMAD_EINVAL, "field not available in the plain model"};
+        }
+    }
+
+    void set_field(int field, const double* in) override
+    {
+        switch (field) {
+        case SMAD_FIELD_PSI:
+            upload(s, psi.p, n, in);
+            preloop();
+            break;
+        case SMAD_FIELD_PSI_HAT:
+            upload(s, psihat.p, n, in);
+            Launch<T>::axis(s, psihat.p, w.p, d, 0, REDFT01, T(1), twx.tw);
+            Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
+            LAUNCHCHK();
+            psi_valid = false;
+            prev_valid = false;
+            break;
+        case SMAD_FIELD_N_HAT: upload(s, nhat.p, n, in); break;
+        default: throw ApiError{SMAD_EINVAL, "field cannot be set in the plain model"};
+        }
+    }
+
+    int reduce(int what, double* out) override
+    {
+        if (what != SMAD_RED_L1) throw ApiError{SMAD_EINVAL, "unknown reduction"};
+        if (!prev_valid) throw ApiError{SMAD_ESTATE, "L1 needs at least one step since init"};
+        materialize();
+        Launch<T>::l1(s, psi.p, psi_prev.p, (long)n, part.p, nparts, red.p);
+        LAUNCHCHK();
+        double h[3];
+        HIPCHK(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        out[0] = h[0];
+        out[1] = h[1];
+        out[2] = h[0] / (p.dt * h[1]);
+        return h[2] > 0 ? SMAD_ENONFINITE : SMAD_OK;
+    }
+};
+
+bool length_ok(int n)
+{
+    return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 384 || n == 512;
+}
+
+}  // namespace
+
+struct smad_ctx {
+    smad_params p{};
+    hipStream_t stream = nullptr;
+    std::unique_ptr<EngineBase> eng;
+    std::string err;
+    long steps = 0;
+};
+
+namespace {
+
+template <typename F>
+int guarded(smad_ctx* c, F&& f)
+{
+    if (!c) return SMAD_EINVAL;
+    try {
+        return f();
+    } catch (const HipError& e) {
+        c->err = e.msg;
+        return SMAD_EHIP;
+    } catch (const ApiError& e) {
+        c->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return SMAD_EINVAL;
+    }
+}
+
+}  // namespace
+
+namespace {
+template <typename T>
+void r2r3d_impl(int nx, int ny, int nz, const int kinds[3], const double* in, double* out)
+{
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{s};
+    Dims d{nx, ny, nz};
+    size_t n = (size_t)d.total();
+    DevBuf<T> buf;
+    buf.alloc(n);
+    TwTable<T> tx, ty, tz;
+    tx.make(nx, s);
+    ty.make(ny, s);
+    tz.make(nz, s);
+    upload(s, buf.p, n, in);
+    Launch<T>::axis(s, buf.p, buf.p, d, 2, kinds[2], T(1), tz.tw);
+    Launch<T>::axis(s, buf.p, buf.p, d, 1, kinds[1], T(1), ty.tw);
+    Launch<T>::axis(s, buf.p, buf.p, d, 0, kinds[0], T(1), tx.tw);
+    LAUNCHCHK();
+    (void)download(s, buf.p, n, out);
+}
+}  // namespace
+
+extern "C" {
+
+int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
+{
+    g_init_error.clear();
+    if (!params || !psi0 || !out) {
+        g_init_error = "smad_init: null argument";
+        return SMAD_EINVAL;
+    }
+    *out = nullptr;
+    const smad_params& p = *params;
+    if (!length_ok(p.nx) || !length_ok(p.ny) || !length_ok(p.nz)) {
+        g_init_error = "smad_init: each grid size must be one of 16, 32, 64, 128, 256, 384, 512";
+        return SMAD_EINVAL;
+    }
+    if (p.precision != SMAD_FP64 && p.precision != SMAD_FP32_MIXED) {
+        g_init_error = "smad_init: precision must be 64 or 32";
+        return SMAD_EINVAL;
+    }
+    if (p.model != SMAD_MODEL_PLAIN) {
+        g_init_error = "smad_init: model not supported by this build";
+        return SMAD_EINVAL;
+    }
+    if (!(p.dt > 0) || !(p.nw > 0) || !(p.q0 > 0)) {
+        g_init_error = "smad_init: dt, nw and q0 must be positive";
+        return SMAD_EINVAL;
+    }
+    auto* c = new smad_ctx;
+    c->p = p;
+    int rc = SMAD_OK;
+    try {
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        if (p.precision == SMAD_FP64)
+            c->eng.reset(new PlainEngine<double>(p, c->stream));
+        else
+            c->eng.reset(new PlainEngine<float>(p, c->stream));
+        c->eng->init(psi0);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (const HipError& e) {
+        g_init_error = e.msg;
+        rc = SMAD_EHIP;
+    } catch (const ApiError& e) {
+        g_init_error = e.msg;
+        rc = e.code;
+    } catch (const std::exception& e) {
+        g_init_error = e.what();
+        rc = SMAD_EINVAL;
+    }
+    if (rc != SMAD_OK) {
+        smad_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return SMAD_OK;
+}
+
+int smad_step(smad_ctx* c, int nsteps)
+{
+    return guarded(c, [&] {
+        if (nsteps < 0) throw ApiError{SMAD_EINVAL, "nsteps < 0"};
+        c->eng->step(nsteps, nullptr);
+        c->steps += nsteps;
+        return SMAD_OK;
+    });
+}
+
+int smad_time_steps(smad_ctx* c, int nsteps, double* ms)
+{
+    return guarded(c, [&] {
+        if (nsteps < 1 || !ms) throw ApiError{SMAD_EINVAL, "nsteps < 1 or null output"};
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        struct G {
+            hipEvent_t a, b;
+            ~G()
+            {
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+            }
+        } g{e0, e1};
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipEventRecord(e0, c->stream));
+        c->eng->step(nsteps, nullptr);
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        *ms = t;
+        c->steps += nsteps;
+        return SMAD_OK;
+    });
+}
+
+int smad_profile_steps(smad_ctx* c, int nsteps, double* kernel_ms, int* launches, int nmax,
+                       int* nclasses)
+{
+    return guarded(c, [&] {
+        if (nsteps < 1 || !kernel_ms || !launches || !nclasses || nmax < 1)
+            throw ApiError{SMAD_EINVAL, "bad arguments"};
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<double> ms;
+        std::vector<int> cnt;
+        {
+            Prof prof(c->stream);
+            c->eng->step(nsteps, &prof);
+            prof.collect(ms, cnt);
+        }
+        c->steps += nsteps;
+        int nc = (int)ms.size();
+        *nclasses = nc;
+        for (int i = 0; i < nmax; ++i) {
+            kernel_ms[i] = i < nc ? ms[i] : 0.0;
+            launches[i] = i < nc ? cnt[i] : 0;
+        }
+        return SMAD_OK;
+    });
+}
+
+const char* smad_kernel_class_name(const smad_ctx* c, int cls)
+{
+    if (!c || !c->eng) return "";
+    const char* n = c->eng->kernel_class(cls);
+    return n ? n : "";
+}
+
+int smad_fetch(smad_ctx* c, int field, double* host_out)
+{
+    return guarded(c, [&] {
+        if (!host_out) throw ApiError{SMAD_EINVAL, "null output buffer"};
+        int rc = c->eng->fetch(field, host_out);
+        if (rc == SMAD_ENONFINITE) c->err = "fetched field holds NaN or Inf";
+        return rc;
+    });
+}
+
+int smad_set_field(smad_ctx* c, int field, const double* host_in)
+{
+    return guarded(c, [&] {
+        if (!host_in) throw ApiError{SMAD_EINVAL, "null input buffer"};
+        c->eng->set_field(field, host_in);
+        return SMAD_OK;
+    });
+}
+
+int smad_reduce(smad_ctx* c, int what, double* out)
+{
+    return guarded(c, [&] {
+        if (!out) throw ApiError{SMAD_EINVAL, "null output buffer"};
+        int rc = c->eng->reduce(what, out);
+        if (rc == SMAD_ENONFINITE) c->err = "psi holds NaN or Inf";
+        return rc;
+    });
+}
+
+int smad_sync(smad_ctx* c)
+{
+    return guarded(c, [&] {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return SMAD_OK;
+    });
+}
+
+
+int smad_r2r_3d(int nx, int ny, int nz, const int kinds[3], int precision, const double* in,
+                double* out)
+{
+    g_init_error.clear();
+    if (!kinds || !in || !out || !length_ok(nx) || !length_ok(ny) || !length_ok(nz)) {
+        g_init_error = "smad_r2r_3d: bad argument or unsupported size";
+        return SMAD_EINVAL;
+    }
+    for (int a = 0; a < 3; ++a)
+        if (kinds[a] < 0 || kinds[a] > 3) {
+            g_init_error = "smad_r2r_3d: bad kind";
+            return SMAD_EINVAL;
+        }
+    try {
+        if (precision == SMAD_FP64) r2r3d_impl<double>(nx, ny, nz, kinds, in, out);
+        else if (precision == SMAD_FP32_MIXED) r2r3d_impl<float>(nx, ny, nz, kinds, in, out);
+        else {
+            g_init_error = "smad_r2r_3d: precision must be 64 or 32";
+            return SMAD_EINVAL;
+        }
+    } catch (const HipError& e) {
+        g_init_error = e.msg;
+        return SMAD_EHIP;
+    } catch (const ApiError& e) {
+        g_init_error = e.msg;
+        return e.code;
+    }
+    return SMAD_OK;
+}
+
+void* smad_stream(smad_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+long smad_steps_taken(const smad_ctx* c) { return c ? c->steps : -1; }
+
+void smad_destroy(smad_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->eng.reset();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* smad_last_error(const smad_ctx* c)
+{
+    if (!c) return g_init_error.c_str();
+    return c->err.c_str();
+}
+
+}  // extern "C"
