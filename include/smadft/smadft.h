@@ -113,6 +113,18 @@ int smad_reduce(smad_ctx *ctx, int what, double *out);
 /* Wait for all work on the context stream. */
 int smad_sync(smad_ctx *ctx);
 
+/* ----------------------------------------------------------------------
+ * Adjoint (plain model; no counterpart in the reference, which has no
+ * gradients).  smad_forward_tape runs nsteps steps from the pre-loop state
+ * (right after smad_init or smad_set_field(PSI)), saving (psi_hat, N_hat)
+ * every ckpt_every steps.  smad_adjoint then takes the cotangent
+ * dL/dpsi_K (host, FP64) and returns dL/dpsi_0 (host) and
+ * dL/d(eps, alpha, beta, gamma) by the discrete adjoint of every step,
+ * recomputing each segment from its checkpoint.  Afterwards the context is
+ * left at step K, as after the forward run. */
+int smad_forward_tape(smad_ctx *ctx, int nsteps, int ckpt_every);
+int smad_adjoint(smad_ctx *ctx, const double *dl_dpsik, double *dl_dpsi0, double *dl_dparams);
+
 /* Enqueue nsteps steps between two HIP events recorded on the context
  * stream and wait; *ms = elapsed device time in milliseconds.  The
  * reference's only timers are std::clock / chrono around the whole loop

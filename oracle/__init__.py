@@ -48,7 +48,10 @@ def lib():
         L.orc_l1_sums.argtypes = [dp, dp, ctypes.c_long, dp]
         L.orc_adv_preloop.argtypes = [pp, dp, dp]
         L.orc_adv_step.argtypes = [pp, dp, dp, dp, dp, dp, ctypes.c_int]
-        for f in ("orc_r2r1d_naive", "orc_r2r1d", "orc_r2r3d", "orc_plain_preloop",
+        L.orc_adv_stokes.argtypes = [pp, dp, dp, dp, dp, dp, dp]
+        L.orc_adv_stokes.restype = None
+        L.orc_plain_adjoint.argtypes = [pp, dp, ctypes.c_int, dp, dp, dp, dp]
+        for f in ("orc_plain_adjoint", "orc_r2r1d_naive", "orc_r2r1d", "orc_r2r3d", "orc_plain_preloop",
                   "orc_plain_step", "orc_l1_sums", "orc_adv_preloop", "orc_adv_step"):
             getattr(L, f).restype = None
         _LIB = L
@@ -118,3 +121,31 @@ def l1_sums(new, old):
     b = np.ascontiguousarray(old, dtype=np.float64)
     lib().orc_l1_sums(_p(a), _p(b), a.size, _p(out))
     return out
+
+
+def plain_adjoint(p, psi0, nsteps, g_psik):
+    """Returns (psi_K, dJ/dpsi0, dJ/d(eps, alpha, beta, gamma)) for the
+    cotangent g_psik = dJ/dpsi_K."""
+    psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
+    g = np.ascontiguousarray(g_psik, dtype=np.float64)
+    psik = np.empty_like(psi0)
+    d0 = np.empty_like(psi0)
+    dp = np.zeros(4)
+    op = oparams(p)
+    lib().orc_plain_adjoint(ctypes.byref(op), _p(psi0), int(nsteps), _p(g), _p(psik), _p(d0), _p(dp))
+    return psik, d0, dp
+
+
+def plain_forward(p, psi0, nsteps):
+    o = PlainOracle(p, psi0)
+    o.step(nsteps)
+    return o.psi
+
+
+def adv_stokes(p, sx, sy, sz):
+    """Velocity projection alone (spectral in, spectral out)."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (sx, sy, sz)]
+    out = [np.empty_like(a[0]) for _ in range(3)]
+    op = oparams(p)
+    lib().orc_adv_stokes(ctypes.byref(op), *[_p(v) for v in a], *[_p(v) for v in out])
+    return tuple(out)

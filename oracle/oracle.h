@@ -53,6 +53,15 @@ void orc_l1_sums(const double *a_new, const double *b_old, long n, double *out);
 void orc_adv_preloop(const orc_params *p, const double *psi, double *nq);
 void orc_adv_step(const orc_params *p, double *psi, double *nq, double *nq_past, double *grad,
                   double *vel, int nsteps);
+/* the velocity projection alone (adv/cosAdvTrackMult.cpp:1103-1274) */
+void orc_adv_stokes(const orc_params *p, const double *sx, const double *sy, const double *sz,
+                    double *vx, double *vy, double *vz);
+
+/* --------------------------------------------------- plain adjoint -- */
+/* K plain steps from psi0, then the discrete adjoint for the cotangent
+ * g_psik = dJ/dpsi^K.  Outputs psi^K, dJ/dpsi0 and dJ/d(eps, alpha, beta, gamma). */
+void orc_plain_adjoint(const orc_params *p, const double *psi0, int nsteps, const double *g_psik,
+                       double *psik_out, double *dpsi0, double *dparams);
 
 #ifdef __cplusplus
 }

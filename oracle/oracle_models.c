@@ -135,6 +135,52 @@ void orc_l1_sums(const double *a_new, const double *b_old, long n, double *out)
 
 /* -------------------------------------------------------- advective model -- */
 
+/* Stokes projection of adv/cosAdvTrackMult.cpp:1103-1274 on the global
+ * (single-slab) grid: outputs are fully overwritten (0 where the reference
+ * leaves its zero-filled arrays untouched). */
+void orc_adv_stokes(const orc_params *p, const double *sx, const double *sy, const double *sz,
+                    double *vx, double *vy, double *vz)
+{
+    grid_t g;
+    grid_make(p, &g);
+    const double nu = p->nu, sc = g.scale;
+    memset(vx, 0, sizeof(double) * g.n);
+    memset(vy, 0, sizeof(double) * g.n);
+    memset(vz, 0, sizeof(double) * g.n);
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    if (i < g.nx - 1) {
+                        double syv = (j != 0) ? sy[IDX(i + 1, j - 1, k)] : 0.0;
+                        double szv = (k != 0) ? sz[IDX(i + 1, j, k - 1)] : 0.0;
+                        double mq2 = pow(g.sx[i], 2) + pow(g.qy[j], 2) + pow(g.qz[k], 2);
+                        double cm1 = sc / (nu * mq2), cm2 = g.sx[i] / mq2;
+                        double dot = g.sx[i] * sx[id] + g.qy[j] * syv + g.qz[k] * szv;
+                        vx[id] = cm1 * (sx[id] - cm2 * dot);
+                    }
+                    if (j < g.ny - 1 && k < g.nz - 1) {
+                        double sxv = (i != 0) ? sx[IDX(i - 1, j + 1, k)] : 0.0;
+                        double szv = (k != 0) ? sz[IDX(i, j + 1, k - 1)] : 0.0;
+                        double mq2 = pow(g.qx[i], 2) + pow(g.sy[j], 2) + pow(g.qz[k], 2);
+                        double cm1 = sc / (nu * mq2), cm2 = g.sy[j] / mq2;
+                        double dot = g.qx[i] * sxv + g.sy[j] * sy[id] + g.qz[k] * szv;
+                        vy[id] = cm1 * (sy[id] - cm2 * dot);
+
+                        sxv = (i != 0) ? sx[IDX(i - 1, j, k + 1)] : 0.0;
+                        double syv = (j != 0) ? sy[IDX(i, j - 1, k + 1)] : 0.0;
+                        mq2 = pow(g.qx[i], 2) + pow(g.qy[j], 2) + pow(g.sz[k], 2);
+                        cm1 = sc / (nu * mq2);
+                        cm2 = g.sz[k] / mq2;
+                        dot = g.qx[i] * sxv + g.qy[j] * syv + g.sz[k] * sz[id];
+                        vz[id] = cm1 * (sz[id] - cm2 * dot);
+                    }
+                }
+    grid_free(&g);
+}
+
+
 void orc_adv_preloop(const orc_params *p, const double *psi, double *nq)
 {
     /* adv/cosAdvTrackMult.cpp:760-777: N without advection */
@@ -147,7 +193,7 @@ void orc_adv_step(const orc_params *p, double *psi, double *nq, double *nq_past,
     grid_t g;
     grid_make(p, &g);
     const long n = g.n;
-    const double dtd2 = p->dt / 2, nu = p->nu, sc = g.scale;
+    const double dtd2 = p->dt / 2, sc = g.scale;
     double *psiq = malloc(sizeof(double) * n);
     double *gx = calloc(n, sizeof(double)), *gy = calloc(n, sizeof(double)), *gz = calloc(n, sizeof(double));
     double *vx = calloc(n, sizeof(double)), *vy = calloc(n, sizeof(double)), *vz = calloc(n, sizeof(double));
@@ -200,37 +246,8 @@ void orc_adv_step(const orc_params *p, double *psi, double *nq, double *nq_past,
         orc_r2r3d(sy, sy, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_RODFT10, ORC_REDFT10);
         orc_r2r3d(sz, sz, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_REDFT10, ORC_RODFT10);
 
-        /* Stokes projection :1103-1274, global (single-slab) indexing */
-#pragma omp parallel for collapse(2) schedule(static)
-        for (int i = 0; i < g.nx; ++i)
-            for (int j = 0; j < g.ny; ++j)
-                for (int k = 0; k < g.nz; ++k) {
-                    long id = IDX(i, j, k);
-                    if (i < g.nx - 1) {
-                        double syv = (j != 0) ? sy[IDX(i + 1, j - 1, k)] : 0.0;
-                        double szv = (k != 0) ? sz[IDX(i + 1, j, k - 1)] : 0.0;
-                        double mq2 = pow(g.sx[i], 2) + pow(g.qy[j], 2) + pow(g.qz[k], 2);
-                        double cm1 = sc / (nu * mq2), cm2 = g.sx[i] / mq2;
-                        double dot = g.sx[i] * sx[id] + g.qy[j] * syv + g.qz[k] * szv;
-                        vx[id] = cm1 * (sx[id] - cm2 * dot);
-                    }
-                    if (j < g.ny - 1 && k < g.nz - 1) {
-                        double sxv = (i != 0) ? sx[IDX(i - 1, j + 1, k)] : 0.0;
-                        double szv = (k != 0) ? sz[IDX(i, j + 1, k - 1)] : 0.0;
-                        double mq2 = pow(g.qx[i], 2) + pow(g.sy[j], 2) + pow(g.qz[k], 2);
-                        double cm1 = sc / (nu * mq2), cm2 = g.sy[j] / mq2;
-                        double dot = g.qx[i] * sxv + g.sy[j] * sy[id] + g.qz[k] * szv;
-                        vy[id] = cm1 * (sy[id] - cm2 * dot);
-
-                        sxv = (i != 0) ? sx[IDX(i - 1, j, k + 1)] : 0.0;
-                        double syv = (j != 0) ? sy[IDX(i, j - 1, k + 1)] : 0.0;
-                        mq2 = pow(g.qx[i], 2) + pow(g.qy[j], 2) + pow(g.sz[k], 2);
-                        cm1 = sc / (nu * mq2);
-                        cm2 = g.sz[k] / mq2;
-                        dot = g.qx[i] * sxv + g.qy[j] * syv + g.sz[k] * sz[id];
-                        vz[id] = cm1 * (sz[id] - cm2 * dot);
-                    }
-                }
+        /* Stokes projection :1103-1274 */
+        orc_adv_stokes(p, sx, sy, sz, vx, vy, vz);
         orc_r2r3d(vx, vx, g.nx, g.ny, g.nz, ORC_RODFT01, ORC_REDFT01, ORC_REDFT01); /* :1276-1286 */
         orc_r2r3d(vy, vy, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_RODFT01, ORC_REDFT01);
         orc_r2r3d(vz, vz, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_REDFT01, ORC_RODFT01);
@@ -258,5 +275,112 @@ void orc_adv_step(const orc_params *p, double *psi, double *nq, double *nq_past,
     if (vel) { memcpy(vel, vx, sizeof(double) * n); memcpy(vel + n, vy, sizeof(double) * n); memcpy(vel + 2 * n, vz, sizeof(double) * n); }
     free(psiq); free(gx); free(gy); free(gz); free(vx); free(vy); free(vz);
     free(sp2); free(sx); free(sy); free(sz); free(nr);
+    grid_free(&g);
+}
+
+/* ------------------------------------------------------ plain adjoint -- */
+/*
+ * Discrete adjoint of K plain steps (no reference counterpart: the
+ * reference has no adjoint).  Forward in the scaled spectral state
+ *   psi^n = iCT(ph^n),  Nh^n = CT(f(psi^n)),  f = beta psi^3 - gamma psi^5,
+ *   ph^{n+1} = a ph^n + c (3 Nh^n - Nh^{n-1}),  Nh^{-1} = Nh^0,
+ *   a = C1/C2, c = s dt/2 / C2,  ph^0 = s CT(psi^0).
+ * With lt = D lambda (D = prod over axes of diag(2,1,...,1)) the backward
+ * recursion needs no D:
+ *   lt^K = CT(g),  g = dJ/dpsi^K,  lt^{K+1} = 0
+ *   mt^n = 3c lt^{n+1} - c lt^{n+2}   (2c instead of 3c at n = 0)
+ *   r^n  = iCT(mt^n);  lt^n = a lt^{n+1} + CT(f'(psi^n) r^n)
+ *   dJ/dpsi^0 = s iCT(lt^0)
+ *   dJ/dbeta  = sum_n <r^n, psi^n^3>,  dJ/dgamma = -sum_n <r^n, psi^n^5>
+ *   dJ/dL(q)  = sum_n w lt^{n+1} dt/2/C2 (ph^n + ph^{n+1}),  w = D^{-1}
+ *   dJ/deps = sum_q dJ/dL,  dJ/dalpha = -sum_q (|q|^2 - q0^2)^2 dJ/dL.
+ * Pinned by central finite differences and the dot-product identity in
+ * tests/test_oracle.py.
+ */
+void orc_plain_adjoint(const orc_params *p, const double *psi0, int nsteps, const double *g_psik,
+                       double *psik_out, double *dpsi0, double *dparams)
+{
+    grid_t g;
+    grid_make(p, &g);
+    const long n = g.n;
+    const double dtd2 = p->dt / 2, sc = g.scale;
+    const int K = nsteps;
+    double *psi = malloc(sizeof(double) * n * (K + 1));
+    double *ph = malloc(sizeof(double) * n * (K + 1));
+    double *nq = malloc(sizeof(double) * n), *nqp = malloc(sizeof(double) * n);
+    double *tmp = malloc(sizeof(double) * n);
+
+    /* forward, reference order (no-adv/cosNoAdvTrackMult.cpp:630-805) */
+    memcpy(psi, psi0, sizeof(double) * n);
+    ct(&g, psi, ph);
+    for (long i = 0; i < n; ++i) ph[i] *= sc;
+    nonlin(p, psi, tmp, n);
+    ct(&g, tmp, nq);
+    for (int s = 0; s < K; ++s) {
+        const double *ps = psi + (long)s * n;
+        memcpy(nqp, nq, sizeof(double) * n);
+        nonlin(p, ps, tmp, n);
+        ct(&g, tmp, nq);
+        double *phn = ph + (long)(s + 1) * n;
+        ct(&g, ps, phn);
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    double al = alin_at(p, &g, i, j, k);
+                    double c1 = 1.0 + dtd2 * al, c2 = 1.0 - dtd2 * al;
+                    phn[id] = sc * (c1 * phn[id] + dtd2 * (3.0 * nq[id] - nqp[id])) / c2;
+                }
+        ict(&g, phn, psi + (long)(s + 1) * n);
+    }
+    if (psik_out) memcpy(psik_out, psi + (long)K * n, sizeof(double) * n);
+
+    /* backward */
+    double *lt1 = calloc(n, sizeof(double)), *lt2 = calloc(n, sizeof(double));
+    double *mt = malloc(sizeof(double) * n);
+    double deps = 0.0, dalpha = 0.0, dbeta = 0.0, dgamma = 0.0;
+    ct(&g, g_psik, lt1); /* lt^K */
+    for (int s = K - 1; s >= 0; --s) {
+        /* lt1 = lt^{s+1}, lt2 = lt^{s+2} */
+        const double *phs = ph + (long)s * n, *phs1 = ph + (long)(s + 1) * n;
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    double mq2 = pow(g.qx[i], 2) + pow(g.qy[j], 2) + pow(g.qz[k], 2);
+                    double t = mq2 - p->q0 * p->q0;
+                    double al = p->eps - p->alpha * t * t;
+                    double c2 = 1.0 - dtd2 * al;
+                    double c = sc * dtd2 / c2;
+                    double w = (i == 0 ? 0.5 : 1.0) * (j == 0 ? 0.5 : 1.0) * (k == 0 ? 0.5 : 1.0);
+                    double term = w * lt1[id] * dtd2 / c2 * (phs[id] + phs1[id]);
+                    deps += term;
+                    dalpha += -t * t * term;
+                    mt[id] = (s >= 1 ? 3.0 : 2.0) * c * lt1[id] - c * lt2[id];
+                }
+        ict(&g, mt, mt); /* r^s */
+        const double *ps = psi + (long)s * n;
+        for (long id = 0; id < n; ++id) {
+            double x = ps[id], x2 = x * x;
+            dbeta += mt[id] * x2 * x;
+            dgamma -= mt[id] * x2 * x2 * x;
+            tmp[id] = (3.0 * p->beta * x2 - 5.0 * p->gamma * x2 * x2) * mt[id];
+        }
+        ct(&g, tmp, tmp);
+        /* lt^s = a lt^{s+1} + CT(...) -> stored in lt2, then swap */
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    double al = alin_at(p, &g, i, j, k);
+                    double a = (1.0 + dtd2 * al) / (1.0 - dtd2 * al);
+                    lt2[id] = a * lt1[id] + tmp[id];
+                }
+        double *sw = lt1; lt1 = lt2; lt2 = sw;
+    }
+    ict(&g, lt1, dpsi0);
+    for (long i = 0; i < n; ++i) dpsi0[i] *= sc;
+    dparams[0] = deps; dparams[1] = dalpha; dparams[2] = dbeta; dparams[3] = dgamma;
+    free(psi); free(ph); free(nq); free(nqp); free(tmp); free(lt1); free(lt2); free(mt);
     grid_free(&g);
 }

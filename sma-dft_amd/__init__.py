@@ -37,7 +37,7 @@ EXPORTS = (
     "smad_sync", "smad_stream", "smad_steps_taken", "smad_destroy",
     "smad_last_error", "smad_ic_focal_conic", "smad_ic_slab", "smad_ic_curved",
     "smad_ic_add_noise", "smad_default_params", "smad_r2r_3d", "smad_time_steps",
-    "smad_profile_steps", "smad_kernel_class_name",
+    "smad_profile_steps", "smad_kernel_class_name", "smad_forward_tape", "smad_adjoint",
 )
 
 REDFT10, REDFT01, RODFT10, RODFT01 = 0, 1, 2, 3
@@ -129,6 +129,8 @@ def lib() -> ctypes.CDLL:
         L.smad_ic_add_noise.argtypes = [P(CParams), ctypes.c_double, ctypes.c_ulonglong, dp]
         L.smad_default_params.argtypes = [ctypes.c_int, ctypes.c_int, P(CParams)]
         L.smad_time_steps.argtypes = [vp, ctypes.c_int, dp]
+        L.smad_forward_tape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.smad_adjoint.argtypes = [vp, dp, dp, dp]
         L.smad_profile_steps.argtypes = [vp, ctypes.c_int, dp, P(ctypes.c_int), ctypes.c_int,
                                          P(ctypes.c_int)]
         L.smad_kernel_class_name.argtypes = [vp, ctypes.c_int]
@@ -241,6 +243,22 @@ class Stepper:
     def l1(self) -> float:
         """L1 = sum|psi_new - psi_old| / (dt sum|psi_new|) over the last step."""
         return float(self.reduce(RED_L1)[2])
+
+    def forward_tape(self, nsteps: int, ckpt_every: int):
+        """nsteps steps from the pre-loop state, checkpointing every
+        ckpt_every steps for a following :meth:`adjoint`."""
+        self._check(lib().smad_forward_tape(self._h, int(nsteps), int(ckpt_every)))
+
+    def adjoint(self, dl_dpsik: np.ndarray):
+        """Discrete adjoint of the taped run: returns (dL/dpsi0,
+        dL/d(eps, alpha, beta, gamma)) for the cotangent dL/dpsi_K."""
+        g = np.ascontiguousarray(dl_dpsik, dtype=np.float64)
+        if g.shape != self.params.shape:
+            raise ValueError("shape mismatch")
+        d0 = np.empty_like(g)
+        dp = np.zeros(4)
+        self._check(lib().smad_adjoint(self._h, _dptr(g), _dptr(d0), _dptr(dp)))
+        return d0, dp
 
     def time_steps(self, n: int) -> float:
         """Run n steps between two HIP events on the context stream; returns ms."""

@@ -74,14 +74,34 @@ inline int num_tiles(const Dims& d, int axis, int NL)
     return (axis == 1 ? d.nx : d.ny) * nct;
 }
 
-// Iterate the N x NL elements of a tile so that a wavefront touches
-// contiguous global memory: for the contiguous axis p runs fastest, for the
-// strided axes c runs fastest.
+// 16-byte vector of the storage type (2 doubles or 4 floats).
+template <typename T>
+struct alignas(16) V16 {
+    static constexpr int W = 16 / sizeof(T);
+    T v[W];
+};
+
+// Iterate a strided tile (axis x or y) in 16-byte groups: f(p, c0, gi) for
+// columns c0 .. c0+W-1 of row p.  A wavefront covers 64 groups, i.e. whole
+// 128 B row segments.  Requires a full tile (ncol == NL).
+template <typename T, int N, int NL, typename F>
+__device__ __forceinline__ void rows_vec(const TileGeo& g, F&& f)
+{
+    constexpr int W = V16<T>::W;
+    constexpr int GPR = NL / W;
+    constexpr int TOT = N * GPR;
+#pragma unroll 8
+    for (int e = threadIdx.x; e < TOT; e += WG) {
+        int p = e / GPR, c0 = (e - p * GPR) * W;
+        f(p, c0, g.base + (long)p * g.stride + c0);
+    }
+}
+
+// Scalar iteration of any tile (partial tiles, small grids).
 template <int N, int NL, typename F>
 __device__ __forceinline__ void tile_for(const TileGeo& g, int axis, F&& f)
 {
     constexpr int TOT = N * NL;
-#pragma unroll 4
     for (int e = threadIdx.x; e < TOT; e += WG) {
         int p, c;
         if (axis == 2) { c = e / N; p = e - c * N; }
@@ -92,26 +112,135 @@ __device__ __forceinline__ void tile_for(const TileGeo& g, int axis, F&& f)
     }
 }
 
+// Load a strided tile into LDS: lds[c*LS + pos(p)] = sgn(p) * src[(p, c)].
+template <typename T, int N, int NL, int LS, typename Pos>
+__device__ __forceinline__ void load_rows(const TileGeo& g, const T* src, T* lds, Pos pos)
+{
+    if (g.ncol == NL) {
+        rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
+            V16<T> v = *reinterpret_cast<const V16<T>*>(src + gi);
+            int q;
+            T sg = pos(p, q);
+#pragma unroll
+            for (int t = 0; t < V16<T>::W; ++t) lds[(c0 + t) * LS + q] = sg * v.v[t];
+        });
+    } else {
+        tile_for<N, NL>(g, 1, [&](int p, int c, long gi, bool ok) {
+            int q;
+            T sg = pos(p, q);
+            lds[c * LS + q] = ok ? sg * src[gi] : T(0);
+        });
+    }
+}
+
+// Store a strided tile from LDS: dst[(p, c)] = scale * sgn(p) * lds[c*LS + pos(p)].
+template <typename T, int N, int NL, int LS, typename Pos>
+__device__ __forceinline__ void store_rows(const TileGeo& g, T* dst, const T* lds, Pos pos, T scale)
+{
+    if (g.ncol == NL) {
+        rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
+            int q;
+            T sg = scale * pos(p, q);
+            V16<T> v;
+#pragma unroll
+            for (int t = 0; t < V16<T>::W; ++t) v.v[t] = sg * lds[(c0 + t) * LS + q];
+            *reinterpret_cast<V16<T>*>(dst + gi) = v;
+        });
+    } else {
+        tile_for<N, NL>(g, 1, [&](int p, int c, long gi, bool ok) {
+            int q;
+            T sg = scale * pos(p, q);
+            if (ok) dst[gi] = sg * lds[c * LS + q];
+        });
+    }
+}
+
+// z lines are contiguous: each wavefront loads, transforms and stores only
+// its own NL/4 lines, so the z passes need no workgroup barrier after the
+// twiddle table is in LDS.  f(p, c, gi) per 16-byte group of line c.
+template <typename T, int N, int NL, typename F>
+__device__ __forceinline__ void zlines_vec(const Dims& d, F&& f)
+{
+    constexpr int W = V16<T>::W;
+    constexpr int LPW = NL / (WG / 64);  // lines per wavefront
+    constexpr int TOT = LPW * N / W;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long nlines = (long)d.nx * d.ny;
+    const long l0 = (long)blockIdx.x * NL + wv * LPW;
+#pragma unroll 4
+    for (int e = lane; e < TOT; e += 64) {
+        int lc = (e * W) / N;
+        int p = e * W - lc * N;
+        if (l0 + lc < nlines) f(p, wv * LPW + lc, (l0 + lc) * N + p);
+    }
+}
+
+// Makhoul input position for a forward kind / natural position otherwise,
+// with the DST sign and reversal folded in (see in_pos / in_neg).
+template <typename T, int N>
+struct InMap {
+    int kind;
+    __device__ __forceinline__ T operator()(int p, int& q) const
+    {
+        q = in_pos<N>(kind, p);
+        return in_neg(kind, p) ? T(-1) : T(1);
+    }
+};
+template <typename T, int N>
+struct OutMap {
+    int kind;
+    __device__ __forceinline__ T operator()(int p, int& q) const
+    {
+        q = out_pos<N>(kind, p);
+        return out_neg(kind, p) ? T(-1) : T(1);
+    }
+};
+
 // ------------------------------------------------------------------------
 // Generic single-axis r2r pass: out = scale * R2R_kind(in) along `axis`.
+// in may equal out.
 template <typename T, int N>
-__global__ __launch_bounds__(WG) void k_axis(const T* in, T* out, Dims d,
-                                             int axis, int kind, T scale, Tw<T> tw)
+__global__ __launch_bounds__(WG) void k_axis(const T* in, T* out, Dims d, int axis, int kind,
+                                             T scale, Tw<T> tw)
 {
     constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
     __shared__ T lds[NL * LS];
-    TileGeo g = tile_geo(d, axis, blockIdx.x, NL);
-    tile_for<N, NL>(g, axis, [&](int p, int c, long gi, bool ok) {
-        T v = ok ? in[gi] : T(0);
-        lds[c * LS + in_pos<N>(kind, p)] = in_neg(kind, p) ? -v : v;
-    });
-    __syncthreads();
-    int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
-    r2r_line<T, N, TPL>(lds + l * LS, u, kind, tw);
-    tile_for<N, NL>(g, axis, [&](int p, int c, long gi, bool ok) {
-        T v = lds[c * LS + out_pos<N>(kind, p)];
-        if (ok) out[gi] = scale * (out_neg(kind, p) ? -v : v);
-    });
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    if (axis == 2) {
+        __syncthreads();
+        InMap<T, N> im{kind};
+        zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+            V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
+#pragma unroll
+            for (int t = 0; t < V16<T>::W; ++t) {
+                int q;
+                T sg = im(p + t, q);
+                lds[c * LS + q] = sg * v.v[t];
+            }
+        });
+        lsync<true>();
+        r2r_line<T, N, TPL, true>(lds + l * LS, u, kind, ct);
+        OutMap<T, N> om{kind};
+        zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+            V16<T> v;
+#pragma unroll
+            for (int t = 0; t < V16<T>::W; ++t) {
+                int q;
+                T sg = scale * om(p + t, q);
+                v.v[t] = sg * lds[c * LS + q];
+            }
+            *reinterpret_cast<V16<T>*>(out + gi) = v;
+        });
+    } else {
+        TileGeo g = tile_geo(d, axis, blockIdx.x, NL);
+        load_rows<T, N, NL, LS>(g, in, lds, InMap<T, N>{kind});
+        __syncthreads();
+        r2r_line<T, N, TPL, true>(lds + l * LS, u, kind, ct);
+        __syncthreads();
+        store_rows<T, N, NL, LS>(g, out, lds, OutMap<T, N>{kind}, scale);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -123,99 +252,413 @@ struct PlainCoef {
 
 // z pass: psi = REDFT01_z(w) (or psi read directly when from_real),
 // optionally store psi, N = beta psi^3 - gamma psi^5, out = REDFT10_z(N).
+// in may equal out.  Each wavefront works on its own lines only.
 template <typename T, int N>
-__global__ __launch_bounds__(WG) void k_plain_z(const T* in, T* out,
-                                                T* __restrict__ psi_out, Dims d, PlainCoef cf,
-                                                int from_real, Tw<T> tw)
+__global__ __launch_bounds__(WG) void k_plain_z(const T* in, T* out, T* __restrict__ psi_out,
+                                                Dims d, PlainCoef cf, int from_real, Tw<T> tw)
 {
     constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
+    constexpr int W = V16<T>::W;
     __shared__ T lds[NL * LS];
-    TileGeo g = tile_geo(d, 2, blockIdx.x, NL);
-    int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
-    if (from_real) {
-        tile_for<N, NL>(g, 2, [&](int p, int c, long gi, bool ok) {
-            lds[c * LS + mk_pos<N>(p)] = ok ? in[gi] : T(0);
-        });
-        __syncthreads();
-    } else {
-        tile_for<N, NL>(g, 2, [&](int p, int c, long gi, bool ok) {
-            lds[c * LS + p] = ok ? in[gi] : T(0);
-        });
-        __syncthreads();
-        dct3_line<T, N, TPL>(lds + l * LS, u, tw);
-    }
-    // LDS now holds psi at Makhoul positions, which is exactly the input
-    // layout of the forward transform: apply the nonlinearity in place.
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    __syncthreads();
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+        V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
+#pragma unroll
+        for (int t = 0; t < W; ++t) lds[c * LS + (from_real ? mk_pos<N>(p + t) : p + t)] = v.v[t];
+    });
+    lsync<true>();
+    if (!from_real) dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    // LDS holds psi at Makhoul positions -- the input layout of the forward
+    // transform -- so the nonlinearity is applied in place.
     if (psi_out) {
-        tile_for<N, NL>(g, 2, [&](int p, int c, long gi, bool ok) {
-            if (ok) psi_out[gi] = lds[c * LS + mk_pos<N>(p)];
+        zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+            V16<T> v;
+#pragma unroll
+            for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + mk_pos<N>(p + t)];
+            *reinterpret_cast<V16<T>*>(psi_out + gi) = v;
         });
-        __syncthreads();
+        lsync<true>();
     }
     const T beta = T(cf.beta), gamma = T(cf.gamma);
-    for (int e = threadIdx.x; e < NL * N; e += WG) {
-        int c = e / N, q = e - c * N;
-        T ps = lds[c * LS + q];
-        T p2 = ps * ps;
-        T p3 = p2 * ps;
-        lds[c * LS + q] = beta * p3 - gamma * (p3 * p2);
+    {
+        constexpr int LPW = NL / (WG / 64);
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll 4
+        for (int e = lane; e < LPW * N; e += 64) {
+            int c = wv * LPW + e / N, q = e % N;
+            T ps = lds[c * LS + q];
+            T p2 = ps * ps;
+            T p3 = p2 * ps;
+            lds[c * LS + q] = beta * p3 - gamma * (p3 * p2);
+        }
     }
-    __syncthreads();
-    dct2_line<T, N, TPL>(lds + l * LS, u, tw);
-    tile_for<N, NL>(g, 2, [&](int p, int c, long gi, bool ok) {
-        if (ok) out[gi] = lds[c * LS + p];
+    lsync<true>();
+    dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+        V16<T> v;
+#pragma unroll
+        for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + p + t];
+        *reinterpret_cast<V16<T>*>(out + gi) = v;
     });
+}
+
+// CN/AB2 update of one spectral value, FP64 arithmetic:
+//   psihat' = (C1 psihat + dtd2 s (3 Nq - Nq_past)) / C2,
+//   C1 = 1 + dtd2 aLin, C2 = 1 - dtd2 aLin, aLin = eps - alpha (|q|^2 - q0^2)^2.
+__device__ __forceinline__ double cn_ab2(const PlainCoef& cf, double mq2, double ph, double nq, double np)
+{
+    double t = mq2 - cf.q0sq;
+    double al = cf.eps - cf.alpha * t * t;
+    double c1 = 1.0 + cf.dtd2 * al, c2 = 1.0 - cf.dtd2 * al;
+    return (c1 * ph + cf.dtd2 * cf.scale * (3.0 * nq - np)) / c2;
 }
 
 // x pass: Nq = REDFT10_x(w); CN/AB2 update of psihat with the history nhat
 // (in place: nhat holds Nq_past on entry, Nq on exit); w = REDFT01_x(psihat').
-// psihat is the pre-scaled spectral field:
-//   psihat' = (C1 psihat + dtd2 s (3 Nq - Nq_past)) / C2,
-//   C1 = 1 + dtd2 aLin, C2 = 1 - dtd2 aLin, aLin = eps - alpha (|q|^2 - q0^2)^2,
-// evaluated in FP64 whatever the storage precision.
+// psihat is the pre-scaled spectral field.
 template <typename T, int N>
 __global__ __launch_bounds__(WG) void k_plain_x(T* __restrict__ w, T* __restrict__ psihat,
-                                                T* __restrict__ nhat, Dims d, PlainCoef cf,
-                                                Tw<T> tw)
+                                                T* __restrict__ nhat, T* __restrict__ s_tape,
+                                                Dims d, PlainCoef cf, int inverse, Tw<T> tw)
 {
     constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
+    constexpr int W = V16<T>::W;
     __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
     TileGeo g = tile_geo(d, 0, blockIdx.x, NL);
-    int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
-    tile_for<N, NL>(g, 0, [&](int p, int c, long gi, bool ok) {
-        lds[c * LS + mk_pos<N>(p)] = ok ? w[gi] : T(0);
-    });
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    struct Mk {
+        __device__ T operator()(int p, int& q) const { q = mk_pos<N>(p); return T(1); }
+    };
+    struct Nat {
+        __device__ T operator()(int p, int& q) const { q = p; return T(1); }
+    };
+    load_rows<T, N, NL, LS>(g, w, lds, Mk{});
     __syncthreads();
-    dct2_line<T, N, TPL>(lds + l * LS, u, tw);
+    dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    __syncthreads();
     // spectral update; tile column c is k = k0 + c at fixed j
     const int nct = (d.nz + NL - 1) / NL;
     const int j = blockIdx.x / nct;
     const int k0 = (blockIdx.x % nct) * NL;
     const double qy = j * cf.qfy;
-    tile_for<N, NL>(g, 0, [&](int p, int c, long gi, bool ok) {
-        if (ok) {
-            double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
-            double mq2 = qx * qx + qy * qy + qz * qz;
-            double t = mq2 - cf.q0sq;
-            double al = cf.eps - cf.alpha * t * t;
-            double c1 = 1.0 + cf.dtd2 * al, c2 = 1.0 - cf.dtd2 * al;
-            double nq = (double)lds[c * LS + p];
-            double np = (double)nhat[gi];
-            double ph = (double)psihat[gi];
-            double r = (c1 * ph + cf.dtd2 * cf.scale * (3.0 * nq - np)) / c2;
-            T rt = T(r);
-            psihat[gi] = rt;
-            nhat[gi] = T(nq);
-            lds[c * LS + p] = rt;
-        } else {
-            lds[c * LS + p] = T(0);
+    if (g.ncol == NL) {
+        rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
+            V16<T> ph = *reinterpret_cast<const V16<T>*>(psihat + gi);
+            V16<T> np = *reinterpret_cast<const V16<T>*>(nhat + gi);
+            V16<T> nq, out, st;
+            const double qx = p * cf.qfx;
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                double qz = (k0 + c0 + t) * cf.qfz;
+                double mq2 = qx * qx + qy * qy + qz * qz;
+                T nqt = lds[(c0 + t) * LS + p];
+                double r = cn_ab2(cf, mq2, (double)ph.v[t], (double)nqt, (double)np.v[t]);
+                nq.v[t] = nqt;
+                out.v[t] = T(r);
+                st.v[t] = T((double)ph.v[t] + (double)out.v[t]);
+                lds[(c0 + t) * LS + p] = out.v[t];
+            }
+            *reinterpret_cast<V16<T>*>(psihat + gi) = out;
+            *reinterpret_cast<V16<T>*>(nhat + gi) = nq;
+            if (s_tape) *reinterpret_cast<V16<T>*>(s_tape + gi) = st;
+        });
+    } else {
+        tile_for<N, NL>(g, 0, [&](int p, int c, long gi, bool ok) {
+            if (ok) {
+                double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
+                double mq2 = qx * qx + qy * qy + qz * qz;
+                T nqt = lds[c * LS + p];
+                double ph = (double)psihat[gi];
+                double r = cn_ab2(cf, mq2, ph, (double)nqt, (double)nhat[gi]);
+                if (s_tape) s_tape[gi] = T(ph + (double)T(r));
+                psihat[gi] = T(r);
+                nhat[gi] = nqt;
+                lds[c * LS + p] = T(r);
+            } else {
+                lds[c * LS + p] = T(0);
+            }
+        });
+    }
+    if (!inverse) return;
+    __syncthreads();
+    dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    __syncthreads();
+    store_rows<T, N, NL, LS>(g, w, lds, Mk{}, T(1));
+}
+
+// ------------------------------------------------------------------------
+// Advective model (adv/cosAdvTrackMult.cpp:827-1369).  Pointwise and
+// spectral stages with the reference's index shifts and truncations on the
+// global (single-slab) grid; the transforms are separate axis passes.
+struct AdvCoef {
+    PlainCoef pc;
+    double nu;
+};
+
+__device__ __forceinline__ void unflat(long id, const Dims& d, int& i, int& j, int& k)
+{
+    k = (int)(id % d.nz);
+    long r = id / d.nz;
+    j = (int)(r % d.ny);
+    i = (int)(r / d.ny);
+}
+
+// From the scaled spectral psi: DST-shift gradients (:866-959) and the
+// linear part of mu, -aLin psihat (:980-991).
+template <typename T>
+__global__ __launch_bounds__(WG) void k_adv_spec(const T* __restrict__ ph, T* __restrict__ gx,
+                                                 T* __restrict__ gy, T* __restrict__ gz,
+                                                 T* __restrict__ sp, Dims d, AdvCoef ac)
+{
+    const PlainCoef& cf = ac.pc;
+    long n = d.total();
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        const long sx = (long)d.ny * d.nz, sy = d.nz;
+        gx[id] = i < d.nx - 1 ? T(-(i + 1) * cf.qfx * (double)ph[id + sx]) : T(0);
+        gy[id] = j < d.ny - 1 ? T(-(j + 1) * cf.qfy * (double)ph[id + sy]) : T(0);
+        gz[id] = k < d.nz - 1 ? T(-(k + 1) * cf.qfz * (double)ph[id + 1]) : T(0);
+        double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
+        double t = qx * qx + qy * qy + qz * qz - cf.q0sq;
+        double al = cf.eps - cf.alpha * t * t;
+        sp[id] = T(-al * (double)ph[id]);
+    }
+}
+
+// S = mu grad psi, mu = Sp2 - beta psi^3 + gamma psi^5 (:996-1015); S
+// overwrites grad psi's buffers only through separate outputs.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_adv_force(const T* __restrict__ psi, const T* __restrict__ sp2,
+                                                  const T* __restrict__ gx, const T* __restrict__ gy,
+                                                  const T* __restrict__ gz, T* __restrict__ sx,
+                                                  T* __restrict__ sy, T* __restrict__ sz, long n,
+                                                  AdvCoef ac)
+{
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        double x = (double)psi[id], x2 = x * x, x3 = x2 * x;
+        double mu = (double)sp2[id] - ac.pc.beta * x3 + ac.pc.gamma * x3 * x2;
+        sx[id] = T(mu * (double)gx[id]);
+        sy[id] = T(mu * (double)gy[id]);
+        sz[id] = T(mu * (double)gz[id]);
+    }
+}
+
+// Stokes projection with the reference's cross-index reads (:1103-1274):
+//  ux[i,j,k] = CM1x (Sx - CM2x (qsx Sx + qy Sy[i+1,j-1,k] + qz Sz[i+1,j,k-1])),  i <= Nx-2
+//  uy[i,j,k] = CM1y (Sy - CM2y (qx Sx[i-1,j+1,k] + qsy Sy + qz Sz[i,j+1,k-1])),  j <= Ny-2, k <= Nz-2
+//  uz[i,j,k] = CM1z (Sz - CM2z (qx Sx[i-1,j,k+1] + qy Sy[i,j-1,k+1] + qsz Sz)),  j <= Ny-2, k <= Nz-2
+// CM1 = s/(nu |q|^2), CM2 = qs/|q|^2 with the DST wavenumber on the own axis;
+// out-of-range neighbours read as 0, all other entries are 0.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_adv_stokes(const T* __restrict__ Sx, const T* __restrict__ Sy,
+                                                   const T* __restrict__ Sz, T* __restrict__ ux,
+                                                   T* __restrict__ uy, T* __restrict__ uz, Dims d,
+                                                   AdvCoef ac)
+{
+    const PlainCoef& cf = ac.pc;
+    long n = d.total();
+    const long sx = (long)d.ny * d.nz, sy = d.nz;
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
+        double qsx = (i + 1) * cf.qfx, qsy = (j + 1) * cf.qfy, qsz = (k + 1) * cf.qfz;
+        double vx = 0.0, vy = 0.0, vz = 0.0;
+        if (i < d.nx - 1) {
+            double syv = j != 0 ? (double)Sy[id + sx - sy] : 0.0;
+            double szv = k != 0 ? (double)Sz[id + sx - 1] : 0.0;
+            double mq2 = qsx * qsx + qy * qy + qz * qz;
+            double dot = qsx * (double)Sx[id] + qy * syv + qz * szv;
+            vx = cf.scale / (ac.nu * mq2) * ((double)Sx[id] - qsx / mq2 * dot);
+        }
+        if (j < d.ny - 1 && k < d.nz - 1) {
+            double sxv = i != 0 ? (double)Sx[id - sx + sy] : 0.0;
+            double szv = k != 0 ? (double)Sz[id + sy - 1] : 0.0;
+            double mq2 = qx * qx + qsy * qsy + qz * qz;
+            double dot = qx * sxv + qsy * (double)Sy[id] + qz * szv;
+            vy = cf.scale / (ac.nu * mq2) * ((double)Sy[id] - qsy / mq2 * dot);
+            sxv = i != 0 ? (double)Sx[id - sx + 1] : 0.0;
+            double syv = j != 0 ? (double)Sy[id - sy + 1] : 0.0;
+            mq2 = qx * qx + qy * qy + qsz * qsz;
+            dot = qx * sxv + qy * syv + qsz * (double)Sz[id];
+            vz = cf.scale / (ac.nu * mq2) * ((double)Sz[id] - qsz / mq2 * dot);
+        }
+        ux[id] = T(vx);
+        uy[id] = T(vy);
+        uz[id] = T(vz);
+    }
+}
+
+// Nr = beta psi^3 - gamma psi^5 - u . grad psi (:1290-1303)
+template <typename T>
+__global__ __launch_bounds__(WG) void k_adv_nl(const T* __restrict__ psi, const T* __restrict__ ux,
+                                               const T* __restrict__ uy, const T* __restrict__ uz,
+                                               const T* __restrict__ gx, const T* __restrict__ gy,
+                                               const T* __restrict__ gz, T* __restrict__ nr, long n,
+                                               AdvCoef ac)
+{
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        double x = (double)psi[id], x2 = x * x, x3 = x2 * x;
+        double v = ac.pc.beta * x3 - ac.pc.gamma * x3 * x2 - (double)ux[id] * (double)gx[id] -
+                   (double)uy[id] * (double)gy[id] - (double)uz[id] * (double)gz[id];
+        nr[id] = T(v);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Discrete adjoint of the plain step (no reference counterpart).  With
+// lt = D lambda, D = prod over axes of diag(2,1,...,1), one backward step is
+//   lt^{n+1} = a lt^{n+2} + REDFT10^3(g^{n+1})
+//   mt^n     = cm c lt^{n+1} - c lt^{n+2}     (cm = 3, or 2 at n = 0)
+//   r^n      = REDFT01^3(mt^n),  g^n = f'(psi^n) r^n
+// with a = C1/C2 and c = s dt/2 / C2, fused into the same four passes as the
+// forward step.  Parameter gradients accumulate per workgroup in FP64:
+//   d eps   += w lt^{n+1} dt/2/C2 (ph^n + ph^{n+1}),  d alpha += -(|q|^2-q0^2)^2 (same)
+//   d beta  += r^n psi^3,  d gamma += -r^n psi^5
+// with w = D^{-1} (1/2 per axis at wavenumber index 0).
+
+// Block-sum NV values and add them to acc[blockIdx.x*NV + q] (one slot per
+// workgroup, so repeated launches accumulate without atomics).
+template <int NV>
+__device__ __forceinline__ void block_accumulate(double (&v)[NV], double* __restrict__ acc)
+{
+    __shared__ double red[NV][WG / 64];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double x = v[q];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double t = 0.0;
+        for (int wv = 0; wv < WG / 64; ++wv) t += red[threadIdx.x][wv];
+        acc[blockIdx.x * NV + threadIdx.x] += t;
+    }
+}
+
+// x pass.  mode 0: backward step (cm = 3), mode 1: backward step n = 0
+// (cm = 2), mode 2: final, w = s REDFT01_x(a lbuf + REDFT10_x(G)).
+template <typename T, int N>
+__global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__ lbuf,
+                                              const T* __restrict__ s_tape, Dims d, PlainCoef cf,
+                                              int mode, double* __restrict__ acc, Tw<T> tw)
+{
+    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
+    __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    TileGeo g = tile_geo(d, 0, blockIdx.x, NL);
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    struct Mk {
+        __device__ T operator()(int p, int& q) const { q = mk_pos<N>(p); return T(1); }
+    };
+    load_rows<T, N, NL, LS>(g, w, lds, Mk{});
+    __syncthreads();
+    dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    __syncthreads();
+    const int nct = (d.nz + NL - 1) / NL;
+    const int j = blockIdx.x / nct;
+    const int k0 = (blockIdx.x % nct) * NL;
+    const double qy = j * cf.qfy;
+    const double cm = mode == 1 ? 2.0 : 3.0;
+    double sums[2] = {0.0, 0.0};
+    auto point = [&](int p, int c, T& lb, T st) {
+        double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
+        double t = qx * qx + qy * qy + qz * qz - cf.q0sq;
+        double al = cf.eps - cf.alpha * t * t;
+        double c2 = 1.0 - cf.dtd2 * al;
+        double a = (1.0 + cf.dtd2 * al) / c2;
+        double lold = (double)lb;
+        double lnew = a * lold + (double)lds[c * LS + p];
+        if (mode == 2) {
+            lds[c * LS + p] = T(cf.scale * lnew);
+            return;
+        }
+        double cc = cf.scale * cf.dtd2 / c2;
+        double wgt = (p == 0 ? 0.5 : 1.0) * (j == 0 ? 0.5 : 1.0) * (k0 + c == 0 ? 0.5 : 1.0);
+        double term = wgt * lnew * cf.dtd2 / c2 * (double)st;
+        sums[0] += term;
+        sums[1] -= t * t * term;
+        lb = T(lnew);
+        lds[c * LS + p] = T(cm * cc * lnew - cc * lold);
+    };
+    if (g.ncol == NL) {
+        rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
+            V16<T> lb = *reinterpret_cast<const V16<T>*>(lbuf + gi);
+            V16<T> st;
+            if (mode != 2) st = *reinterpret_cast<const V16<T>*>(s_tape + gi);
+#pragma unroll
+            for (int t = 0; t < V16<T>::W; ++t) point(p, c0 + t, lb.v[t], mode != 2 ? st.v[t] : T(0));
+            if (mode != 2) *reinterpret_cast<V16<T>*>(lbuf + gi) = lb;
+        });
+    } else {
+        tile_for<N, NL>(g, 0, [&](int p, int c, long gi, bool ok) {
+            if (ok) {
+                T lb = lbuf[gi];
+                point(p, c, lb, mode != 2 ? s_tape[gi] : T(0));
+                if (mode != 2) lbuf[gi] = lb;
+            } else {
+                lds[c * LS + p] = T(0);
+            }
+        });
+    }
+    if (mode != 2) block_accumulate<2>(sums, acc);
+    __syncthreads();
+    dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    __syncthreads();
+    store_rows<T, N, NL, LS>(g, w, lds, Mk{}, T(1));
+}
+
+// z pass: r = REDFT01_z(w); g = (3 beta psi^2 - 5 gamma psi^4) r with psi^n
+// from the tape; sums r psi^3, -r psi^5; w = REDFT10_z(g).
+template <typename T, int N>
+__global__ __launch_bounds__(WG) void k_adj_z(T* w, const T* __restrict__ psi_tape, Dims d,
+                                              PlainCoef cf, double* __restrict__ acc, Tw<T> tw)
+{
+    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
+    constexpr int W = V16<T>::W;
+    __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    __syncthreads();
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+        V16<T> v = *reinterpret_cast<const V16<T>*>(w + gi);
+#pragma unroll
+        for (int t = 0; t < W; ++t) lds[c * LS + p + t] = v.v[t];
+    });
+    lsync<true>();
+    dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    double sums[2] = {0.0, 0.0};
+    const double b3 = 3.0 * cf.beta, g5 = 5.0 * cf.gamma;
+    zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+        V16<T> ps = *reinterpret_cast<const V16<T>*>(psi_tape + gi);
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            int q = c * LS + mk_pos<N>(p + t);
+            double r = (double)lds[q];
+            double x = (double)ps.v[t], x2 = x * x;
+            sums[0] += r * x2 * x;
+            sums[1] -= r * x2 * x2 * x;
+            lds[q] = T((b3 * x2 - g5 * x2 * x2) * r);
         }
     });
-    __syncthreads();
-    dct3_line<T, N, TPL>(lds + l * LS, u, tw);
-    tile_for<N, NL>(g, 0, [&](int p, int c, long gi, bool ok) {
-        if (ok) w[gi] = lds[c * LS + mk_pos<N>(p)];
+    lsync<true>();
+    dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+        V16<T> v;
+#pragma unroll
+        for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + p + t];
+        *reinterpret_cast<V16<T>*>(w + gi) = v;
     });
+    block_accumulate<2>(sums, acc);
 }
 
 // ------------------------------------------------------------------------
@@ -289,8 +732,21 @@ struct Launch {
                      const Tw<T>& tw);
     static void plain_z(hipStream_t s, const T* in, T* out, T* psi_out, Dims d,
                         const PlainCoef& cf, int from_real, const Tw<T>& tw);
-    static void plain_x(hipStream_t s, T* w, T* psihat, T* nhat, Dims d, const PlainCoef& cf,
-                        const Tw<T>& tw);
+    static void plain_x(hipStream_t s, T* w, T* psihat, T* nhat, T* s_tape, Dims d,
+                        const PlainCoef& cf, const Tw<T>& tw, int inverse = 1);
+    static void adv_spec(hipStream_t s, const T* ph, T* gx, T* gy, T* gz, T* sp, Dims d,
+                         const AdvCoef& ac);
+    static void adv_force(hipStream_t s, const T* psi, const T* sp2, const T* gx, const T* gy,
+                          const T* gz, T* sx, T* sy, T* sz, Dims d, const AdvCoef& ac);
+    static void adv_stokes(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, T* ux, T* uy,
+                           T* uz, Dims d, const AdvCoef& ac);
+    static void adv_nl(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
+                       const T* gx, const T* gy, const T* gz, T* nr, Dims d, const AdvCoef& ac);
+    static void adj_x(hipStream_t s, T* w, T* lbuf, const T* s_tape, Dims d, const PlainCoef& cf,
+                      int mode, double* acc, const Tw<T>& tw);
+    static void adj_z(hipStream_t s, T* w, const T* psi_tape, Dims d, const PlainCoef& cf,
+                      double* acc, const Tw<T>& tw);
+    static void fold(hipStream_t s, const double* part, int nparts, int nv, double* out);
     static void l1(hipStream_t s, const T* a, const T* b, long n, double* part, int nparts,
                    double* out);
 };

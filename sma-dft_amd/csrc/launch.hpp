@@ -4,6 +4,8 @@
 
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace smad {
 
 #define SMAD_LENGTHS(X) X(16) X(32) X(64) X(128) X(256) X(384) X(512)
@@ -39,11 +41,71 @@ void Launch<T>::plain_z(hipStream_t s, const T* in, T* out, T* psi_out, Dims d,
 }
 
 template <typename T>
-void Launch<T>::plain_x(hipStream_t s, T* w, T* psihat, T* nhat, Dims d, const PlainCoef& cf,
-                        const Tw<T>& tw)
+void Launch<T>::plain_x(hipStream_t s, T* w, T* psihat, T* nhat, T* s_tape, Dims d,
+                        const PlainCoef& cf, const Tw<T>& tw, int inverse)
 {
     dim3 grid(num_tiles(d, 0, Lines<T>::NL));
-    SMAD_DISPATCH(d.nx, k_plain_x, grid, w, psihat, nhat, d, cf, tw);
+    SMAD_DISPATCH(d.nx, k_plain_x, grid, w, psihat, nhat, s_tape, d, cf, inverse, tw);
+}
+
+inline dim3 flat_grid(long n)
+{
+    long b = (n + WG - 1) / WG;
+    return dim3((unsigned)std::min(b, 8192L));
+}
+
+template <typename T>
+void Launch<T>::adv_spec(hipStream_t s, const T* ph, T* gx, T* gy, T* gz, T* sp, Dims d,
+                         const AdvCoef& ac)
+{
+    hipLaunchKernelGGL(k_adv_spec<T>, flat_grid(d.total()), dim3(WG), 0, s, ph, gx, gy, gz, sp, d, ac);
+}
+
+template <typename T>
+void Launch<T>::adv_force(hipStream_t s, const T* psi, const T* sp2, const T* gx, const T* gy,
+                          const T* gz, T* sx, T* sy, T* sz, Dims d, const AdvCoef& ac)
+{
+    hipLaunchKernelGGL(k_adv_force<T>, flat_grid(d.total()), dim3(WG), 0, s, psi, sp2, gx, gy, gz,
+                       sx, sy, sz, d.total(), ac);
+}
+
+template <typename T>
+void Launch<T>::adv_stokes(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, T* ux, T* uy,
+                           T* uz, Dims d, const AdvCoef& ac)
+{
+    hipLaunchKernelGGL(k_adv_stokes<T>, flat_grid(d.total()), dim3(WG), 0, s, Sx, Sy, Sz, ux, uy,
+                       uz, d, ac);
+}
+
+template <typename T>
+void Launch<T>::adv_nl(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
+                       const T* gx, const T* gy, const T* gz, T* nr, Dims d, const AdvCoef& ac)
+{
+    hipLaunchKernelGGL(k_adv_nl<T>, flat_grid(d.total()), dim3(WG), 0, s, psi, ux, uy, uz, gx, gy,
+                       gz, nr, d.total(), ac);
+}
+
+template <typename T>
+void Launch<T>::adj_x(hipStream_t s, T* w, T* lbuf, const T* s_tape, Dims d, const PlainCoef& cf,
+                      int mode, double* acc, const Tw<T>& tw)
+{
+    dim3 grid(num_tiles(d, 0, Lines<T>::NL));
+    SMAD_DISPATCH(d.nx, k_adj_x, grid, w, lbuf, s_tape, d, cf, mode, acc, tw);
+}
+
+template <typename T>
+void Launch<T>::adj_z(hipStream_t s, T* w, const T* psi_tape, Dims d, const PlainCoef& cf,
+                      double* acc, const Tw<T>& tw)
+{
+    dim3 grid(num_tiles(d, 2, Lines<T>::NL));
+    SMAD_DISPATCH(d.nz, k_adj_z, grid, w, psi_tape, d, cf, acc, tw);
+}
+
+template <typename T>
+void Launch<T>::fold(hipStream_t s, const double* part, int nparts, int nv, double* out)
+{
+    if (nv == 2) hipLaunchKernelGGL(k_fold<2>, dim3(1), dim3(WG), 0, s, part, nparts, out);
+    else if (nv == 3) hipLaunchKernelGGL(k_fold<3>, dim3(1), dim3(WG), 0, s, part, nparts, out);
 }
 
 template <typename T>

@@ -17,7 +17,8 @@
 // reversal:  RODFT10(x)_k = REDFT10((-1)^n x_n)_{N-1-k},
 //            RODFT01(X)_k = (-1)^k REDFT01(X_{N-1-n})_k.
 // The complex FFT is a mixed-radix Stockham (radix 2/3/4/8) run in LDS by
-// TPL threads per line, one workgroup barrier per stage.
+// TPL threads per line; a line lives inside one wavefront, so the stages only
+// need wavefront-scope synchronisation.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -54,16 +55,53 @@ __device__ __forceinline__ cpx<T> rot(cpx<T> a)
     return INV ? cpx<T>{-a.im, a.re} : cpx<T>{a.im, -a.re};
 }
 
-// Twiddle tables for one line length N (M = N/2), all in the storage precision:
-//   fft[m] = e^{-2 pi i m/M}  (m < M)
-//   h[k]   = e^{-i pi k/2N}   (k < N)   Makhoul post-twiddle
-//   s[k]   = e^{-2 pi i k/N}  (k < M)   real-FFT split twiddle
+// All twiddles of a length-N line are powers of omega = e^{-2 pi i/4N}:
+//   FFT stage   W_M^m = omega^{8m}     (M = N/2)
+//   Makhoul     h_k   = omega^{k}      = e^{-i pi k/2N}
+//   real split  s_k   = omega^{4k}     = e^{-2 pi i k/N}
+// They are read from a quarter-wave table c[j] = cos(pi j/2N), j = 0..N,
+// kept in LDS (N+1 values) and unfolded by quadrant.
 template <typename T>
 struct Tw {
-    const cpx<T>* fft;
-    const cpx<T>* h;
-    const cpx<T>* s;
+    const T* cosq;  // global copy of the quarter-wave table (N+1 values)
 };
+
+template <typename T, int N>
+__device__ __forceinline__ cpx<T> omega(const T* __restrict__ c, int e)
+{
+    const int q = e / N;
+    const int r = e - q * N;
+    const T a = c[r], b = c[N - r];
+    switch (q & 3) {
+    case 0: return cpx<T>{a, -b};
+    case 1: return cpx<T>{-b, -a};
+    case 2: return cpx<T>{-a, b};
+    default: return cpx<T>{b, a};
+    }
+}
+
+// Copy the quarter-wave table into LDS (caller synchronises).
+template <typename T, int N>
+__device__ __forceinline__ void load_cosq(T* dst, const Tw<T>& tw)
+{
+    for (int j = threadIdx.x; j <= N; j += blockDim.x) dst[j] = tw.cosq[j];
+}
+
+// Synchronisation of the threads that share a line.  Lines never straddle a
+// wavefront (TPL divides 64), so inside the transforms a wavefront-scope
+// barrier with LDS fences suffices; WAVE = false falls back to the
+// workgroup barrier.
+template <bool WAVE>
+__device__ __forceinline__ void lsync()
+{
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
 
 // ------------------------------------------------------------ small DFTs --
 
@@ -139,9 +177,10 @@ __device__ __forceinline__ void dftR(cpx<T>* a)
 // butterfly b reads line[b + r M/R], multiplies by W^{r (b mod Ns) M/(Ns R)},
 // and writes line[(b/Ns) Ns R + (b mod Ns) + r Ns].  All reads land in
 // registers before a barrier, so the stage is in place.
-template <typename T, int M, int R, int TPL, bool INV>
-__device__ __forceinline__ void stage(cpx<T>* line, int u, int Ns, const cpx<T>* __restrict__ tw)
+template <typename T, int N, int R, int TPL, bool INV, bool WAVE>
+__device__ __forceinline__ void stage(cpx<T>* line, int u, int Ns, const T* __restrict__ ct)
 {
+    constexpr int M = N / 2;
     constexpr int NB = M / R;
     constexpr int PER = (NB + TPL - 1) / TPL;
     cpx<T> v[PER][R];
@@ -153,7 +192,7 @@ __device__ __forceinline__ void stage(cpx<T>* line, int u, int Ns, const cpx<T>*
             for (int r = 0; r < R; ++r) v[p][r] = line[b + r * NB];
         }
     }
-    __syncthreads();
+    lsync<WAVE>();
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
         int b = u + p * TPL;
@@ -162,7 +201,7 @@ __device__ __forceinline__ void stage(cpx<T>* line, int u, int Ns, const cpx<T>*
             int step = M / (Ns * R);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                cpx<T> w = tw[k * r * step];
+                cpx<T> w = omega<T, N>(ct, 8 * k * r * step);
                 if (INV) w.im = -w.im;
                 v[p][r] = mul(v[p][r], w);
             }
@@ -172,36 +211,37 @@ __device__ __forceinline__ void stage(cpx<T>* line, int u, int Ns, const cpx<T>*
             for (int r = 0; r < R; ++r) line[base + r * Ns] = v[p][r];
         }
     }
-    __syncthreads();
+    lsync<WAVE>();
 }
 
-template <typename T, int M, int TPL, bool INV, int... Rs>
+template <typename T, int N, int TPL, bool INV, bool WAVE, int... Rs>
 struct FFTRun;
-template <typename T, int M, int TPL, bool INV>
-struct FFTRun<T, M, TPL, INV> {
-    static __device__ __forceinline__ void run(cpx<T>*, int, int, const cpx<T>*) {}
+template <typename T, int N, int TPL, bool INV, bool WAVE>
+struct FFTRun<T, N, TPL, INV, WAVE> {
+    static __device__ __forceinline__ void run(cpx<T>*, int, int, const T*) {}
 };
-template <typename T, int M, int TPL, bool INV, int R, int... Rs>
-struct FFTRun<T, M, TPL, INV, R, Rs...> {
-    static __device__ __forceinline__ void run(cpx<T>* line, int u, int Ns, const cpx<T>* tw)
+template <typename T, int N, int TPL, bool INV, bool WAVE, int R, int... Rs>
+struct FFTRun<T, N, TPL, INV, WAVE, R, Rs...> {
+    static __device__ __forceinline__ void run(cpx<T>* line, int u, int Ns, const T* ct)
     {
-        stage<T, M, R, TPL, INV>(line, u, Ns, tw);
-        FFTRun<T, M, TPL, INV, Rs...>::run(line, u, Ns * R, tw);
+        stage<T, N, R, TPL, INV, WAVE>(line, u, Ns, ct);
+        FFTRun<T, N, TPL, INV, WAVE, Rs...>::run(line, u, Ns * R, ct);
     }
 };
 
-// Unnormalised M-point complex FFT, forward (e^-) or inverse (e^+).
-template <typename T, int M, int TPL, bool INV>
-__device__ __forceinline__ void fft(cpx<T>* line, int u, const cpx<T>* tw)
+// Unnormalised M = N/2 point complex FFT, forward (e^-) or inverse (e^+).
+template <typename T, int N, int TPL, bool INV, bool WAVE>
+__device__ __forceinline__ void fft(cpx<T>* line, int u, const T* ct)
 {
-    if constexpr (M == 4) FFTRun<T, M, TPL, INV, 4>::run(line, u, 1, tw);
-    else if constexpr (M == 8) FFTRun<T, M, TPL, INV, 8>::run(line, u, 1, tw);
-    else if constexpr (M == 16) FFTRun<T, M, TPL, INV, 4, 4>::run(line, u, 1, tw);
-    else if constexpr (M == 32) FFTRun<T, M, TPL, INV, 4, 8>::run(line, u, 1, tw);
-    else if constexpr (M == 64) FFTRun<T, M, TPL, INV, 8, 8>::run(line, u, 1, tw);
-    else if constexpr (M == 128) FFTRun<T, M, TPL, INV, 4, 4, 8>::run(line, u, 1, tw);
-    else if constexpr (M == 192) FFTRun<T, M, TPL, INV, 3, 8, 8>::run(line, u, 1, tw);
-    else if constexpr (M == 256) FFTRun<T, M, TPL, INV, 4, 8, 8>::run(line, u, 1, tw);
+    constexpr int M = N / 2;
+    if constexpr (M == 4) FFTRun<T, N, TPL, INV, WAVE, 4>::run(line, u, 1, ct);
+    else if constexpr (M == 8) FFTRun<T, N, TPL, INV, WAVE, 8>::run(line, u, 1, ct);
+    else if constexpr (M == 16) FFTRun<T, N, TPL, INV, WAVE, 4, 4>::run(line, u, 1, ct);
+    else if constexpr (M == 32) FFTRun<T, N, TPL, INV, WAVE, 4, 8>::run(line, u, 1, ct);
+    else if constexpr (M == 64) FFTRun<T, N, TPL, INV, WAVE, 8, 8>::run(line, u, 1, ct);
+    else if constexpr (M == 128) FFTRun<T, N, TPL, INV, WAVE, 4, 4, 8>::run(line, u, 1, ct);
+    else if constexpr (M == 192) FFTRun<T, N, TPL, INV, WAVE, 3, 8, 8>::run(line, u, 1, ct);
+    else if constexpr (M == 256) FFTRun<T, N, TPL, INV, WAVE, 4, 8, 8>::run(line, u, 1, ct);
     else static_assert(M == 4, "unsupported FFT length");
 }
 
@@ -216,12 +256,12 @@ __device__ __forceinline__ int mk_pos(int p)
 // ------------------------------------------------ DCT-II / DCT-III lines --
 // DCT-II: on entry line (as reals) holds x at Makhoul positions, i.e.
 // real[mk_pos(p)] = x_p.  On exit real[k] = Y_k (natural order).
-template <typename T, int N, int TPL>
-__device__ __forceinline__ void dct2_line(T* lr, int u, const Tw<T>& tw)
+template <typename T, int N, int TPL, bool WAVE = true>
+__device__ __forceinline__ void dct2_line(T* lr, int u, const T* ct)
 {
     constexpr int M = N / 2;
     cpx<T>* z = reinterpret_cast<cpx<T>*>(lr);
-    fft<T, M, TPL, false>(z, u, tw.fft);
+    fft<T, N, TPL, false, WAVE>(z, u, ct);
     // split + post-twiddle, pairs (k, M-k) for k = 0..M/2
     constexpr int NP = M / 2 + 1;
     constexpr int PER = (NP + TPL - 1) / TPL;
@@ -234,19 +274,19 @@ __device__ __forceinline__ void dct2_line(T* lr, int u, const Tw<T>& tw)
             cpx<T> zm = z[(M - k) % M];
             cpx<T> A = mk(zk.re + zm.re, zk.im - zm.im);
             cpx<T> B = mk(zk.re - zm.re, zk.im + zm.im);
-            cpx<T> sB = mul(tw.s[k], B);
+            cpx<T> sB = mul(omega<T, N>(ct, 4 * k), B);
             // V_k = A/2 - i sB/2 ; V_{M-k} = conj(A)/2 - i conj(sB)/2
             cpx<T> vk = mk(T(0.5) * (A.re + sB.im), T(0.5) * (A.im - sB.re));
             cpx<T> vm = mk(T(0.5) * (A.re - sB.im), T(-0.5) * (A.im + sB.re));
-            cpx<T> a = mul(tw.h[k], vk);
-            cpx<T> b = mul(tw.h[M - k], vm);
+            cpx<T> a = mul(omega<T, N>(ct, k), vk);
+            cpx<T> b = mul(omega<T, N>(ct, M - k), vm);
             y[q][0] = T(2) * a.re;   // Y_k
             y[q][1] = T(-2) * a.im;  // Y_{N-k}
             y[q][2] = T(2) * b.re;   // Y_{M-k}
             y[q][3] = T(-2) * b.im;  // Y_{M+k}
         }
     }
-    __syncthreads();
+    lsync<WAVE>();
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         int k = u + q * TPL;
@@ -263,13 +303,13 @@ __device__ __forceinline__ void dct2_line(T* lr, int u, const Tw<T>& tw)
             }
         }
     }
-    __syncthreads();
+    lsync<WAVE>();
 }
 
 // DCT-III: on entry real[k] = X_k (natural order).  On exit the line holds v,
 // i.e. output sample p is real[mk_pos(p)].
-template <typename T, int N, int TPL>
-__device__ __forceinline__ void dct3_line(T* lr, int u, const Tw<T>& tw)
+template <typename T, int N, int TPL, bool WAVE = true>
+__device__ __forceinline__ void dct3_line(T* lr, int u, const T* ct)
 {
     constexpr int M = N / 2;
     cpx<T>* z = reinterpret_cast<cpx<T>*>(lr);
@@ -285,22 +325,22 @@ __device__ __forceinline__ void dct3_line(T* lr, int u, const Tw<T>& tw)
             T xmk = lr[M - k];
             T xpk = (k == 0) ? lr[M] : lr[M + k];
             // V'_k = conj(h_k)(X_k - i X_{N-k}),  V'_{k+M} = conj(h_{k+M})(X_{M+k} - i X_{M-k})
-            cpx<T> v0 = mul(conj(tw.h[k]), mk(xk, -xnk));
-            cpx<T> v1 = mul(conj(tw.h[k + M]), mk(xpk, -xmk));
+            cpx<T> v0 = mul(conj(omega<T, N>(ct, k)), mk(xk, -xnk));
+            cpx<T> v1 = mul(conj(omega<T, N>(ct, k + M)), mk(xpk, -xmk));
             cpx<T> E = add(v0, v1);
-            cpx<T> O = mul(sub(v0, v1), conj(tw.s[k]));
+            cpx<T> O = mul(sub(v0, v1), conj(omega<T, N>(ct, 4 * k)));
             zo[q][0] = mk(E.re - O.im, E.im + O.re);  // E + i O
             if (k > 0 && k != M - k) {
                 // V'_{M-k} = conj(h_{M-k})(X_{M-k} - i X_{M+k}),  V'_{N-k} = conj(h_{N-k})(X_{N-k} - i X_k)
-                cpx<T> w0 = mul(conj(tw.h[M - k]), mk(xmk, -xpk));
-                cpx<T> w1 = mul(conj(tw.h[N - k]), mk(xnk, -xk));
+                cpx<T> w0 = mul(conj(omega<T, N>(ct, M - k)), mk(xmk, -xpk));
+                cpx<T> w1 = mul(conj(omega<T, N>(ct, N - k)), mk(xnk, -xk));
                 cpx<T> E2 = add(w0, w1);
-                cpx<T> O2 = mul(sub(w0, w1), conj(tw.s[M - k]));
+                cpx<T> O2 = mul(sub(w0, w1), conj(omega<T, N>(ct, 4 * (M - k))));
                 zo[q][1] = mk(E2.re - O2.im, E2.im + O2.re);
             }
         }
     }
-    __syncthreads();
+    lsync<WAVE>();
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         int k = u + q * TPL;
@@ -309,8 +349,8 @@ __device__ __forceinline__ void dct3_line(T* lr, int u, const Tw<T>& tw)
             if (k > 0 && k != M - k) z[M - k] = zo[q][1];
         }
     }
-    __syncthreads();
-    fft<T, M, TPL, true>(z, u, tw.fft);
+    lsync<WAVE>();
+    fft<T, N, TPL, true, WAVE>(z, u, ct);
 }
 
 // Where global sample p of a line goes in LDS before a forward/inverse
@@ -339,11 +379,11 @@ __device__ __forceinline__ int out_pos(int kind, int k)
 __device__ __forceinline__ bool out_neg(int kind, int k) { return kind == RODFT01 && (k & 1); }
 __device__ __forceinline__ bool is_fwd(int kind) { return kind == REDFT10 || kind == RODFT10; }
 
-template <typename T, int N, int TPL>
-__device__ __forceinline__ void r2r_line(T* lr, int u, int kind, const Tw<T>& tw)
+template <typename T, int N, int TPL, bool WAVE = true>
+__device__ __forceinline__ void r2r_line(T* lr, int u, int kind, const T* ct)
 {
-    if (is_fwd(kind)) dct2_line<T, N, TPL>(lr, u, tw);
-    else dct3_line<T, N, TPL>(lr, u, tw);
+    if (is_fwd(kind)) dct2_line<T, N, TPL, WAVE>(lr, u, ct);
+    else dct3_line<T, N, TPL, WAVE>(lr, u, ct);
 }
 
 }  // namespace smad

@@ -70,34 +70,22 @@ struct DevBuf {
     }
 };
 
-// Twiddle tables for one line length, computed in long double on the host.
+// Quarter-wave cosine table c[j] = cos(pi j / 2N), j = 0..N, for one line
+// length, computed in long double on the host and rounded once.
 template <typename T>
 struct TwTable {
-    DevBuf<cpx<T>> buf;
+    DevBuf<T> buf;
     Tw<T> tw{};
     void make(int n, hipStream_t s)
     {
-        const int m = n / 2;
-        std::vector<cpx<T>> h(m + n + m);
+        std::vector<T> h(n + 1);
         const long double pi = 3.141592653589793238462643383279502884L;
-        for (int j = 0; j < m; ++j) {
-            long double a = 2.0L * pi * j / m;
-            h[j] = cpx<T>{T(cosl(a)), T(-sinl(a))};
-        }
-        for (int k = 0; k < n; ++k) {
-            long double a = pi * k / (2.0L * n);
-            h[m + k] = cpx<T>{T(cosl(a)), T(-sinl(a))};
-        }
-        for (int k = 0; k < m; ++k) {
-            long double a = 2.0L * pi * k / n;
-            h[m + n + k] = cpx<T>{T(cosl(a)), T(-sinl(a))};
-        }
+        for (int j = 0; j <= n; ++j) h[j] = T(cosl(pi * j / (2.0L * n)));
+        h[n] = T(0);
         buf.alloc(h.size());
-        HIPCHK(hipMemcpyAsync(buf.p, h.data(), h.size() * sizeof(cpx<T>), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(buf.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));
-        tw.fft = buf.p;
-        tw.h = buf.p + m;
-        tw.s = buf.p + m + n;
+        tw.cosq = buf.p;
     }
 };
 
@@ -111,6 +99,8 @@ struct EngineBase {
     virtual int fetch(int field, double* out) = 0;
     virtual void set_field(int field, const double* in) = 0;
     virtual int reduce(int what, double* out) = 0;
+    virtual void forward_tape(int K, int ck) = 0;
+    virtual void adjoint(const double* g_psik, double* dpsi0, double* dparams) = 0;
 };
 
 template <typename T>
@@ -245,6 +235,7 @@ struct PlainEngine : EngineBase {
     {
         upload(s, psi.p, n, psi0);
         preloop();
+        fresh = true;
     }
 
     const char* kernel_class(int c) const override
@@ -256,24 +247,164 @@ struct PlainEngine : EngineBase {
         return (c >= 0 && c < 4) ? names[c] : nullptr;
     }
 
+    // One loop-body iteration.  psi_out receives psi^n (the real field this
+    // step starts from), s_tape receives psihat^n + psihat^{n+1}.
+    void step_one(T* psi_out, T* s_tape, Prof* prof)
+    {
+        Launch<T>::plain_z(s, w.p, w.p, psi_out, d, cf, 0, twz.tw);
+        if (prof) prof->mark(0);
+        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
+        if (prof) prof->mark(1);
+        Launch<T>::plain_x(s, w.p, psihat.p, nhat.p, s_tape, d, cf, twx.tw);
+        if (prof) prof->mark(2);
+        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
+        if (prof) prof->mark(3);
+    }
+
     void step(int nsteps, Prof* prof) override
     {
-        for (int it = 0; it < nsteps; ++it) {
-            bool last = it == nsteps - 1;
-            Launch<T>::plain_z(s, w.p, w.p, last ? psi_prev.p : nullptr, d, cf, 0, twz.tw);
-            if (prof) prof->mark(0);
-            Launch<T>::axis(s, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
-            if (prof) prof->mark(1);
-            Launch<T>::plain_x(s, w.p, psihat.p, nhat.p, d, cf, twx.tw);
-            if (prof) prof->mark(2);
-            Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
-            if (prof) prof->mark(3);
-        }
+        for (int it = 0; it < nsteps; ++it) step_one(it == nsteps - 1 ? psi_prev.p : nullptr, nullptr, prof);
         LAUNCHCHK();
         if (nsteps > 0) {
             psi_valid = false;
             prev_valid = true;
+            fresh = false;
         }
+    }
+
+    // ------------------------------------------------------------ adjoint --
+    // Checkpointed forward: (psihat, nhat) saved at the start of every
+    // segment of `ck` steps; the last segment also records its tape (psi^n
+    // and psihat^n + psihat^{n+1} per step), so it is not recomputed.
+    int tape_K = 0, tape_ck = 0, tape_nseg = 0;
+    std::vector<std::unique_ptr<DevBuf<T>>> ck_ph, ck_nh, tp_psi, tp_s;
+    DevBuf<T> end_ph, end_nh, lbuf;
+    DevBuf<double> acc_x, acc_z, gsum;
+    bool fresh = true;
+
+    void copy_dd(T* dst, const T* src)
+    {
+        HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    }
+
+    void restore_w()  // w = REDFT01_y REDFT01_x (psihat)
+    {
+        Launch<T>::axis(s, psihat.p, w.p, d, 0, REDFT01, T(1), twx.tw);
+        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
+    }
+
+    void run_segment(int nsteps, bool record)
+    {
+        for (int m = 0; m < nsteps; ++m)
+            step_one(record ? tp_psi[m]->p : nullptr, record ? tp_s[m]->p : nullptr, nullptr);
+        LAUNCHCHK();
+    }
+
+    void forward_tape(int K, int ck) override
+    {
+        if (!fresh) throw ApiError{SMAD_ESTATE, "forward_tape must start from the pre-loop state (after init or set_field(PSI))"};
+        if (K < 1 || ck < 1) throw ApiError{SMAD_EINVAL, "K and ckpt_every must be >= 1"};
+        ck = std::min(ck, K);
+        int nseg = (K + ck - 1) / ck;
+        if ((int)ck_ph.size() < nseg) {
+            ck_ph.resize(nseg);
+            ck_nh.resize(nseg);
+        }
+        for (int g = 0; g < nseg; ++g) {
+            if (!ck_ph[g]) {
+                ck_ph[g].reset(new DevBuf<T>);
+                ck_ph[g]->alloc(n);
+                ck_nh[g].reset(new DevBuf<T>);
+                ck_nh[g]->alloc(n);
+            }
+        }
+        if ((int)tp_psi.size() < ck) {
+            tp_psi.resize(ck);
+            tp_s.resize(ck);
+        }
+        for (int m = 0; m < ck; ++m) {
+            if (!tp_psi[m]) {
+                tp_psi[m].reset(new DevBuf<T>);
+                tp_psi[m]->alloc(n);
+                tp_s[m].reset(new DevBuf<T>);
+                tp_s[m]->alloc(n);
+            }
+        }
+        if (!end_ph.p) {
+            end_ph.alloc(n);
+            end_nh.alloc(n);
+            lbuf.alloc(n);
+            acc_x.alloc(2 * (size_t)num_tiles(d, 0, Lines<T>::NL));
+            acc_z.alloc(2 * (size_t)num_tiles(d, 2, Lines<T>::NL));
+            gsum.alloc(4);
+        }
+        for (int g = 0; g < nseg; ++g) {
+            copy_dd(ck_ph[g]->p, psihat.p);
+            copy_dd(ck_nh[g]->p, nhat.p);
+            int ns = std::min(ck, K - g * ck);
+            run_segment(ns, g == nseg - 1);
+        }
+        copy_dd(end_ph.p, psihat.p);
+        copy_dd(end_nh.p, nhat.p);
+        tape_K = K;
+        tape_ck = ck;
+        tape_nseg = nseg;
+        psi_valid = false;
+        prev_valid = false;
+        fresh = false;
+    }
+
+    void adjoint(const double* g_psik, double* dpsi0, double* dparams) override
+    {
+        if (tape_K < 1) throw ApiError{SMAD_ESTATE, "adjoint needs a preceding forward_tape"};
+        const int K = tape_K, ck = tape_ck, nseg = tape_nseg;
+        // lt^K = REDFT10^3(g): z and y here, x inside the first adjoint x pass
+        upload(s, psi.p, n, g_psik);
+        Launch<T>::axis(s, psi.p, w.p, d, 2, REDFT10, T(1), twz.tw);
+        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
+        HIPCHK(hipMemsetAsync(lbuf.p, 0, n * sizeof(T), s));
+        HIPCHK(hipMemsetAsync(acc_x.p, 0, acc_x.n * sizeof(double), s));
+        HIPCHK(hipMemsetAsync(acc_z.p, 0, acc_z.n * sizeof(double), s));
+        // w holds the partial transform of the cotangent; recomputation
+        // below uses w too, so park it in psi (unused during the adjoint)
+        for (int g = nseg - 1; g >= 0; --g) {
+            int n0 = g * ck, ns = std::min(ck, K - n0);
+            if (g != nseg - 1) {
+                copy_dd(psi.p, w.p);
+                copy_dd(psihat.p, ck_ph[g]->p);
+                copy_dd(nhat.p, ck_nh[g]->p);
+                restore_w();
+                run_segment(ns, true);
+                copy_dd(w.p, psi.p);
+            }
+            for (int m = ns - 1; m >= 0; --m) {
+                int nn = n0 + m;
+                Launch<T>::adj_x(s, w.p, lbuf.p, tp_s[m]->p, d, cf, nn >= 1 ? 0 : 1, acc_x.p, twx.tw);
+                Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
+                Launch<T>::adj_z(s, w.p, tp_psi[m]->p, d, cf, acc_z.p, twz.tw);
+                Launch<T>::axis(s, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
+            }
+            LAUNCHCHK();
+        }
+        // dJ/dpsi0 = s REDFT01^3(a lt^1 + REDFT10_x(G))
+        Launch<T>::adj_x(s, w.p, lbuf.p, nullptr, d, cf, 2, acc_x.p, twx.tw);
+        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
+        Launch<T>::axis(s, w.p, psi.p, d, 2, REDFT01, T(1), twz.tw);
+        Launch<T>::fold(s, acc_x.p, (int)(acc_x.n / 2), 2, gsum.p);
+        Launch<T>::fold(s, acc_z.p, (int)(acc_z.n / 2), 2, gsum.p + 2);
+        LAUNCHCHK();
+        double h[4];
+        HIPCHK(hipMemcpyAsync(h, gsum.p, sizeof(h), hipMemcpyDeviceToHost, s));
+        (void)download(s, psi.p, n, dpsi0);
+        for (int q = 0; q < 4; ++q) dparams[q] = h[q];
+        // leave the context at the end of the taped forward run
+        copy_dd(psihat.p, end_ph.p);
+        copy_dd(nhat.p, end_nh.p);
+        restore_w();
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(s));
+        psi_valid = false;
+        prev_valid = false;
     }
 
     void materialize()
@@ -301,6 +432,7 @@ struct PlainEngine : EngineBase {
         case SMAD_FIELD_PSI:
             upload(s, psi.p, n, in);
             preloop();
+            fresh = true;
             break;
         case SMAD_FIELD_PSI_HAT:
             upload(s, psihat.p, n, in);
@@ -309,8 +441,12 @@ struct PlainEngine : EngineBase {
             LAUNCHCHK();
             psi_valid = false;
             prev_valid = false;
+            fresh = false;
             break;
-        case SMAD_FIELD_N_HAT: upload(s, nhat.p, n, in); break;
+        case SMAD_FIELD_N_HAT:
+            upload(s, nhat.p, n, in);
+            fresh = false;
+            break;
         default: throw ApiError{SMAD_EINVAL, "field cannot be set in the plain model"};
         }
     }
@@ -329,6 +465,195 @@ struct PlainEngine : EngineBase {
         out[1] = h[1];
         out[2] = h[0] / (p.dt * h[1]);
         return h[2] > 0 ? SMAD_ENONFINITE : SMAD_OK;
+    }
+};
+
+// ------------------------------------------------------------ advective --
+// adv/cosAdvTrackMult.cpp:827-1369 with psihat (scaled) as the carried state.
+// Each 3D transform is three axis passes; pointwise and spectral stages are
+// separate kernels.
+template <typename T>
+struct AdvEngine : EngineBase {
+    smad_params p;
+    hipStream_t s;
+    Dims d;
+    size_t n;
+    AdvCoef ac{};
+    TwTable<T> twx, twy, twz;
+    DevBuf<T> ph, nhat, psi, psi_prev, gx, gy, gz, sp, sx, sy, sz, ux, uy, uz, nr;
+    DevBuf<double> part, red;
+    int nparts = 1024;
+    bool psi_valid = false, prev_valid = false;
+
+    AdvEngine(const smad_params& pp, hipStream_t st) : p(pp), s(st)
+    {
+        d = Dims{p.nx, p.ny, p.nz};
+        n = (size_t)d.total();
+        PlainCoef& cf = ac.pc;
+        const double pi = M_PI;
+        double dx = 2.0 * pi / (p.q0 * p.nw);
+        cf.qfx = pi / (dx * p.nx);
+        cf.qfy = pi / (dx * p.ny);
+        cf.qfz = pi / (dx * p.nz);
+        cf.eps = p.eps;
+        cf.alpha = p.alpha;
+        cf.q0sq = p.q0 * p.q0;
+        cf.beta = p.beta;
+        cf.gamma = p.gamma;
+        cf.dtd2 = p.dt / 2;
+        cf.scale = 0.125 / ((double)p.nx * p.ny * p.nz);
+        ac.nu = p.nu;
+        twx.make(p.nx, s);
+        twy.make(p.ny, s);
+        twz.make(p.nz, s);
+        for (DevBuf<T>* b : {&ph, &nhat, &psi, &psi_prev, &gx, &gy, &gz, &sp, &sx, &sy, &sz, &ux, &uy, &uz, &nr})
+            b->alloc(n);
+        part.alloc(3 * nparts);
+        red.alloc(8);
+    }
+
+    const char* kernel_class(int c) const override
+    {
+        static const char* names[] = {"axis passes (r2r)", "adv_spec", "adv_force", "adv_stokes",
+                                      "adv_nl", "plain_x: REDFT10_x + CN/AB2 update"};
+        return (c >= 0 && c < 6) ? names[c] : nullptr;
+    }
+
+    // out = scale * R2R(kx, ky, kz)(in), passes z, y, x
+    void tr3(const T* in, T* out, int kx, int ky, int kz, T scale = T(1))
+    {
+        Launch<T>::axis(s, in, out, d, 2, kz, T(1), twz.tw);
+        Launch<T>::axis(s, out, out, d, 1, ky, T(1), twy.tw);
+        Launch<T>::axis(s, out, out, d, 0, kx, scale, twx.tw);
+    }
+
+    void preloop()
+    {
+        // Nq^0 = CT(beta psi^3 - gamma psi^5), no advection (:760-777)
+        Launch<T>::plain_z(s, psi.p, nr.p, nullptr, d, ac.pc, 1, twz.tw);
+        Launch<T>::axis(s, nr.p, nr.p, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(s, nr.p, nhat.p, d, 0, REDFT10, T(1), twx.tw);
+        tr3(psi.p, ph.p, REDFT10, REDFT10, REDFT10, T(ac.pc.scale));
+        LAUNCHCHK();
+        psi_valid = true;
+        prev_valid = false;
+    }
+
+    void init(const double* psi0) override
+    {
+        upload(s, psi.p, n, psi0);
+        preloop();
+    }
+
+    void step_one(bool keep_prev, Prof* prof)
+    {
+        auto mk = [&](int c) { if (prof) prof->mark(c); };
+        Launch<T>::adv_spec(s, ph.p, gx.p, gy.p, gz.p, sp.p, d, ac);
+        mk(1);
+        tr3(ph.p, psi.p, REDFT01, REDFT01, REDFT01);
+        mk(0);
+        if (keep_prev) HIPCHK(hipMemcpyAsync(psi_prev.p, psi.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        tr3(gx.p, gx.p, RODFT01, REDFT01, REDFT01);
+        tr3(gy.p, gy.p, REDFT01, RODFT01, REDFT01);
+        tr3(gz.p, gz.p, REDFT01, REDFT01, RODFT01);
+        tr3(sp.p, sp.p, REDFT01, REDFT01, REDFT01);
+        mk(0);
+        Launch<T>::adv_force(s, psi.p, sp.p, gx.p, gy.p, gz.p, sx.p, sy.p, sz.p, d, ac);
+        mk(2);
+        tr3(sx.p, sx.p, RODFT10, REDFT10, REDFT10);
+        tr3(sy.p, sy.p, REDFT10, RODFT10, REDFT10);
+        tr3(sz.p, sz.p, REDFT10, REDFT10, RODFT10);
+        mk(0);
+        Launch<T>::adv_stokes(s, sx.p, sy.p, sz.p, ux.p, uy.p, uz.p, d, ac);
+        mk(3);
+        tr3(ux.p, ux.p, RODFT01, REDFT01, REDFT01);
+        tr3(uy.p, uy.p, REDFT01, RODFT01, REDFT01);
+        tr3(uz.p, uz.p, REDFT01, REDFT01, RODFT01);
+        mk(0);
+        Launch<T>::adv_nl(s, psi.p, ux.p, uy.p, uz.p, gx.p, gy.p, gz.p, nr.p, d, ac);
+        mk(4);
+        Launch<T>::axis(s, nr.p, nr.p, d, 2, REDFT10, T(1), twz.tw);
+        Launch<T>::axis(s, nr.p, nr.p, d, 1, REDFT10, T(1), twy.tw);
+        mk(0);
+        Launch<T>::plain_x(s, nr.p, ph.p, nhat.p, nullptr, d, ac.pc, twx.tw, 0);
+        mk(5);
+    }
+
+    void step(int nsteps, Prof* prof) override
+    {
+        for (int it = 0; it < nsteps; ++it) step_one(it == nsteps - 1, prof);
+        LAUNCHCHK();
+        if (nsteps > 0) {
+            psi_valid = false;
+            prev_valid = true;
+        }
+    }
+
+    void materialize()
+    {
+        if (!psi_valid) {
+            tr3(ph.p, psi.p, REDFT01, REDFT01, REDFT01);
+            LAUNCHCHK();
+            psi_valid = true;
+        }
+    }
+
+    int fetch(int field, double* out) override
+    {
+        switch (field) {
+        case SMAD_FIELD_PSI: materialize(); return download(s, psi.p, n, out);
+        case SMAD_FIELD_PSI_HAT: return download(s, ph.p, n, out);
+        case SMAD_FIELD_N_HAT: return download(s, nhat.p, n, out);
+        case SMAD_FIELD_GRAD_X: return download(s, gx.p, n, out);
+        case SMAD_FIELD_GRAD_Y: return download(s, gy.p, n, out);
+        case SMAD_FIELD_GRAD_Z: return download(s, gz.p, n, out);
+        case SMAD_FIELD_VEL_X: return download(s, ux.p, n, out);
+        case SMAD_FIELD_VEL_Y: return download(s, uy.p, n, out);
+        case SMAD_FIELD_VEL_Z: return download(s, uz.p, n, out);
+        default: throw ApiError{SMAD_EINVAL, "unknown field"};
+        }
+    }
+
+    void set_field(int field, const double* in) override
+    {
+        switch (field) {
+        case SMAD_FIELD_PSI:
+            upload(s, psi.p, n, in);
+            preloop();
+            break;
+        case SMAD_FIELD_PSI_HAT:
+            upload(s, ph.p, n, in);
+            psi_valid = false;
+            prev_valid = false;
+            break;
+        case SMAD_FIELD_N_HAT: upload(s, nhat.p, n, in); break;
+        default: throw ApiError{SMAD_EINVAL, "field cannot be set in the advective model"};
+        }
+    }
+
+    int reduce(int what, double* out) override
+    {
+        if (what != SMAD_RED_L1) throw ApiError{SMAD_EINVAL, "unknown reduction"};
+        if (!prev_valid) throw ApiError{SMAD_ESTATE, "L1 needs at least one step since init"};
+        materialize();
+        Launch<T>::l1(s, psi.p, psi_prev.p, (long)n, part.p, nparts, red.p);
+        LAUNCHCHK();
+        double h[3];
+        HIPCHK(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        out[0] = h[0];
+        out[1] = h[1];
+        out[2] = h[0] / (p.dt * h[1]);
+        return h[2] > 0 ? SMAD_ENONFINITE : SMAD_OK;
+    }
+
+    void forward_tape(int, int) override
+    {
+        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
+    }
+    void adjoint(const double*, double*, double*) override
+    {
+        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
     }
 };
 
@@ -415,8 +740,12 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
         g_init_error = "smad_init: precision must be 64 or 32";
         return SMAD_EINVAL;
     }
-    if (p.model != SMAD_MODEL_PLAIN) {
+    if (p.model != SMAD_MODEL_PLAIN && p.model != SMAD_MODEL_ADV) {
         g_init_error = "smad_init: model not supported by this build";
+        return SMAD_EINVAL;
+    }
+    if (p.model == SMAD_MODEL_ADV && !(p.nu > 0)) {
+        g_init_error = "smad_init: nu must be positive";
         return SMAD_EINVAL;
     }
     if (!(p.dt > 0) || !(p.nw > 0) || !(p.q0 > 0)) {
@@ -428,10 +757,14 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
     int rc = SMAD_OK;
     try {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        if (p.precision == SMAD_FP64)
-            c->eng.reset(new PlainEngine<double>(p, c->stream));
-        else
-            c->eng.reset(new PlainEngine<float>(p, c->stream));
+        const bool f64 = p.precision == SMAD_FP64;
+        if (p.model == SMAD_MODEL_PLAIN) {
+            if (f64) c->eng.reset(new PlainEngine<double>(p, c->stream));
+            else c->eng.reset(new PlainEngine<float>(p, c->stream));
+        } else {
+            if (f64) c->eng.reset(new AdvEngine<double>(p, c->stream));
+            else c->eng.reset(new AdvEngine<float>(p, c->stream));
+        }
         c->eng->init(psi0);
         HIPCHK(hipStreamSynchronize(c->stream));
     } catch (const HipError& e) {
@@ -458,6 +791,24 @@ int smad_step(smad_ctx* c, int nsteps)
         if (nsteps < 0) throw ApiError{SMAD_EINVAL, "nsteps < 0"};
         c->eng->step(nsteps, nullptr);
         c->steps += nsteps;
+        return SMAD_OK;
+    });
+}
+
+int smad_forward_tape(smad_ctx* c, int nsteps, int ckpt_every)
+{
+    return guarded(c, [&] {
+        c->eng->forward_tape(nsteps, ckpt_every);
+        c->steps += nsteps;
+        return SMAD_OK;
+    });
+}
+
+int smad_adjoint(smad_ctx* c, const double* dl_dpsik, double* dl_dpsi0, double* dl_dparams)
+{
+    return guarded(c, [&] {
+        if (!dl_dpsik || !dl_dpsi0 || !dl_dparams) throw ApiError{SMAD_EINVAL, "null argument"};
+        c->eng->adjoint(dl_dpsik, dl_dpsi0, dl_dparams);
         return SMAD_OK;
     });
 }

@@ -1,0 +1,90 @@
+"""GPU parity of the checkpointed discrete adjoint of the plain step against
+the FP64 oracle adjoint (which is itself pinned by central differences)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(smadft, shape=(16, 32, 32), prec=64):
+    p = smadft.Params.default(0, shape[0])
+    p.nx, p.ny, p.nz = shape
+    p.precision = prec
+    psi0 = smadft.ic_slab(p, 1.328, 1e-2, seed=3)
+    tgt = np.random.default_rng(0).standard_normal(psi0.shape) * 0.1
+    return p, psi0, tgt
+
+
+@pytest.mark.parametrize("K,ck", [(24, 5), (10, 10), (1, 1), (13, 4)])
+def test_adjoint_fp64_matches_oracle(smadft, oracle, K, ck):
+    p, psi0, tgt = setup(smadft)
+    psik_o = oracle.plain_forward(p, psi0, K)
+    g = 2 * (psik_o - tgt)
+    _, d0_o, dp_o = oracle.plain_adjoint(p, psi0, K, g)
+    with smadft.Stepper(p, psi0) as s:
+        s.forward_tape(K, ck)
+        psik = s.fetch()
+        d0, dp = s.adjoint(g)
+        after = s.fetch()
+    assert rel_l2(psik, psik_o) < 1e-11
+    assert rel_l2(after, psik_o) < 1e-11  # context left at step K
+    assert rel_l2(d0, d0_o) < 1e-9
+    for q in range(4):
+        assert dp[q] == pytest.approx(dp_o[q], rel=1e-9, abs=1e-9 * np.abs(dp_o).max())
+
+
+def test_adjoint_fp32_mixed_close_to_oracle(smadft, oracle):
+    p, psi0, tgt = setup(smadft, prec=32)
+    K = 20
+    g = 2 * (oracle.plain_forward(p, psi0, K) - tgt)
+    _, d0_o, dp_o = oracle.plain_adjoint(p, psi0, K, g)
+    with smadft.Stepper(p, psi0) as s:
+        s.forward_tape(K, 5)
+        d0, dp = s.adjoint(g)
+    assert rel_l2(d0, d0_o) < 1e-4
+    assert np.max(np.abs(dp - dp_o) / np.abs(dp_o)) < 1e-3
+
+
+def test_adjoint_state_errors(smadft):
+    p, psi0, tgt = setup(smadft, shape=(16, 16, 16))
+    with smadft.Stepper(p, psi0) as s:
+        with pytest.raises(smadft.SmadError) as e:
+            s.adjoint(tgt)
+        assert e.value.code == smadft.ESTATE
+        s.step(3)
+        with pytest.raises(smadft.SmadError) as e:
+            s.forward_tape(5, 2)
+        assert e.value.code == smadft.ESTATE
+
+
+def test_autograd_rollout_matches_oracle_and_fd(smadft, oracle):
+    import importlib
+    import torch
+    ag = importlib.import_module("sma-dft_amd.autograd")
+    p, psi0, tgt = setup(smadft, shape=(16, 16, 32))
+    K = 16
+    theta = torch.tensor([p.eps, p.alpha, p.beta, p.gamma], dtype=torch.float64, requires_grad=True)
+    x0 = torch.tensor(psi0, requires_grad=True)
+    t = torch.tensor(tgt)
+    loss = ((ag.rollout(x0, theta, p, K, 4) - t) ** 2).sum()
+    loss.backward()
+    g = 2 * (oracle.plain_forward(p, psi0, K) - tgt)
+    _, d0_o, dp_o = oracle.plain_adjoint(p, psi0, K, g)
+    assert rel_l2(x0.grad.numpy(), d0_o) < 1e-9
+    assert np.allclose(theta.grad.numpy(), dp_o, rtol=1e-9)
+    # central differences of the GPU loss in the parameters, h = 1e-6
+    import dataclasses
+    h = 1e-6
+
+    def J(**kw):
+        q = dataclasses.replace(p, **kw)
+        with smadft.Stepper(q, psi0) as s:
+            s.step(K)
+            return float(np.sum((s.fetch() - tgt) ** 2))
+
+    for i, name in enumerate(["eps", "alpha", "beta", "gamma"]):
+        v = getattr(p, name)
+        fd = (J(**{name: v + h}) - J(**{name: v - h})) / (2 * h)
+        assert theta.grad[i].item() == pytest.approx(fd, rel=1e-6), name

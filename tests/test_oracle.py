@@ -132,3 +132,46 @@ def test_l1_sums(oracle):
     a, b = rng.standard_normal(1000), rng.standard_normal(1000)
     s = oracle.l1_sums(a, b)
     assert np.allclose(s, [np.abs(a - b).sum(), np.abs(a).sum()], rtol=1e-13)
+
+
+def _adj_setup(smadft):
+    p = _params(smadft, (16, 8, 16), dt=0.1)
+    psi0 = smadft.ic_slab(p, 1.328, 1e-2, seed=3)
+    tgt = np.random.default_rng(0).standard_normal(psi0.shape) * 0.1
+    return p, psi0, tgt
+
+
+def test_plain_adjoint_central_fd(smadft, oracle):
+    import copy
+    p, psi0, tgt = _adj_setup(smadft)
+    K = 12
+
+    def J(pp, x0):
+        return float(np.sum((oracle.plain_forward(pp, x0, K) - tgt) ** 2))
+
+    g = 2 * (oracle.plain_forward(p, psi0, K) - tgt)
+    psik, d0, dp = oracle.plain_adjoint(p, psi0, K, g)
+    h = 1e-6
+    for idx, name in enumerate(["eps", "alpha", "beta", "gamma"]):
+        pp = copy.copy(p); setattr(pp, name, getattr(p, name) + h)
+        pm = copy.copy(p); setattr(pm, name, getattr(p, name) - h)
+        fd = (J(pp, psi0) - J(pm, psi0)) / (2 * h)
+        assert dp[idx] == pytest.approx(fd, rel=1e-6), name
+    v = np.random.default_rng(1).standard_normal(psi0.shape)
+    fd = (J(p, psi0 + h * v) - J(p, psi0 - h * v)) / (2 * h)
+    assert float(np.sum(d0 * v)) == pytest.approx(fd, rel=1e-6)
+
+
+def test_plain_adjoint_linear_dot_product(smadft, oracle):
+    # beta = gamma = 0: the K-step map is linear, M psi0 = psi_K, and the
+    # adjoint must satisfy <M x, y> = <x, M^T y> to roundoff
+    p, _, _ = _adj_setup(smadft)
+    p.beta = 0.0
+    p.gamma = 0.0
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((p.nx, p.ny, p.nz))
+    y = rng.standard_normal(x.shape)
+    K = 7
+    mx = oracle.plain_forward(p, x, K)
+    _, mty, _ = oracle.plain_adjoint(p, x, K, y)
+    assert float(np.sum(mx * y)) == pytest.approx(float(np.sum(x * mty)), rel=1e-12)
