@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--acc-steps", type=int, default=1000)
     ap.add_argument("--prof-steps", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--adj-steps", type=int, default=200)
+    ap.add_argument("--adj-ckpt", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-fp", choices=["64", "32"], default=None,
                     help="time one precision only (for profiler runs)")
@@ -124,6 +126,37 @@ def main():
             acc[label] = {"rel_l2_fp32_vs_fp64": err, "norm_fp64": float(np.linalg.norm(out[64]))}
             log(f"accuracy {label}: rel L2 = {err:.3e}")
 
+    # adjoint / forward time ratio: K steps with checkpoints every ckpt,
+    # (taped forward + adjoint) over the plain forward of the same K steps
+    adj = None
+    if args.adj_steps > 0 and args.only_fp is None:
+        adj = {"K": args.adj_steps, "ckpt_every": args.adj_ckpt}
+        grads = {}
+        for prec in (64, 32):
+            p = smadft.Params.default(smadft.MODEL_PLAIN, n)
+            p.precision = prec
+            ic = smadft.ic_curved(p64, 1.328, 1e-3, seed=1)
+            with smadft.Stepper(p, ic) as s:
+                s.step(5)
+                s.set_field(smadft.FIELD_PSI, ic)
+                fwd_ms = s.time_steps(args.adj_steps)
+                s.set_field(smadft.FIELD_PSI, ic)
+                s.forward_tape(args.adj_steps, args.adj_ckpt)
+                psik = s.fetch()
+                target = smadft.ic_curved(p64, 1.328, 1e-3, seed=5)
+                d0, dp = s.adjoint(2.0 * (psik - target))
+                tape_ms, adj_ms = s.adjoint_timing()
+            grads[prec] = (d0, dp)
+            adj[f"fp{prec}"] = {"forward_ms": fwd_ms, "forward_tape_ms": tape_ms,
+                                "adjoint_ms": adj_ms, "ratio": (tape_ms + adj_ms) / fwd_ms}
+            log(f"adjoint fp{prec}: fwd {fwd_ms:.2f} ms, taped fwd {tape_ms:.2f} ms, "
+                f"adjoint {adj_ms:.2f} ms, ratio {(tape_ms + adj_ms) / fwd_ms:.3f}")
+        d64, p64g = grads[64]
+        d32, p32g = grads[32]
+        adj["fp32_grad_rel_err"] = {
+            "dpsi0": float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64)),
+            "dtheta": [float(abs(a - b) / abs(b)) for a, b in zip(p32g, p64g)]}
+
     # CPU baseline: the FP64 oracle on the host cores, bounded sample
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
@@ -164,7 +197,8 @@ def main():
             "accuracy_after_steps": args.acc_steps,
             "accuracy": acc,
         },
-        "adjoint_fwd_ratio": None,
+        "adjoint_fwd_ratio": adj["fp64"]["ratio"] if adj else None,
+        "adjoint": adj,
         "kernels": kernels,
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

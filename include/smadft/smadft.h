@@ -6,6 +6,7 @@
  * inline body of `while (L1 > limL1)` in each main()
  *   plain      no-adv/cosNoAdvTrackMult.cpp:729-1147 (cosNoAdvTrackSing.cpp:681-1072)
  *   advective  adv/cosAdvTrackMult.cpp:827-1742
+ *   quasi      quasi/quasi.cpp:1099-2498
  * and below it the FFTW-MPI calls fftw_mpi_plan_r2r_3d / fftw_execute.  This
  * header cuts at that seam: everything above the loop (parameters, initial
  * conditions, L1 cadence, output writers) stays host code, and the loop body
@@ -74,14 +75,20 @@ enum {
     SMAD_FIELD_GRAD_Z = 5,
     SMAD_FIELD_VEL_X = 6,   /* real velocity of the last step (advective) */
     SMAD_FIELD_VEL_Y = 7,
-    SMAD_FIELD_VEL_Z = 8
+    SMAD_FIELD_VEL_Z = 8,
+    SMAD_FIELD_PRESSURE = 9, /* real pressure (quasi), FP64 storage */
+    SMAD_FIELD_RHO = 10,     /* filtered density of the last step (quasi) */
+    SMAD_FIELD_ENERGY = 11,  /* free-energy density e of the last step (quasi) */
+    SMAD_FIELD_DIVV_HAT = 12 /* REDFT10^3 of the imposed divergence (quasi, unscaled) */
 };
 
 /* Reductions (FP64 on the device, deterministic order). */
 enum {
-    SMAD_RED_L1 = 0 /* out[0] = sum|psi_n+1 - psi_n| over the last step taken,
-                       out[1] = sum|psi_n+1|, out[2] = out[0]/(dt out[1]);
-                       reference L1 check, no-adv/cosNoAdvTrackMult.cpp:814-842 */
+    SMAD_RED_L1 = 0,    /* out[0] = sum|psi_n+1 - psi_n| over the last step taken,
+                           out[1] = sum|psi_n+1|, out[2] = out[0]/(dt out[1]);
+                           reference L1 check, no-adv/cosNoAdvTrackMult.cpp:814-842 */
+    SMAD_RED_MASS = 1,  /* quasi: out[0] = sum(rho) dx^3 (quasi/quasi.cpp:2238-2243) */
+    SMAD_RED_ENERGY = 2 /* quasi: out[0] = sum(e) dx^3 (quasi/quasi.cpp:2244-2248) */
 };
 
 /* Create a context and run the reference pre-loop on psi0 (host, FP64,
@@ -124,6 +131,10 @@ int smad_sync(smad_ctx *ctx);
  * left at step K, as after the forward run. */
 int smad_forward_tape(smad_ctx *ctx, int nsteps, int ckpt_every);
 int smad_adjoint(smad_ctx *ctx, const double *dl_dpsik, double *dl_dpsi0, double *dl_dparams);
+
+/* Device time (HIP events, ms) of the last smad_forward_tape and of the
+ * device part of the last smad_adjoint (host copies excluded). */
+int smad_adjoint_timing(const smad_ctx *ctx, double *fwd_tape_ms, double *adjoint_ms);
 
 /* Enqueue nsteps steps between two HIP events recorded on the context
  * stream and wait; *ms = elapsed device time in milliseconds.  The

@@ -48,6 +48,10 @@ def lib():
         L.orc_l1_sums.argtypes = [dp, dp, ctypes.c_long, dp]
         L.orc_adv_preloop.argtypes = [pp, dp, dp]
         L.orc_adv_step.argtypes = [pp, dp, dp, dp, dp, dp, ctypes.c_int]
+        L.orc_quasi_preloop.argtypes = [pp, ctypes.POINTER(QState)]
+        L.orc_quasi_step.argtypes = [pp, ctypes.POINTER(QState), ctypes.c_int, dp]
+        L.orc_quasi_preloop.restype = None
+        L.orc_quasi_step.restype = None
         L.orc_adv_stokes.argtypes = [pp, dp, dp, dp, dp, dp, dp]
         L.orc_adv_stokes.restype = None
         L.orc_plain_adjoint.argtypes = [pp, dp, ctypes.c_int, dp, dp, dp, dp]
@@ -79,6 +83,36 @@ def r2r3d(x, kinds):
     y = np.empty_like(x)
     lib().orc_r2r3d(_p(x), _p(y), *x.shape, *kinds)
     return y
+
+
+class QState(ctypes.Structure):
+    _fields_ = [(nm, ctypes.POINTER(ctypes.c_double)) for nm in
+                ("psi", "psiq", "psiq_old", "nl", "rhoq", "rho", "rho_old", "divv", "vel", "p",
+                 "energy", "grad_psi")] + [("nloop", ctypes.c_int)]
+
+
+class QuasiOracle:
+    """Reference-order quasi-incompressible stepper (quasi/quasi.cpp:990-2251)."""
+
+    def __init__(self, p, psi0):
+        self.op = oparams(p)
+        sh = tuple(np.shape(psi0))
+        self.arrays = {}
+        for nm, _ in QState._fields_[:-1]:
+            shape = (3,) + sh if nm in ("vel", "grad_psi") else sh
+            self.arrays[nm] = np.zeros(shape)
+        self.arrays["psi"][...] = psi0
+        self.st = QState(**{nm: _p(a) for nm, a in self.arrays.items()}, nloop=0)
+        lib().orc_quasi_preloop(ctypes.byref(self.op), ctypes.byref(self.st))
+        self.sums = np.zeros(4)
+
+    def __getattr__(self, nm):
+        if nm in ("arrays", "op", "st"):
+            raise AttributeError(nm)
+        return self.arrays[nm]
+
+    def step(self, n=1):
+        lib().orc_quasi_step(ctypes.byref(self.op), ctypes.byref(self.st), int(n), _p(self.sums))
 
 
 class PlainOracle:

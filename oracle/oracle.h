@@ -57,6 +57,17 @@ void orc_adv_step(const orc_params *p, double *psi, double *nq, double *nq_past,
 void orc_adv_stokes(const orc_params *p, const double *sx, const double *sy, const double *sz,
                     double *vx, double *vy, double *vz);
 
+/* ------------------------------------------- quasi-incompressible model -- */
+/* quasi/quasi.cpp:990-2251.  All arrays nx*ny*nz (vel, grad_psi: 3x). */
+typedef struct {
+    double *psi, *psiq, *psiq_old, *nl, *rhoq, *rho, *rho_old, *divv, *vel, *p, *energy, *grad_psi;
+    int nloop;
+} orc_quasi_state;
+void orc_quasi_preloop(const orc_params *p, orc_quasi_state *st);
+/* sums (may be NULL): sum|psi-psi_old|, sum|psi|, mass sum(rho) dx^3, energy sum(e) dx^3
+ * for the last step taken */
+void orc_quasi_step(const orc_params *p, orc_quasi_state *st, int nsteps, double *sums);
+
 /* --------------------------------------------------- plain adjoint -- */
 /* K plain steps from psi0, then the discrete adjoint for the cotangent
  * g_psik = dJ/dpsi^K.  Outputs psi^K, dJ/dpsi0 and dJ/d(eps, alpha, beta, gamma). */

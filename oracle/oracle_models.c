@@ -384,3 +384,233 @@ void orc_plain_adjoint(const orc_params *p, const double *psi0, int nsteps, cons
     free(psi); free(ph); free(nq); free(nqp); free(tmp); free(lt1); free(lt2); free(mt);
     grid_free(&g);
 }
+
+/* ---------------------------------------------- quasi-incompressible -- */
+/*
+ * quasi/quasi.cpp:990-1036 (pre-loop) and :1099-2251 (loop body), global
+ * (single-slab) indexing.  The state is kept in an orc_quasi_state; every
+ * array is nx*ny*nz doubles.
+ */
+static void dst_shift(const grid_t *gp, const double *ah, double *gx, double *gy, double *gz)
+{
+    grid_t g = *gp;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                long id = IDX(i, j, k);
+                gx[id] = i < g.nx - 1 ? -g.sx[i] * ah[IDX(i + 1, j, k)] : 0.0;
+                gy[id] = j < g.ny - 1 ? -g.sy[j] * ah[IDX(i, j + 1, k)] : 0.0;
+                gz[id] = k < g.nz - 1 ? -g.sz[k] * ah[IDX(i, j, k + 1)] : 0.0;
+            }
+}
+
+static void ist3(const grid_t *g, double *gx, double *gy, double *gz)
+{
+    orc_r2r3d(gx, gx, g->nx, g->ny, g->nz, ORC_RODFT01, ORC_REDFT01, ORC_REDFT01);
+    orc_r2r3d(gy, gy, g->nx, g->ny, g->nz, ORC_REDFT01, ORC_RODFT01, ORC_REDFT01);
+    orc_r2r3d(gz, gz, g->nx, g->ny, g->nz, ORC_REDFT01, ORC_REDFT01, ORC_RODFT01);
+}
+
+static double mq2c_at(const grid_t *gp, int i, int j, int k)
+{
+    return pow(gp->qx[i], 2) + pow(gp->qy[j], 2) + pow(gp->qz[k], 2);
+}
+
+void orc_quasi_preloop(const orc_params *p, orc_quasi_state *st)
+{
+    grid_t g;
+    grid_make(p, &g);
+    const long n = g.n;
+    double *tmp = malloc(sizeof(double) * n);
+    ct(&g, st->psi, st->psiq);                                   /* :1002-1004 */
+    for (long i = 0; i < n; ++i) st->psiq[i] *= g.scale;         /* :1009-1016 */
+    ict(&g, st->psiq, st->psi);                                  /* :1018-1020 */
+    nonlin(p, st->psi, tmp, n);                                  /* :1022-1030 */
+    memcpy(st->psiq_old, st->psiq, sizeof(double) * n);          /* :1025 */
+    ct(&g, tmp, st->nl);                                         /* :1036-1038 */
+    memset(st->rhoq, 0, sizeof(double) * n);
+    memset(st->rho_old, 0, sizeof(double) * n);
+    memset(st->vel, 0, sizeof(double) * 3 * n);
+    memset(st->divv, 0, sizeof(double) * n);
+    st->nloop = 0;
+    free(tmp);
+    grid_free(&g);
+}
+
+void orc_quasi_step(const orc_params *p, orc_quasi_state *st, int nsteps, double *sums)
+{
+    grid_t g;
+    grid_make(p, &g);
+    const long n = g.n;
+    const double dt = p->dt, dtd2 = dt / 2, sc = g.scale, nu = p->nu, q02 = p->q0 * p->q0;
+    const double sig2 = p->filter_sigma * p->filter_sigma;
+    double *nl_old = malloc(sizeof(double) * n), *f = malloc(sizeof(double) * n);
+    double *gpx = malloc(sizeof(double) * n), *gpy = malloc(sizeof(double) * n), *gpz = malloc(sizeof(double) * n);
+    double *grx = malloc(sizeof(double) * n), *gry = malloc(sizeof(double) * n), *grz = malloc(sizeof(double) * n);
+    double *rho = st->rho, *lap = malloc(sizeof(double) * n), *tmp = malloc(sizeof(double) * n);
+    double *e = st->energy, *mu = malloc(sizeof(double) * n);
+    double *Sx = malloc(sizeof(double) * n), *Sy = malloc(sizeof(double) * n), *Sz = malloc(sizeof(double) * n);
+    double *pr = st->p, *psi_old = malloc(sizeof(double) * n), *psiq_prev = malloc(sizeof(double) * n);
+    double *vx = st->vel, *vy = st->vel + n, *vz = st->vel + 2 * n;
+
+    for (int s = 0; s < nsteps; ++s) {
+        st->nloop++;
+        const int nloop = st->nloop;
+        memcpy(nl_old, st->nl, sizeof(double) * n);                         /* :1117 */
+        /* f = iCT(alpha (q0^2 - |q|^2) psiq) :1121-1138 */
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    f[id] = p->alpha * (q02 - mq2c_at(&g, i, j, k)) * st->psiq[id];
+                }
+        ict(&g, f, f);
+        /* gradients of psi and (previous, filtered) rho :1147-1303 */
+        dst_shift(&g, st->psiq, gpx, gpy, gpz);
+        dst_shift(&g, st->rhoq, grx, gry, grz);
+        ist3(&g, gpx, gpy, gpz);
+        ist3(&g, grx, gry, grz);
+        /* density law + Gaussian filter :1322-1360 */
+        for (long id = 0; id < n; ++id)
+            rho[id] = p->kp * sqrt(q02 * pow(st->psi[id], 2) + pow(gpx[id], 2) + pow(gpy[id], 2) +
+                                   pow(gpz[id], 2)) + p->rho0;
+        ct(&g, rho, st->rhoq);
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    st->rhoq[id] = sc * st->rhoq[id] * exp(-sig2 * mq2c_at(&g, i, j, k) * 2);
+                }
+        ict(&g, st->rhoq, rho);
+        /* divv and lap(rho f) :1365-1438 */
+        if (nloop > p->div_switch) {
+            for (long id = 0; id < n; ++id) {
+                st->divv[id] = -((rho[id] - st->rho_old[id]) / dt + vx[id] * grx[id] + vy[id] * gry[id] +
+                                 vz[id] * grz[id]) / (0.5 * (rho[id] + st->rho_old[id]));
+                lap[id] = rho[id] * f[id];
+            }
+            ct(&g, lap, lap);
+            for (int i = 0; i < g.nx; ++i)
+                for (int j = 0; j < g.ny; ++j)
+                    for (int k = 0; k < g.nz; ++k) {
+                        long id = IDX(i, j, k);
+                        lap[id] = -sc * mq2c_at(&g, i, j, k) * lap[id];
+                    }
+            ict(&g, lap, lap);
+        } else {
+            for (int i = 0; i < g.nx; ++i)
+                for (int j = 0; j < g.ny; ++j)
+                    for (int k = 0; k < g.nz; ++k) {
+                        long id = IDX(i, j, k);
+                        double m2 = mq2c_at(&g, i, j, k);
+                        lap[id] = -m2 * p->alpha * (q02 - m2) * st->psiq[id];
+                    }
+            ict(&g, lap, lap);
+            for (long id = 0; id < n; ++id) lap[id] = rho[id] * lap[id];
+        }
+        memcpy(st->rho_old, rho, sizeof(double) * n);                         /* :1440 */
+        ct(&g, st->divv, st->divv);                                           /* :1442-1444 */
+        /* energy, mu, S :1449-1488 */
+        for (long id = 0; id < n; ++id) {
+            double x = st->psi[id];
+            e[id] = -0.5 * p->eps * pow(x, 2) + 0.5 * pow(f[id], 2) - 0.25 * p->beta * pow(x, 4) +
+                    (p->gamma / 6) * pow(x, 6);
+            mu[id] = lap[id] + rho[id] * (f[id] - p->eps * x - p->beta * pow(x, 3) + p->gamma * pow(x, 5));
+            Sx[id] = mu[id] * gpx[id] + e[id] * grx[id];
+            Sy[id] = mu[id] * gpy[id] + e[id] * gry[id];
+            Sz[id] = mu[id] * gpz[id] + e[id] * grz[id];
+        }
+        orc_r2r3d(Sx, Sx, g.nx, g.ny, g.nz, ORC_RODFT10, ORC_REDFT10, ORC_REDFT10);  /* :1494-1504 */
+        orc_r2r3d(Sy, Sy, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_RODFT10, ORC_REDFT10);
+        orc_r2r3d(Sz, Sz, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_REDFT10, ORC_RODFT10);
+        /* pressure :1510-1592 */
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    double sxv = i > 0 ? Sx[IDX(i - 1, j, k)] : 0.0;
+                    double syv = j > 0 ? Sy[IDX(i, j - 1, k)] : 0.0;
+                    double szv = k > 0 ? Sz[IDX(i, j, k - 1)] : 0.0;
+                    if (i + j + k != 0)
+                        pr[id] = sc * (-(g.qx[i] * sxv + g.qy[j] * syv + g.qz[k] * szv) / mq2c_at(&g, i, j, k) +
+                                       (p->lambda + 2 * nu) * st->divv[id]);
+                    else
+                        pr[id] = 0.0;
+                }
+        ict(&g, pr, pr);
+        /* velocity: solenoidal projection + irrotational part :1599-2109 */
+        memset(vx, 0, sizeof(double) * 3 * n);
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    if (i < g.nx - 1) {
+                        double syv = (j != 0) ? Sy[IDX(i + 1, j - 1, k)] : 0.0;
+                        double szv = (k != 0) ? Sz[IDX(i + 1, j, k - 1)] : 0.0;
+                        double mq2 = pow(g.sx[i], 2) + pow(g.qy[j], 2) + pow(g.qz[k], 2);
+                        double cm1 = sc / (nu * mq2), cm2 = g.sx[i] / mq2;
+                        double dot = g.sx[i] * Sx[id] + g.qy[j] * syv + g.qz[k] * szv;
+                        vx[id] = cm1 * (Sx[id] - cm2 * dot) + cm1 * (nu * g.sx[i] * st->divv[IDX(i + 1, j, k)]);
+                    }
+                    if (j < g.ny - 1 && k < g.nz - 1) {
+                        double sxv = (i != 0) ? Sx[IDX(i - 1, j + 1, k)] : 0.0;
+                        double szv = (k != 0) ? Sz[IDX(i, j + 1, k - 1)] : 0.0;
+                        double mq2 = pow(g.qx[i], 2) + pow(g.sy[j], 2) + pow(g.qz[k], 2);
+                        double cm1 = sc / (nu * mq2), cm2 = g.sy[j] / mq2;
+                        double dot = g.qx[i] * sxv + g.sy[j] * Sy[id] + g.qz[k] * szv;
+                        vy[id] = cm1 * (Sy[id] - cm2 * dot) + cm1 * (nu * g.sy[j] * st->divv[IDX(i, j + 1, k)]);
+                        sxv = (i != 0) ? Sx[IDX(i - 1, j, k + 1)] : 0.0;
+                        double syv = (j != 0) ? Sy[IDX(i, j - 1, k + 1)] : 0.0;
+                        mq2 = pow(g.qx[i], 2) + pow(g.qy[j], 2) + pow(g.sz[k], 2);
+                        cm1 = sc / (nu * mq2);
+                        cm2 = g.sz[k] / mq2;
+                        dot = g.qx[i] * sxv + g.qy[j] * syv + g.sz[k] * Sz[id];
+                        vz[id] = cm1 * (Sz[id] - cm2 * dot) + cm1 * (nu * g.sz[k] * st->divv[IDX(i, j, k + 1)]);
+                    }
+                }
+        ist3(&g, vx, vy, vz);                                                 /* :2113-2123 */
+        /* Nl :2128-2154 */
+        for (long id = 0; id < n; ++id) {
+            double v = -mu[id] - vx[id] * gpx[id] - vy[id] * gpy[id] - vz[id] * gpz[id];
+            if (nloop > p->div_switch + 1)
+                v += (pr[id] + rho[id] * e[id]) * p->kp *
+                     sqrt(pow(grx[id], 2) + pow(gry[id], 2) + pow(grz[id], 2)) / pow(rho[id], 2);
+            st->nl[id] = v;
+        }
+        ct(&g, st->nl, st->nl);                                               /* :2158-2160 */
+        if (nloop == 1) memcpy(nl_old, st->nl, sizeof(double) * n);          /* :2162-2164 */
+        /* BDF2/AB2 update :2169-2182 */
+        memcpy(psiq_prev, st->psiq, sizeof(double) * n);
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    double m2 = mq2c_at(&g, i, j, k);
+                    double al = p->eps - p->alpha * pow(q02 - m2, 2);
+                    double c1 = 2.0 - dt * p->rho_m * al, c2 = 1.5 - dt * p->rho_m * al;
+                    st->psiq[id] = (c1 * st->psiq[id] - 0.5 * st->psiq_old[id] +
+                                    dtd2 * sc * (3.0 * st->nl[id] - nl_old[id])) / c2;
+                }
+        memcpy(st->psiq_old, psiq_prev, sizeof(double) * n);
+        memcpy(psi_old, st->psi, sizeof(double) * n);                         /* :2186 */
+        ict(&g, st->psiq, st->psi);                                           /* :2187-2190 */
+        memcpy(st->grad_psi, gpx, sizeof(double) * n);
+        memcpy(st->grad_psi + n, gpy, sizeof(double) * n);
+        memcpy(st->grad_psi + 2 * n, gpz, sizeof(double) * n);
+    }
+    if (sums) { /* :2194-2251 for the last step */
+        double sa = 0, sb = 0, sr = 0, se = 0;
+        for (long id = 0; id < n; ++id) {
+            sa += fabs(st->psi[id] - psi_old[id]);
+            sb += fabs(st->psi[id]);
+            sr += rho[id];
+            se += e[id];
+        }
+        double dv = g.dx * g.dx * g.dx;
+        sums[0] = sa; sums[1] = sb; sums[2] = sr * dv; sums[3] = se * dv;
+    }
+    free(nl_old); free(f); free(gpx); free(gpy); free(gpz); free(grx); free(gry); free(grz);
+    free(lap); free(tmp); free(mu); free(Sx); free(Sy); free(Sz); free(psi_old); free(psiq_prev);
+    grid_free(&g);
+}

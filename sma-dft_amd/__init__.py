@@ -26,8 +26,9 @@ FP64, FP32_MIXED = 64, 32
 FIELD_PSI, FIELD_PSI_HAT, FIELD_N_HAT = 0, 1, 2
 FIELD_GRAD_X, FIELD_GRAD_Y, FIELD_GRAD_Z = 3, 4, 5
 FIELD_VEL_X, FIELD_VEL_Y, FIELD_VEL_Z = 6, 7, 8
+FIELD_PRESSURE, FIELD_RHO, FIELD_ENERGY, FIELD_DIVV_HAT = 9, 10, 11, 12
 
-RED_L1 = 0
+RED_L1, RED_MASS, RED_ENERGY = 0, 1, 2
 
 OK, EINVAL, EHIP, ENOMEM, ENONFINITE, ESTATE = 0, -1, -2, -3, -4, -5
 
@@ -37,7 +38,7 @@ EXPORTS = (
     "smad_sync", "smad_stream", "smad_steps_taken", "smad_destroy",
     "smad_last_error", "smad_ic_focal_conic", "smad_ic_slab", "smad_ic_curved",
     "smad_ic_add_noise", "smad_default_params", "smad_r2r_3d", "smad_time_steps",
-    "smad_profile_steps", "smad_kernel_class_name", "smad_forward_tape", "smad_adjoint",
+    "smad_profile_steps", "smad_kernel_class_name", "smad_forward_tape", "smad_adjoint", "smad_adjoint_timing",
 )
 
 REDFT10, REDFT01, RODFT10, RODFT01 = 0, 1, 2, 3
@@ -131,6 +132,7 @@ def lib() -> ctypes.CDLL:
         L.smad_time_steps.argtypes = [vp, ctypes.c_int, dp]
         L.smad_forward_tape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.smad_adjoint.argtypes = [vp, dp, dp, dp]
+        L.smad_adjoint_timing.argtypes = [vp, dp, dp]
         L.smad_profile_steps.argtypes = [vp, ctypes.c_int, dp, P(ctypes.c_int), ctypes.c_int,
                                          P(ctypes.c_int)]
         L.smad_kernel_class_name.argtypes = [vp, ctypes.c_int]
@@ -259,6 +261,12 @@ class Stepper:
         dp = np.zeros(4)
         self._check(lib().smad_adjoint(self._h, _dptr(g), _dptr(d0), _dptr(dp)))
         return d0, dp
+
+    def adjoint_timing(self):
+        """(device ms of the last forward_tape, device ms of the last adjoint)."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().smad_adjoint_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def time_steps(self, n: int) -> float:
         """Run n steps between two HIP events on the context stream; returns ms."""

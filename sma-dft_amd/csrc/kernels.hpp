@@ -511,6 +511,251 @@ __global__ __launch_bounds__(WG) void k_adv_nl(const T* __restrict__ psi, const 
     }
 }
 
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* out);
+
+// ------------------------------------------------------------------------
+// Quasi-incompressible model (quasi/quasi.cpp:1099-2251), pointwise and
+// spectral stages on the global grid.  Arithmetic in FP64; the pressure
+// field is FP64 storage in every precision mode.
+struct QuasiCoef {
+    PlainCoef pc;
+    double nu, lambda, rho0, kp, rho_m, sig2, dt;
+    int pre_switch;  // 1 while nloop <= div_switch
+};
+
+__device__ __forceinline__ double mq2_at(const PlainCoef& cf, int i, int j, int k)
+{
+    double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
+    return qx * qx + qy * qy + qz * qz;
+}
+
+// f_hat = alpha (q0^2 - |q|^2) psihat (:1121-1131); DST-shift gradients of
+// psihat and of the previous filtered rhohat (:1147-1256); before the
+// switch also lap_hat = -|q|^2 alpha (q0^2 - |q|^2) psihat (:1411-1421).
+template <typename T>
+__global__ __launch_bounds__(WG) void k_q_spec(const T* __restrict__ ph, const T* __restrict__ rq,
+                                               T* __restrict__ fh, T* __restrict__ gpx,
+                                               T* __restrict__ gpy, T* __restrict__ gpz,
+                                               T* __restrict__ grx, T* __restrict__ gry,
+                                               T* __restrict__ grz, T* __restrict__ laph, Dims d,
+                                               QuasiCoef qc)
+{
+    const PlainCoef& cf = qc.pc;
+    const long n = d.total(), sx = (long)d.ny * d.nz, sy = d.nz;
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        double m2 = mq2_at(cf, i, j, k);
+        double phv = (double)ph[id];
+        fh[id] = T(cf.alpha * (cf.q0sq - m2) * phv);
+        if (laph) laph[id] = T(-m2 * cf.alpha * (cf.q0sq - m2) * phv);
+        double ax = -(i + 1) * cf.qfx, ay = -(j + 1) * cf.qfy, az = -(k + 1) * cf.qfz;
+        bool bx = i < d.nx - 1, by = j < d.ny - 1, bz = k < d.nz - 1;
+        gpx[id] = bx ? T(ax * (double)ph[id + sx]) : T(0);
+        gpy[id] = by ? T(ay * (double)ph[id + sy]) : T(0);
+        gpz[id] = bz ? T(az * (double)ph[id + 1]) : T(0);
+        grx[id] = bx ? T(ax * (double)rq[id + sx]) : T(0);
+        gry[id] = by ? T(ay * (double)rq[id + sy]) : T(0);
+        grz[id] = bz ? T(az * (double)rq[id + 1]) : T(0);
+    }
+}
+
+// rho = kp sqrt(q0^2 psi^2 + |grad psi|^2) + rho0 (:1322-1340)
+template <typename T>
+__global__ __launch_bounds__(WG) void k_q_rho(const T* __restrict__ psi, const T* __restrict__ gx,
+                                              const T* __restrict__ gy, const T* __restrict__ gz,
+                                              T* __restrict__ rho, long n, QuasiCoef qc)
+{
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        double x = (double)psi[id], a = (double)gx[id], b = (double)gy[id], c = (double)gz[id];
+        rho[id] = T(qc.kp * sqrt(qc.pc.q0sq * x * x + a * a + b * b + c * c) + qc.rho0);
+    }
+}
+
+// spectral multipliers: mode 0: s exp(-2 sig^2 |q|^2) (rho filter, :1350-1354),
+// mode 1: -s |q|^2 (laplacian of rho f, :1393-1401)
+template <typename T>
+__global__ __launch_bounds__(WG) void k_q_specmul(T* __restrict__ a, Dims d, QuasiCoef qc, int mode)
+{
+    const long n = d.total();
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        double m2 = mq2_at(qc.pc, i, j, k);
+        double f = mode == 0 ? qc.pc.scale * exp(-qc.sig2 * m2 * 2.0) : -qc.pc.scale * m2;
+        a[id] = T(f * (double)a[id]);
+    }
+}
+
+// after the switch (:1369-1384): divv = -((rho - rho_old)/dt + u . grad rho) / (rho + rho_old)/2,
+// rf = rho f.  Before it: lap = rho * lap.  Then rho_old = rho.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_q_divv(const T* __restrict__ rho, T* __restrict__ rho_old,
+                                               const T* __restrict__ ux, const T* __restrict__ uy,
+                                               const T* __restrict__ uz, const T* __restrict__ grx,
+                                               const T* __restrict__ gry, const T* __restrict__ grz,
+                                               const T* __restrict__ f, T* __restrict__ divv,
+                                               T* __restrict__ lap, long n, QuasiCoef qc)
+{
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        double r = (double)rho[id], ro = (double)rho_old[id];
+        if (qc.pre_switch) {
+            lap[id] = T(r * (double)lap[id]);
+        } else {
+            double adv = (double)ux[id] * (double)grx[id] + (double)uy[id] * (double)gry[id] +
+                         (double)uz[id] * (double)grz[id];
+            divv[id] = T(-((r - ro) / qc.dt + adv) / (0.5 * (r + ro)));
+            lap[id] = T(r * (double)f[id]);
+        }
+        rho_old[id] = rho[id];
+    }
+}
+
+// e, mu and S = mu grad psi + e grad rho (:1449-1488)
+template <typename T>
+__global__ __launch_bounds__(WG) void k_q_force(const T* __restrict__ psi, const T* __restrict__ f,
+                                                const T* __restrict__ lap, const T* __restrict__ rho,
+                                                const T* __restrict__ gpx, const T* __restrict__ gpy,
+                                                const T* __restrict__ gpz, const T* __restrict__ grx,
+                                                const T* __restrict__ gry, const T* __restrict__ grz,
+                                                T* __restrict__ e, T* __restrict__ mu,
+                                                T* __restrict__ sx, T* __restrict__ sy,
+                                                T* __restrict__ sz, long n, QuasiCoef qc)
+{
+    const PlainCoef& cf = qc.pc;
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        double x = (double)psi[id], x2 = x * x, x3 = x2 * x, fv = (double)f[id];
+        double ev = -0.5 * cf.eps * x2 + 0.5 * fv * fv - 0.25 * cf.beta * x2 * x2 + (cf.gamma / 6) * x3 * x3;
+        double mv = (double)lap[id] + (double)rho[id] * (fv - cf.eps * x - cf.beta * x3 + cf.gamma * x3 * x2);
+        e[id] = T(ev);
+        mu[id] = T(mv);
+        sx[id] = T(mv * (double)gpx[id] + ev * (double)grx[id]);
+        sy[id] = T(mv * (double)gpy[id] + ev * (double)gry[id]);
+        sz[id] = T(mv * (double)gpz[id] + ev * (double)grz[id]);
+    }
+}
+
+// pressure (:1510-1592) and velocity (:1599-2109), FP64 arithmetic:
+//  p_hat = s (-(qx Sx[i-1] + qy Sy[j-1] + qz Sz[k-1])/|q|^2 + (lambda + 2 nu) divv_hat), p_hat[0] = 0
+//  u = solenoidal projection of S (as the advective model) + CM1 nu qs divv_hat[own axis + 1]
+template <typename T>
+__global__ __launch_bounds__(WG) void k_q_pvel(const T* __restrict__ Sx, const T* __restrict__ Sy,
+                                               const T* __restrict__ Sz, const T* __restrict__ dv,
+                                               double* __restrict__ ph, T* __restrict__ ux,
+                                               T* __restrict__ uy, T* __restrict__ uz, Dims d,
+                                               QuasiCoef qc)
+{
+    const PlainCoef& cf = qc.pc;
+    const long n = d.total(), sx = (long)d.ny * d.nz, sy = d.nz;
+    const double nu = qc.nu, sc = cf.scale;
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
+        double qsx = (i + 1) * cf.qfx, qsy = (j + 1) * cf.qfy, qsz = (k + 1) * cf.qfz;
+        double sxv = i > 0 ? (double)Sx[id - sx] : 0.0;
+        double syv = j > 0 ? (double)Sy[id - sy] : 0.0;
+        double szv = k > 0 ? (double)Sz[id - 1] : 0.0;
+        ph[id] = (i + j + k != 0)
+                     ? sc * (-(qx * sxv + qy * syv + qz * szv) / (qx * qx + qy * qy + qz * qz) +
+                             (qc.lambda + 2 * nu) * (double)dv[id])
+                     : 0.0;
+        double vx = 0.0, vy = 0.0, vz = 0.0;
+        if (i < d.nx - 1) {
+            double s2 = j != 0 ? (double)Sy[id + sx - sy] : 0.0;
+            double s3 = k != 0 ? (double)Sz[id + sx - 1] : 0.0;
+            double mq2 = qsx * qsx + qy * qy + qz * qz;
+            double cm1 = sc / (nu * mq2);
+            double dot = qsx * (double)Sx[id] + qy * s2 + qz * s3;
+            vx = cm1 * ((double)Sx[id] - qsx / mq2 * dot) + cm1 * (nu * qsx * (double)dv[id + sx]);
+        }
+        if (j < d.ny - 1 && k < d.nz - 1) {
+            double s1 = i != 0 ? (double)Sx[id - sx + sy] : 0.0;
+            double s3 = k != 0 ? (double)Sz[id + sy - 1] : 0.0;
+            double mq2 = qx * qx + qsy * qsy + qz * qz;
+            double cm1 = sc / (nu * mq2);
+            double dot = qx * s1 + qsy * (double)Sy[id] + qz * s3;
+            vy = cm1 * ((double)Sy[id] - qsy / mq2 * dot) + cm1 * (nu * qsy * (double)dv[id + sy]);
+            s1 = i != 0 ? (double)Sx[id - sx + 1] : 0.0;
+            double s2 = j != 0 ? (double)Sy[id - sy + 1] : 0.0;
+            mq2 = qx * qx + qy * qy + qsz * qsz;
+            cm1 = sc / (nu * mq2);
+            dot = qx * s1 + qy * s2 + qsz * (double)Sz[id];
+            vz = cm1 * ((double)Sz[id] - qsz / mq2 * dot) + cm1 * (nu * qsz * (double)dv[id + 1]);
+        }
+        ux[id] = T(vx);
+        uy[id] = T(vy);
+        uz[id] = T(vz);
+    }
+}
+
+// Nl = -mu - u . grad psi (+ (p + rho e) kp |grad rho| / rho^2 after the
+// switch step + 1) (:2128-2154)
+template <typename T>
+__global__ __launch_bounds__(WG) void k_q_nl(const T* __restrict__ mu, const T* __restrict__ ux,
+                                             const T* __restrict__ uy, const T* __restrict__ uz,
+                                             const T* __restrict__ gpx, const T* __restrict__ gpy,
+                                             const T* __restrict__ gpz, const double* __restrict__ pr,
+                                             const T* __restrict__ rho, const T* __restrict__ e,
+                                             const T* __restrict__ grx, const T* __restrict__ gry,
+                                             const T* __restrict__ grz, T* __restrict__ nl, long n,
+                                             QuasiCoef qc, int press_term)
+{
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        double v = -(double)mu[id] - (double)ux[id] * (double)gpx[id] - (double)uy[id] * (double)gpy[id] -
+                   (double)uz[id] * (double)gpz[id];
+        if (press_term) {
+            double r = (double)rho[id], a = (double)grx[id], b = (double)gry[id], c = (double)grz[id];
+            v += (pr[id] + r * (double)e[id]) * qc.kp * sqrt(a * a + b * b + c * c) / (r * r);
+        }
+        nl[id] = T(v);
+    }
+}
+
+// BDF2/AB2 update (:2169-2182), C1 = 2 - dt rho_m aLin, C2 = 1.5 - dt rho_m aLin:
+//   psihat' = (C1 psihat - psihat_old/2 + dt/2 s (3 Nl - Nl_old)) / C2
+// then psihat_old = psihat, Nl_old = Nl (the next step's history).
+template <typename T>
+__global__ __launch_bounds__(WG) void k_q_update(T* __restrict__ ph, T* __restrict__ pho,
+                                                 const T* __restrict__ nl, T* __restrict__ nlo,
+                                                 Dims d, QuasiCoef qc)
+{
+    const PlainCoef& cf = qc.pc;
+    const long n = d.total();
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        double t = cf.q0sq - mq2_at(cf, i, j, k);
+        double al = cf.eps - cf.alpha * t * t;
+        double c1 = 2.0 - qc.dt * qc.rho_m * al, c2 = 1.5 - qc.dt * qc.rho_m * al;
+        double p0 = (double)ph[id], nv = (double)nl[id];
+        double r = (c1 * p0 - 0.5 * (double)pho[id] + cf.dtd2 * cf.scale * (3.0 * nv - (double)nlo[id])) / c2;
+        pho[id] = ph[id];
+        ph[id] = T(r);
+        nlo[id] = nl[id];
+    }
+}
+
+// sums {sum a, #non-finite(a)} in FP64
+template <typename T>
+__global__ __launch_bounds__(WG) void k_sum_partial(const T* __restrict__ a, long n, double* __restrict__ part)
+{
+    double v[3] = {0.0, 0.0, 0.0};
+    for (long i = (long)blockIdx.x * WG + threadIdx.x; i < n; i += (long)gridDim.x * WG) {
+        double x = (double)a[i];
+        if (!isfinite(x)) v[2] += 1.0;
+        v[0] += x;
+    }
+    double o[3];
+    block_sum<3>(v, o);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x * 3 + 0] = o[0];
+        part[blockIdx.x * 3 + 1] = o[1];
+        part[blockIdx.x * 3 + 2] = o[2];
+    }
+}
+
 // ------------------------------------------------------------------------
 // Discrete adjoint of the plain step (no reference counterpart).  With
 // lt = D lambda, D = prod over axes of diag(2,1,...,1), one backward step is
@@ -747,6 +992,26 @@ struct Launch {
     static void adj_z(hipStream_t s, T* w, const T* psi_tape, Dims d, const PlainCoef& cf,
                       double* acc, const Tw<T>& tw);
     static void fold(hipStream_t s, const double* part, int nparts, int nv, double* out);
+    static void sum(hipStream_t s, const T* a, long n, double* part, int nparts, double* out);
+    static void q_spec(hipStream_t s, const T* ph, const T* rq, T* fh, T* gpx, T* gpy, T* gpz,
+                       T* grx, T* gry, T* grz, T* laph, Dims d, const QuasiCoef& qc);
+    static void q_rho(hipStream_t s, const T* psi, const T* gx, const T* gy, const T* gz, T* rho,
+                      Dims d, const QuasiCoef& qc);
+    static void q_specmul(hipStream_t s, T* a, Dims d, const QuasiCoef& qc, int mode);
+    static void q_divv(hipStream_t s, const T* rho, T* rho_old, const T* ux, const T* uy,
+                       const T* uz, const T* grx, const T* gry, const T* grz, const T* f, T* divv,
+                       T* lap, Dims d, const QuasiCoef& qc);
+    static void q_force(hipStream_t s, const T* psi, const T* f, const T* lap, const T* rho,
+                        const T* gpx, const T* gpy, const T* gpz, const T* grx, const T* gry,
+                        const T* grz, T* e, T* mu, T* sx, T* sy, T* sz, Dims d, const QuasiCoef& qc);
+    static void q_pvel(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, const T* dv,
+                       double* ph, T* ux, T* uy, T* uz, Dims d, const QuasiCoef& qc);
+    static void q_nl(hipStream_t s, const T* mu, const T* ux, const T* uy, const T* uz,
+                     const T* gpx, const T* gpy, const T* gpz, const double* pr, const T* rho,
+                     const T* e, const T* grx, const T* gry, const T* grz, T* nl, Dims d,
+                     const QuasiCoef& qc, int press_term);
+    static void q_update(hipStream_t s, T* ph, T* pho, const T* nl, T* nlo, Dims d,
+                         const QuasiCoef& qc);
     static void l1(hipStream_t s, const T* a, const T* b, long n, double* part, int nparts,
                    double* out);
 };

@@ -116,4 +116,75 @@ void Launch<T>::l1(hipStream_t s, const T* a, const T* b, long n, double* part, 
     hipLaunchKernelGGL(k_fold<3>, dim3(1), dim3(WG), 0, s, part, nparts, out);
 }
 
+#define SMAD_FLAT(KERNEL, ...) \
+    hipLaunchKernelGGL(KERNEL<T>, flat_grid(d.total()), dim3(WG), 0, s, __VA_ARGS__)
+
+template <typename T>
+void Launch<T>::sum(hipStream_t s, const T* a, long n, double* part, int nparts, double* out)
+{
+    hipLaunchKernelGGL(k_sum_partial<T>, dim3(nparts), dim3(WG), 0, s, a, n, part);
+    hipLaunchKernelGGL(k_fold<3>, dim3(1), dim3(WG), 0, s, part, nparts, out);
+}
+
+template <typename T>
+void Launch<T>::q_spec(hipStream_t s, const T* ph, const T* rq, T* fh, T* gpx, T* gpy, T* gpz,
+                       T* grx, T* gry, T* grz, T* laph, Dims d, const QuasiCoef& qc)
+{
+    SMAD_FLAT(k_q_spec, ph, rq, fh, gpx, gpy, gpz, grx, gry, grz, laph, d, qc);
+}
+
+template <typename T>
+void Launch<T>::q_rho(hipStream_t s, const T* psi, const T* gx, const T* gy, const T* gz, T* rho,
+                      Dims d, const QuasiCoef& qc)
+{
+    SMAD_FLAT(k_q_rho, psi, gx, gy, gz, rho, d.total(), qc);
+}
+
+template <typename T>
+void Launch<T>::q_specmul(hipStream_t s, T* a, Dims d, const QuasiCoef& qc, int mode)
+{
+    SMAD_FLAT(k_q_specmul, a, d, qc, mode);
+}
+
+template <typename T>
+void Launch<T>::q_divv(hipStream_t s, const T* rho, T* rho_old, const T* ux, const T* uy,
+                       const T* uz, const T* grx, const T* gry, const T* grz, const T* f, T* divv,
+                       T* lap, Dims d, const QuasiCoef& qc)
+{
+    SMAD_FLAT(k_q_divv, rho, rho_old, ux, uy, uz, grx, gry, grz, f, divv, lap, d.total(), qc);
+}
+
+template <typename T>
+void Launch<T>::q_force(hipStream_t s, const T* psi, const T* f, const T* lap, const T* rho,
+                        const T* gpx, const T* gpy, const T* gpz, const T* grx, const T* gry,
+                        const T* grz, T* e, T* mu, T* sx, T* sy, T* sz, Dims d, const QuasiCoef& qc)
+{
+    SMAD_FLAT(k_q_force, psi, f, lap, rho, gpx, gpy, gpz, grx, gry, grz, e, mu, sx, sy, sz,
+              d.total(), qc);
+}
+
+template <typename T>
+void Launch<T>::q_pvel(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, const T* dv,
+                       double* ph, T* ux, T* uy, T* uz, Dims d, const QuasiCoef& qc)
+{
+    SMAD_FLAT(k_q_pvel, Sx, Sy, Sz, dv, ph, ux, uy, uz, d, qc);
+}
+
+template <typename T>
+void Launch<T>::q_nl(hipStream_t s, const T* mu, const T* ux, const T* uy, const T* uz,
+                     const T* gpx, const T* gpy, const T* gpz, const double* pr, const T* rho,
+                     const T* e, const T* grx, const T* gry, const T* grz, T* nl, Dims d,
+                     const QuasiCoef& qc, int press_term)
+{
+    SMAD_FLAT(k_q_nl, mu, ux, uy, uz, gpx, gpy, gpz, pr, rho, e, grx, gry, grz, nl, d.total(), qc,
+              press_term);
+}
+
+template <typename T>
+void Launch<T>::q_update(hipStream_t s, T* ph, T* pho, const T* nl, T* nlo, Dims d,
+                         const QuasiCoef& qc)
+{
+    SMAD_FLAT(k_q_update, ph, pho, nl, nlo, d, qc);
+}
+
 }  // namespace smad

@@ -91,6 +91,28 @@ struct TwTable {
 
 struct Prof;
 
+// Device time between two points on a stream.
+struct EvPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EvPair()
+    {
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+    }
+    ~EvPair()
+    {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    double ms()
+    {
+        HIPCHK(hipEventSynchronize(b));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, a, b));
+        return t;
+    }
+};
+
 struct EngineBase {
     virtual ~EngineBase() = default;
     virtual void init(const double* psi0) = 0;
@@ -100,6 +122,7 @@ struct EngineBase {
     virtual void set_field(int field, const double* in) = 0;
     virtual int reduce(int what, double* out) = 0;
     virtual void forward_tape(int K, int ck) = 0;
+    double last_fwd_ms = 0.0, last_adj_ms = 0.0;
     virtual void adjoint(const double* g_psik, double* dpsi0, double* dparams) = 0;
 };
 
@@ -338,6 +361,8 @@ struct PlainEngine : EngineBase {
             acc_z.alloc(2 * (size_t)num_tiles(d, 2, Lines<T>::NL));
             gsum.alloc(4);
         }
+        EvPair ev;
+        HIPCHK(hipEventRecord(ev.a, s));
         for (int g = 0; g < nseg; ++g) {
             copy_dd(ck_ph[g]->p, psihat.p);
             copy_dd(ck_nh[g]->p, nhat.p);
@@ -346,6 +371,8 @@ struct PlainEngine : EngineBase {
         }
         copy_dd(end_ph.p, psihat.p);
         copy_dd(end_nh.p, nhat.p);
+        HIPCHK(hipEventRecord(ev.b, s));
+        last_fwd_ms = ev.ms();
         tape_K = K;
         tape_ck = ck;
         tape_nseg = nseg;
@@ -360,6 +387,8 @@ struct PlainEngine : EngineBase {
         const int K = tape_K, ck = tape_ck, nseg = tape_nseg;
         // lt^K = REDFT10^3(g): z and y here, x inside the first adjoint x pass
         upload(s, psi.p, n, g_psik);
+        EvPair ev;
+        HIPCHK(hipEventRecord(ev.a, s));
         Launch<T>::axis(s, psi.p, w.p, d, 2, REDFT10, T(1), twz.tw);
         Launch<T>::axis(s, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
         HIPCHK(hipMemsetAsync(lbuf.p, 0, n * sizeof(T), s));
@@ -393,6 +422,8 @@ struct PlainEngine : EngineBase {
         Launch<T>::fold(s, acc_x.p, (int)(acc_x.n / 2), 2, gsum.p);
         Launch<T>::fold(s, acc_z.p, (int)(acc_z.n / 2), 2, gsum.p + 2);
         LAUNCHCHK();
+        HIPCHK(hipEventRecord(ev.b, s));
+        last_adj_ms = ev.ms();
         double h[4];
         HIPCHK(hipMemcpyAsync(h, gsum.p, sizeof(h), hipMemcpyDeviceToHost, s));
         (void)download(s, psi.p, n, dpsi0);
@@ -657,6 +688,265 @@ struct AdvEngine : EngineBase {
     }
 };
 
+// ---------------------------------------------------------------- quasi --
+// quasi/quasi.cpp:990-2251.  Separate transform passes and pointwise /
+// spectral kernels; the pressure field is FP64 in every precision mode and
+// the pressure / velocity projection runs in FP64 arithmetic.
+template <typename T>
+struct QuasiEngine : EngineBase {
+    smad_params p;
+    hipStream_t s;
+    Dims d;
+    size_t n;
+    QuasiCoef qc{};
+    TwTable<T> twx, twy, twz;
+    TwTable<double> tdx, tdy, tdz;
+    DevBuf<T> ph, pho, nl, nlo, rhoq, rho, rho_old, divv, ux, uy, uz, psi, psi_prev, f, gpx, gpy,
+        gpz, grx, gry, grz, lap, e, mu, sx, sy, sz;
+    DevBuf<double> pr, part, red;
+    int nparts = 1024;
+    int nloop = 0;
+    bool psi_valid = false, prev_valid = false;
+
+    QuasiEngine(const smad_params& pp, hipStream_t st) : p(pp), s(st)
+    {
+        d = Dims{p.nx, p.ny, p.nz};
+        n = (size_t)d.total();
+        PlainCoef& cf = qc.pc;
+        const double pi = M_PI;
+        double dx = 2.0 * pi / (p.q0 * p.nw);
+        cf.qfx = pi / (dx * p.nx);
+        cf.qfy = pi / (dx * p.ny);
+        cf.qfz = pi / (dx * p.nz);
+        cf.eps = p.eps;
+        cf.alpha = p.alpha;
+        cf.q0sq = p.q0 * p.q0;
+        cf.beta = p.beta;
+        cf.gamma = p.gamma;
+        cf.dtd2 = p.dt / 2;
+        cf.scale = 0.125 / ((double)p.nx * p.ny * p.nz);
+        qc.nu = p.nu;
+        qc.lambda = p.lambda;
+        qc.rho0 = p.rho0;
+        qc.kp = p.kp;
+        qc.rho_m = p.rho_m;
+        qc.sig2 = p.filter_sigma * p.filter_sigma;
+        qc.dt = p.dt;
+        twx.make(p.nx, s);
+        twy.make(p.ny, s);
+        twz.make(p.nz, s);
+        tdx.make(p.nx, s);
+        tdy.make(p.ny, s);
+        tdz.make(p.nz, s);
+        for (DevBuf<T>* b : {&ph, &pho, &nl, &nlo, &rhoq, &rho, &rho_old, &divv, &ux, &uy, &uz, &psi,
+                             &psi_prev, &f, &gpx, &gpy, &gpz, &grx, &gry, &grz, &lap, &e, &mu, &sx,
+                             &sy, &sz})
+            b->alloc(n);
+        pr.alloc(n);
+        part.alloc(3 * nparts);
+        red.alloc(8);
+    }
+
+    const char* kernel_class(int c) const override
+    {
+        static const char* names[] = {"axis passes (r2r)", "pointwise / spectral stages",
+                                      "pressure + velocity projection (FP64)"};
+        return (c >= 0 && c < 3) ? names[c] : nullptr;
+    }
+
+    void tr3(const T* in, T* out, int kx, int ky, int kz, T scale = T(1))
+    {
+        Launch<T>::axis(s, in, out, d, 2, kz, T(1), twz.tw);
+        Launch<T>::axis(s, out, out, d, 1, ky, T(1), twy.tw);
+        Launch<T>::axis(s, out, out, d, 0, kx, scale, twx.tw);
+    }
+    void ist3(T* x, T* y, T* z)
+    {
+        tr3(x, x, RODFT01, REDFT01, REDFT01);
+        tr3(y, y, REDFT01, RODFT01, REDFT01);
+        tr3(z, z, REDFT01, REDFT01, RODFT01);
+    }
+    void zero(DevBuf<T>& b) { HIPCHK(hipMemsetAsync(b.p, 0, n * sizeof(T), s)); }
+
+    void preloop()  // quasi.cpp:990-1036
+    {
+        tr3(psi.p, ph.p, REDFT10, REDFT10, REDFT10, T(qc.pc.scale));
+        tr3(ph.p, psi.p, REDFT01, REDFT01, REDFT01);
+        Launch<T>::plain_z(s, psi.p, nl.p, nullptr, d, qc.pc, 1, twz.tw);
+        Launch<T>::axis(s, nl.p, nl.p, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(s, nl.p, nl.p, d, 0, REDFT10, T(1), twx.tw);
+        HIPCHK(hipMemcpyAsync(pho.p, ph.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        zero(rhoq);
+        zero(rho_old);
+        zero(ux);
+        zero(uy);
+        zero(uz);
+        zero(divv);
+        LAUNCHCHK();
+        nloop = 0;
+        psi_valid = true;
+        prev_valid = false;
+    }
+
+    void init(const double* psi0) override
+    {
+        upload(s, psi.p, n, psi0);
+        preloop();
+    }
+
+    void step_one(bool keep_prev, Prof* prof)
+    {
+        auto mk = [&](int c) { if (prof) prof->mark(c); };
+        ++nloop;
+        qc.pre_switch = nloop <= p.div_switch ? 1 : 0;
+        // psi of this step from psihat (the reference carries it from the
+        // previous step's inverse transform)
+        if (!psi_valid) tr3(ph.p, psi.p, REDFT01, REDFT01, REDFT01);
+        mk(0);
+        if (keep_prev) HIPCHK(hipMemcpyAsync(psi_prev.p, psi.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        Launch<T>::q_spec(s, ph.p, rhoq.p, f.p, gpx.p, gpy.p, gpz.p, grx.p, gry.p, grz.p,
+                          qc.pre_switch ? lap.p : nullptr, d, qc);
+        mk(1);
+        tr3(f.p, f.p, REDFT01, REDFT01, REDFT01);
+        ist3(gpx.p, gpy.p, gpz.p);
+        ist3(grx.p, gry.p, grz.p);
+        mk(0);
+        Launch<T>::q_rho(s, psi.p, gpx.p, gpy.p, gpz.p, rho.p, d, qc);
+        mk(1);
+        tr3(rho.p, rhoq.p, REDFT10, REDFT10, REDFT10);
+        mk(0);
+        Launch<T>::q_specmul(s, rhoq.p, d, qc, 0);
+        mk(1);
+        tr3(rhoq.p, rho.p, REDFT01, REDFT01, REDFT01);
+        if (qc.pre_switch) tr3(lap.p, lap.p, REDFT01, REDFT01, REDFT01);
+        mk(0);
+        Launch<T>::q_divv(s, rho.p, rho_old.p, ux.p, uy.p, uz.p, grx.p, gry.p, grz.p, f.p, divv.p,
+                          lap.p, d, qc);
+        mk(1);
+        if (!qc.pre_switch) {
+            tr3(lap.p, lap.p, REDFT10, REDFT10, REDFT10);
+            mk(0);
+            Launch<T>::q_specmul(s, lap.p, d, qc, 1);
+            mk(1);
+            tr3(lap.p, lap.p, REDFT01, REDFT01, REDFT01);
+            tr3(divv.p, divv.p, REDFT10, REDFT10, REDFT10);
+            mk(0);
+        }
+        Launch<T>::q_force(s, psi.p, f.p, lap.p, rho.p, gpx.p, gpy.p, gpz.p, grx.p, gry.p, grz.p,
+                           e.p, mu.p, sx.p, sy.p, sz.p, d, qc);
+        mk(1);
+        tr3(sx.p, sx.p, RODFT10, REDFT10, REDFT10);
+        tr3(sy.p, sy.p, REDFT10, RODFT10, REDFT10);
+        tr3(sz.p, sz.p, REDFT10, REDFT10, RODFT10);
+        mk(0);
+        Launch<T>::q_pvel(s, sx.p, sy.p, sz.p, divv.p, pr.p, ux.p, uy.p, uz.p, d, qc);
+        mk(2);
+        Launch<double>::axis(s, pr.p, pr.p, d, 2, REDFT01, 1.0, tdz.tw);
+        Launch<double>::axis(s, pr.p, pr.p, d, 1, REDFT01, 1.0, tdy.tw);
+        Launch<double>::axis(s, pr.p, pr.p, d, 0, REDFT01, 1.0, tdx.tw);
+        ist3(ux.p, uy.p, uz.p);
+        mk(0);
+        Launch<T>::q_nl(s, mu.p, ux.p, uy.p, uz.p, gpx.p, gpy.p, gpz.p, pr.p, rho.p, e.p, grx.p,
+                        gry.p, grz.p, nl.p, d, qc, nloop > p.div_switch + 1 ? 1 : 0);
+        mk(1);
+        tr3(nl.p, nl.p, REDFT10, REDFT10, REDFT10);
+        mk(0);
+        if (nloop == 1) HIPCHK(hipMemcpyAsync(nlo.p, nl.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        Launch<T>::q_update(s, ph.p, pho.p, nl.p, nlo.p, d, qc);
+        mk(1);
+        psi_valid = false;
+    }
+
+    void step(int nsteps, Prof* prof) override
+    {
+        for (int it = 0; it < nsteps; ++it) step_one(it == nsteps - 1, prof);
+        LAUNCHCHK();
+        if (nsteps > 0) prev_valid = true;
+    }
+
+    void materialize()
+    {
+        if (!psi_valid) {
+            tr3(ph.p, psi.p, REDFT01, REDFT01, REDFT01);
+            LAUNCHCHK();
+            psi_valid = true;
+        }
+    }
+
+    int fetch(int field, double* out) override
+    {
+        switch (field) {
+        case SMAD_FIELD_PSI: materialize(); return download(s, psi.p, n, out);
+        case SMAD_FIELD_PSI_HAT: return download(s, ph.p, n, out);
+        case SMAD_FIELD_N_HAT: return download(s, nl.p, n, out);
+        case SMAD_FIELD_GRAD_X: return download(s, gpx.p, n, out);
+        case SMAD_FIELD_GRAD_Y: return download(s, gpy.p, n, out);
+        case SMAD_FIELD_GRAD_Z: return download(s, gpz.p, n, out);
+        case SMAD_FIELD_VEL_X: return download(s, ux.p, n, out);
+        case SMAD_FIELD_VEL_Y: return download(s, uy.p, n, out);
+        case SMAD_FIELD_VEL_Z: return download(s, uz.p, n, out);
+        case SMAD_FIELD_PRESSURE: return download(s, pr.p, n, out);
+        case SMAD_FIELD_RHO: return download(s, rho.p, n, out);
+        case SMAD_FIELD_ENERGY: return download(s, e.p, n, out);
+        case SMAD_FIELD_DIVV_HAT: return download(s, divv.p, n, out);
+        default: throw ApiError{SMAD_EINVAL, "unknown field"};
+        }
+    }
+
+    void set_field(int field, const double* in) override
+    {
+        if (field != SMAD_FIELD_PSI) throw ApiError{SMAD_EINVAL, "only psi can be set in the quasi model"};
+        upload(s, psi.p, n, in);
+        preloop();
+    }
+
+    double field_sum(const T* a, double* nonfinite)
+    {
+        Launch<T>::sum(s, a, (long)n, part.p, nparts, red.p);
+        LAUNCHCHK();
+        double h[3];
+        HIPCHK(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        *nonfinite += h[2];
+        return h[0];
+    }
+
+    int reduce(int what, double* out) override
+    {
+        if (!prev_valid) throw ApiError{SMAD_ESTATE, "reductions need at least one step since init"};
+        const double dx = 2.0 * M_PI / (p.q0 * p.nw);
+        double bad = 0.0;
+        if (what == SMAD_RED_L1) {
+            materialize();
+            Launch<T>::l1(s, psi.p, psi_prev.p, (long)n, part.p, nparts, red.p);
+            LAUNCHCHK();
+            double h[3];
+            HIPCHK(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            out[0] = h[0];
+            out[1] = h[1];
+            out[2] = h[0] / (p.dt * h[1]);
+            bad = h[2];
+        } else if (what == SMAD_RED_MASS) {
+            out[0] = field_sum(rho.p, &bad) * dx * dx * dx;
+        } else if (what == SMAD_RED_ENERGY) {
+            out[0] = field_sum(e.p, &bad) * dx * dx * dx;
+        } else {
+            throw ApiError{SMAD_EINVAL, "unknown reduction"};
+        }
+        return bad > 0 ? SMAD_ENONFINITE : SMAD_OK;
+    }
+
+    void forward_tape(int, int) override
+    {
+        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
+    }
+    void adjoint(const double*, double*, double*) override
+    {
+        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
+    }
+};
+
 bool length_ok(int n)
 {
     return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 384 || n == 512;
@@ -740,11 +1030,11 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
         g_init_error = "smad_init: precision must be 64 or 32";
         return SMAD_EINVAL;
     }
-    if (p.model != SMAD_MODEL_PLAIN && p.model != SMAD_MODEL_ADV) {
-        g_init_error = "smad_init: model not supported by this build";
+    if (p.model != SMAD_MODEL_PLAIN && p.model != SMAD_MODEL_ADV && p.model != SMAD_MODEL_QUASI) {
+        g_init_error = "smad_init: unknown model";
         return SMAD_EINVAL;
     }
-    if (p.model == SMAD_MODEL_ADV && !(p.nu > 0)) {
+    if (p.model != SMAD_MODEL_PLAIN && !(p.nu > 0)) {
         g_init_error = "smad_init: nu must be positive";
         return SMAD_EINVAL;
     }
@@ -761,9 +1051,12 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
         if (p.model == SMAD_MODEL_PLAIN) {
             if (f64) c->eng.reset(new PlainEngine<double>(p, c->stream));
             else c->eng.reset(new PlainEngine<float>(p, c->stream));
-        } else {
+        } else if (p.model == SMAD_MODEL_ADV) {
             if (f64) c->eng.reset(new AdvEngine<double>(p, c->stream));
             else c->eng.reset(new AdvEngine<float>(p, c->stream));
+        } else {
+            if (f64) c->eng.reset(new QuasiEngine<double>(p, c->stream));
+            else c->eng.reset(new QuasiEngine<float>(p, c->stream));
         }
         c->eng->init(psi0);
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -811,6 +1104,14 @@ int smad_adjoint(smad_ctx* c, const double* dl_dpsik, double* dl_dpsi0, double* 
         c->eng->adjoint(dl_dpsik, dl_dpsi0, dl_dparams);
         return SMAD_OK;
     });
+}
+
+int smad_adjoint_timing(const smad_ctx* c, double* fwd_tape_ms, double* adjoint_ms)
+{
+    if (!c || !c->eng || !fwd_tape_ms || !adjoint_ms) return SMAD_EINVAL;
+    *fwd_tape_ms = c->eng->last_fwd_ms;
+    *adjoint_ms = c->eng->last_adj_ms;
+    return SMAD_OK;
 }
 
 int smad_time_steps(smad_ctx* c, int nsteps, double* ms)
