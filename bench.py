@@ -106,7 +106,16 @@ def main():
                 "alg_GBps": round(alg_bytes[k.split(':')[0]] / (ms / c * 1e-3) / 1e9, 1)}
                for k, ms, c in prof]
     for kd in kernels:
-        log(f"kernel {kd}")
+        log(f"kernel fp{prec_main} {kd}")
+    kernels32 = None
+    if 32 in steppers and prec_main != 32:
+        F32 = n ** 3 * 4
+        ab32 = {"plain_z": 2 * F32, "axis_y": 2 * F32, "plain_x": 6 * F32}
+        kernels32 = [{"kernel": k, "avg_us": round(1e3 * ms / c, 2), "launches": c,
+                      "alg_GBps": round(ab32[k.split(':')[0]] / (ms / c * 1e-3) / 1e9, 1)}
+                     for k, ms, c in steppers[32].profile_steps(args.prof_steps)]
+        for kd in kernels32:
+            log(f"kernel fp32 {kd}")
     traffic = traffic_from_profiles(key, n, f"fp{prec_main}")
 
     # FP32-mixed error against FP64 after acc_steps, on the workload IC and on
@@ -129,10 +138,10 @@ def main():
     # adjoint / forward time ratio: K steps with checkpoints every ckpt,
     # (taped forward + adjoint) over the plain forward of the same K steps
     adj = None
-    if args.adj_steps > 0 and args.only_fp is None:
+    if args.adj_steps > 0:
         adj = {"K": args.adj_steps, "ckpt_every": args.adj_ckpt}
         grads = {}
-        for prec in (64, 32):
+        for prec in precs:
             p = smadft.Params.default(smadft.MODEL_PLAIN, n)
             p.precision = prec
             ic = smadft.ic_curved(p64, 1.328, 1e-3, seed=1)
@@ -151,11 +160,12 @@ def main():
                                 "adjoint_ms": adj_ms, "ratio": (tape_ms + adj_ms) / fwd_ms}
             log(f"adjoint fp{prec}: fwd {fwd_ms:.2f} ms, taped fwd {tape_ms:.2f} ms, "
                 f"adjoint {adj_ms:.2f} ms, ratio {(tape_ms + adj_ms) / fwd_ms:.3f}")
-        d64, p64g = grads[64]
-        d32, p32g = grads[32]
-        adj["fp32_grad_rel_err"] = {
-            "dpsi0": float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64)),
-            "dtheta": [float(abs(a - b) / abs(b)) for a, b in zip(p32g, p64g)]}
+        if 64 in grads and 32 in grads:
+            d64, p64g = grads[64]
+            d32, p32g = grads[32]
+            adj["fp32_grad_rel_err"] = {
+                "dpsi0": float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64)),
+                "dtheta": [float(abs(a - b) / abs(b)) for a, b in zip(p32g, p64g)]}
 
     # CPU baseline: the FP64 oracle on the host cores, bounded sample
     cpu = None
@@ -197,9 +207,10 @@ def main():
             "accuracy_after_steps": args.acc_steps,
             "accuracy": acc,
         },
-        "adjoint_fwd_ratio": adj["fp64"]["ratio"] if adj else None,
+        "adjoint_fwd_ratio": adj["fp64"]["ratio"] if adj and "fp64" in adj else None,
         "adjoint": adj,
         "kernels": kernels,
+        "kernels_fp32": kernels32,
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "alg_bytes_per_launch": alg_bytes[key],

@@ -17,12 +17,22 @@
 
 namespace smad {
 
-constexpr int WG = 256;
+constexpr int WG = 512;
 
 template <typename T>
 struct Lines {
-    static constexpr int NL = sizeof(T) == 8 ? 16 : 32;  // lines per workgroup
+    static constexpr int NL = sizeof(T) == 8 ? 16 : 32;  // lines per workgroup (strided axes)
     static constexpr int TPL = WG / NL;                     // threads per line
+};
+// z lines are contiguous, so their tile width is free of coalescing
+// constraints: 16 threads per line in both precisions (every radix stage's
+// butterflies stay busy for N = 256), 16 lines per 256-thread workgroup in
+// FP64 and 32 lines per 512-thread workgroup in FP32.
+template <typename T>
+struct LinesZ {
+    static constexpr int WGZ = sizeof(T) == 8 ? 256 : 512;
+    static constexpr int TPL = 16;
+    static constexpr int NL = WGZ / TPL;
 };
 
 struct Dims {
@@ -158,11 +168,11 @@ __device__ __forceinline__ void store_rows(const TileGeo& g, T* dst, const T* ld
 // z lines are contiguous: each wavefront loads, transforms and stores only
 // its own NL/4 lines, so the z passes need no workgroup barrier after the
 // twiddle table is in LDS.  f(p, c, gi) per 16-byte group of line c.
-template <typename T, int N, int NL, typename F>
+template <typename T, int N, int NL, int WGK, typename F>
 __device__ __forceinline__ void zlines_vec(const Dims& d, F&& f)
 {
     constexpr int W = V16<T>::W;
-    constexpr int LPW = NL / (WG / 64);  // lines per wavefront
+    constexpr int LPW = NL / (WGK / 64);  // lines per wavefront
     constexpr int TOT = LPW * N / W;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long nlines = (long)d.nx * d.ny;
@@ -200,18 +210,19 @@ struct OutMap {
 // Generic single-axis r2r pass: out = scale * R2R_kind(in) along `axis`.
 // in may equal out.
 template <typename T, int N>
-__global__ __launch_bounds__(WG) void k_axis(const T* in, T* out, Dims d, int axis, int kind,
-                                             T scale, Tw<T> tw)
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_axis_z(const T* in, T* out, Dims d, int kind, T scale,
+                                               Tw<T> tw)
 {
-    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
+    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = N + 2;
+    constexpr int WGZ = LinesZ<T>::WGZ;
     __shared__ T lds[NL * LS];
     __shared__ T ct[N + 1];
     load_cosq<T, N>(ct, tw);
     const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
-    if (axis == 2) {
+    {
         __syncthreads();
         InMap<T, N> im{kind};
-        zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+        zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
             V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
 #pragma unroll
             for (int t = 0; t < V16<T>::W; ++t) {
@@ -223,7 +234,7 @@ __global__ __launch_bounds__(WG) void k_axis(const T* in, T* out, Dims d, int ax
         lsync<true>();
         r2r_line<T, N, TPL, true>(lds + l * LS, u, kind, ct);
         OutMap<T, N> om{kind};
-        zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+        zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
             V16<T> v;
 #pragma unroll
             for (int t = 0; t < V16<T>::W; ++t) {
@@ -233,14 +244,24 @@ __global__ __launch_bounds__(WG) void k_axis(const T* in, T* out, Dims d, int ax
             }
             *reinterpret_cast<V16<T>*>(out + gi) = v;
         });
-    } else {
-        TileGeo g = tile_geo(d, axis, blockIdx.x, NL);
-        load_rows<T, N, NL, LS>(g, in, lds, InMap<T, N>{kind});
-        __syncthreads();
-        r2r_line<T, N, TPL, true>(lds + l * LS, u, kind, ct);
-        __syncthreads();
-        store_rows<T, N, NL, LS>(g, out, lds, OutMap<T, N>{kind}, scale);
     }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(WG) void k_axis(const T* in, T* out, Dims d, int axis, int kind,
+                                             T scale, Tw<T> tw)
+{
+    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
+    __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    TileGeo g = tile_geo(d, axis, blockIdx.x, NL);
+    load_rows<T, N, NL, LS>(g, in, lds, InMap<T, N>{kind});
+    __syncthreads();
+    r2r_line<T, N, TPL, true>(lds + l * LS, u, kind, ct);
+    __syncthreads();
+    store_rows<T, N, NL, LS>(g, out, lds, OutMap<T, N>{kind}, scale);
 }
 
 // ------------------------------------------------------------------------
@@ -254,17 +275,18 @@ struct PlainCoef {
 // optionally store psi, N = beta psi^3 - gamma psi^5, out = REDFT10_z(N).
 // in may equal out.  Each wavefront works on its own lines only.
 template <typename T, int N>
-__global__ __launch_bounds__(WG) void k_plain_z(const T* in, T* out, T* __restrict__ psi_out,
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_plain_z(const T* in, T* out, T* __restrict__ psi_out,
                                                 Dims d, PlainCoef cf, int from_real, Tw<T> tw)
 {
-    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
+    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = N + 2;
+    constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
     __shared__ T lds[NL * LS];
     __shared__ T ct[N + 1];
     load_cosq<T, N>(ct, tw);
     __syncthreads();
     const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
-    zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
 #pragma unroll
         for (int t = 0; t < W; ++t) lds[c * LS + (from_real ? mk_pos<N>(p + t) : p + t)] = v.v[t];
@@ -274,7 +296,7 @@ __global__ __launch_bounds__(WG) void k_plain_z(const T* in, T* out, T* __restri
     // LDS holds psi at Makhoul positions -- the input layout of the forward
     // transform -- so the nonlinearity is applied in place.
     if (psi_out) {
-        zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+        zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
             V16<T> v;
 #pragma unroll
             for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + mk_pos<N>(p + t)];
@@ -284,7 +306,7 @@ __global__ __launch_bounds__(WG) void k_plain_z(const T* in, T* out, T* __restri
     }
     const T beta = T(cf.beta), gamma = T(cf.gamma);
     {
-        constexpr int LPW = NL / (WG / 64);
+        constexpr int LPW = NL / (WGZ / 64);
         const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll 4
         for (int e = lane; e < LPW * N; e += 64) {
@@ -297,7 +319,7 @@ __global__ __launch_bounds__(WG) void k_plain_z(const T* in, T* out, T* __restri
     }
     lsync<true>();
     dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
-    zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v;
 #pragma unroll
         for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + p + t];
@@ -770,10 +792,10 @@ __global__ __launch_bounds__(WG) void k_sum_partial(const T* __restrict__ a, lon
 
 // Block-sum NV values and add them to acc[blockIdx.x*NV + q] (one slot per
 // workgroup, so repeated launches accumulate without atomics).
-template <int NV>
+template <int NV, int WGK = WG>
 __device__ __forceinline__ void block_accumulate(double (&v)[NV], double* __restrict__ acc)
 {
-    __shared__ double red[NV][WG / 64];
+    __shared__ double red[NV][WGK / 64];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         double x = v[q];
@@ -783,7 +805,7 @@ __device__ __forceinline__ void block_accumulate(double (&v)[NV], double* __rest
     __syncthreads();
     if (threadIdx.x < NV) {
         double t = 0.0;
-        for (int wv = 0; wv < WG / 64; ++wv) t += red[threadIdx.x][wv];
+        for (int wv = 0; wv < WGK / 64; ++wv) t += red[threadIdx.x][wv];
         acc[blockIdx.x * NV + threadIdx.x] += t;
     }
 }
@@ -864,17 +886,18 @@ __global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__
 // z pass: r = REDFT01_z(w); g = (3 beta psi^2 - 5 gamma psi^4) r with psi^n
 // from the tape; sums r psi^3, -r psi^5; w = REDFT10_z(g).
 template <typename T, int N>
-__global__ __launch_bounds__(WG) void k_adj_z(T* w, const T* __restrict__ psi_tape, Dims d,
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restrict__ psi_tape, Dims d,
                                               PlainCoef cf, double* __restrict__ acc, Tw<T> tw)
 {
-    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
+    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = N + 2;
+    constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
     __shared__ T lds[NL * LS];
     __shared__ T ct[N + 1];
     load_cosq<T, N>(ct, tw);
     __syncthreads();
     const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
-    zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v = *reinterpret_cast<const V16<T>*>(w + gi);
 #pragma unroll
         for (int t = 0; t < W; ++t) lds[c * LS + p + t] = v.v[t];
@@ -883,7 +906,7 @@ __global__ __launch_bounds__(WG) void k_adj_z(T* w, const T* __restrict__ psi_ta
     dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
     double sums[2] = {0.0, 0.0};
     const double b3 = 3.0 * cf.beta, g5 = 5.0 * cf.gamma;
-    zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> ps = *reinterpret_cast<const V16<T>*>(psi_tape + gi);
 #pragma unroll
         for (int t = 0; t < W; ++t) {
@@ -897,13 +920,13 @@ __global__ __launch_bounds__(WG) void k_adj_z(T* w, const T* __restrict__ psi_ta
     });
     lsync<true>();
     dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
-    zlines_vec<T, N, NL>(d, [&](int p, int c, long gi) {
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v;
 #pragma unroll
         for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + p + t];
         *reinterpret_cast<V16<T>*>(w + gi) = v;
     });
-    block_accumulate<2>(sums, acc);
+    block_accumulate<2, WGZ>(sums, acc);
 }
 
 // ------------------------------------------------------------------------

@@ -10,17 +10,20 @@ namespace smad {
 
 #define SMAD_LENGTHS(X) X(16) X(32) X(64) X(128) X(256) X(384) X(512)
 
-#define SMAD_DISPATCH(n, KERNEL, GRID, ...)                                             \
+#define SMAD_DISPATCH_B(n, KERNEL, GRID, BLOCK, ...)                                    \
     switch (n) {                                                                       \
-    case 16: hipLaunchKernelGGL((KERNEL<T, 16>), GRID, dim3(WG), 0, s, __VA_ARGS__); break;   \
-    case 32: hipLaunchKernelGGL((KERNEL<T, 32>), GRID, dim3(WG), 0, s, __VA_ARGS__); break;   \
-    case 64: hipLaunchKernelGGL((KERNEL<T, 64>), GRID, dim3(WG), 0, s, __VA_ARGS__); break;   \
-    case 128: hipLaunchKernelGGL((KERNEL<T, 128>), GRID, dim3(WG), 0, s, __VA_ARGS__); break; \
-    case 256: hipLaunchKernelGGL((KERNEL<T, 256>), GRID, dim3(WG), 0, s, __VA_ARGS__); break; \
-    case 384: hipLaunchKernelGGL((KERNEL<T, 384>), GRID, dim3(WG), 0, s, __VA_ARGS__); break; \
-    case 512: hipLaunchKernelGGL((KERNEL<T, 512>), GRID, dim3(WG), 0, s, __VA_ARGS__); break; \
+    case 16: hipLaunchKernelGGL((KERNEL<T, 16>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
+    case 32: hipLaunchKernelGGL((KERNEL<T, 32>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
+    case 64: hipLaunchKernelGGL((KERNEL<T, 64>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
+    case 128: hipLaunchKernelGGL((KERNEL<T, 128>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 256: hipLaunchKernelGGL((KERNEL<T, 256>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 384: hipLaunchKernelGGL((KERNEL<T, 384>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 512: hipLaunchKernelGGL((KERNEL<T, 512>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
     default: break;                                                                    \
     }
+#define SMAD_DISPATCH(n, KERNEL, GRID, ...) SMAD_DISPATCH_B(n, KERNEL, GRID, dim3(WG), __VA_ARGS__)
+#define SMAD_DISPATCH_Z(n, KERNEL, GRID, ...) \
+    SMAD_DISPATCH_B(n, KERNEL, GRID, dim3(LinesZ<T>::WGZ), __VA_ARGS__)
 
 inline int axis_len(const Dims& d, int axis) { return axis == 0 ? d.nx : axis == 1 ? d.ny : d.nz; }
 
@@ -28,16 +31,21 @@ template <typename T>
 void Launch<T>::axis(hipStream_t s, const T* in, T* out, Dims d, int axis, int kind, T scale,
                      const Tw<T>& tw)
 {
-    dim3 grid(num_tiles(d, axis, Lines<T>::NL));
-    SMAD_DISPATCH(axis_len(d, axis), k_axis, grid, in, out, d, axis, kind, scale, tw);
+    if (axis == 2) {
+        dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+        SMAD_DISPATCH_Z(d.nz, k_axis_z, grid, in, out, d, kind, scale, tw);
+    } else {
+        dim3 grid(num_tiles(d, axis, Lines<T>::NL));
+        SMAD_DISPATCH(axis_len(d, axis), k_axis, grid, in, out, d, axis, kind, scale, tw);
+    }
 }
 
 template <typename T>
 void Launch<T>::plain_z(hipStream_t s, const T* in, T* out, T* psi_out, Dims d,
                         const PlainCoef& cf, int from_real, const Tw<T>& tw)
 {
-    dim3 grid(num_tiles(d, 2, Lines<T>::NL));
-    SMAD_DISPATCH(d.nz, k_plain_z, grid, in, out, psi_out, d, cf, from_real, tw);
+    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+    SMAD_DISPATCH_Z(d.nz, k_plain_z, grid, in, out, psi_out, d, cf, from_real, tw);
 }
 
 template <typename T>
@@ -97,8 +105,8 @@ template <typename T>
 void Launch<T>::adj_z(hipStream_t s, T* w, const T* psi_tape, Dims d, const PlainCoef& cf,
                       double* acc, const Tw<T>& tw)
 {
-    dim3 grid(num_tiles(d, 2, Lines<T>::NL));
-    SMAD_DISPATCH(d.nz, k_adj_z, grid, w, psi_tape, d, cf, acc, tw);
+    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+    SMAD_DISPATCH_Z(d.nz, k_adj_z, grid, w, psi_tape, d, cf, acc, tw);
 }
 
 template <typename T>

@@ -358,7 +358,7 @@ struct PlainEngine : EngineBase {
             end_nh.alloc(n);
             lbuf.alloc(n);
             acc_x.alloc(2 * (size_t)num_tiles(d, 0, Lines<T>::NL));
-            acc_z.alloc(2 * (size_t)num_tiles(d, 2, Lines<T>::NL));
+            acc_z.alloc(2 * (size_t)num_tiles(d, 2, LinesZ<T>::NL));
             gsum.alloc(4);
         }
         EvPair ev;
