@@ -94,13 +94,13 @@ struct alignas(16) V16 {
 // Iterate a strided tile (axis x or y) in 16-byte groups: f(p, c0, gi) for
 // columns c0 .. c0+W-1 of row p.  A wavefront covers 64 groups, i.e. whole
 // 128 B row segments.  Requires a full tile (ncol == NL).
-template <typename T, int N, int NL, typename F>
+template <typename T, int N, int NL, int UNROLL = 8, typename F>
 __device__ __forceinline__ void rows_vec(const TileGeo& g, F&& f)
 {
     constexpr int W = V16<T>::W;
     constexpr int GPR = NL / W;
     constexpr int TOT = N * GPR;
-#pragma unroll 8
+#pragma unroll UNROLL
     for (int e = threadIdx.x; e < TOT; e += WG) {
         int p = e / GPR, c0 = (e - p * GPR) * W;
         f(p, c0, g.base + (long)p * g.stride + c0);
@@ -342,9 +342,9 @@ __device__ __forceinline__ double cn_ab2(const PlainCoef& cf, double mq2, double
 // (in place: nhat holds Nq_past on entry, Nq on exit); w = REDFT01_x(psihat').
 // psihat is the pre-scaled spectral field.
 template <typename T, int N>
-__global__ __launch_bounds__(WG) void k_plain_x(T* __restrict__ w, T* __restrict__ psihat,
-                                                T* __restrict__ nhat, T* __restrict__ s_tape,
-                                                Dims d, PlainCoef cf, int inverse, Tw<T> tw)
+__global__ __launch_bounds__(WG) void k_plain_x(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
+                                                T* __restrict__ nhat, Dims d, PlainCoef cf,
+                                                int inverse, Tw<T> tw)
 {
     constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
     constexpr int W = V16<T>::W;
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(WG) void k_plain_x(T* __restrict__ w, T* __restrict
     struct Nat {
         __device__ T operator()(int p, int& q) const { q = p; return T(1); }
     };
-    load_rows<T, N, NL, LS>(g, w, lds, Mk{});
+    load_rows<T, N, NL, LS>(g, a_in, lds, Mk{});
     __syncthreads();
     dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
     __syncthreads();
@@ -370,9 +370,9 @@ __global__ __launch_bounds__(WG) void k_plain_x(T* __restrict__ w, T* __restrict
     const double qy = j * cf.qfy;
     if (g.ncol == NL) {
         rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
-            V16<T> ph = *reinterpret_cast<const V16<T>*>(psihat + gi);
+            V16<T> ph = *reinterpret_cast<const V16<T>*>(ph_in + gi);
             V16<T> np = *reinterpret_cast<const V16<T>*>(nhat + gi);
-            V16<T> nq, out, st;
+            V16<T> nq, out;
             const double qx = p * cf.qfx;
 #pragma unroll
             for (int t = 0; t < W; ++t) {
@@ -382,12 +382,10 @@ __global__ __launch_bounds__(WG) void k_plain_x(T* __restrict__ w, T* __restrict
                 double r = cn_ab2(cf, mq2, (double)ph.v[t], (double)nqt, (double)np.v[t]);
                 nq.v[t] = nqt;
                 out.v[t] = T(r);
-                st.v[t] = T((double)ph.v[t] + (double)out.v[t]);
                 lds[(c0 + t) * LS + p] = out.v[t];
             }
-            *reinterpret_cast<V16<T>*>(psihat + gi) = out;
+            *reinterpret_cast<V16<T>*>(ph_out + gi) = out;
             *reinterpret_cast<V16<T>*>(nhat + gi) = nq;
-            if (s_tape) *reinterpret_cast<V16<T>*>(s_tape + gi) = st;
         });
     } else {
         tile_for<N, NL>(g, 0, [&](int p, int c, long gi, bool ok) {
@@ -395,10 +393,8 @@ __global__ __launch_bounds__(WG) void k_plain_x(T* __restrict__ w, T* __restrict
                 double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
                 double mq2 = qx * qx + qy * qy + qz * qz;
                 T nqt = lds[c * LS + p];
-                double ph = (double)psihat[gi];
-                double r = cn_ab2(cf, mq2, ph, (double)nqt, (double)nhat[gi]);
-                if (s_tape) s_tape[gi] = T(ph + (double)T(r));
-                psihat[gi] = T(r);
+                double r = cn_ab2(cf, mq2, (double)ph_in[gi], (double)nqt, (double)nhat[gi]);
+                ph_out[gi] = T(r);
                 nhat[gi] = nqt;
                 lds[c * LS + p] = T(r);
             } else {
@@ -410,7 +406,7 @@ __global__ __launch_bounds__(WG) void k_plain_x(T* __restrict__ w, T* __restrict
     __syncthreads();
     dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
     __syncthreads();
-    store_rows<T, N, NL, LS>(g, w, lds, Mk{}, T(1));
+    store_rows<T, N, NL, LS>(g, w_out, lds, Mk{}, T(1));
 }
 
 // ------------------------------------------------------------------------
@@ -811,10 +807,15 @@ __device__ __forceinline__ void block_accumulate(double (&v)[NV], double* __rest
 }
 
 // x pass.  mode 0: backward step (cm = 3), mode 1: backward step n = 0
-// (cm = 2), mode 2: final, w = s REDFT01_x(a lbuf + REDFT10_x(G)).
+// (cm = 2); both read ph = psihat^{n+1}.  mode 2: final pass with
+// ph = psihat^0: w = s REDFT01_x(a lbuf + REDFT10_x(w)).
+// The L-gradient is grouped by psihat index m:
+//   dJ/dL = sum_m w dt/2/C2 psihat^m (lt^{m+1} + lt^m),  lt^{K+1} = 0,
+// the lt^m term absent at m = 0; the backward step for n handles m = n+1
+// with lt^{n+1} (new) and lt^{n+2} (lbuf on entry).
 template <typename T, int N>
 __global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__ lbuf,
-                                              const T* __restrict__ s_tape, Dims d, PlainCoef cf,
+                                              const T* __restrict__ ph, Dims d, PlainCoef cf,
                                               int mode, double* __restrict__ acc, Tw<T> tw)
 {
     constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
@@ -836,47 +837,46 @@ __global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__
     const double qy = j * cf.qfy;
     const double cm = mode == 1 ? 2.0 : 3.0;
     double sums[2] = {0.0, 0.0};
-    auto point = [&](int p, int c, T& lb, T st) {
+    auto point = [&](int p, int c, T& lb, T phv) {
         double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
         double t = qx * qx + qy * qy + qz * qz - cf.q0sq;
         double al = cf.eps - cf.alpha * t * t;
-        double c2 = 1.0 - cf.dtd2 * al;
-        double a = (1.0 + cf.dtd2 * al) / c2;
+        double h = cf.dtd2 / (1.0 - cf.dtd2 * al);  // dt/2 / C2
+        double a = (1.0 + cf.dtd2 * al) * (h / cf.dtd2);
         double lold = (double)lb;
         double lnew = a * lold + (double)lds[c * LS + p];
+        double wgt = (p == 0 ? 0.5 : 1.0) * (j == 0 ? 0.5 : 1.0) * (k0 + c == 0 ? 0.5 : 1.0);
+        double term = wgt * h * (double)phv * (mode == 2 ? lold : lnew + lold);
+        sums[0] += term;
+        sums[1] -= t * t * term;
         if (mode == 2) {
             lds[c * LS + p] = T(cf.scale * lnew);
             return;
         }
-        double cc = cf.scale * cf.dtd2 / c2;
-        double wgt = (p == 0 ? 0.5 : 1.0) * (j == 0 ? 0.5 : 1.0) * (k0 + c == 0 ? 0.5 : 1.0);
-        double term = wgt * lnew * cf.dtd2 / c2 * (double)st;
-        sums[0] += term;
-        sums[1] -= t * t * term;
+        double cc = cf.scale * h;
         lb = T(lnew);
-        lds[c * LS + p] = T(cm * cc * lnew - cc * lold);
+        lds[c * LS + p] = T(cc * (cm * lnew - lold));
     };
     if (g.ncol == NL) {
-        rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
+        rows_vec<T, N, NL, 1>(g, [&](int p, int c0, long gi) {
             V16<T> lb = *reinterpret_cast<const V16<T>*>(lbuf + gi);
-            V16<T> st;
-            if (mode != 2) st = *reinterpret_cast<const V16<T>*>(s_tape + gi);
+            V16<T> pv = *reinterpret_cast<const V16<T>*>(ph + gi);
 #pragma unroll
-            for (int t = 0; t < V16<T>::W; ++t) point(p, c0 + t, lb.v[t], mode != 2 ? st.v[t] : T(0));
+            for (int t = 0; t < V16<T>::W; ++t) point(p, c0 + t, lb.v[t], pv.v[t]);
             if (mode != 2) *reinterpret_cast<V16<T>*>(lbuf + gi) = lb;
         });
     } else {
         tile_for<N, NL>(g, 0, [&](int p, int c, long gi, bool ok) {
             if (ok) {
                 T lb = lbuf[gi];
-                point(p, c, lb, mode != 2 ? s_tape[gi] : T(0));
+                point(p, c, lb, ph[gi]);
                 if (mode != 2) lbuf[gi] = lb;
             } else {
                 lds[c * LS + p] = T(0);
             }
         });
     }
-    if (mode != 2) block_accumulate<2>(sums, acc);
+    block_accumulate<2>(sums, acc);
     __syncthreads();
     dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
     __syncthreads();
@@ -1000,8 +1000,8 @@ struct Launch {
                      const Tw<T>& tw);
     static void plain_z(hipStream_t s, const T* in, T* out, T* psi_out, Dims d,
                         const PlainCoef& cf, int from_real, const Tw<T>& tw);
-    static void plain_x(hipStream_t s, T* w, T* psihat, T* nhat, T* s_tape, Dims d,
-                        const PlainCoef& cf, const Tw<T>& tw, int inverse = 1);
+    static void plain_x(hipStream_t s, const T* a_in, T* w_out, const T* ph_in, T* ph_out,
+                        T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse = 1);
     static void adv_spec(hipStream_t s, const T* ph, T* gx, T* gy, T* gz, T* sp, Dims d,
                          const AdvCoef& ac);
     static void adv_force(hipStream_t s, const T* psi, const T* sp2, const T* gx, const T* gy,
@@ -1010,7 +1010,7 @@ struct Launch {
                            T* uz, Dims d, const AdvCoef& ac);
     static void adv_nl(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
                        const T* gx, const T* gy, const T* gz, T* nr, Dims d, const AdvCoef& ac);
-    static void adj_x(hipStream_t s, T* w, T* lbuf, const T* s_tape, Dims d, const PlainCoef& cf,
+    static void adj_x(hipStream_t s, T* w, T* lbuf, const T* ph, Dims d, const PlainCoef& cf,
                       int mode, double* acc, const Tw<T>& tw);
     static void adj_z(hipStream_t s, T* w, const T* psi_tape, Dims d, const PlainCoef& cf,
                       double* acc, const Tw<T>& tw);

@@ -49,11 +49,11 @@ void Launch<T>::plain_z(hipStream_t s, const T* in, T* out, T* psi_out, Dims d,
 }
 
 template <typename T>
-void Launch<T>::plain_x(hipStream_t s, T* w, T* psihat, T* nhat, T* s_tape, Dims d,
-                        const PlainCoef& cf, const Tw<T>& tw, int inverse)
+void Launch<T>::plain_x(hipStream_t s, const T* a_in, T* w_out, const T* ph_in, T* ph_out,
+                        T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse)
 {
     dim3 grid(num_tiles(d, 0, Lines<T>::NL));
-    SMAD_DISPATCH(d.nx, k_plain_x, grid, w, psihat, nhat, s_tape, d, cf, inverse, tw);
+    SMAD_DISPATCH(d.nx, k_plain_x, grid, a_in, w_out, ph_in, ph_out, nhat, d, cf, inverse, tw);
 }
 
 inline dim3 flat_grid(long n)
@@ -94,11 +94,11 @@ void Launch<T>::adv_nl(hipStream_t s, const T* psi, const T* ux, const T* uy, co
 }
 
 template <typename T>
-void Launch<T>::adj_x(hipStream_t s, T* w, T* lbuf, const T* s_tape, Dims d, const PlainCoef& cf,
+void Launch<T>::adj_x(hipStream_t s, T* w, T* lbuf, const T* ph, Dims d, const PlainCoef& cf,
                       int mode, double* acc, const Tw<T>& tw)
 {
     dim3 grid(num_tiles(d, 0, Lines<T>::NL));
-    SMAD_DISPATCH(d.nx, k_adj_x, grid, w, lbuf, s_tape, d, cf, mode, acc, tw);
+    SMAD_DISPATCH(d.nx, k_adj_x, grid, w, lbuf, ph, d, cf, mode, acc, tw);
 }
 
 template <typename T>

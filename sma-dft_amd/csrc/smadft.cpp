@@ -270,15 +270,16 @@ struct PlainEngine : EngineBase {
         return (c >= 0 && c < 4) ? names[c] : nullptr;
     }
 
-    // One loop-body iteration.  psi_out receives psi^n (the real field this
-    // step starts from), s_tape receives psihat^n + psihat^{n+1}.
-    void step_one(T* psi_out, T* s_tape, Prof* prof)
+    // One loop-body iteration: psihat is read from ph_in and written to
+    // ph_out (they alias in ordinary stepping); psi_out, if set, receives
+    // psi^n, the real field the step starts from.
+    void step_one(const T* ph_in, T* ph_out, T* psi_out, Prof* prof)
     {
         Launch<T>::plain_z(s, w.p, w.p, psi_out, d, cf, 0, twz.tw);
         if (prof) prof->mark(0);
         Launch<T>::axis(s, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
         if (prof) prof->mark(1);
-        Launch<T>::plain_x(s, w.p, psihat.p, nhat.p, s_tape, d, cf, twx.tw);
+        Launch<T>::plain_x(s, w.p, w.p, ph_in, ph_out, nhat.p, d, cf, twx.tw, 1);
         if (prof) prof->mark(2);
         Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
         if (prof) prof->mark(3);
@@ -286,7 +287,8 @@ struct PlainEngine : EngineBase {
 
     void step(int nsteps, Prof* prof) override
     {
-        for (int it = 0; it < nsteps; ++it) step_one(it == nsteps - 1 ? psi_prev.p : nullptr, nullptr, prof);
+        for (int it = 0; it < nsteps; ++it)
+            step_one(psihat.p, psihat.p, it == nsteps - 1 ? psi_prev.p : nullptr, prof);
         LAUNCHCHK();
         if (nsteps > 0) {
             psi_valid = false;
@@ -297,11 +299,14 @@ struct PlainEngine : EngineBase {
 
     // ------------------------------------------------------------ adjoint --
     // Checkpointed forward: (psihat, nhat) saved at the start of every
-    // segment of `ck` steps; the last segment also records its tape (psi^n
-    // and psihat^n + psihat^{n+1} per step), so it is not recomputed.
+    // segment of `ck` steps.  A recorded segment writes psi^m (m-th step of
+    // the segment, from its z pass) to tape slot m and steps psihat through
+    // rotating slots (slot 0 is the state buffer, slot m+1 receives
+    // psihat^{m+1}), so psihat costs no extra traffic.  The last segment is
+    // recorded by the forward run itself and is not recomputed.
     int tape_K = 0, tape_ck = 0, tape_nseg = 0;
-    std::vector<std::unique_ptr<DevBuf<T>>> ck_ph, ck_nh, tp_psi, tp_s;
-    DevBuf<T> end_ph, end_nh, lbuf;
+    std::vector<std::unique_ptr<DevBuf<T>>> ck_ph, ck_nh, ck_w, tp_psi, sl_ph;
+    DevBuf<T> end_ph, end_nh, lbuf, gbuf;
     DevBuf<double> acc_x, acc_z, gsum;
     bool fresh = true;
 
@@ -316,11 +321,29 @@ struct PlainEngine : EngineBase {
         Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
     }
 
-    void run_segment(int nsteps, bool record)
+    T* slot_ph(int m) { return m == 0 ? psihat.p : sl_ph[m - 1]->p; }
+
+    // last_z_only: the segment's final step only needs its z pass (psi^{ns-1}
+    // for the tape); psihat^{ns} is the next segment's checkpoint.
+    void run_segment(int ns, bool record, bool last_z_only = false)
     {
-        for (int m = 0; m < nsteps; ++m)
-            step_one(record ? tp_psi[m]->p : nullptr, record ? tp_s[m]->p : nullptr, nullptr);
+        for (int m = 0; m < ns; ++m) {
+            if (!record) step_one(psihat.p, psihat.p, nullptr, nullptr);
+            else if (last_z_only && m == ns - 1)
+                Launch<T>::plain_z(s, w.p, w.p, tp_psi[m]->p, d, cf, 0, twz.tw);
+            else step_one(slot_ph(m), slot_ph(m + 1), tp_psi[m]->p, nullptr);
+        }
         LAUNCHCHK();
+    }
+
+    static void ensure(std::vector<std::unique_ptr<DevBuf<T>>>& v, int count, size_t n)
+    {
+        if ((int)v.size() < count) v.resize(count);
+        for (int i = 0; i < count; ++i)
+            if (!v[i]) {
+                v[i].reset(new DevBuf<T>);
+                v[i]->alloc(n);
+            }
     }
 
     void forward_tape(int K, int ck) override
@@ -329,34 +352,13 @@ struct PlainEngine : EngineBase {
         if (K < 1 || ck < 1) throw ApiError{SMAD_EINVAL, "K and ckpt_every must be >= 1"};
         ck = std::min(ck, K);
         int nseg = (K + ck - 1) / ck;
-        if ((int)ck_ph.size() < nseg) {
-            ck_ph.resize(nseg);
-            ck_nh.resize(nseg);
-        }
-        for (int g = 0; g < nseg; ++g) {
-            if (!ck_ph[g]) {
-                ck_ph[g].reset(new DevBuf<T>);
-                ck_ph[g]->alloc(n);
-                ck_nh[g].reset(new DevBuf<T>);
-                ck_nh[g]->alloc(n);
-            }
-        }
-        if ((int)tp_psi.size() < ck) {
-            tp_psi.resize(ck);
-            tp_s.resize(ck);
-        }
-        for (int m = 0; m < ck; ++m) {
-            if (!tp_psi[m]) {
-                tp_psi[m].reset(new DevBuf<T>);
-                tp_psi[m]->alloc(n);
-                tp_s[m].reset(new DevBuf<T>);
-                tp_s[m]->alloc(n);
-            }
-        }
+        ensure(ck_ph, nseg, n);
+        ensure(ck_nh, nseg, n);
+        ensure(ck_w, nseg, n);
+        ensure(tp_psi, ck, n);
+        ensure(sl_ph, ck, n);
         if (!end_ph.p) {
-            end_ph.alloc(n);
-            end_nh.alloc(n);
-            lbuf.alloc(n);
+            for (DevBuf<T>* b : {&end_ph, &end_nh, &lbuf, &gbuf}) b->alloc(n);
             acc_x.alloc(2 * (size_t)num_tiles(d, 0, Lines<T>::NL));
             acc_z.alloc(2 * (size_t)num_tiles(d, 2, LinesZ<T>::NL));
             gsum.alloc(4);
@@ -366,13 +368,17 @@ struct PlainEngine : EngineBase {
         for (int g = 0; g < nseg; ++g) {
             copy_dd(ck_ph[g]->p, psihat.p);
             copy_dd(ck_nh[g]->p, nhat.p);
+            if (g != nseg - 1) copy_dd(ck_w[g]->p, w.p);
             int ns = std::min(ck, K - g * ck);
             run_segment(ns, g == nseg - 1);
+            if (g == nseg - 1) {
+                copy_dd(end_ph.p, slot_ph(ns));
+                copy_dd(end_nh.p, nhat.p);
+            }
         }
-        copy_dd(end_ph.p, psihat.p);
-        copy_dd(end_nh.p, nhat.p);
         HIPCHK(hipEventRecord(ev.b, s));
         last_fwd_ms = ev.ms();
+        copy_dd(psihat.p, end_ph.p);  // w already holds step K
         tape_K = K;
         tape_ck = ck;
         tape_nseg = nseg;
@@ -385,40 +391,39 @@ struct PlainEngine : EngineBase {
     {
         if (tape_K < 1) throw ApiError{SMAD_ESTATE, "adjoint needs a preceding forward_tape"};
         const int K = tape_K, ck = tape_ck, nseg = tape_nseg;
-        // lt^K = REDFT10^3(g): z and y here, x inside the first adjoint x pass
         upload(s, psi.p, n, g_psik);
         EvPair ev;
         HIPCHK(hipEventRecord(ev.a, s));
-        Launch<T>::axis(s, psi.p, w.p, d, 2, REDFT10, T(1), twz.tw);
-        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
+        // lt^K = REDFT10^3(g): z and y here, x inside the first adjoint x pass
+        Launch<T>::axis(s, psi.p, gbuf.p, d, 2, REDFT10, T(1), twz.tw);
+        Launch<T>::axis(s, gbuf.p, gbuf.p, d, 1, REDFT10, T(1), twy.tw);
         HIPCHK(hipMemsetAsync(lbuf.p, 0, n * sizeof(T), s));
         HIPCHK(hipMemsetAsync(acc_x.p, 0, acc_x.n * sizeof(double), s));
         HIPCHK(hipMemsetAsync(acc_z.p, 0, acc_z.n * sizeof(double), s));
-        // w holds the partial transform of the cotangent; recomputation
-        // below uses w too, so park it in psi (unused during the adjoint)
         for (int g = nseg - 1; g >= 0; --g) {
             int n0 = g * ck, ns = std::min(ck, K - n0);
-            if (g != nseg - 1) {
-                copy_dd(psi.p, w.p);
-                copy_dd(psihat.p, ck_ph[g]->p);
+            // slot 0 is the segment's starting psihat
+            copy_dd(psihat.p, ck_ph[g]->p);
+            const bool last = g == nseg - 1;
+            if (!last) {
                 copy_dd(nhat.p, ck_nh[g]->p);
-                restore_w();
-                run_segment(ns, true);
-                copy_dd(w.p, psi.p);
+                copy_dd(w.p, ck_w[g]->p);
+                run_segment(ns, true, true);
             }
             for (int m = ns - 1; m >= 0; --m) {
                 int nn = n0 + m;
-                Launch<T>::adj_x(s, w.p, lbuf.p, tp_s[m]->p, d, cf, nn >= 1 ? 0 : 1, acc_x.p, twx.tw);
-                Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
-                Launch<T>::adj_z(s, w.p, tp_psi[m]->p, d, cf, acc_z.p, twz.tw);
-                Launch<T>::axis(s, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
+                const T* ph_next = (!last && m == ns - 1) ? ck_ph[g + 1]->p : slot_ph(m + 1);
+                Launch<T>::adj_x(s, gbuf.p, lbuf.p, ph_next, d, cf, nn >= 1 ? 0 : 1, acc_x.p, twx.tw);
+                Launch<T>::axis(s, gbuf.p, gbuf.p, d, 1, REDFT01, T(1), twy.tw);
+                Launch<T>::adj_z(s, gbuf.p, tp_psi[m]->p, d, cf, acc_z.p, twz.tw);
+                Launch<T>::axis(s, gbuf.p, gbuf.p, d, 1, REDFT10, T(1), twy.tw);
             }
             LAUNCHCHK();
         }
-        // dJ/dpsi0 = s REDFT01^3(a lt^1 + REDFT10_x(G))
-        Launch<T>::adj_x(s, w.p, lbuf.p, nullptr, d, cf, 2, acc_x.p, twx.tw);
-        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
-        Launch<T>::axis(s, w.p, psi.p, d, 2, REDFT01, T(1), twz.tw);
+        // dJ/dpsi0 = s REDFT01^3(a lt^1 + REDFT10_x(G)); psihat holds psihat^0
+        Launch<T>::adj_x(s, gbuf.p, lbuf.p, psihat.p, d, cf, 2, acc_x.p, twx.tw);
+        Launch<T>::axis(s, gbuf.p, gbuf.p, d, 1, REDFT01, T(1), twy.tw);
+        Launch<T>::axis(s, gbuf.p, psi.p, d, 2, REDFT01, T(1), twz.tw);
         Launch<T>::fold(s, acc_x.p, (int)(acc_x.n / 2), 2, gsum.p);
         Launch<T>::fold(s, acc_z.p, (int)(acc_z.n / 2), 2, gsum.p + 2);
         LAUNCHCHK();
@@ -606,7 +611,7 @@ struct AdvEngine : EngineBase {
         Launch<T>::axis(s, nr.p, nr.p, d, 2, REDFT10, T(1), twz.tw);
         Launch<T>::axis(s, nr.p, nr.p, d, 1, REDFT10, T(1), twy.tw);
         mk(0);
-        Launch<T>::plain_x(s, nr.p, ph.p, nhat.p, nullptr, d, ac.pc, twx.tw, 0);
+        Launch<T>::plain_x(s, nr.p, nr.p, ph.p, ph.p, nhat.p, d, ac.pc, twx.tw, 0);
         mk(5);
     }
 
