@@ -132,13 +132,13 @@ __device__ __forceinline__ void load_rows(const TileGeo& g, const T* src, T* lds
             int q;
             T sg = pos(p, q);
 #pragma unroll
-            for (int t = 0; t < V16<T>::W; ++t) lds[(c0 + t) * LS + q] = sg * v.v[t];
+            for (int t = 0; t < V16<T>::W; ++t) lds[(c0 + t) * LS + PR(q)] = sg * v.v[t];
         });
     } else {
         tile_for<N, NL>(g, 1, [&](int p, int c, long gi, bool ok) {
             int q;
             T sg = pos(p, q);
-            lds[c * LS + q] = ok ? sg * src[gi] : T(0);
+            lds[c * LS + PR(q)] = ok ? sg * src[gi] : T(0);
         });
     }
 }
@@ -153,14 +153,14 @@ __device__ __forceinline__ void store_rows(const TileGeo& g, T* dst, const T* ld
             T sg = scale * pos(p, q);
             V16<T> v;
 #pragma unroll
-            for (int t = 0; t < V16<T>::W; ++t) v.v[t] = sg * lds[(c0 + t) * LS + q];
+            for (int t = 0; t < V16<T>::W; ++t) v.v[t] = sg * lds[(c0 + t) * LS + PR(q)];
             *reinterpret_cast<V16<T>*>(dst + gi) = v;
         });
     } else {
         tile_for<N, NL>(g, 1, [&](int p, int c, long gi, bool ok) {
             int q;
             T sg = scale * pos(p, q);
-            if (ok) dst[gi] = sg * lds[c * LS + q];
+            if (ok) dst[gi] = sg * lds[c * LS + PR(q)];
         });
     }
 }
@@ -213,7 +213,7 @@ template <typename T, int N>
 __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_axis_z(const T* in, T* out, Dims d, int kind, T scale,
                                                Tw<T> tw)
 {
-    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = N + 2;
+    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     __shared__ T lds[NL * LS];
     __shared__ T ct[N + 1];
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_axis_z(const T* in, T* out, 
             for (int t = 0; t < V16<T>::W; ++t) {
                 int q;
                 T sg = im(p + t, q);
-                lds[c * LS + q] = sg * v.v[t];
+                lds[c * LS + PR(q)] = sg * v.v[t];
             }
         });
         lsync<true>();
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_axis_z(const T* in, T* out, 
             for (int t = 0; t < V16<T>::W; ++t) {
                 int q;
                 T sg = scale * om(p + t, q);
-                v.v[t] = sg * lds[c * LS + q];
+                v.v[t] = sg * lds[c * LS + PR(q)];
             }
             *reinterpret_cast<V16<T>*>(out + gi) = v;
         });
@@ -251,7 +251,7 @@ template <typename T, int N>
 __global__ __launch_bounds__(WG) void k_axis(const T* in, T* out, Dims d, int axis, int kind,
                                              T scale, Tw<T> tw)
 {
-    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
+    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
     __shared__ T lds[NL * LS];
     __shared__ T ct[N + 1];
     load_cosq<T, N>(ct, tw);
@@ -278,7 +278,7 @@ template <typename T, int N>
 __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_plain_z(const T* in, T* out, T* __restrict__ psi_out,
                                                 Dims d, PlainCoef cf, int from_real, Tw<T> tw)
 {
-    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = N + 2;
+    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
     __shared__ T lds[NL * LS];
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_plain_z(const T* in, T* out,
     zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
 #pragma unroll
-        for (int t = 0; t < W; ++t) lds[c * LS + (from_real ? mk_pos<N>(p + t) : p + t)] = v.v[t];
+        for (int t = 0; t < W; ++t) lds[c * LS + PR((from_real ? mk_pos<N>(p + t) : p + t))] = v.v[t];
     });
     lsync<true>();
     if (!from_real) dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_plain_z(const T* in, T* out,
         zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
             V16<T> v;
 #pragma unroll
-            for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + mk_pos<N>(p + t)];
+            for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + PR(mk_pos<N>(p + t))];
             *reinterpret_cast<V16<T>*>(psi_out + gi) = v;
         });
         lsync<true>();
@@ -311,10 +311,10 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_plain_z(const T* in, T* out,
 #pragma unroll 4
         for (int e = lane; e < LPW * N; e += 64) {
             int c = wv * LPW + e / N, q = e % N;
-            T ps = lds[c * LS + q];
+            T ps = lds[c * LS + PR(q)];
             T p2 = ps * ps;
             T p3 = p2 * ps;
-            lds[c * LS + q] = beta * p3 - gamma * (p3 * p2);
+            lds[c * LS + PR(q)] = beta * p3 - gamma * (p3 * p2);
         }
     }
     lsync<true>();
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_plain_z(const T* in, T* out,
     zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v;
 #pragma unroll
-        for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + p + t];
+        for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + PR(p + t)];
         *reinterpret_cast<V16<T>*>(out + gi) = v;
     });
 }
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(WG) void k_plain_x(const T* a_in, T* w_out, const T
                                                 T* __restrict__ nhat, Dims d, PlainCoef cf,
                                                 int inverse, Tw<T> tw)
 {
-    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
+    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
     constexpr int W = V16<T>::W;
     __shared__ T lds[NL * LS];
     __shared__ T ct[N + 1];
@@ -378,11 +378,11 @@ __global__ __launch_bounds__(WG) void k_plain_x(const T* a_in, T* w_out, const T
             for (int t = 0; t < W; ++t) {
                 double qz = (k0 + c0 + t) * cf.qfz;
                 double mq2 = qx * qx + qy * qy + qz * qz;
-                T nqt = lds[(c0 + t) * LS + p];
+                T nqt = lds[(c0 + t) * LS + PR(p)];
                 double r = cn_ab2(cf, mq2, (double)ph.v[t], (double)nqt, (double)np.v[t]);
                 nq.v[t] = nqt;
                 out.v[t] = T(r);
-                lds[(c0 + t) * LS + p] = out.v[t];
+                lds[(c0 + t) * LS + PR(p)] = out.v[t];
             }
             *reinterpret_cast<V16<T>*>(ph_out + gi) = out;
             *reinterpret_cast<V16<T>*>(nhat + gi) = nq;
@@ -392,13 +392,13 @@ __global__ __launch_bounds__(WG) void k_plain_x(const T* a_in, T* w_out, const T
             if (ok) {
                 double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
                 double mq2 = qx * qx + qy * qy + qz * qz;
-                T nqt = lds[c * LS + p];
+                T nqt = lds[c * LS + PR(p)];
                 double r = cn_ab2(cf, mq2, (double)ph_in[gi], (double)nqt, (double)nhat[gi]);
                 ph_out[gi] = T(r);
                 nhat[gi] = nqt;
-                lds[c * LS + p] = T(r);
+                lds[c * LS + PR(p)] = T(r);
             } else {
-                lds[c * LS + p] = T(0);
+                lds[c * LS + PR(p)] = T(0);
             }
         });
     }
@@ -818,7 +818,7 @@ __global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__
                                               const T* __restrict__ ph, Dims d, PlainCoef cf,
                                               int mode, double* __restrict__ acc, Tw<T> tw)
 {
-    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = N + 2;
+    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
     __shared__ T lds[NL * LS];
     __shared__ T ct[N + 1];
     load_cosq<T, N>(ct, tw);
@@ -844,18 +844,18 @@ __global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__
         double h = cf.dtd2 / (1.0 - cf.dtd2 * al);  // dt/2 / C2
         double a = (1.0 + cf.dtd2 * al) * (h / cf.dtd2);
         double lold = (double)lb;
-        double lnew = a * lold + (double)lds[c * LS + p];
+        double lnew = a * lold + (double)lds[c * LS + PR(p)];
         double wgt = (p == 0 ? 0.5 : 1.0) * (j == 0 ? 0.5 : 1.0) * (k0 + c == 0 ? 0.5 : 1.0);
         double term = wgt * h * (double)phv * (mode == 2 ? lold : lnew + lold);
         sums[0] += term;
         sums[1] -= t * t * term;
         if (mode == 2) {
-            lds[c * LS + p] = T(cf.scale * lnew);
+            lds[c * LS + PR(p)] = T(cf.scale * lnew);
             return;
         }
         double cc = cf.scale * h;
         lb = T(lnew);
-        lds[c * LS + p] = T(cc * (cm * lnew - lold));
+        lds[c * LS + PR(p)] = T(cc * (cm * lnew - lold));
     };
     if (g.ncol == NL) {
         rows_vec<T, N, NL, 1>(g, [&](int p, int c0, long gi) {
@@ -872,7 +872,7 @@ __global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__
                 point(p, c, lb, ph[gi]);
                 if (mode != 2) lbuf[gi] = lb;
             } else {
-                lds[c * LS + p] = T(0);
+                lds[c * LS + PR(p)] = T(0);
             }
         });
     }
@@ -889,7 +889,7 @@ template <typename T, int N>
 __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restrict__ psi_tape, Dims d,
                                               PlainCoef cf, double* __restrict__ acc, Tw<T> tw)
 {
-    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = N + 2;
+    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
     __shared__ T lds[NL * LS];
@@ -900,7 +900,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restr
     zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v = *reinterpret_cast<const V16<T>*>(w + gi);
 #pragma unroll
-        for (int t = 0; t < W; ++t) lds[c * LS + p + t] = v.v[t];
+        for (int t = 0; t < W; ++t) lds[c * LS + PR(p + t)] = v.v[t];
     });
     lsync<true>();
     dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
@@ -910,7 +910,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restr
         V16<T> ps = *reinterpret_cast<const V16<T>*>(psi_tape + gi);
 #pragma unroll
         for (int t = 0; t < W; ++t) {
-            int q = c * LS + mk_pos<N>(p + t);
+            int q = c * LS + PR(mk_pos<N>(p + t));
             double r = (double)lds[q];
             double x = (double)ps.v[t], x2 = x * x;
             sums[0] += r * x2 * x;
@@ -923,7 +923,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restr
     zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v;
 #pragma unroll
-        for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + p + t];
+        for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + PR(p + t)];
         *reinterpret_cast<V16<T>*>(w + gi) = v;
     });
     block_accumulate<2, WGZ>(sums, acc);

@@ -103,6 +103,20 @@ __device__ __forceinline__ void lsync()
     }
 }
 
+// LDS line layout: one 16-byte pad after every 8 complex values spreads the
+// Stockham read/write patterns over the LDS banks.  Complex index m lives at
+// PC(m); real index q (re/im of complex q/2) lives at PR(q).
+__device__ __forceinline__ constexpr int PC(int m) { return m + (m >> 3); }
+__device__ __forceinline__ constexpr int PR(int q) { return q + ((q >> 4) << 1); }
+// reals per line in LDS: the padded length rounded up to 2 (mod 32), so
+// consecutive lines of a tile start 8 B (FP32) / 16 B (FP64) apart modulo
+// the 128 B bank width and the column-wise tile loads stay conflict-free.
+template <int N>
+struct LineLS {
+    static constexpr int raw = N + 2 * ((N - 1) >> 4);
+    static constexpr int value = raw + ((2 - raw % 32) + 32) % 32;
+};
+
 // ------------------------------------------------------------ small DFTs --
 
 template <typename T, bool INV>
@@ -189,7 +203,7 @@ __device__ __forceinline__ void stage(cpx<T>* line, int u, int Ns, const T* __re
         int b = u + p * TPL;
         if (NB % TPL == 0 || b < NB) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[p][r] = line[b + r * NB];
+            for (int r = 0; r < R; ++r) v[p][r] = line[PC(b + r * NB)];
         }
     }
     lsync<WAVE>();
@@ -208,7 +222,7 @@ __device__ __forceinline__ void stage(cpx<T>* line, int u, int Ns, const T* __re
             dftR<T, R, INV>(v[p]);
             int base = (b / Ns) * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) line[base + r * Ns] = v[p][r];
+            for (int r = 0; r < R; ++r) line[PC(base + r * Ns)] = v[p][r];
         }
     }
     lsync<WAVE>();
@@ -255,7 +269,7 @@ __device__ __forceinline__ int mk_pos(int p)
 
 // ------------------------------------------------ DCT-II / DCT-III lines --
 // DCT-II: on entry line (as reals) holds x at Makhoul positions, i.e.
-// real[mk_pos(p)] = x_p.  On exit real[k] = Y_k (natural order).
+// real[PR(mk_pos(p))] = x_p.  On exit real[PR(k)] = Y_k (natural order).
 template <typename T, int N, int TPL, bool WAVE = true>
 __device__ __forceinline__ void dct2_line(T* lr, int u, const T* ct)
 {
@@ -270,8 +284,8 @@ __device__ __forceinline__ void dct2_line(T* lr, int u, const T* ct)
     for (int q = 0; q < PER; ++q) {
         int k = u + q * TPL;
         if (k < NP) {
-            cpx<T> zk = z[k];
-            cpx<T> zm = z[(M - k) % M];
+            cpx<T> zk = z[PC(k)];
+            cpx<T> zm = z[PC((M - k) % M)];
             cpx<T> A = mk(zk.re + zm.re, zk.im - zm.im);
             cpx<T> B = mk(zk.re - zm.re, zk.im + zm.im);
             cpx<T> sB = mul(omega<T, N>(ct, 4 * k), B);
@@ -291,23 +305,23 @@ __device__ __forceinline__ void dct2_line(T* lr, int u, const T* ct)
     for (int q = 0; q < PER; ++q) {
         int k = u + q * TPL;
         if (k < NP) {
-            lr[k] = y[q][0];
+            lr[PR(k)] = y[q][0];
             if (k > 0) {
-                lr[N - k] = y[q][1];
+                lr[PR(N - k)] = y[q][1];
                 if (k != M - k) {
-                    lr[M - k] = y[q][2];
-                    lr[M + k] = y[q][3];
+                    lr[PR(M - k)] = y[q][2];
+                    lr[PR(M + k)] = y[q][3];
                 }
             } else {
-                lr[M] = y[q][2];
+                lr[PR(M)] = y[q][2];
             }
         }
     }
     lsync<WAVE>();
 }
 
-// DCT-III: on entry real[k] = X_k (natural order).  On exit the line holds v,
-// i.e. output sample p is real[mk_pos(p)].
+// DCT-III: on entry real[PR(k)] = X_k (natural order).  On exit the line
+// holds v, i.e. output sample p is real[PR(mk_pos(p))].
 template <typename T, int N, int TPL, bool WAVE = true>
 __device__ __forceinline__ void dct3_line(T* lr, int u, const T* ct)
 {
@@ -320,10 +334,10 @@ __device__ __forceinline__ void dct3_line(T* lr, int u, const T* ct)
     for (int q = 0; q < PER; ++q) {
         int k = u + q * TPL;
         if (k < NP) {
-            T xk = lr[k];
-            T xnk = (k == 0) ? T(0) : lr[N - k];
-            T xmk = lr[M - k];
-            T xpk = (k == 0) ? lr[M] : lr[M + k];
+            T xk = lr[PR(k)];
+            T xnk = (k == 0) ? T(0) : lr[PR(N - k)];
+            T xmk = lr[PR(M - k)];
+            T xpk = (k == 0) ? lr[PR(M)] : lr[PR(M + k)];
             // V'_k = conj(h_k)(X_k - i X_{N-k}),  V'_{k+M} = conj(h_{k+M})(X_{M+k} - i X_{M-k})
             cpx<T> v0 = mul(conj(omega<T, N>(ct, k)), mk(xk, -xnk));
             cpx<T> v1 = mul(conj(omega<T, N>(ct, k + M)), mk(xpk, -xmk));
@@ -345,8 +359,8 @@ __device__ __forceinline__ void dct3_line(T* lr, int u, const T* ct)
     for (int q = 0; q < PER; ++q) {
         int k = u + q * TPL;
         if (k < NP) {
-            z[k] = zo[q][0];
-            if (k > 0 && k != M - k) z[M - k] = zo[q][1];
+            z[PC(k)] = zo[q][0];
+            if (k > 0 && k != M - k) z[PC(M - k)] = zo[q][1];
         }
     }
     lsync<WAVE>();
