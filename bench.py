@@ -59,6 +59,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-ckpt", type=int, default=20)
+    ap.add_argument("--adj-reps", type=int, default=3)
+    ap.add_argument("--model-steps", type=int, default=10)
+    ap.add_argument("--adv-grid", type=int, default=256)
+    ap.add_argument("--adv-acc", type=int, default=200)
+    ap.add_argument("--quasi-grid", type=int, default=384)
+    ap.add_argument("--quasi-acc", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-fp", choices=["64", "32"], default=None,
                     help="time one precision only (for profiler runs)")
@@ -139,7 +145,8 @@ def main():
     # (taped forward + adjoint) over the plain forward of the same K steps
     adj = None
     if args.adj_steps > 0:
-        adj = {"K": args.adj_steps, "ckpt_every": args.adj_ckpt}
+        adj = {"K": args.adj_steps, "ckpt_every": args.adj_ckpt,
+               "timing": f"best of {args.adj_reps} runs per leg"}
         grads = {}
         for prec in precs:
             p = smadft.Params.default(smadft.MODEL_PLAIN, n)
@@ -147,14 +154,21 @@ def main():
             ic = smadft.ic_curved(p64, 1.328, 1e-3, seed=1)
             with smadft.Stepper(p, ic) as s:
                 s.step(5)
-                s.set_field(smadft.FIELD_PSI, ic)
-                fwd_ms = s.time_steps(args.adj_steps)
-                s.set_field(smadft.FIELD_PSI, ic)
-                s.forward_tape(args.adj_steps, args.adj_ckpt)
-                psik = s.fetch()
+                # best of `adj_reps` runs of each leg (device time, HIP events)
+                fwd_ms = float("inf")
+                for _ in range(args.adj_reps):
+                    s.set_field(smadft.FIELD_PSI, ic)
+                    fwd_ms = min(fwd_ms, s.time_steps(args.adj_steps))
                 target = smadft.ic_curved(p64, 1.328, 1e-3, seed=5)
-                d0, dp = s.adjoint(2.0 * (psik - target))
-                tape_ms, adj_ms = s.adjoint_timing()
+                tape_ms = adj_ms = float("inf")
+                for _ in range(args.adj_reps):
+                    s.set_field(smadft.FIELD_PSI, ic)
+                    s.forward_tape(args.adj_steps, args.adj_ckpt)
+                    psik = s.fetch()
+                    d0, dp = s.adjoint(2.0 * (psik - target))
+                    t_ms, a_ms = s.adjoint_timing()
+                    if t_ms + a_ms < tape_ms + adj_ms:
+                        tape_ms, adj_ms = t_ms, a_ms
             grads[prec] = (d0, dp)
             adj[f"fp{prec}"] = {"forward_ms": fwd_ms, "forward_tape_ms": tape_ms,
                                 "adjoint_ms": adj_ms, "ratio": (tape_ms + adj_ms) / fwd_ms}
@@ -166,6 +180,45 @@ def main():
             adj["fp32_grad_rel_err"] = {
                 "dpsi0": float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64)),
                 "dtheta": [float(abs(a - b) / abs(b)) for a, b in zip(p32g, p64g)]}
+
+    # advective (256^3, config 2) and quasi-incompressible (384^3, config 3)
+    # models: steps/s in both precisions and per-field FP32-mixed error
+    models = {}
+    if args.model_steps > 0 and args.only_fp is None:
+        for label, model, grid, nacc in (("advective", smadft.MODEL_ADV, args.adv_grid, args.adv_acc),
+                                         ("quasi", smadft.MODEL_QUASI, args.quasi_grid, args.quasi_acc)):
+            pm = smadft.Params.default(model, grid)
+            ic = smadft.ic_curved(pm, 1.34164 if model == smadft.MODEL_QUASI else 1.328, 1e-3, seed=2)
+            entry = {"grid": [grid] * 3, "dt": pm.dt}
+            fields = {"psi": smadft.FIELD_PSI, "ux": smadft.FIELD_VEL_X, "uy": smadft.FIELD_VEL_Y,
+                      "uz": smadft.FIELD_VEL_Z}
+            if model == smadft.MODEL_QUASI:
+                pm.div_switch = min(pm.div_switch, max(1, nacc // 2))
+                fields.update({"p": smadft.FIELD_PRESSURE, "rho": smadft.FIELD_RHO})
+                entry["div_switch"] = pm.div_switch
+            out = {}
+            for prec in (64, 32):
+                pm.precision = prec
+                with smadft.Stepper(pm, ic) as s:
+                    s.step(2)
+                    ms = s.time_steps(args.model_steps)
+                    entry[f"fp{prec}_steps_per_s"] = 1e3 * args.model_steps / ms
+                    log(f"{label} {grid}^3 fp{prec}: {1e3 * args.model_steps / ms:.1f} steps/s")
+                    if nacc > 0:
+                        s.set_field(smadft.FIELD_PSI, ic)
+                        s.step(nacc)
+                        try:
+                            out[prec] = {k: s.fetch(f) for k, f in fields.items()}
+                        except smadft.SmadError as e:
+                            entry[f"fp{prec}_error"] = str(e)
+                            nacc = 0
+            if nacc > 0:
+                entry["fp32_rel_l2_vs_fp64"] = {
+                    k: float(np.linalg.norm(out[32][k] - out[64][k]) / max(np.linalg.norm(out[64][k]), 1e-300))
+                    for k in fields}
+                entry["accuracy_after_steps"] = nacc
+                log(f"{label} fp32 errors: {entry['fp32_rel_l2_vs_fp64']}")
+            models[label] = entry
 
     # CPU baseline: the FP64 oracle on the host cores, bounded sample
     cpu = None
@@ -209,6 +262,7 @@ def main():
         },
         "adjoint_fwd_ratio": adj["fp64"]["ratio"] if adj and "fp64" in adj else None,
         "adjoint": adj,
+        "models": models,
         "kernels": kernels,
         "kernels_fp32": kernels32,
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved,
