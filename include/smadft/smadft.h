@@ -130,6 +130,13 @@ int smad_sync(smad_ctx *ctx);
  * recomputing each segment from its checkpoint.  Afterwards the context is
  * left at step K, as after the forward run. */
 int smad_forward_tape(smad_ctx *ctx, int nsteps, int ckpt_every);
+
+/* Device memory (GiB) the forward run may spend on tape beyond the one
+ * segment it always records.  Segments are taped from the end while the
+ * budget allows; the others are recomputed from their checkpoints during
+ * smad_adjoint.  0 (default) = strict checkpointing: only the last segment
+ * is recorded.  Takes effect at the next smad_forward_tape. */
+int smad_set_tape_budget(smad_ctx *ctx, double gib);
 int smad_adjoint(smad_ctx *ctx, const double *dl_dpsik, double *dl_dpsi0, double *dl_dparams);
 
 /* Device time (HIP events, ms) of the last smad_forward_tape and of the

@@ -38,7 +38,7 @@ EXPORTS = (
     "smad_sync", "smad_stream", "smad_steps_taken", "smad_destroy",
     "smad_last_error", "smad_ic_focal_conic", "smad_ic_slab", "smad_ic_curved",
     "smad_ic_add_noise", "smad_default_params", "smad_r2r_3d", "smad_time_steps",
-    "smad_profile_steps", "smad_kernel_class_name", "smad_forward_tape", "smad_adjoint", "smad_adjoint_timing",
+    "smad_profile_steps", "smad_kernel_class_name", "smad_forward_tape", "smad_adjoint", "smad_adjoint_timing", "smad_set_tape_budget",
 )
 
 REDFT10, REDFT01, RODFT10, RODFT01 = 0, 1, 2, 3
@@ -132,6 +132,7 @@ def lib() -> ctypes.CDLL:
         L.smad_time_steps.argtypes = [vp, ctypes.c_int, dp]
         L.smad_forward_tape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.smad_adjoint.argtypes = [vp, dp, dp, dp]
+        L.smad_set_tape_budget.argtypes = [vp, ctypes.c_double]
         L.smad_adjoint_timing.argtypes = [vp, dp, dp]
         L.smad_profile_steps.argtypes = [vp, ctypes.c_int, dp, P(ctypes.c_int), ctypes.c_int,
                                          P(ctypes.c_int)]
@@ -250,6 +251,11 @@ class Stepper:
         """nsteps steps from the pre-loop state, checkpointing every
         ckpt_every steps for a following :meth:`adjoint`."""
         self._check(lib().smad_forward_tape(self._h, int(nsteps), int(ckpt_every)))
+
+    def set_tape_budget(self, gib: float):
+        """HBM (GiB) the next forward_tape may spend on tape beyond one
+        segment; 0 = strict checkpointing."""
+        self._check(lib().smad_set_tape_budget(self._h, float(gib)))
 
     def adjoint(self, dl_dpsik: np.ndarray):
         """Discrete adjoint of the taped run: returns (dL/dpsi0,

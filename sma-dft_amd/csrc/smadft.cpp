@@ -122,6 +122,7 @@ struct EngineBase {
     virtual void set_field(int field, const double* in) = 0;
     virtual int reduce(int what, double* out) = 0;
     virtual void forward_tape(int K, int ck) = 0;
+    virtual void set_tape_budget(double) {}
     double last_fwd_ms = 0.0, last_adj_ms = 0.0;
     virtual void adjoint(const double* g_psik, double* dpsi0, double* dparams) = 0;
 };
@@ -273,10 +274,11 @@ struct PlainEngine : EngineBase {
     // One loop-body iteration: psihat is read from ph_in and written to
     // ph_out (they alias in ordinary stepping); psi_out, if set, receives
     // psi^n, the real field the step starts from.
-    void step_one(const T* ph_in, T* ph_out, T* psi_out, Prof* prof)
+    void step_one(const T* ph_in, T* ph_out, T* psi_out, Prof* prof, bool z_only = false)
     {
         Launch<T>::plain_z(s, w.p, w.p, psi_out, d, cf, 0, twz.tw);
         if (prof) prof->mark(0);
+        if (z_only) return;
         Launch<T>::axis(s, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
         if (prof) prof->mark(1);
         Launch<T>::plain_x(s, w.p, w.p, ph_in, ph_out, nhat.p, d, cf, twx.tw, 1);
@@ -298,42 +300,31 @@ struct PlainEngine : EngineBase {
     }
 
     // ------------------------------------------------------------ adjoint --
-    // Checkpointed forward: (psihat, nhat) saved at the start of every
-    // segment of `ck` steps.  A recorded segment writes psi^m (m-th step of
-    // the segment, from its z pass) to tape slot m and steps psihat through
-    // rotating slots (slot 0 is the state buffer, slot m+1 receives
-    // psihat^{m+1}), so psihat costs no extra traffic.  The last segment is
-    // recorded by the forward run itself and is not recomputed.
-    int tape_K = 0, tape_ck = 0, tape_nseg = 0;
-    std::vector<std::unique_ptr<DevBuf<T>>> ck_ph, ck_nh, ck_w, tp_psi, sl_ph;
+    // Checkpointed forward: (psihat, nhat, w) saved at the start of every
+    // segment of `ck` steps.  A recorded segment writes psi^m (from its z
+    // pass) to tape slot m and steps psihat through rotating slots (slot 0 is
+    // the segment's checkpoint, slot m+1 receives psihat^{m+1}), so
+    // recording psihat costs no extra traffic.  The forward run records the
+    // last segments directly, as many as the tape budget allows (at least
+    // one); every other segment is recomputed from its checkpoint into a
+    // scratch tape when the adjoint reaches it.
+    struct Tape {
+        std::vector<std::unique_ptr<DevBuf<T>>> psi, ph;  // ph[m-1] holds slot m
+    };
+    int tape_K = 0, tape_ck = 0, tape_nseg = 0, tape_first = 0;  // segments >= tape_first are taped
+    double tape_budget = 0.0;                                    // bytes beyond one segment
+    std::vector<std::unique_ptr<DevBuf<T>>> ck_ph, ck_nh, ck_w;
+    std::vector<std::unique_ptr<Tape>> tapes;  // per segment (taped ones), plus scratch
+    std::unique_ptr<Tape> scratch;
     DevBuf<T> end_ph, end_nh, lbuf, gbuf;
     DevBuf<double> acc_x, acc_z, gsum;
     bool fresh = true;
 
+    void set_tape_budget(double bytes) override { tape_budget = bytes > 0 ? bytes : 0.0; }
+
     void copy_dd(T* dst, const T* src)
     {
         HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-    }
-
-    void restore_w()  // w = REDFT01_y REDFT01_x (psihat)
-    {
-        Launch<T>::axis(s, psihat.p, w.p, d, 0, REDFT01, T(1), twx.tw);
-        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
-    }
-
-    T* slot_ph(int m) { return m == 0 ? psihat.p : sl_ph[m - 1]->p; }
-
-    // last_z_only: the segment's final step only needs its z pass (psi^{ns-1}
-    // for the tape); psihat^{ns} is the next segment's checkpoint.
-    void run_segment(int ns, bool record, bool last_z_only = false)
-    {
-        for (int m = 0; m < ns; ++m) {
-            if (!record) step_one(psihat.p, psihat.p, nullptr, nullptr);
-            else if (last_z_only && m == ns - 1)
-                Launch<T>::plain_z(s, w.p, w.p, tp_psi[m]->p, d, cf, 0, twz.tw);
-            else step_one(slot_ph(m), slot_ph(m + 1), tp_psi[m]->p, nullptr);
-        }
-        LAUNCHCHK();
     }
 
     static void ensure(std::vector<std::unique_ptr<DevBuf<T>>>& v, int count, size_t n)
@@ -345,18 +336,41 @@ struct PlainEngine : EngineBase {
                 v[i]->alloc(n);
             }
     }
+    void ensure_tape(std::unique_ptr<Tape>& t, int ck)
+    {
+        if (!t) t.reset(new Tape);
+        ensure(t->psi, ck, n);
+        ensure(t->ph, ck, n);
+    }
+
+    // slot m of segment g on tape t: slot 0 is the checkpoint
+    const T* slot(const Tape& t, int g, int m) const { return m == 0 ? ck_ph[g]->p : t.ph[m - 1]->p; }
+
+    // recorded run of ns steps of segment g from (w, ck_ph[g], nhat) into t
+    void record(Tape& t, int g, int ns, bool last_z_only)
+    {
+        for (int m = 0; m < ns; ++m) {
+            bool zo = last_z_only && m == ns - 1;
+            step_one(slot(t, g, m), zo ? nullptr : t.ph[m]->p, t.psi[m]->p, nullptr, zo);
+        }
+        LAUNCHCHK();
+    }
 
     void forward_tape(int K, int ck) override
     {
         if (!fresh) throw ApiError{SMAD_ESTATE, "forward_tape must start from the pre-loop state (after init or set_field(PSI))"};
         if (K < 1 || ck < 1) throw ApiError{SMAD_EINVAL, "K and ckpt_every must be >= 1"};
         ck = std::min(ck, K);
-        int nseg = (K + ck - 1) / ck;
+        const int nseg = (K + ck - 1) / ck;
+        const double seg_bytes = 2.0 * ck * n * sizeof(T);
+        const int ntaped = std::min(nseg, 1 + (int)(tape_budget / seg_bytes));
+        const int first = nseg - ntaped;
         ensure(ck_ph, nseg, n);
         ensure(ck_nh, nseg, n);
         ensure(ck_w, nseg, n);
-        ensure(tp_psi, ck, n);
-        ensure(sl_ph, ck, n);
+        if ((int)tapes.size() < nseg) tapes.resize(nseg);
+        for (int g = first; g < nseg; ++g) ensure_tape(tapes[g], ck);
+        if (first > 0) ensure_tape(scratch, ck);
         if (!end_ph.p) {
             for (DevBuf<T>* b : {&end_ph, &end_nh, &lbuf, &gbuf}) b->alloc(n);
             acc_x.alloc(2 * (size_t)num_tiles(d, 0, Lines<T>::NL));
@@ -368,20 +382,24 @@ struct PlainEngine : EngineBase {
         for (int g = 0; g < nseg; ++g) {
             copy_dd(ck_ph[g]->p, psihat.p);
             copy_dd(ck_nh[g]->p, nhat.p);
-            if (g != nseg - 1) copy_dd(ck_w[g]->p, w.p);
             int ns = std::min(ck, K - g * ck);
-            run_segment(ns, g == nseg - 1);
-            if (g == nseg - 1) {
-                copy_dd(end_ph.p, slot_ph(ns));
-                copy_dd(end_nh.p, nhat.p);
+            if (g < first) {
+                copy_dd(ck_w[g]->p, w.p);
+                for (int m = 0; m < ns; ++m) step_one(psihat.p, psihat.p, nullptr, nullptr, false);
+                LAUNCHCHK();
+            } else {
+                record(*tapes[g], g, ns, false);
+                copy_dd(psihat.p, tapes[g]->ph[ns - 1]->p);
             }
         }
+        copy_dd(end_ph.p, psihat.p);
+        copy_dd(end_nh.p, nhat.p);
         HIPCHK(hipEventRecord(ev.b, s));
         last_fwd_ms = ev.ms();
-        copy_dd(psihat.p, end_ph.p);  // w already holds step K
         tape_K = K;
         tape_ck = ck;
         tape_nseg = nseg;
+        tape_first = first;
         psi_valid = false;
         prev_valid = false;
         fresh = false;
@@ -401,27 +419,28 @@ struct PlainEngine : EngineBase {
         HIPCHK(hipMemsetAsync(acc_x.p, 0, acc_x.n * sizeof(double), s));
         HIPCHK(hipMemsetAsync(acc_z.p, 0, acc_z.n * sizeof(double), s));
         for (int g = nseg - 1; g >= 0; --g) {
-            int n0 = g * ck, ns = std::min(ck, K - n0);
-            // slot 0 is the segment's starting psihat
-            copy_dd(psihat.p, ck_ph[g]->p);
-            const bool last = g == nseg - 1;
-            if (!last) {
+            const int n0 = g * ck, ns = std::min(ck, K - n0);
+            const bool taped = g >= tape_first;
+            Tape& t = taped ? *tapes[g] : *scratch;
+            if (!taped) {
+                // recompute from the checkpoint; the final step needs only its
+                // z pass because psihat at the segment end is checkpoint g+1
                 copy_dd(nhat.p, ck_nh[g]->p);
                 copy_dd(w.p, ck_w[g]->p);
-                run_segment(ns, true, true);
+                record(t, g, ns, true);
             }
             for (int m = ns - 1; m >= 0; --m) {
                 int nn = n0 + m;
-                const T* ph_next = (!last && m == ns - 1) ? ck_ph[g + 1]->p : slot_ph(m + 1);
+                const T* ph_next = (!taped && m == ns - 1) ? ck_ph[g + 1]->p : t.ph[m]->p;
                 Launch<T>::adj_x(s, gbuf.p, lbuf.p, ph_next, d, cf, nn >= 1 ? 0 : 1, acc_x.p, twx.tw);
                 Launch<T>::axis(s, gbuf.p, gbuf.p, d, 1, REDFT01, T(1), twy.tw);
-                Launch<T>::adj_z(s, gbuf.p, tp_psi[m]->p, d, cf, acc_z.p, twz.tw);
+                Launch<T>::adj_z(s, gbuf.p, t.psi[m]->p, d, cf, acc_z.p, twz.tw);
                 Launch<T>::axis(s, gbuf.p, gbuf.p, d, 1, REDFT10, T(1), twy.tw);
             }
             LAUNCHCHK();
         }
-        // dJ/dpsi0 = s REDFT01^3(a lt^1 + REDFT10_x(G)); psihat holds psihat^0
-        Launch<T>::adj_x(s, gbuf.p, lbuf.p, psihat.p, d, cf, 2, acc_x.p, twx.tw);
+        // dJ/dpsi0 = s REDFT01^3(a lt^1 + REDFT10_x(G)), psihat^0 = checkpoint 0
+        Launch<T>::adj_x(s, gbuf.p, lbuf.p, ck_ph[0]->p, d, cf, 2, acc_x.p, twx.tw);
         Launch<T>::axis(s, gbuf.p, gbuf.p, d, 1, REDFT01, T(1), twy.tw);
         Launch<T>::axis(s, gbuf.p, psi.p, d, 2, REDFT01, T(1), twz.tw);
         Launch<T>::fold(s, acc_x.p, (int)(acc_x.n / 2), 2, gsum.p);
@@ -436,7 +455,8 @@ struct PlainEngine : EngineBase {
         // leave the context at the end of the taped forward run
         copy_dd(psihat.p, end_ph.p);
         copy_dd(nhat.p, end_nh.p);
-        restore_w();
+        Launch<T>::axis(s, psihat.p, w.p, d, 0, REDFT01, T(1), twx.tw);
+        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
         LAUNCHCHK();
         HIPCHK(hipStreamSynchronize(s));
         psi_valid = false;
@@ -1098,6 +1118,15 @@ int smad_forward_tape(smad_ctx* c, int nsteps, int ckpt_every)
     return guarded(c, [&] {
         c->eng->forward_tape(nsteps, ckpt_every);
         c->steps += nsteps;
+        return SMAD_OK;
+    });
+}
+
+int smad_set_tape_budget(smad_ctx* c, double gib)
+{
+    return guarded(c, [&] {
+        if (!(gib >= 0)) throw ApiError{SMAD_EINVAL, "budget must be >= 0"};
+        c->eng->set_tape_budget(gib * 1073741824.0);
         return SMAD_OK;
     });
 }

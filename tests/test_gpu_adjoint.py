@@ -35,6 +35,24 @@ def test_adjoint_fp64_matches_oracle(smadft, oracle, K, ck):
         assert dp[q] == pytest.approx(dp_o[q], rel=1e-9, abs=1e-9 * np.abs(dp_o).max())
 
 
+@pytest.mark.parametrize("gib", [0.0026, 1.0])
+def test_adjoint_tape_budget_bit_identical(smadft, gib):
+    # taping more segments replaces recomputation by stored copies of the
+    # same values: forward state and gradients must not change by one bit
+    p, psi0, tgt = setup(smadft)
+    K, ck = 24, 5
+    out = {}
+    for b in (0.0, gib):
+        with smadft.Stepper(p, psi0) as s:
+            s.set_tape_budget(b)
+            s.forward_tape(K, ck)
+            psik = s.fetch()
+            d0, dp = s.adjoint(2 * (psik - tgt))
+            out[b] = (psik, d0, dp, s.fetch())
+    for a, b in zip(out[0.0], out[gib]):
+        assert np.array_equal(a, b)
+
+
 def test_adjoint_fp32_mixed_close_to_oracle(smadft, oracle):
     p, psi0, tgt = setup(smadft, prec=32)
     K = 20
@@ -57,6 +75,9 @@ def test_adjoint_state_errors(smadft):
         with pytest.raises(smadft.SmadError) as e:
             s.forward_tape(5, 2)
         assert e.value.code == smadft.ESTATE
+        with pytest.raises(smadft.SmadError) as e:
+            s.set_tape_budget(-1.0)
+        assert e.value.code == smadft.EINVAL
 
 
 def test_autograd_rollout_matches_oracle_and_fd(smadft, oracle):
