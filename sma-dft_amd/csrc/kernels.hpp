@@ -274,9 +274,9 @@ struct PlainCoef {
 // z pass: psi = REDFT01_z(w) (or psi read directly when from_real),
 // optionally store psi, N = beta psi^3 - gamma psi^5, out = REDFT10_z(N).
 // in may equal out.  Each wavefront works on its own lines only.
-template <typename T, int N>
-__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_plain_z(const T* in, T* out, T* __restrict__ psi_out,
-                                                Dims d, PlainCoef cf, int from_real, Tw<T> tw)
+template <typename T, int N, bool from_real>
+__device__ __forceinline__ void plain_z_body(const T* in, T* out, T* __restrict__ psi_out, Dims d,
+                                             const PlainCoef& cf, const Tw<T>& tw)
 {
     constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
@@ -292,10 +292,9 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_plain_z(const T* in, T* out,
         for (int t = 0; t < W; ++t) lds[c * LS + PR((from_real ? mk_pos<N>(p + t) : p + t))] = v.v[t];
     });
     lsync<true>();
-    if (!from_real) dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
-    // LDS holds psi at Makhoul positions -- the input layout of the forward
-    // transform -- so the nonlinearity is applied in place.
-    if (psi_out) {
+    const T beta = T(cf.beta), gamma = T(cf.gamma);
+    // psi at Makhoul positions in LDS -> psi_out in natural order
+    auto store_psi = [&]() {
         zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
             V16<T> v;
 #pragma unroll
@@ -303,8 +302,27 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_plain_z(const T* in, T* out,
             *reinterpret_cast<V16<T>*>(psi_out + gi) = v;
         });
         lsync<true>();
+    };
+    if constexpr (OpFused<N, TPL>::value && !from_real) {
+        {
+            auto nl = [&](T x) {
+                T x2 = x * x, x3 = x2 * x;
+                return beta * x3 - gamma * (x3 * x2);
+            };
+            dct3_map_dct2<T, N, TPL, true>(lds + l * LS, u, ct, nl, psi_out != nullptr, store_psi);
+            zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+                V16<T> v;
+#pragma unroll
+                for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + PR(p + t)];
+                *reinterpret_cast<V16<T>*>(out + gi) = v;
+            });
+            return;
+        }
     }
-    const T beta = T(cf.beta), gamma = T(cf.gamma);
+    if constexpr (!from_real) dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    // LDS holds psi at Makhoul positions -- the input layout of the forward
+    // transform -- so the nonlinearity is applied in place.
+    if (psi_out) store_psi();
     {
         constexpr int LPW = NL / (WGZ / 64);
         const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -325,6 +343,21 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_plain_z(const T* in, T* out,
         for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + PR(p + t)];
         *reinterpret_cast<V16<T>*>(out + gi) = v;
     });
+}
+
+// The spectral-input step kernel and the real-input pre-loop kernel are
+// separate so that each carries only its own register footprint.
+template <typename T, int N>
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_plain_z(const T* in, T* out, T* __restrict__ psi_out,
+                                                            Dims d, PlainCoef cf, Tw<T> tw)
+{
+    plain_z_body<T, N, false>(in, out, psi_out, d, cf, tw);
+}
+template <typename T, int N>
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_plain_z_real(const T* in, T* out, T* __restrict__ psi_out,
+                                                                 Dims d, PlainCoef cf, Tw<T> tw)
+{
+    plain_z_body<T, N, true>(in, out, psi_out, d, cf, tw);
 }
 
 // CN/AB2 update of one spectral value, FP64 arithmetic:

@@ -45,7 +45,11 @@ void Launch<T>::plain_z(hipStream_t s, const T* in, T* out, T* psi_out, Dims d,
                         const PlainCoef& cf, int from_real, const Tw<T>& tw)
 {
     dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
-    SMAD_DISPATCH_Z(d.nz, k_plain_z, grid, in, out, psi_out, d, cf, from_real, tw);
+    if (from_real) {
+        SMAD_DISPATCH_Z(d.nz, k_plain_z_real, grid, in, out, psi_out, d, cf, tw);
+    } else {
+        SMAD_DISPATCH_Z(d.nz, k_plain_z, grid, in, out, psi_out, d, cf, tw);
+    }
 }
 
 template <typename T>

@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace smad {
 
 enum Kind : int { REDFT10 = 0, REDFT01 = 1, RODFT10 = 2, RODFT01 = 3 };
@@ -189,57 +191,144 @@ __device__ __forceinline__ void dftR(cpx<T>* a)
 // One radix-R stage over a line of M complex values held in LDS, executed by
 // TPL cooperating threads (lane u of the line).  Stockham/Govindaraju form:
 // butterfly b reads line[b + r M/R], multiplies by W^{r (b mod Ns) M/(Ns R)},
-// and writes line[(b/Ns) Ns R + (b mod Ns) + r Ns].  All reads land in
-// registers before a barrier, so the stage is in place.
-template <typename T, int N, int R, int TPL, bool INV, bool WAVE>
-__device__ __forceinline__ void stage(cpx<T>* line, int u, int Ns, const T* __restrict__ ct)
+// and writes line[(b/Ns) Ns R + (b mod Ns) + r Ns].  A stage is split into
+// its LDS read, its arithmetic and its LDS write, so that the first and the
+// last stage of a transform can take their inputs from, or leave their
+// outputs in, registers.
+template <int N, int R, int TPL>
+struct Bf {
+    static constexpr int M = N / 2;
+    static constexpr int NB = M / R;                  // butterflies per stage
+    static constexpr int PER = (NB + TPL - 1) / TPL;  // butterflies per thread
+    static __device__ __forceinline__ bool live(int b) { return NB % TPL == 0 || b < NB; }
+};
+
+template <typename T, int N, int R, int TPL>
+using BfRegs = cpx<T>[Bf<N, R, TPL>::PER][R];
+
+template <typename T, int N, int R, int TPL>
+__device__ __forceinline__ void bf_load(const cpx<T>* line, int u, BfRegs<T, N, R, TPL>& v)
 {
-    constexpr int M = N / 2;
-    constexpr int NB = M / R;
-    constexpr int PER = (NB + TPL - 1) / TPL;
-    cpx<T> v[PER][R];
+    using B = Bf<N, R, TPL>;
 #pragma unroll
-    for (int p = 0; p < PER; ++p) {
+    for (int p = 0; p < B::PER; ++p) {
         int b = u + p * TPL;
-        if (NB % TPL == 0 || b < NB) {
+        if (B::live(b)) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[p][r] = line[PC(b + r * NB)];
+            for (int r = 0; r < R; ++r) v[p][r] = line[PC(b + r * B::NB)];
         }
     }
-    lsync<WAVE>();
+}
+
+// twiddle + radix-R DFT; FIRST = stage with Ns = 1 (no twiddle)
+template <typename T, int N, int R, int TPL, bool INV, bool FIRST>
+__device__ __forceinline__ void bf_compute(BfRegs<T, N, R, TPL>& v, int u, int Ns, const T* __restrict__ ct)
+{
+    using B = Bf<N, R, TPL>;
 #pragma unroll
-    for (int p = 0; p < PER; ++p) {
+    for (int p = 0; p < B::PER; ++p) {
         int b = u + p * TPL;
-        if (NB % TPL == 0 || b < NB) {
-            int k = b % Ns;
-            int step = M / (Ns * R);
+        if (B::live(b)) {
+            if constexpr (!FIRST) {
+                int k = b % Ns;
+                int step = B::M / (Ns * R);
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                cpx<T> w = omega<T, N>(ct, 8 * k * r * step);
-                if (INV) w.im = -w.im;
-                v[p][r] = mul(v[p][r], w);
+                for (int r = 1; r < R; ++r) {
+                    cpx<T> w = omega<T, N>(ct, 8 * k * r * step);
+                    if (INV) w.im = -w.im;
+                    v[p][r] = mul(v[p][r], w);
+                }
             }
             dftR<T, R, INV>(v[p]);
-            int base = (b / Ns) * Ns * R + k;
+        }
+    }
+}
+
+template <typename T, int N, int R, int TPL>
+__device__ __forceinline__ void bf_store(cpx<T>* line, int u, int Ns, const BfRegs<T, N, R, TPL>& v)
+{
+    using B = Bf<N, R, TPL>;
+#pragma unroll
+    for (int p = 0; p < B::PER; ++p) {
+        int b = u + p * TPL;
+        if (B::live(b)) {
+            int base = (b / Ns) * Ns * R + b % Ns;
 #pragma unroll
             for (int r = 0; r < R; ++r) line[PC(base + r * Ns)] = v[p][r];
         }
     }
+}
+
+// All reads land in registers before the barrier, so a stage is in place.
+template <typename T, int N, int R, int TPL, bool INV, bool WAVE>
+__device__ __forceinline__ void stage(cpx<T>* line, int u, int Ns, const T* __restrict__ ct)
+{
+    BfRegs<T, N, R, TPL> v;
+    bf_load<T, N, R, TPL>(line, u, v);
+    lsync<WAVE>();
+    bf_compute<T, N, R, TPL, INV, false>(v, u, Ns, ct);
+    bf_store<T, N, R, TPL>(line, u, Ns, v);
     lsync<WAVE>();
 }
 
-template <typename T, int N, int TPL, bool INV, bool WAVE, int... Rs>
-struct FFTRun;
+// Radix lists.  The forward plan is the inverse plan reversed, so that the
+// last inverse stage and the first forward stage have the same butterflies
+// (butterfly b holds elements b + r M/R in both) and a pointwise operation
+// between an inverse and a forward transform can stay in registers.
+template <int... Rs>
+struct RList {
+    static constexpr int size = sizeof...(Rs);
+};
+template <class L>
+struct RHead;
+template <int R, int... Rs>
+struct RHead<RList<R, Rs...>> {
+    static constexpr int value = R;
+    using rest = RList<Rs...>;
+};
+template <int R, class L>
+struct RPrepend;
+template <int R, int... Rs>
+struct RPrepend<R, RList<Rs...>> {
+    using type = RList<R, Rs...>;
+};
+template <class L>
+struct RTail;
+template <int R>
+struct RTail<RList<R>> {
+    static constexpr int value = R;
+    using init = RList<>;
+};
+template <int R, int R2, int... Rs>
+struct RTail<RList<R, R2, Rs...>> {
+    static constexpr int value = RTail<RList<R2, Rs...>>::value;
+    using init = typename RPrepend<R, typename RTail<RList<R2, Rs...>>::init>::type;
+};
+
+template <int M>
+struct Plan;
+template <> struct Plan<4> { using inv = RList<4>; using fwd = RList<4>; };
+template <> struct Plan<8> { using inv = RList<8>; using fwd = RList<8>; };
+template <> struct Plan<16> { using inv = RList<4, 4>; using fwd = RList<4, 4>; };
+template <> struct Plan<32> { using inv = RList<4, 8>; using fwd = RList<8, 4>; };
+template <> struct Plan<64> { using inv = RList<8, 8>; using fwd = RList<8, 8>; };
+template <> struct Plan<128> { using inv = RList<4, 4, 8>; using fwd = RList<8, 4, 4>; };
+template <> struct Plan<192> { using inv = RList<3, 8, 8>; using fwd = RList<8, 8, 3>; };
+template <> struct Plan<256> { using inv = RList<4, 8, 8>; using fwd = RList<8, 8, 4>; };
+
+// Stages of a radix list, starting at span Ns.
+template <typename T, int N, int TPL, bool INV, bool WAVE, class L>
+struct Stages;
 template <typename T, int N, int TPL, bool INV, bool WAVE>
-struct FFTRun<T, N, TPL, INV, WAVE> {
+struct Stages<T, N, TPL, INV, WAVE, RList<>> {
     static __device__ __forceinline__ void run(cpx<T>*, int, int, const T*) {}
 };
 template <typename T, int N, int TPL, bool INV, bool WAVE, int R, int... Rs>
-struct FFTRun<T, N, TPL, INV, WAVE, R, Rs...> {
+struct Stages<T, N, TPL, INV, WAVE, RList<R, Rs...>> {
     static __device__ __forceinline__ void run(cpx<T>* line, int u, int Ns, const T* ct)
     {
         stage<T, N, R, TPL, INV, WAVE>(line, u, Ns, ct);
-        FFTRun<T, N, TPL, INV, WAVE, Rs...>::run(line, u, Ns * R, ct);
+        Stages<T, N, TPL, INV, WAVE, RList<Rs...>>::run(line, u, Ns * R, ct);
     }
 };
 
@@ -247,16 +336,9 @@ struct FFTRun<T, N, TPL, INV, WAVE, R, Rs...> {
 template <typename T, int N, int TPL, bool INV, bool WAVE>
 __device__ __forceinline__ void fft(cpx<T>* line, int u, const T* ct)
 {
-    constexpr int M = N / 2;
-    if constexpr (M == 4) FFTRun<T, N, TPL, INV, WAVE, 4>::run(line, u, 1, ct);
-    else if constexpr (M == 8) FFTRun<T, N, TPL, INV, WAVE, 8>::run(line, u, 1, ct);
-    else if constexpr (M == 16) FFTRun<T, N, TPL, INV, WAVE, 4, 4>::run(line, u, 1, ct);
-    else if constexpr (M == 32) FFTRun<T, N, TPL, INV, WAVE, 4, 8>::run(line, u, 1, ct);
-    else if constexpr (M == 64) FFTRun<T, N, TPL, INV, WAVE, 8, 8>::run(line, u, 1, ct);
-    else if constexpr (M == 128) FFTRun<T, N, TPL, INV, WAVE, 4, 4, 8>::run(line, u, 1, ct);
-    else if constexpr (M == 192) FFTRun<T, N, TPL, INV, WAVE, 3, 8, 8>::run(line, u, 1, ct);
-    else if constexpr (M == 256) FFTRun<T, N, TPL, INV, WAVE, 4, 8, 8>::run(line, u, 1, ct);
-    else static_assert(M == 4, "unsupported FFT length");
+    using P = Plan<N / 2>;
+    Stages<T, N, TPL, INV, WAVE, typename std::conditional<INV, typename P::inv, typename P::fwd>::type>::run(
+        line, u, 1, ct);
 }
 
 // ------------------------------------------------------ Makhoul index map --
@@ -268,10 +350,116 @@ __device__ __forceinline__ int mk_pos(int p)
 }
 
 // ------------------------------------------------ DCT-II / DCT-III lines --
-// DCT-II: on entry line (as reals) holds x at Makhoul positions, i.e.
-// real[PR(mk_pos(p))] = x_p.  On exit real[PR(k)] = Y_k (natural order).
-template <typename T, int N, int TPL, bool WAVE = true>
-__device__ __forceinline__ void dct2_line(T* lr, int u, const T* ct)
+// DCT-III pre-twiddle: packed complex input z_k of the inverse FFT from the
+// four spectral values X_k, X_{N-k}, X_{M+k}, X_{M-k} (X_N = 0):
+//   v0 = conj(h_k)(X_k - i X_{N-k}),  v1 = conj(h_{k+M})(X_{M+k} - i X_{M-k}),
+//   z_k = (v0 + v1) + i (v0 - v1) conj(s_k).
+template <typename T, int N>
+__device__ __forceinline__ cpx<T> dct3_pre(const T* lr, int k, const T* __restrict__ ct)
+{
+    constexpr int M = N / 2;
+    T xk = lr[PR(k)];
+    T xnk = (k == 0) ? T(0) : lr[PR(N - k)];
+    T xpk = lr[PR(M + k)];
+    T xmk = lr[PR(M - k)];
+    cpx<T> v0 = mul(conj(omega<T, N>(ct, k)), mk(xk, -xnk));
+    cpx<T> v1 = mul(conj(omega<T, N>(ct, k + M)), mk(xpk, -xmk));
+    cpx<T> E = add(v0, v1);
+    cpx<T> O = mul(sub(v0, v1), conj(omega<T, N>(ct, 4 * k)));
+    return mk(E.re - O.im, E.im + O.re);  // E + i O
+}
+
+// DCT-II real split + post-twiddle of output pair (Z_k, Z_{M-k}):
+//   A = Z_k + conj(Z_{M-k}),  B = Z_k - conj(Z_{M-k}),
+//   V_k = A/2 - i s_k B/2,  Y_k = 2 Re(h_k V_k),  Y_{N-k} = -2 Im(h_k V_k).
+// For k = 0 the second value is Y_M = 2 Re(h_M (conj(A)/2 - i conj(s_0 B)/2)).
+template <typename T, int N>
+__device__ __forceinline__ void dct2_post(cpx<T> zk, cpx<T> zm, int k, const T* __restrict__ ct, T& y0, T& y1)
+{
+    constexpr int M = N / 2;
+    cpx<T> A = mk(zk.re + zm.re, zk.im - zm.im);
+    cpx<T> B = mk(zk.re - zm.re, zk.im + zm.im);
+    cpx<T> sB = mul(omega<T, N>(ct, 4 * k), B);
+    if (k != 0) {
+        cpx<T> a = mul(omega<T, N>(ct, k), mk(T(0.5) * (A.re + sB.im), T(0.5) * (A.im - sB.re)));
+        y0 = T(2) * a.re;
+        y1 = T(-2) * a.im;
+    } else {
+        cpx<T> a = mk(T(0.5) * (A.re + sB.im), T(0.5) * (A.im - sB.re));
+        cpx<T> bm = mul(omega<T, N>(ct, M), mk(T(0.5) * (A.re - sB.im), T(-0.5) * (A.im + sB.re)));
+        y0 = T(2) * a.re;
+        y1 = T(2) * bm.re;
+    }
+}
+
+// First inverse stage with the DCT-III pre-twiddle folded into its loads
+// (it reads the spectral values in natural order from the line).
+template <typename T, int N, int R, int TPL, bool WAVE>
+__device__ __forceinline__ void dct3_first_stage(T* lr, int u, const T* __restrict__ ct)
+{
+    using B = Bf<N, R, TPL>;
+    BfRegs<T, N, R, TPL> v;
+#pragma unroll
+    for (int p = 0; p < B::PER; ++p) {
+        int b = u + p * TPL;
+        if (B::live(b)) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[p][r] = dct3_pre<T, N>(lr, b + r * B::NB, ct);
+        }
+    }
+    lsync<WAVE>();
+    bf_compute<T, N, R, TPL, true, true>(v, u, 1, ct);
+    bf_store<T, N, R, TPL>(reinterpret_cast<cpx<T>*>(lr), u, 1, v);
+    lsync<WAVE>();
+}
+
+// Output of the last forward stage (span M/R, so butterfly b holds Z at
+// b + r M/R): fetch the partner values Z_{M-k} from lane (TPL - u) mod TPL
+// and write Y_k, Y_{N-k} (and Y_M from k = 0) in natural order.  Requires
+// TPL to divide the butterfly count.  The caller synchronises before the
+// LDS writes land on data other lanes still read.
+template <typename T, int N, int R, int TPL>
+__device__ __forceinline__ void dct2_last_split(T* lr, int u, const BfRegs<T, N, R, TPL>& v,
+                                                const T* __restrict__ ct)
+{
+    using B = Bf<N, R, TPL>;
+    static_assert(B::NB % TPL == 0, "split fusion needs TPL | butterflies");
+    constexpr int PER = B::PER;
+    const int src = (TPL - u) & (TPL - 1);
+    cpx<T> pv[PER][R];  // partner lane's values, butterfly index mirrored
+#pragma unroll
+    for (int p = 0; p < PER; ++p)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            pv[p][r].re = __shfl(v[PER - 1 - p][r].re, src, TPL);
+            pv[p][r].im = __shfl(v[PER - 1 - p][r].im, src, TPL);
+        }
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int b = u + p * TPL;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int k = b + r * B::NB;
+            // Z_{M-k}: butterfly NB - b (elem R-1-r) for b > 0; for b = 0
+            // the own butterfly, element (R - r) mod R
+            cpx<T> zm;
+            if (u != 0) {
+                zm = pv[p][R - 1 - r];
+            } else {
+                const int pp = (PER - p) % PER;
+                zm = (p == 0) ? v[0][(R - r) % R] : v[pp][R - 1 - r];
+            }
+            T y0, y1;
+            dct2_post<T, N>(v[p][r], zm, k, ct, y0, y1);
+            lr[PR(k)] = y0;
+            lr[PR(k == 0 ? N / 2 : N - k)] = y1;
+        }
+    }
+}
+// DCT-II with the real split done through LDS after a full FFT (lines whose
+// last stage has fewer butterflies than threads).
+template <typename T, int N, int TPL, bool WAVE>
+__device__ __forceinline__ void dct2_line_lds(T* lr, int u, const T* ct)
 {
     constexpr int M = N / 2;
     cpx<T>* z = reinterpret_cast<cpx<T>*>(lr);
@@ -320,51 +508,97 @@ __device__ __forceinline__ void dct2_line(T* lr, int u, const T* ct)
     lsync<WAVE>();
 }
 
+template <int N, int TPL>
+struct SplitFused {
+    static constexpr int RL = RTail<typename Plan<N / 2>::fwd>::value;
+    static constexpr bool value = ((N / 2) / RL) % TPL == 0;
+};
+
+// DCT-II: on entry line (as reals) holds x at Makhoul positions, i.e.
+// real[PR(mk_pos(p))] = x_p.  On exit real[PR(k)] = Y_k (natural order).
+template <typename T, int N, int TPL, bool WAVE = true>
+__device__ __forceinline__ void dct2_line(T* lr, int u, const T* ct)
+{
+    constexpr int M = N / 2;
+    using F = typename Plan<M>::fwd;
+    if constexpr (SplitFused<N, TPL>::value) {
+        constexpr int RL = RTail<F>::value;
+        cpx<T>* z = reinterpret_cast<cpx<T>*>(lr);
+        Stages<T, N, TPL, false, WAVE, typename RTail<F>::init>::run(z, u, 1, ct);
+        BfRegs<T, N, RL, TPL> v;
+        bf_load<T, N, RL, TPL>(z, u, v);
+        lsync<WAVE>();
+        bf_compute<T, N, RL, TPL, false, false>(v, u, M / RL, ct);
+        dct2_last_split<T, N, RL, TPL>(lr, u, v, ct);
+        lsync<WAVE>();
+    } else {
+        dct2_line_lds<T, N, TPL, WAVE>(lr, u, ct);
+    }
+}
+
 // DCT-III: on entry real[PR(k)] = X_k (natural order).  On exit the line
 // holds v, i.e. output sample p is real[PR(mk_pos(p))].
 template <typename T, int N, int TPL, bool WAVE = true>
 __device__ __forceinline__ void dct3_line(T* lr, int u, const T* ct)
 {
+    using I = typename Plan<N / 2>::inv;
+    constexpr int R0 = RHead<I>::value;
+    dct3_first_stage<T, N, R0, TPL, WAVE>(lr, u, ct);
+    Stages<T, N, TPL, true, WAVE, typename RHead<I>::rest>::run(reinterpret_cast<cpx<T>*>(lr), u, R0, ct);
+}
+
+// DCT-III, a pointwise map of the physical samples, DCT-II, with the last
+// inverse stage, the map and the first forward stage in registers.  The map
+// does not depend on the sample position.  When `spill`, the physical line
+// is also written to LDS (at Makhoul positions) and `mid()` runs in between;
+// mid() must end with an lsync.
+template <int N, int TPL>
+struct OpFused {
+    using I = typename Plan<N / 2>::inv;
+    static constexpr bool value = I::size >= 2 && SplitFused<N, TPL>::value;
+};
+
+template <typename T, int N, int TPL, bool WAVE, typename Op, typename Mid>
+__device__ __forceinline__ void dct3_map_dct2(T* lr, int u, const T* ct, Op&& op, bool spill, Mid&& mid)
+{
     constexpr int M = N / 2;
+    using I = typename Plan<M>::inv;
+    using F = typename Plan<M>::fwd;
+    constexpr int R0 = RHead<I>::value;
+    constexpr int RM = RTail<I>::value;
+    static_assert(RM == RHead<F>::value, "forward plan must start with the last inverse radix");
+    using Bm = Bf<N, RM, TPL>;
     cpx<T>* z = reinterpret_cast<cpx<T>*>(lr);
-    constexpr int NP = M / 2 + 1;
-    constexpr int PER = (NP + TPL - 1) / TPL;
-    cpx<T> zo[PER][2];
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        int k = u + q * TPL;
-        if (k < NP) {
-            T xk = lr[PR(k)];
-            T xnk = (k == 0) ? T(0) : lr[PR(N - k)];
-            T xmk = lr[PR(M - k)];
-            T xpk = (k == 0) ? lr[PR(M)] : lr[PR(M + k)];
-            // V'_k = conj(h_k)(X_k - i X_{N-k}),  V'_{k+M} = conj(h_{k+M})(X_{M+k} - i X_{M-k})
-            cpx<T> v0 = mul(conj(omega<T, N>(ct, k)), mk(xk, -xnk));
-            cpx<T> v1 = mul(conj(omega<T, N>(ct, k + M)), mk(xpk, -xmk));
-            cpx<T> E = add(v0, v1);
-            cpx<T> O = mul(sub(v0, v1), conj(omega<T, N>(ct, 4 * k)));
-            zo[q][0] = mk(E.re - O.im, E.im + O.re);  // E + i O
-            if (k > 0 && k != M - k) {
-                // V'_{M-k} = conj(h_{M-k})(X_{M-k} - i X_{M+k}),  V'_{N-k} = conj(h_{N-k})(X_{N-k} - i X_k)
-                cpx<T> w0 = mul(conj(omega<T, N>(ct, M - k)), mk(xmk, -xpk));
-                cpx<T> w1 = mul(conj(omega<T, N>(ct, N - k)), mk(xnk, -xk));
-                cpx<T> E2 = add(w0, w1);
-                cpx<T> O2 = mul(sub(w0, w1), conj(omega<T, N>(ct, 4 * (M - k))));
-                zo[q][1] = mk(E2.re - O2.im, E2.im + O2.re);
-            }
-        }
-    }
+    dct3_first_stage<T, N, R0, TPL, WAVE>(lr, u, ct);
+    Stages<T, N, TPL, true, WAVE, typename RTail<typename RHead<I>::rest>::init>::run(z, u, R0, ct);
+    BfRegs<T, N, RM, TPL> v;
+    bf_load<T, N, RM, TPL>(z, u, v);
     lsync<WAVE>();
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        int k = u + q * TPL;
-        if (k < NP) {
-            z[PC(k)] = zo[q][0];
-            if (k > 0 && k != M - k) z[PC(M - k)] = zo[q][1];
-        }
+    bf_compute<T, N, RM, TPL, true, false>(v, u, Bm::NB, ct);
+    if (spill) {
+        bf_store<T, N, RM, TPL>(z, u, Bm::NB, v);
+        lsync<WAVE>();
+        mid();
     }
+#pragma unroll
+    for (int p = 0; p < Bm::PER; ++p)
+#pragma unroll
+        for (int r = 0; r < RM; ++r) {
+            v[p][r].re = op(v[p][r].re);
+            v[p][r].im = op(v[p][r].im);
+        }
+    bf_compute<T, N, RM, TPL, false, true>(v, u, 1, ct);
+    bf_store<T, N, RM, TPL>(z, u, 1, v);
     lsync<WAVE>();
-    fft<T, N, TPL, true, WAVE>(z, u, ct);
+    using FR = typename RHead<F>::rest;
+    constexpr int RL = RTail<FR>::value;
+    Stages<T, N, TPL, false, WAVE, typename RTail<FR>::init>::run(z, u, RM, ct);
+    BfRegs<T, N, RL, TPL> w;
+    bf_load<T, N, RL, TPL>(z, u, w);
+    lsync<WAVE>();
+    bf_compute<T, N, RL, TPL, false, false>(w, u, M / RL, ct);
+    dct2_last_split<T, N, RL, TPL>(lr, u, w, ct);
+    lsync<WAVE>();
 }
 
 // Where global sample p of a line goes in LDS before a forward/inverse
