@@ -82,6 +82,43 @@ __device__ __forceinline__ cpx<T> omega(const T* __restrict__ c, int e)
     }
 }
 
+// cos(pi j / 96), j = 0..96, correctly rounded to double.  Every constant
+// twiddle ratio used below is e^{-i pi j/96} for an integer j: e^{-i pi r/(4R)}
+// and e^{-i pi r/R} for R in {3, 4, 8}, and e^{-i pi/4}.
+constexpr double kCosPi96[97] = {
+    1.0, 0.9994645874763657, 0.9978589232386035, 0.9951847266721969,
+    0.9914448613738104, 0.986643332084879, 0.9807852804032304, 0.9738769792773336,
+    0.9659258262890683, 0.9569403357322088, 0.9469301294951057, 0.9359059267573256,
+    0.9238795325112867, 0.9108638249211758, 0.8968727415326883, 0.881921264348355,
+    0.8660254037844386, 0.8492021815265789, 0.8314696123025452, 0.8128466845916152,
+    0.7933533402912352, 0.773010453362737, 0.7518398074789774, 0.7298640726978357,
+    0.7071067811865476, 0.6835923020228712, 0.6593458151000688, 0.6343932841636455,
+    0.6087614290087207, 0.5824776968678022, 0.5555702330196022, 0.528067850650368,
+    0.5, 0.47139673682599764, 0.4422886902190013, 0.4127070298043947,
+    0.3826834323650898, 0.3522500479212335, 0.3214394653031616, 0.2902846772544624,
+    0.25881904510252074, 0.2270762630343732, 0.19509032201612828, 0.16289547339458874,
+    0.1305261922200516, 0.0980171403295606, 0.06540312923014306, 0.032719082821776144,
+    0.0, -0.032719082821776144, -0.06540312923014306, -0.0980171403295606,
+    -0.1305261922200516, -0.16289547339458874, -0.19509032201612828, -0.2270762630343732,
+    -0.25881904510252074, -0.2902846772544624, -0.3214394653031616, -0.3522500479212335,
+    -0.3826834323650898, -0.4127070298043947, -0.4422886902190013, -0.47139673682599764,
+    -0.5, -0.528067850650368, -0.5555702330196022, -0.5824776968678022,
+    -0.6087614290087207, -0.6343932841636455, -0.6593458151000688, -0.6835923020228712,
+    -0.7071067811865476, -0.7298640726978357, -0.7518398074789774, -0.773010453362737,
+    -0.7933533402912352, -0.8128466845916152, -0.8314696123025452, -0.8492021815265789,
+    -0.8660254037844386, -0.881921264348355, -0.8968727415326883, -0.9108638249211758,
+    -0.9238795325112867, -0.9359059267573256, -0.9469301294951057, -0.9569403357322088,
+    -0.9659258262890683, -0.9738769792773336, -0.9807852804032304, -0.986643332084879,
+    -0.9914448613738104, -0.9951847266721969, -0.9978589232386035, -0.9994645874763657,
+    -1.0};
+
+// e^{-i pi j/96}, 0 <= j <= 96 (a compile-time constant once loops unroll)
+template <typename T>
+__device__ __forceinline__ cpx<T> cis96(int j)
+{
+    return cpx<T>{T(kCosPi96[j]), -T(kCosPi96[j <= 48 ? 48 - j : j - 48])};
+}
+
 // Copy the quarter-wave table into LDS (caller synchronises).
 template <typename T, int N>
 __device__ __forceinline__ void load_cosq(T* dst, const Tw<T>& tw)
@@ -220,6 +257,29 @@ __device__ __forceinline__ void bf_load(const cpx<T>* line, int u, BfRegs<T, N, 
     }
 }
 
+// w[r] = omega^{e r}, r = 1..R-1, from at most three table reads; the other
+// powers are products of two looked-up ones.
+template <typename T, int N, int R>
+__device__ __forceinline__ void tw_powers(const T* __restrict__ ct, int e, cpx<T> (&w)[R])
+{
+    if constexpr (R == 8) {
+        w[1] = omega<T, N>(ct, e);
+        w[2] = omega<T, N>(ct, 2 * e);
+        w[4] = omega<T, N>(ct, 4 * e);
+        w[3] = mul(w[1], w[2]);
+        w[5] = mul(w[1], w[4]);
+        w[6] = mul(w[2], w[4]);
+        w[7] = mul(w[3], w[4]);
+    } else if constexpr (R == 4) {
+        w[1] = omega<T, N>(ct, e);
+        w[2] = omega<T, N>(ct, 2 * e);
+        w[3] = mul(w[1], w[2]);
+    } else {
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[r] = omega<T, N>(ct, r * e);
+    }
+}
+
 // twiddle + radix-R DFT; FIRST = stage with Ns = 1 (no twiddle)
 template <typename T, int N, int R, int TPL, bool INV, bool FIRST>
 __device__ __forceinline__ void bf_compute(BfRegs<T, N, R, TPL>& v, int u, int Ns, const T* __restrict__ ct)
@@ -230,13 +290,13 @@ __device__ __forceinline__ void bf_compute(BfRegs<T, N, R, TPL>& v, int u, int N
         int b = u + p * TPL;
         if (B::live(b)) {
             if constexpr (!FIRST) {
-                int k = b % Ns;
-                int step = B::M / (Ns * R);
+                const int e = 8 * (b % Ns) * (B::M / (Ns * R));
+                cpx<T> w[R];
+                tw_powers<T, N, R>(ct, e, w);
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    cpx<T> w = omega<T, N>(ct, 8 * k * r * step);
-                    if (INV) w.im = -w.im;
-                    v[p][r] = mul(v[p][r], w);
+                    if (INV) w[r].im = -w[r].im;
+                    v[p][r] = mul(v[p][r], w[r]);
                 }
             }
             dftR<T, R, INV>(v[p]);
@@ -353,41 +413,44 @@ __device__ __forceinline__ int mk_pos(int p)
 // DCT-III pre-twiddle: packed complex input z_k of the inverse FFT from the
 // four spectral values X_k, X_{N-k}, X_{M+k}, X_{M-k} (X_N = 0):
 //   v0 = conj(h_k)(X_k - i X_{N-k}),  v1 = conj(h_{k+M})(X_{M+k} - i X_{M-k}),
-//   z_k = (v0 + v1) + i (v0 - v1) conj(s_k).
+//   z_k = (v0 + v1) + i (v0 - v1) conj(s_k),
+// with h = h_k = omega^k, s = s_k = omega^{4k} given, h_{k+M} = h_k e^{-i pi/4}.
 template <typename T, int N>
-__device__ __forceinline__ cpx<T> dct3_pre(const T* lr, int k, const T* __restrict__ ct)
+__device__ __forceinline__ cpx<T> dct3_pre(const T* lr, int k, cpx<T> h, cpx<T> s)
 {
     constexpr int M = N / 2;
     T xk = lr[PR(k)];
     T xnk = (k == 0) ? T(0) : lr[PR(N - k)];
     T xpk = lr[PR(M + k)];
     T xmk = lr[PR(M - k)];
-    cpx<T> v0 = mul(conj(omega<T, N>(ct, k)), mk(xk, -xnk));
-    cpx<T> v1 = mul(conj(omega<T, N>(ct, k + M)), mk(xpk, -xmk));
+    const cpx<T> hc = conj(h);
+    const cpx<T> hm = mul(hc, conj(cis96<T>(24)));
+    cpx<T> v0 = mul(hc, mk(xk, -xnk));
+    cpx<T> v1 = mul(hm, mk(xpk, -xmk));
     cpx<T> E = add(v0, v1);
-    cpx<T> O = mul(sub(v0, v1), conj(omega<T, N>(ct, 4 * k)));
+    cpx<T> O = mul(sub(v0, v1), conj(s));
     return mk(E.re - O.im, E.im + O.re);  // E + i O
 }
 
 // DCT-II real split + post-twiddle of output pair (Z_k, Z_{M-k}):
 //   A = Z_k + conj(Z_{M-k}),  B = Z_k - conj(Z_{M-k}),
-//   V_k = A/2 - i s_k B/2,  Y_k = 2 Re(h_k V_k),  Y_{N-k} = -2 Im(h_k V_k).
-// For k = 0 the second value is Y_M = 2 Re(h_M (conj(A)/2 - i conj(s_0 B)/2)).
+//   V_k = A/2 - i s_k B/2,  Y_k = 2 Re(h_k V_k),  Y_{N-k} = -2 Im(h_k V_k),
+// with h = h_k, s = s_k given.  For k = 0 the second value is
+// Y_M = 2 Re(h_M (conj(A)/2 - i conj(s_0 B)/2)), h_M = e^{-i pi/4}.
 template <typename T, int N>
-__device__ __forceinline__ void dct2_post(cpx<T> zk, cpx<T> zm, int k, const T* __restrict__ ct, T& y0, T& y1)
+__device__ __forceinline__ void dct2_post(cpx<T> zk, cpx<T> zm, bool k0, cpx<T> h, cpx<T> s, T& y0, T& y1)
 {
-    constexpr int M = N / 2;
     cpx<T> A = mk(zk.re + zm.re, zk.im - zm.im);
     cpx<T> B = mk(zk.re - zm.re, zk.im + zm.im);
-    cpx<T> sB = mul(omega<T, N>(ct, 4 * k), B);
-    if (k != 0) {
-        cpx<T> a = mul(omega<T, N>(ct, k), mk(T(0.5) * (A.re + sB.im), T(0.5) * (A.im - sB.re)));
+    cpx<T> sB = mul(s, B);
+    cpx<T> vk = mk(T(0.5) * (A.re + sB.im), T(0.5) * (A.im - sB.re));
+    if (!k0) {
+        cpx<T> a = mul(h, vk);
         y0 = T(2) * a.re;
         y1 = T(-2) * a.im;
     } else {
-        cpx<T> a = mk(T(0.5) * (A.re + sB.im), T(0.5) * (A.im - sB.re));
-        cpx<T> bm = mul(omega<T, N>(ct, M), mk(T(0.5) * (A.re - sB.im), T(-0.5) * (A.im + sB.re)));
-        y0 = T(2) * a.re;
+        cpx<T> bm = mul(cis96<T>(24), mk(T(0.5) * (A.re - sB.im), T(-0.5) * (A.im + sB.re)));
+        y0 = T(2) * vk.re;
         y1 = T(2) * bm.re;
     }
 }
@@ -403,8 +466,13 @@ __device__ __forceinline__ void dct3_first_stage(T* lr, int u, const T* __restri
     for (int p = 0; p < B::PER; ++p) {
         int b = u + p * TPL;
         if (B::live(b)) {
+            // element r of butterfly b is k = b + r M/R: h_k = h_b e^{-i pi r/4R},
+            // s_k = s_b e^{-i pi r/R}
+            const cpx<T> hb = omega<T, N>(ct, b), sb = omega<T, N>(ct, 4 * b);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[p][r] = dct3_pre<T, N>(lr, b + r * B::NB, ct);
+            for (int r = 0; r < R; ++r)
+                v[p][r] = dct3_pre<T, N>(lr, b + r * B::NB, mul(hb, cis96<T>(r * (24 / R))),
+                                         mul(sb, cis96<T>(r * (96 / R))));
         }
     }
     lsync<WAVE>();
@@ -437,6 +505,7 @@ __device__ __forceinline__ void dct2_last_split(T* lr, int u, const BfRegs<T, N,
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
         const int b = u + p * TPL;
+        const cpx<T> hb = omega<T, N>(ct, b), sb = omega<T, N>(ct, 4 * b);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int k = b + r * B::NB;
@@ -450,7 +519,8 @@ __device__ __forceinline__ void dct2_last_split(T* lr, int u, const BfRegs<T, N,
                 zm = (p == 0) ? v[0][(R - r) % R] : v[pp][R - 1 - r];
             }
             T y0, y1;
-            dct2_post<T, N>(v[p][r], zm, k, ct, y0, y1);
+            dct2_post<T, N>(v[p][r], zm, k == 0, mul(hb, cis96<T>(r * (24 / R))),
+                            mul(sb, cis96<T>(r * (96 / R))), y0, y1);
             lr[PR(k)] = y0;
             lr[PR(k == 0 ? N / 2 : N - k)] = y1;
         }

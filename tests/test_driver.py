@@ -1,0 +1,90 @@
+"""smadft_run, the host driver with the reference programs' loop control
+(no-adv/cosNoAdvTrackMult.cpp:692-850): argument handling on the CPU, and on
+the GPU the L1.dat / psi.dat / psi.bin it writes against the library driven
+directly through the ABI."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "sma-dft_amd", "smadft_run")
+
+
+def run(args, **kw):
+    if not os.path.exists(EXE):
+        pytest.fail("smadft_run is not built (make -C sma-dft_amd)")
+    return subprocess.run([EXE] + args, capture_output=True, text=True, timeout=300, **kw)
+
+
+@pytest.mark.parametrize("args", [[], ["nosuch"], ["adv", "1", "0.7"], ["quasi", "1"],
+                                  ["plain", "--bogus"], ["plain", "--n"], ["plain", "--steps", "x"],
+                                  ["plain", "--steps", "0"]])
+def test_usage_errors(args):
+    r = run(args)
+    assert r.returncode == 2
+    assert "usage" in r.stderr
+
+
+def read_bin(path):
+    with open(path, "rb") as f:
+        assert f.read(8) == b"SMADPSI1"
+        nx, ny, nz = struct.unpack("<3i", f.read(12))
+        (steps,) = struct.unpack("<q", f.read(8))
+        data = np.frombuffer(f.read(), dtype=np.float64)
+    assert data.size == nx * ny * nz
+    return (nx, ny, nz), steps, data.reshape(nx, ny, nz)
+
+
+@pytest.mark.gpu
+def test_plain_run_matches_library(smadft, tmp_path):
+    out = tmp_path / "run"
+    r = run(["plain", "--n", "32", "--steps", "300", "--save-every", "2", "--ic", "slab",
+             "--amp", "1.328", "--noise", "1e-3", "--seed", "3", "--out", str(out)])
+    assert r.returncode == 0, r.stderr
+    l1 = np.loadtxt(out / "L1.dat")
+    assert l1.shape == (3,)
+    dims, steps, psi = read_bin(out / "psi.bin")
+    assert dims == (32, 32, 32) and steps == 300
+    txt = np.loadtxt(out / "psi.dat")
+    assert np.allclose(txt, psi.ravel(), rtol=1e-5, atol=1e-6)  # 6 significant digits
+    p = smadft.Params.default(smadft.MODEL_PLAIN, 32)
+    psi0 = smadft.ic_slab(p, 1.328, 1e-3, seed=3)
+    with smadft.Stepper(p, psi0) as s:
+        ref = []
+        for _ in range(3):
+            s.step(100)
+            ref.append(s.l1())
+        ref_psi = s.fetch()
+    assert np.allclose(l1, ref, rtol=1e-5)
+    assert rel_l2(psi, ref_psi) < 1e-13
+
+
+@pytest.mark.gpu
+def test_plain_restart_and_l1_stop(smadft, tmp_path):
+    out = tmp_path / "run"
+    r = run(["plain", "--n", "16", "--steps", "200", "--ic", "slab", "--amp", "1.328",
+             "--out", str(out)])
+    assert r.returncode == 0, r.stderr
+    _, steps, psi200 = read_bin(out / "psi.bin")
+    assert steps == 200
+    # restart re-runs the pre-loop from psi, as the reference's load path
+    r = run(["plain", "--n", "16", "--steps", "100", "--restart", str(out / "psi.bin"),
+             "--out", str(tmp_path / "cont")])
+    assert r.returncode == 0, r.stderr
+    _, steps, psi300 = read_bin(tmp_path / "cont" / "psi.bin")
+    assert steps == 300
+    p = smadft.Params.default(smadft.MODEL_PLAIN, 16)
+    with smadft.Stepper(p, psi200.copy()) as s:
+        s.step(100)
+        assert rel_l2(psi300, s.fetch()) < 1e-13
+    # L1-stop: runs until L1 <= limit, last line of L1.dat below it
+    r = run(["plain", "--n", "16", "--l1", "1e-3", "--max-steps", "20000", "--ic", "slab",
+             "--amp", "1.328", "--out", str(tmp_path / "stop")])
+    assert r.returncode == 0, r.stderr
+    l1 = np.atleast_1d(np.loadtxt(tmp_path / "stop" / "L1.dat"))
+    assert l1[-1] <= 1e-3 and (l1[:-1] > 1e-3).all()
