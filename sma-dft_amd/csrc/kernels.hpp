@@ -25,13 +25,25 @@ struct Lines {
     static constexpr int TPL = WG / NL;                     // threads per line
 };
 // z lines are contiguous, so their tile width is free of coalescing
-// constraints: 16 threads per line in both precisions (every radix stage's
-// butterflies stay busy for N = 256), 16 lines per 256-thread workgroup in
-// FP64 and 32 lines per 512-thread workgroup in FP32.
+// constraints: 512-thread workgroups, 32 threads per line in FP64 (16 lines;
+// half the per-thread registers of 16 threads per line) and 16 in FP32 (32
+// lines).
+#ifndef SMAD_ZWG64
+#define SMAD_ZWG64 512
+#endif
+#ifndef SMAD_ZTPL64
+#define SMAD_ZTPL64 32
+#endif
+#ifndef SMAD_ZWG32
+#define SMAD_ZWG32 512
+#endif
+#ifndef SMAD_ZTPL32
+#define SMAD_ZTPL32 16
+#endif
 template <typename T>
 struct LinesZ {
-    static constexpr int WGZ = sizeof(T) == 8 ? 256 : 512;
-    static constexpr int TPL = 16;
+    static constexpr int WGZ = sizeof(T) == 8 ? SMAD_ZWG64 : SMAD_ZWG32;
+    static constexpr int TPL = sizeof(T) == 8 ? SMAD_ZTPL64 : SMAD_ZTPL32;
     static constexpr int NL = WGZ / TPL;
 };
 
@@ -565,6 +577,210 @@ __global__ __launch_bounds__(WG) void k_adv_nl(const T* __restrict__ psi, const 
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* out);
 
+__device__ __forceinline__ double mq2_at(const PlainCoef& cf, int i, int j, int k);
+
+// ------------------------------------------------------------------------
+// Multi-output axis passes.  Several 1D transforms of fields that differ
+// only by a spectral multiplier or by a DST-shift along the pass axis share
+// their other axis passes, e.g. psi = REDFT01^3(psihat) and
+// grad_x psi = (RODFT01_x o shift_x)(REDFT01_y REDFT01_z psihat): one pass
+// reads its input tile once from HBM and writes every output, re-reading the
+// tile from cache for the second and third output.
+enum PassOp : int {
+    OP_ID = 0,     // x
+    OP_SHIFT = 1,  // DST-shift gradient along the pass axis: y[p] = -(p+1) qf x[p+1], y[N-1] = 0
+    OP_ALIN = 2,   // -aLin(|q|) x, aLin = eps - alpha (|q|^2 - q0^2)^2
+};
+
+template <typename T>
+struct PassSet {
+    T* out[3];
+    int kind[3];
+    int op[3];
+    int n;
+};
+
+// Input sample p (value x) of a line through spectral point (i, j, k): LDS
+// position q for a transform of `kind` and the value stored there.
+template <typename T, int N, int OP>
+__device__ __forceinline__ T pass_in(int kind, double qf, const PlainCoef& cf, int p, T x, int i, int j, int k,
+                                     int& q)
+{
+    if constexpr (OP == OP_SHIFT) {
+        // sample p feeds output sample p-1; sample 0 instead zeroes sample N-1
+        if (p == 0) {
+            q = in_pos<N>(kind, N - 1);
+            return T(0);
+        }
+        q = in_pos<N>(kind, p - 1);
+        T v = T(-p * qf * (double)x);
+        return in_neg(kind, p - 1) ? -v : v;
+    } else {
+        q = in_pos<N>(kind, p);
+        T v = x;
+        if constexpr (OP == OP_ALIN) {
+            double t = mq2_at(cf, i, j, k) - cf.q0sq;
+            v = T(-(cf.eps - cf.alpha * t * t) * (double)x);
+        }
+        return in_neg(kind, p) ? -v : v;
+    }
+}
+
+// One output of a z pass: load with op OP, transform, store.
+template <typename T, int N, int OP>
+__device__ __forceinline__ void zmulti_one(const T* in, T* out, int kind, const Dims& d, const PlainCoef& cf,
+                                           T* lds, const T* ct, int l, int u)
+{
+    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int WGZ = LinesZ<T>::WGZ;
+    constexpr int W = V16<T>::W;
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+        V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
+        int i = 0, j = 0;
+        if constexpr (OP == OP_ALIN) {
+            const long L = gi / N;
+            i = (int)(L / d.ny);
+            j = (int)(L - (long)i * d.ny);
+        }
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            int q;
+            T y = pass_in<T, N, OP>(kind, cf.qfz, cf, p + t, v.v[t], i, j, p + t, q);
+            lds[c * LS + PR(q)] = y;
+        }
+    });
+    lsync<true>();
+    r2r_line<T, N, TPL, true>(lds + l * LS, u, kind, ct);
+    OutMap<T, N> om{kind};
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+        V16<T> v;
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            int q;
+            T sg = om(p + t, q);
+            v.v[t] = sg * lds[c * LS + PR(q)];
+        }
+        *reinterpret_cast<V16<T>*>(out + gi) = v;
+    });
+    lsync<true>();
+}
+
+template <typename T, int N, int... OPS>
+__device__ __forceinline__ void zmulti_body(const T* in, const PassSet<T>& ps, const Dims& d,
+                                            const PlainCoef& cf, const Tw<T>& tw)
+{
+    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    __syncthreads();
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    int o = 0;
+    ((zmulti_one<T, N, OPS>(in, ps.out[o], ps.kind[o], d, cf, lds, ct, l, u), ++o), ...);
+}
+
+// psihat -> (REDFT01_z psihat, DST-shift-z output, -aLin output): the z pass
+// of the advective step's inverse transforms.
+template <typename T, int N>
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_zmulti(const T* in, PassSet<T> ps, Dims d, PlainCoef cf,
+                                                           Tw<T> tw)
+{
+    zmulti_body<T, N, OP_ID, OP_SHIFT, OP_ALIN>(in, ps, d, cf, tw);
+}
+
+// One output of a strided pass.
+template <typename T, int N, int OP>
+__device__ __forceinline__ void multi_one(const T* in, T* out, int kind, const Dims& d, int axis,
+                                          const TileGeo& g, double qf, const PlainCoef& cf, T* lds,
+                                          const T* ct, int l, int u)
+{
+    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
+    constexpr int W = V16<T>::W;
+    __syncthreads();
+    if (g.ncol == NL) {
+        rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
+            V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                int q;
+                T y = pass_in<T, N, OP>(kind, qf, cf, p, v.v[t], 0, 0, 0, q);
+                lds[(c0 + t) * LS + PR(q)] = y;
+            }
+        });
+    } else {
+        tile_for<N, NL>(g, 1, [&](int p, int c, long gi, bool ok) {
+            int q;
+            T y = pass_in<T, N, OP>(kind, qf, cf, p, ok ? in[gi] : T(0), 0, 0, 0, q);
+            lds[c * LS + PR(q)] = ok ? y : T(0);
+        });
+    }
+    __syncthreads();
+    r2r_line<T, N, TPL, true>(lds + l * LS, u, kind, ct);
+    __syncthreads();
+    store_rows<T, N, NL, LS>(g, out, lds, OutMap<T, N>{kind}, T(1));
+}
+
+// Strided axes (0 = x, 1 = y): (DST-shift output, plain output).  The plain
+// output may alias `in` (it is written last).
+template <typename T, int N>
+__global__ __launch_bounds__(WG) void k_multi(const T* in, PassSet<T> ps, Dims d, int axis, PlainCoef cf,
+                                              Tw<T> tw)
+{
+    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
+    __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    const TileGeo g = tile_geo(d, axis, blockIdx.x, NL);
+    const double qf = axis == 0 ? cf.qfx : cf.qfy;
+    multi_one<T, N, OP_SHIFT>(in, ps.out[0], ps.kind[0], d, axis, g, qf, cf, lds, ct, l, u);
+    multi_one<T, N, OP_ID>(in, ps.out[1], ps.kind[1], d, axis, g, qf, cf, lds, ct, l, u);
+}
+
+// REDFT10_z of Nr = beta psi^3 - gamma psi^5 - u . grad psi (:1290-1307),
+// the nonlinear term formed while the z lines are loaded.
+template <typename T, int N>
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_nl_z(const T* __restrict__ psi, const T* __restrict__ ux,
+                                                             const T* __restrict__ uy, const T* __restrict__ uz,
+                                                             const T* __restrict__ gx, const T* __restrict__ gy,
+                                                             const T* __restrict__ gz, T* __restrict__ out,
+                                                             Dims d, AdvCoef ac, Tw<T> tw)
+{
+    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int WGZ = LinesZ<T>::WGZ;
+    constexpr int W = V16<T>::W;
+    __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    __syncthreads();
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+        const V16<T> a = *reinterpret_cast<const V16<T>*>(psi + gi);
+        const V16<T> vx = *reinterpret_cast<const V16<T>*>(ux + gi);
+        const V16<T> vy = *reinterpret_cast<const V16<T>*>(uy + gi);
+        const V16<T> vz = *reinterpret_cast<const V16<T>*>(uz + gi);
+        const V16<T> hx = *reinterpret_cast<const V16<T>*>(gx + gi);
+        const V16<T> hy = *reinterpret_cast<const V16<T>*>(gy + gi);
+        const V16<T> hz = *reinterpret_cast<const V16<T>*>(gz + gi);
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            double x = (double)a.v[t], x2 = x * x, x3 = x2 * x;
+            double v = ac.pc.beta * x3 - ac.pc.gamma * x3 * x2 - (double)vx.v[t] * (double)hx.v[t] -
+                       (double)vy.v[t] * (double)hy.v[t] - (double)vz.v[t] * (double)hz.v[t];
+            lds[c * LS + PR(mk_pos<N>(p + t))] = T(v);
+        }
+    });
+    lsync<true>();
+    dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+        V16<T> v;
+#pragma unroll
+        for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + PR(p + t)];
+        *reinterpret_cast<V16<T>*>(out + gi) = v;
+    });
+}
+
+
 // ------------------------------------------------------------------------
 // Quasi-incompressible model (quasi/quasi.cpp:1099-2251), pointwise and
 // spectral stages on the global grid.  Arithmetic in FP64; the pressure
@@ -1043,6 +1259,13 @@ struct Launch {
                            T* uz, Dims d, const AdvCoef& ac);
     static void adv_nl(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
                        const T* gx, const T* gy, const T* gz, T* nr, Dims d, const AdvCoef& ac);
+    static void zmulti(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, const PlainCoef& cf,
+                       const Tw<T>& tw);
+    static void multi(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, int axis,
+                      const PlainCoef& cf, const Tw<T>& tw);
+    static void adv_nl_z(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
+                         const T* gx, const T* gy, const T* gz, T* out, Dims d, const AdvCoef& ac,
+                         const Tw<T>& tw);
     static void adj_x(hipStream_t s, T* w, T* lbuf, const T* ph, Dims d, const PlainCoef& cf,
                       int mode, double* acc, const Tw<T>& tw);
     static void adj_z(hipStream_t s, T* w, const T* psi_tape, Dims d, const PlainCoef& cf,

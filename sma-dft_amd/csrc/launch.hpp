@@ -5,6 +5,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <stdexcept>
 
 namespace smad {
 
@@ -71,6 +72,35 @@ void Launch<T>::adv_spec(hipStream_t s, const T* ph, T* gx, T* gy, T* gz, T* sp,
                          const AdvCoef& ac)
 {
     hipLaunchKernelGGL(k_adv_spec<T>, flat_grid(d.total()), dim3(WG), 0, s, ph, gx, gy, gz, sp, d, ac);
+}
+
+template <typename T>
+void Launch<T>::zmulti(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, const PlainCoef& cf,
+                       const Tw<T>& tw)
+{
+    if (ps.n != 3 || ps.op[0] != OP_ID || ps.op[1] != OP_SHIFT || ps.op[2] != OP_ALIN)
+        throw std::logic_error("zmulti: unsupported output set");
+    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+    SMAD_DISPATCH_Z(d.nz, k_zmulti, grid, in, ps, d, cf, tw);
+}
+
+template <typename T>
+void Launch<T>::multi(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, int axis,
+                      const PlainCoef& cf, const Tw<T>& tw)
+{
+    if (ps.n != 2 || ps.op[0] != OP_SHIFT || ps.op[1] != OP_ID)
+        throw std::logic_error("multi: unsupported output set");
+    dim3 grid(num_tiles(d, axis, Lines<T>::NL));
+    SMAD_DISPATCH(axis_len(d, axis), k_multi, grid, in, ps, d, axis, cf, tw);
+}
+
+template <typename T>
+void Launch<T>::adv_nl_z(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
+                         const T* gx, const T* gy, const T* gz, T* out, Dims d, const AdvCoef& ac,
+                         const Tw<T>& tw)
+{
+    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+    SMAD_DISPATCH_Z(d.nz, k_adv_nl_z, grid, psi, ux, uy, uz, gx, gy, gz, out, d, ac, tw);
 }
 
 template <typename T>

@@ -89,6 +89,24 @@ struct TwTable {
     }
 };
 
+template <typename T>
+PassSet<T> pass2(T* o0, int k0, int op0, T* o1, int k1, int op1)
+{
+    PassSet<T> ps{};
+    ps.out[0] = o0; ps.kind[0] = k0; ps.op[0] = op0;
+    ps.out[1] = o1; ps.kind[1] = k1; ps.op[1] = op1;
+    ps.n = 2;
+    return ps;
+}
+template <typename T>
+PassSet<T> pass3(T* o0, int k0, int op0, T* o1, int k1, int op1, T* o2, int k2, int op2)
+{
+    PassSet<T> ps = pass2(o0, k0, op0, o1, k1, op1);
+    ps.out[2] = o2; ps.kind[2] = k2; ps.op[2] = op2;
+    ps.n = 3;
+    return ps;
+}
+
 struct Prof;
 
 // Device time between two points on a stream.
@@ -570,8 +588,9 @@ struct AdvEngine : EngineBase {
 
     const char* kernel_class(int c) const override
     {
-        static const char* names[] = {"axis passes (r2r)", "adv_spec", "adv_force", "adv_stokes",
-                                      "adv_nl", "plain_x: REDFT10_x + CN/AB2 update"};
+        static const char* names[] = {"axis passes (r2r)", "inverse passes: psi, grad psi, Sp2",
+                                      "adv_force", "adv_stokes", "adv_nl + REDFT10_z",
+                                      "plain_x: REDFT10_x + CN/AB2 update"};
         return (c >= 0 && c < 6) ? names[c] : nullptr;
     }
 
@@ -604,16 +623,21 @@ struct AdvEngine : EngineBase {
     void step_one(bool keep_prev, Prof* prof)
     {
         auto mk = [&](int c) { if (prof) prof->mark(c); };
-        Launch<T>::adv_spec(s, ph.p, gx.p, gy.p, gz.p, sp.p, d, ac);
+        // psi, grad psi and Sp2 = REDFT01^3(-aLin psihat) (:847-991) with
+        // shared passes: z from psihat, y from B = REDFT01_z psihat, x from
+        // A = REDFT01_y B
+        const PlainCoef& cf = ac.pc;
+        Launch<T>::zmulti(s, ph.p, pass3(psi.p, REDFT01, OP_ID, gz.p, RODFT01, OP_SHIFT, sp.p, REDFT01, OP_ALIN),
+                          d, cf, twz.tw);
+        Launch<T>::multi(s, psi.p, pass2(gy.p, RODFT01, OP_SHIFT, psi.p, REDFT01, OP_ID), d, 1, cf, twy.tw);
+        Launch<T>::axis(s, gz.p, gz.p, d, 1, REDFT01, T(1), twy.tw);
+        Launch<T>::axis(s, sp.p, sp.p, d, 1, REDFT01, T(1), twy.tw);
+        Launch<T>::multi(s, psi.p, pass2(gx.p, RODFT01, OP_SHIFT, psi.p, REDFT01, OP_ID), d, 0, cf, twx.tw);
+        Launch<T>::axis(s, gy.p, gy.p, d, 0, REDFT01, T(1), twx.tw);
+        Launch<T>::axis(s, gz.p, gz.p, d, 0, REDFT01, T(1), twx.tw);
+        Launch<T>::axis(s, sp.p, sp.p, d, 0, REDFT01, T(1), twx.tw);
         mk(1);
-        tr3(ph.p, psi.p, REDFT01, REDFT01, REDFT01);
-        mk(0);
         if (keep_prev) HIPCHK(hipMemcpyAsync(psi_prev.p, psi.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-        tr3(gx.p, gx.p, RODFT01, REDFT01, REDFT01);
-        tr3(gy.p, gy.p, REDFT01, RODFT01, REDFT01);
-        tr3(gz.p, gz.p, REDFT01, REDFT01, RODFT01);
-        tr3(sp.p, sp.p, REDFT01, REDFT01, REDFT01);
-        mk(0);
         Launch<T>::adv_force(s, psi.p, sp.p, gx.p, gy.p, gz.p, sx.p, sy.p, sz.p, d, ac);
         mk(2);
         tr3(sx.p, sx.p, RODFT10, REDFT10, REDFT10);
@@ -626,9 +650,8 @@ struct AdvEngine : EngineBase {
         tr3(uy.p, uy.p, REDFT01, RODFT01, REDFT01);
         tr3(uz.p, uz.p, REDFT01, REDFT01, RODFT01);
         mk(0);
-        Launch<T>::adv_nl(s, psi.p, ux.p, uy.p, uz.p, gx.p, gy.p, gz.p, nr.p, d, ac);
+        Launch<T>::adv_nl_z(s, psi.p, ux.p, uy.p, uz.p, gx.p, gy.p, gz.p, nr.p, d, ac, twz.tw);
         mk(4);
-        Launch<T>::axis(s, nr.p, nr.p, d, 2, REDFT10, T(1), twz.tw);
         Launch<T>::axis(s, nr.p, nr.p, d, 1, REDFT10, T(1), twy.tw);
         mk(0);
         Launch<T>::plain_x(s, nr.p, nr.p, ph.p, ph.p, nhat.p, d, ac.pc, twx.tw, 0);
