@@ -471,29 +471,6 @@ __device__ __forceinline__ void unflat(long id, const Dims& d, int& i, int& j, i
     i = (int)(r / d.ny);
 }
 
-// From the scaled spectral psi: DST-shift gradients (:866-959) and the
-// linear part of mu, -aLin psihat (:980-991).
-template <typename T>
-__global__ __launch_bounds__(WG) void k_adv_spec(const T* __restrict__ ph, T* __restrict__ gx,
-                                                 T* __restrict__ gy, T* __restrict__ gz,
-                                                 T* __restrict__ sp, Dims d, AdvCoef ac)
-{
-    const PlainCoef& cf = ac.pc;
-    long n = d.total();
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        int i, j, k;
-        unflat(id, d, i, j, k);
-        const long sx = (long)d.ny * d.nz, sy = d.nz;
-        gx[id] = i < d.nx - 1 ? T(-(i + 1) * cf.qfx * (double)ph[id + sx]) : T(0);
-        gy[id] = j < d.ny - 1 ? T(-(j + 1) * cf.qfy * (double)ph[id + sy]) : T(0);
-        gz[id] = k < d.nz - 1 ? T(-(k + 1) * cf.qfz * (double)ph[id + 1]) : T(0);
-        double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
-        double t = qx * qx + qy * qy + qz * qz - cf.q0sq;
-        double al = cf.eps - cf.alpha * t * t;
-        sp[id] = T(-al * (double)ph[id]);
-    }
-}
-
 // S = mu grad psi, mu = Sp2 - beta psi^3 + gamma psi^5 (:996-1015); S
 // overwrites grad psi's buffers only through separate outputs.
 template <typename T>
@@ -558,22 +535,6 @@ __global__ __launch_bounds__(WG) void k_adv_stokes(const T* __restrict__ Sx, con
     }
 }
 
-// Nr = beta psi^3 - gamma psi^5 - u . grad psi (:1290-1303)
-template <typename T>
-__global__ __launch_bounds__(WG) void k_adv_nl(const T* __restrict__ psi, const T* __restrict__ ux,
-                                               const T* __restrict__ uy, const T* __restrict__ uz,
-                                               const T* __restrict__ gx, const T* __restrict__ gy,
-                                               const T* __restrict__ gz, T* __restrict__ nr, long n,
-                                               AdvCoef ac)
-{
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        double x = (double)psi[id], x2 = x * x, x3 = x2 * x;
-        double v = ac.pc.beta * x3 - ac.pc.gamma * x3 * x2 - (double)ux[id] * (double)gx[id] -
-                   (double)uy[id] * (double)gy[id] - (double)uz[id] * (double)gz[id];
-        nr[id] = T(v);
-    }
-}
-
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* out);
 
@@ -587,48 +548,95 @@ __device__ __forceinline__ double mq2_at(const PlainCoef& cf, int i, int j, int 
 // reads its input tile once from HBM and writes every output, re-reading the
 // tile from cache for the second and third output.
 enum PassOp : int {
-    OP_ID = 0,     // x
-    OP_SHIFT = 1,  // DST-shift gradient along the pass axis: y[p] = -(p+1) qf x[p+1], y[N-1] = 0
-    OP_ALIN = 2,   // -aLin(|q|) x, aLin = eps - alpha (|q|^2 - q0^2)^2
+    OP_ID = 0,          // x
+    OP_SHIFT = 1,       // DST-shift gradient along the pass axis: y[p] = -(p+1) qf x[p+1], y[N-1] = 0
+    OP_ALIN = 2,        // -aLin(|q|) x, aLin = eps - alpha (|q|^2 - q0^2)^2
+    OP_FQ = 3,          // alpha (q0^2 - |q|^2) x                  (quasi f_hat)
+    OP_LAPF = 4,        // -|q|^2 alpha (q0^2 - |q|^2) x           (quasi lap_hat before the switch)
+    OP_FILT = 5,        // s exp(-2 sig^2 |q|^2) x                 (quasi rho filter)
+    OP_FILT_SHIFT = 6,  // OP_SHIFT of the OP_FILT-filtered input
+    OP_NMQ2 = 7,        // -s |q|^2 x                              (quasi laplacian of rho f)
+};
+
+// Op sets of the z passes with several outputs.
+enum ZSet : int {
+    ZS_ADV = 0,       // (OP_ID, OP_SHIFT, OP_ALIN): psi, grad_z psi, Sp2 of the advective step
+    ZS_QPSI = 1,      // (OP_ID, OP_SHIFT, OP_FQ): psi, grad_z psi, f of the quasi step
+    ZS_QPSI_LAP = 2,  // ZS_QPSI + OP_LAPF (before the divv switch)
+    ZS_QRHO = 3,      // (OP_FILT, OP_FILT_SHIFT): filtered rho and grad_z rho
+    ZS_NMQ2 = 4,      // (OP_NMQ2): laplacian of rho f
+};
+
+// Coefficients of the spectral multipliers.
+struct SpecCoef {
+    PlainCoef pc;
+    double sig2;  // quasi filter width squared
 };
 
 template <typename T>
 struct PassSet {
-    T* out[3];
-    int kind[3];
-    int op[3];
+    T* out[4];
+    int kind[4];
+    int op[4];
     int n;
 };
 
 // Input sample p (value x) of a line through spectral point (i, j, k): LDS
 // position q for a transform of `kind` and the value stored there.
+template <int OP>
+__device__ __forceinline__ double spec_mult(const SpecCoef& sc, int i, int j, int k, double x)
+{
+    const PlainCoef& cf = sc.pc;
+    const double m2 = mq2_at(cf, i, j, k);
+    if constexpr (OP == OP_ALIN) {
+        double t = m2 - cf.q0sq;
+        return -(cf.eps - cf.alpha * t * t) * x;
+    } else if constexpr (OP == OP_FQ) {
+        return cf.alpha * (cf.q0sq - m2) * x;
+    } else if constexpr (OP == OP_LAPF) {
+        return -m2 * cf.alpha * (cf.q0sq - m2) * x;
+    } else if constexpr (OP == OP_FILT || OP == OP_FILT_SHIFT) {
+        return cf.scale * exp(-sc.sig2 * m2 * 2.0) * x;
+    } else {
+        static_assert(OP == OP_NMQ2, "no multiplier for this op");
+        return -cf.scale * m2 * x;
+    }
+}
+
+// Input sample p (value x) of a line through spectral point (i, j, k): LDS
+// position q for a transform of `kind` and the value stored there.
 template <typename T, int N, int OP>
-__device__ __forceinline__ T pass_in(int kind, double qf, const PlainCoef& cf, int p, T x, int i, int j, int k,
+__device__ __forceinline__ T pass_in(int kind, double qf, const SpecCoef& sc, int p, T x, int i, int j, int k,
                                      int& q)
 {
-    if constexpr (OP == OP_SHIFT) {
+    if constexpr (OP == OP_SHIFT || OP == OP_FILT_SHIFT) {
         // sample p feeds output sample p-1; sample 0 instead zeroes sample N-1
         if (p == 0) {
             q = in_pos<N>(kind, N - 1);
             return T(0);
         }
         q = in_pos<N>(kind, p - 1);
-        T v = T(-p * qf * (double)x);
+        double xv = (double)x;
+        if constexpr (OP == OP_FILT_SHIFT) xv = (double)T(spec_mult<OP_FILT>(sc, i, j, k, xv));
+        T v = T(-p * qf * xv);
         return in_neg(kind, p - 1) ? -v : v;
     } else {
         q = in_pos<N>(kind, p);
         T v = x;
-        if constexpr (OP == OP_ALIN) {
-            double t = mq2_at(cf, i, j, k) - cf.q0sq;
-            v = T(-(cf.eps - cf.alpha * t * t) * (double)x);
-        }
+        if constexpr (OP != OP_ID) v = T(spec_mult<OP>(sc, i, j, k, (double)x));
         return in_neg(kind, p) ? -v : v;
     }
 }
 
+template <int OP>
+constexpr bool needs_ijk()
+{
+    return OP != OP_ID && OP != OP_SHIFT;
+}
+
 // One output of a z pass: load with op OP, transform, store.
 template <typename T, int N, int OP>
-__device__ __forceinline__ void zmulti_one(const T* in, T* out, int kind, const Dims& d, const PlainCoef& cf,
+__device__ __forceinline__ void zmulti_one(const T* in, T* out, int kind, const Dims& d, const SpecCoef& sc,
                                            T* lds, const T* ct, int l, int u)
 {
     constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
@@ -637,7 +645,7 @@ __device__ __forceinline__ void zmulti_one(const T* in, T* out, int kind, const 
     zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
         int i = 0, j = 0;
-        if constexpr (OP == OP_ALIN) {
+        if constexpr (needs_ijk<OP>()) {
             const long L = gi / N;
             i = (int)(L / d.ny);
             j = (int)(L - (long)i * d.ny);
@@ -645,7 +653,7 @@ __device__ __forceinline__ void zmulti_one(const T* in, T* out, int kind, const 
 #pragma unroll
         for (int t = 0; t < W; ++t) {
             int q;
-            T y = pass_in<T, N, OP>(kind, cf.qfz, cf, p + t, v.v[t], i, j, p + t, q);
+            T y = pass_in<T, N, OP>(kind, sc.pc.qfz, sc, p + t, v.v[t], i, j, p + t, q);
             lds[c * LS + PR(q)] = y;
         }
     });
@@ -665,9 +673,11 @@ __device__ __forceinline__ void zmulti_one(const T* in, T* out, int kind, const 
     lsync<true>();
 }
 
+// z pass over one input with one output per op in OPS (out[o], kind[o]).
+// An output may alias `in` only as the last one.
 template <typename T, int N, int... OPS>
-__device__ __forceinline__ void zmulti_body(const T* in, const PassSet<T>& ps, const Dims& d,
-                                            const PlainCoef& cf, const Tw<T>& tw)
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_zmulti(const T* in, PassSet<T> ps, Dims d, SpecCoef sc,
+                                                           Tw<T> tw)
 {
     constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
     __shared__ T lds[NL * LS];
@@ -676,41 +686,39 @@ __device__ __forceinline__ void zmulti_body(const T* in, const PassSet<T>& ps, c
     __syncthreads();
     const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
     int o = 0;
-    ((zmulti_one<T, N, OPS>(in, ps.out[o], ps.kind[o], d, cf, lds, ct, l, u), ++o), ...);
-}
-
-// psihat -> (REDFT01_z psihat, DST-shift-z output, -aLin output): the z pass
-// of the advective step's inverse transforms.
-template <typename T, int N>
-__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_zmulti(const T* in, PassSet<T> ps, Dims d, PlainCoef cf,
-                                                           Tw<T> tw)
-{
-    zmulti_body<T, N, OP_ID, OP_SHIFT, OP_ALIN>(in, ps, d, cf, tw);
+    ((zmulti_one<T, N, OPS>(in, ps.out[o], ps.kind[o], d, sc, lds, ct, l, u), ++o), ...);
 }
 
 // One output of a strided pass.
 template <typename T, int N, int OP>
-__device__ __forceinline__ void multi_one(const T* in, T* out, int kind, const Dims& d, int axis,
-                                          const TileGeo& g, double qf, const PlainCoef& cf, T* lds,
-                                          const T* ct, int l, int u)
+__device__ __forceinline__ void multi_one(const T* in, T* out, int kind, int axis, const TileGeo& g, int outer,
+                                          int k0, double qf, const SpecCoef& sc, T* lds, const T* ct, int l,
+                                          int u)
 {
     constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
     constexpr int W = V16<T>::W;
+    auto ijk = [&](int p, int c, int& i, int& j, int& k) {
+        k = k0 + c;
+        if (axis == 0) { i = p; j = outer; }
+        else { i = outer; j = p; }
+    };
     __syncthreads();
     if (g.ncol == NL) {
         rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
             V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
 #pragma unroll
             for (int t = 0; t < W; ++t) {
-                int q;
-                T y = pass_in<T, N, OP>(kind, qf, cf, p, v.v[t], 0, 0, 0, q);
+                int i = 0, j = 0, k = 0, q;
+                if constexpr (needs_ijk<OP>()) ijk(p, c0 + t, i, j, k);
+                T y = pass_in<T, N, OP>(kind, qf, sc, p, v.v[t], i, j, k, q);
                 lds[(c0 + t) * LS + PR(q)] = y;
             }
         });
     } else {
         tile_for<N, NL>(g, 1, [&](int p, int c, long gi, bool ok) {
-            int q;
-            T y = pass_in<T, N, OP>(kind, qf, cf, p, ok ? in[gi] : T(0), 0, 0, 0, q);
+            int i = 0, j = 0, k = 0, q;
+            if constexpr (needs_ijk<OP>()) ijk(p, c, i, j, k);
+            T y = pass_in<T, N, OP>(kind, qf, sc, p, ok ? in[gi] : T(0), i, j, k, q);
             lds[c * LS + PR(q)] = ok ? y : T(0);
         });
     }
@@ -720,10 +728,10 @@ __device__ __forceinline__ void multi_one(const T* in, T* out, int kind, const D
     store_rows<T, N, NL, LS>(g, out, lds, OutMap<T, N>{kind}, T(1));
 }
 
-// Strided axes (0 = x, 1 = y): (DST-shift output, plain output).  The plain
-// output may alias `in` (it is written last).
-template <typename T, int N>
-__global__ __launch_bounds__(WG) void k_multi(const T* in, PassSet<T> ps, Dims d, int axis, PlainCoef cf,
+// Strided axes (0 = x, 1 = y), one output per op in OPS; an output may
+// alias `in` only as the last one.
+template <typename T, int N, int... OPS>
+__global__ __launch_bounds__(WG) void k_multi(const T* in, PassSet<T> ps, Dims d, int axis, SpecCoef sc,
                                               Tw<T> tw)
 {
     constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
@@ -732,9 +740,11 @@ __global__ __launch_bounds__(WG) void k_multi(const T* in, PassSet<T> ps, Dims d
     load_cosq<T, N>(ct, tw);
     const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
     const TileGeo g = tile_geo(d, axis, blockIdx.x, NL);
-    const double qf = axis == 0 ? cf.qfx : cf.qfy;
-    multi_one<T, N, OP_SHIFT>(in, ps.out[0], ps.kind[0], d, axis, g, qf, cf, lds, ct, l, u);
-    multi_one<T, N, OP_ID>(in, ps.out[1], ps.kind[1], d, axis, g, qf, cf, lds, ct, l, u);
+    const int nct = (d.nz + NL - 1) / NL;
+    const int outer = blockIdx.x / nct, k0 = (blockIdx.x % nct) * NL;
+    const double qf = axis == 0 ? sc.pc.qfx : sc.pc.qfy;
+    int o = 0;
+    ((multi_one<T, N, OPS>(in, ps.out[o], ps.kind[o], axis, g, outer, k0, qf, sc, lds, ct, l, u), ++o), ...);
 }
 
 // REDFT10_z of Nr = beta psi^3 - gamma psi^5 - u . grad psi (:1290-1307),
@@ -795,64 +805,6 @@ __device__ __forceinline__ double mq2_at(const PlainCoef& cf, int i, int j, int 
 {
     double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
     return qx * qx + qy * qy + qz * qz;
-}
-
-// f_hat = alpha (q0^2 - |q|^2) psihat (:1121-1131); DST-shift gradients of
-// psihat and of the previous filtered rhohat (:1147-1256); before the
-// switch also lap_hat = -|q|^2 alpha (q0^2 - |q|^2) psihat (:1411-1421).
-template <typename T>
-__global__ __launch_bounds__(WG) void k_q_spec(const T* __restrict__ ph, const T* __restrict__ rq,
-                                               T* __restrict__ fh, T* __restrict__ gpx,
-                                               T* __restrict__ gpy, T* __restrict__ gpz,
-                                               T* __restrict__ grx, T* __restrict__ gry,
-                                               T* __restrict__ grz, T* __restrict__ laph, Dims d,
-                                               QuasiCoef qc)
-{
-    const PlainCoef& cf = qc.pc;
-    const long n = d.total(), sx = (long)d.ny * d.nz, sy = d.nz;
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        int i, j, k;
-        unflat(id, d, i, j, k);
-        double m2 = mq2_at(cf, i, j, k);
-        double phv = (double)ph[id];
-        fh[id] = T(cf.alpha * (cf.q0sq - m2) * phv);
-        if (laph) laph[id] = T(-m2 * cf.alpha * (cf.q0sq - m2) * phv);
-        double ax = -(i + 1) * cf.qfx, ay = -(j + 1) * cf.qfy, az = -(k + 1) * cf.qfz;
-        bool bx = i < d.nx - 1, by = j < d.ny - 1, bz = k < d.nz - 1;
-        gpx[id] = bx ? T(ax * (double)ph[id + sx]) : T(0);
-        gpy[id] = by ? T(ay * (double)ph[id + sy]) : T(0);
-        gpz[id] = bz ? T(az * (double)ph[id + 1]) : T(0);
-        grx[id] = bx ? T(ax * (double)rq[id + sx]) : T(0);
-        gry[id] = by ? T(ay * (double)rq[id + sy]) : T(0);
-        grz[id] = bz ? T(az * (double)rq[id + 1]) : T(0);
-    }
-}
-
-// rho = kp sqrt(q0^2 psi^2 + |grad psi|^2) + rho0 (:1322-1340)
-template <typename T>
-__global__ __launch_bounds__(WG) void k_q_rho(const T* __restrict__ psi, const T* __restrict__ gx,
-                                              const T* __restrict__ gy, const T* __restrict__ gz,
-                                              T* __restrict__ rho, long n, QuasiCoef qc)
-{
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        double x = (double)psi[id], a = (double)gx[id], b = (double)gy[id], c = (double)gz[id];
-        rho[id] = T(qc.kp * sqrt(qc.pc.q0sq * x * x + a * a + b * b + c * c) + qc.rho0);
-    }
-}
-
-// spectral multipliers: mode 0: s exp(-2 sig^2 |q|^2) (rho filter, :1350-1354),
-// mode 1: -s |q|^2 (laplacian of rho f, :1393-1401)
-template <typename T>
-__global__ __launch_bounds__(WG) void k_q_specmul(T* __restrict__ a, Dims d, QuasiCoef qc, int mode)
-{
-    const long n = d.total();
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        int i, j, k;
-        unflat(id, d, i, j, k);
-        double m2 = mq2_at(qc.pc, i, j, k);
-        double f = mode == 0 ? qc.pc.scale * exp(-qc.sig2 * m2 * 2.0) : -qc.pc.scale * m2;
-        a[id] = T(f * (double)a[id]);
-    }
 }
 
 // after the switch (:1369-1384): divv = -((rho - rho_old)/dt + u . grad rho) / (rho + rho_old)/2,
@@ -957,27 +909,100 @@ __global__ __launch_bounds__(WG) void k_q_pvel(const T* __restrict__ Sx, const T
     }
 }
 
-// Nl = -mu - u . grad psi (+ (p + rho e) kp |grad rho| / rho^2 after the
-// switch step + 1) (:2128-2154)
-template <typename T>
-__global__ __launch_bounds__(WG) void k_q_nl(const T* __restrict__ mu, const T* __restrict__ ux,
-                                             const T* __restrict__ uy, const T* __restrict__ uz,
-                                             const T* __restrict__ gpx, const T* __restrict__ gpy,
-                                             const T* __restrict__ gpz, const double* __restrict__ pr,
-                                             const T* __restrict__ rho, const T* __restrict__ e,
-                                             const T* __restrict__ grx, const T* __restrict__ gry,
-                                             const T* __restrict__ grz, T* __restrict__ nl, long n,
-                                             QuasiCoef qc, int press_term)
+// REDFT10_z of rho = kp sqrt(q0^2 psi^2 + |grad psi|^2) + rho0 (:1322-1344),
+// the density law formed while the z lines are loaded.
+template <typename T, int N>
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_rho_z(const T* __restrict__ psi, const T* __restrict__ gx,
+                                                            const T* __restrict__ gy, const T* __restrict__ gz,
+                                                            T* __restrict__ out, Dims d, QuasiCoef qc, Tw<T> tw)
 {
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        double v = -(double)mu[id] - (double)ux[id] * (double)gpx[id] - (double)uy[id] * (double)gpy[id] -
-                   (double)uz[id] * (double)gpz[id];
-        if (press_term) {
-            double r = (double)rho[id], a = (double)grx[id], b = (double)gry[id], c = (double)grz[id];
-            v += (pr[id] + r * (double)e[id]) * qc.kp * sqrt(a * a + b * b + c * c) / (r * r);
+    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int WGZ = LinesZ<T>::WGZ;
+    constexpr int W = V16<T>::W;
+    __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    __syncthreads();
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+        const V16<T> a = *reinterpret_cast<const V16<T>*>(psi + gi);
+        const V16<T> hx = *reinterpret_cast<const V16<T>*>(gx + gi);
+        const V16<T> hy = *reinterpret_cast<const V16<T>*>(gy + gi);
+        const V16<T> hz = *reinterpret_cast<const V16<T>*>(gz + gi);
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            double x = (double)a.v[t], g1 = (double)hx.v[t], g2 = (double)hy.v[t], g3 = (double)hz.v[t];
+            lds[c * LS + PR(mk_pos<N>(p + t))] =
+                T(qc.kp * sqrt(qc.pc.q0sq * x * x + g1 * g1 + g2 * g2 + g3 * g3) + qc.rho0);
         }
-        nl[id] = T(v);
-    }
+    });
+    lsync<true>();
+    dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+        V16<T> v;
+#pragma unroll
+        for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + PR(p + t)];
+        *reinterpret_cast<V16<T>*>(out + gi) = v;
+    });
+}
+
+// REDFT10_z of Nl = -mu - u . grad psi (+ (p + rho e) kp |grad rho| / rho^2
+// after the switch step + 1) (:2128-2164), formed while the z lines are
+// loaded.
+template <typename T, int N>
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_nl_z(const T* __restrict__ mu, const T* __restrict__ ux,
+                                                           const T* __restrict__ uy, const T* __restrict__ uz,
+                                                           const T* __restrict__ gpx, const T* __restrict__ gpy,
+                                                           const T* __restrict__ gpz, const double* __restrict__ pr,
+                                                           const T* __restrict__ rho, const T* __restrict__ e,
+                                                           const T* __restrict__ grx, const T* __restrict__ gry,
+                                                           const T* __restrict__ grz, T* __restrict__ out, Dims d,
+                                                           QuasiCoef qc, int press_term, Tw<T> tw)
+{
+    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int WGZ = LinesZ<T>::WGZ;
+    constexpr int W = V16<T>::W;
+    __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    __syncthreads();
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+        const V16<T> m = *reinterpret_cast<const V16<T>*>(mu + gi);
+        const V16<T> vx = *reinterpret_cast<const V16<T>*>(ux + gi);
+        const V16<T> vy = *reinterpret_cast<const V16<T>*>(uy + gi);
+        const V16<T> vz = *reinterpret_cast<const V16<T>*>(uz + gi);
+        const V16<T> hx = *reinterpret_cast<const V16<T>*>(gpx + gi);
+        const V16<T> hy = *reinterpret_cast<const V16<T>*>(gpy + gi);
+        const V16<T> hz = *reinterpret_cast<const V16<T>*>(gpz + gi);
+        double v[W];
+#pragma unroll
+        for (int t = 0; t < W; ++t)
+            v[t] = -(double)m.v[t] - (double)vx.v[t] * (double)hx.v[t] - (double)vy.v[t] * (double)hy.v[t] -
+                   (double)vz.v[t] * (double)hz.v[t];
+        if (press_term) {
+            const V16<T> r = *reinterpret_cast<const V16<T>*>(rho + gi);
+            const V16<T> ev = *reinterpret_cast<const V16<T>*>(e + gi);
+            const V16<T> ax = *reinterpret_cast<const V16<T>*>(grx + gi);
+            const V16<T> ay = *reinterpret_cast<const V16<T>*>(gry + gi);
+            const V16<T> az = *reinterpret_cast<const V16<T>*>(grz + gi);
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                double rr = (double)r.v[t], a = (double)ax.v[t], b = (double)ay.v[t], cc = (double)az.v[t];
+                v[t] += (pr[gi + t] + rr * (double)ev.v[t]) * qc.kp * sqrt(a * a + b * b + cc * cc) / (rr * rr);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < W; ++t) lds[c * LS + PR(mk_pos<N>(p + t))] = T(v[t]);
+    });
+    lsync<true>();
+    dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+        V16<T> v;
+#pragma unroll
+        for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + PR(p + t)];
+        *reinterpret_cast<V16<T>*>(out + gi) = v;
+    });
 }
 
 // BDF2/AB2 update (:2169-2182), C1 = 2 - dt rho_m aLin, C2 = 1.5 - dt rho_m aLin:
@@ -1251,18 +1276,22 @@ struct Launch {
                         const PlainCoef& cf, int from_real, const Tw<T>& tw);
     static void plain_x(hipStream_t s, const T* a_in, T* w_out, const T* ph_in, T* ph_out,
                         T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse = 1);
-    static void adv_spec(hipStream_t s, const T* ph, T* gx, T* gy, T* gz, T* sp, Dims d,
-                         const AdvCoef& ac);
     static void adv_force(hipStream_t s, const T* psi, const T* sp2, const T* gx, const T* gy,
                           const T* gz, T* sx, T* sy, T* sz, Dims d, const AdvCoef& ac);
     static void adv_stokes(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, T* ux, T* uy,
                            T* uz, Dims d, const AdvCoef& ac);
-    static void adv_nl(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
-                       const T* gx, const T* gy, const T* gz, T* nr, Dims d, const AdvCoef& ac);
-    static void zmulti(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, const PlainCoef& cf,
+    // z pass with one output per op of the set `zs` (ZS_*); strided pass
+    // with the outputs (OP_SHIFT, OP_ID)
+    static void zpass(hipStream_t s, int zs, const T* in, const PassSet<T>& ps, Dims d, const SpecCoef& sc,
+                      const Tw<T>& tw);
+    static void spass(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, int axis, const SpecCoef& sc,
+                      const Tw<T>& tw);
+    static void q_rho_z(hipStream_t s, const T* psi, const T* gx, const T* gy, const T* gz, T* out, Dims d,
+                        const QuasiCoef& qc, const Tw<T>& tw);
+    static void q_nl_z(hipStream_t s, const T* mu, const T* ux, const T* uy, const T* uz, const T* gpx,
+                       const T* gpy, const T* gpz, const double* pr, const T* rho, const T* e, const T* grx,
+                       const T* gry, const T* grz, T* out, Dims d, const QuasiCoef& qc, int press_term,
                        const Tw<T>& tw);
-    static void multi(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, int axis,
-                      const PlainCoef& cf, const Tw<T>& tw);
     static void adv_nl_z(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
                          const T* gx, const T* gy, const T* gz, T* out, Dims d, const AdvCoef& ac,
                          const Tw<T>& tw);
@@ -1272,11 +1301,6 @@ struct Launch {
                       double* acc, const Tw<T>& tw);
     static void fold(hipStream_t s, const double* part, int nparts, int nv, double* out);
     static void sum(hipStream_t s, const T* a, long n, double* part, int nparts, double* out);
-    static void q_spec(hipStream_t s, const T* ph, const T* rq, T* fh, T* gpx, T* gpy, T* gpz,
-                       T* grx, T* gry, T* grz, T* laph, Dims d, const QuasiCoef& qc);
-    static void q_rho(hipStream_t s, const T* psi, const T* gx, const T* gy, const T* gz, T* rho,
-                      Dims d, const QuasiCoef& qc);
-    static void q_specmul(hipStream_t s, T* a, Dims d, const QuasiCoef& qc, int mode);
     static void q_divv(hipStream_t s, const T* rho, T* rho_old, const T* ux, const T* uy,
                        const T* uz, const T* grx, const T* gry, const T* grz, const T* f, T* divv,
                        T* lap, Dims d, const QuasiCoef& qc);
@@ -1285,10 +1309,6 @@ struct Launch {
                         const T* grz, T* e, T* mu, T* sx, T* sy, T* sz, Dims d, const QuasiCoef& qc);
     static void q_pvel(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, const T* dv,
                        double* ph, T* ux, T* uy, T* uz, Dims d, const QuasiCoef& qc);
-    static void q_nl(hipStream_t s, const T* mu, const T* ux, const T* uy, const T* uz,
-                     const T* gpx, const T* gpy, const T* gpz, const double* pr, const T* rho,
-                     const T* e, const T* grx, const T* gry, const T* grz, T* nl, Dims d,
-                     const QuasiCoef& qc, int press_term);
     static void q_update(hipStream_t s, T* ph, T* pho, const T* nl, T* nlo, Dims d,
                          const QuasiCoef& qc);
     static void l1(hipStream_t s, const T* a, const T* b, long n, double* part, int nparts,

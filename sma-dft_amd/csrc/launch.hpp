@@ -67,31 +67,70 @@ inline dim3 flat_grid(long n)
     return dim3((unsigned)std::min(b, 8192L));
 }
 
+#define SMAD_UNPAREN(...) __VA_ARGS__
+#define SMAD_DISPATCH_OPS(n, KERNEL, OPS, GRID, BLOCK, ...)                                              \
+    switch (n) {                                                                                        \
+    case 16: hipLaunchKernelGGL((KERNEL<T, 16, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
+    case 32: hipLaunchKernelGGL((KERNEL<T, 32, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
+    case 64: hipLaunchKernelGGL((KERNEL<T, 64, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
+    case 128: hipLaunchKernelGGL((KERNEL<T, 128, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 256: hipLaunchKernelGGL((KERNEL<T, 256, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 384: hipLaunchKernelGGL((KERNEL<T, 384, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 512: hipLaunchKernelGGL((KERNEL<T, 512, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    default: break;                                                                                     \
+    }
+
 template <typename T>
-void Launch<T>::adv_spec(hipStream_t s, const T* ph, T* gx, T* gy, T* gz, T* sp, Dims d,
-                         const AdvCoef& ac)
+void Launch<T>::zpass(hipStream_t s, int zs, const T* in, const PassSet<T>& ps, Dims d, const SpecCoef& sc,
+                      const Tw<T>& tw)
 {
-    hipLaunchKernelGGL(k_adv_spec<T>, flat_grid(d.total()), dim3(WG), 0, s, ph, gx, gy, gz, sp, d, ac);
+    static const int ops[5][4] = {{OP_ID, OP_SHIFT, OP_ALIN, -1},
+                                  {OP_ID, OP_SHIFT, OP_FQ, -1},
+                                  {OP_ID, OP_SHIFT, OP_FQ, OP_LAPF},
+                                  {OP_FILT, OP_FILT_SHIFT, -1, -1},
+                                  {OP_NMQ2, -1, -1, -1}};
+    if (zs < 0 || zs > ZS_NMQ2) throw std::logic_error("zpass: unknown op set");
+    for (int o = 0; o < 4; ++o)
+        if ((o < ps.n ? ps.op[o] : -1) != ops[zs][o]) throw std::logic_error("zpass: outputs do not match the op set");
+    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL)), block(LinesZ<T>::WGZ);
+    switch (zs) {
+    case ZS_ADV: SMAD_DISPATCH_OPS(d.nz, k_zmulti, (OP_ID, OP_SHIFT, OP_ALIN), grid, block, in, ps, d, sc, tw); break;
+    case ZS_QPSI: SMAD_DISPATCH_OPS(d.nz, k_zmulti, (OP_ID, OP_SHIFT, OP_FQ), grid, block, in, ps, d, sc, tw); break;
+    case ZS_QPSI_LAP:
+        SMAD_DISPATCH_OPS(d.nz, k_zmulti, (OP_ID, OP_SHIFT, OP_FQ, OP_LAPF), grid, block, in, ps, d, sc, tw);
+        break;
+    case ZS_QRHO: SMAD_DISPATCH_OPS(d.nz, k_zmulti, (OP_FILT, OP_FILT_SHIFT), grid, block, in, ps, d, sc, tw); break;
+    default: SMAD_DISPATCH_OPS(d.nz, k_zmulti, (OP_NMQ2), grid, block, in, ps, d, sc, tw); break;
+    }
 }
 
 template <typename T>
-void Launch<T>::zmulti(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, const PlainCoef& cf,
-                       const Tw<T>& tw)
-{
-    if (ps.n != 3 || ps.op[0] != OP_ID || ps.op[1] != OP_SHIFT || ps.op[2] != OP_ALIN)
-        throw std::logic_error("zmulti: unsupported output set");
-    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
-    SMAD_DISPATCH_Z(d.nz, k_zmulti, grid, in, ps, d, cf, tw);
-}
-
-template <typename T>
-void Launch<T>::multi(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, int axis,
-                      const PlainCoef& cf, const Tw<T>& tw)
+void Launch<T>::spass(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, int axis, const SpecCoef& sc,
+                      const Tw<T>& tw)
 {
     if (ps.n != 2 || ps.op[0] != OP_SHIFT || ps.op[1] != OP_ID)
-        throw std::logic_error("multi: unsupported output set");
-    dim3 grid(num_tiles(d, axis, Lines<T>::NL));
-    SMAD_DISPATCH(axis_len(d, axis), k_multi, grid, in, ps, d, axis, cf, tw);
+        throw std::logic_error("spass: outputs must be (OP_SHIFT, OP_ID)");
+    dim3 grid(num_tiles(d, axis, Lines<T>::NL)), block(WG);
+    SMAD_DISPATCH_OPS(axis_len(d, axis), k_multi, (OP_SHIFT, OP_ID), grid, block, in, ps, d, axis, sc, tw);
+}
+
+template <typename T>
+void Launch<T>::q_rho_z(hipStream_t s, const T* psi, const T* gx, const T* gy, const T* gz, T* out, Dims d,
+                        const QuasiCoef& qc, const Tw<T>& tw)
+{
+    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+    SMAD_DISPATCH_Z(d.nz, k_q_rho_z, grid, psi, gx, gy, gz, out, d, qc, tw);
+}
+
+template <typename T>
+void Launch<T>::q_nl_z(hipStream_t s, const T* mu, const T* ux, const T* uy, const T* uz, const T* gpx,
+                       const T* gpy, const T* gpz, const double* pr, const T* rho, const T* e, const T* grx,
+                       const T* gry, const T* grz, T* out, Dims d, const QuasiCoef& qc, int press_term,
+                       const Tw<T>& tw)
+{
+    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+    SMAD_DISPATCH_Z(d.nz, k_q_nl_z, grid, mu, ux, uy, uz, gpx, gpy, gpz, pr, rho, e, grx, gry, grz, out, d, qc,
+                    press_term, tw);
 }
 
 template <typename T>
@@ -117,14 +156,6 @@ void Launch<T>::adv_stokes(hipStream_t s, const T* Sx, const T* Sy, const T* Sz,
 {
     hipLaunchKernelGGL(k_adv_stokes<T>, flat_grid(d.total()), dim3(WG), 0, s, Sx, Sy, Sz, ux, uy,
                        uz, d, ac);
-}
-
-template <typename T>
-void Launch<T>::adv_nl(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
-                       const T* gx, const T* gy, const T* gz, T* nr, Dims d, const AdvCoef& ac)
-{
-    hipLaunchKernelGGL(k_adv_nl<T>, flat_grid(d.total()), dim3(WG), 0, s, psi, ux, uy, uz, gx, gy,
-                       gz, nr, d.total(), ac);
 }
 
 template <typename T>
@@ -169,26 +200,6 @@ void Launch<T>::sum(hipStream_t s, const T* a, long n, double* part, int nparts,
 }
 
 template <typename T>
-void Launch<T>::q_spec(hipStream_t s, const T* ph, const T* rq, T* fh, T* gpx, T* gpy, T* gpz,
-                       T* grx, T* gry, T* grz, T* laph, Dims d, const QuasiCoef& qc)
-{
-    SMAD_FLAT(k_q_spec, ph, rq, fh, gpx, gpy, gpz, grx, gry, grz, laph, d, qc);
-}
-
-template <typename T>
-void Launch<T>::q_rho(hipStream_t s, const T* psi, const T* gx, const T* gy, const T* gz, T* rho,
-                      Dims d, const QuasiCoef& qc)
-{
-    SMAD_FLAT(k_q_rho, psi, gx, gy, gz, rho, d.total(), qc);
-}
-
-template <typename T>
-void Launch<T>::q_specmul(hipStream_t s, T* a, Dims d, const QuasiCoef& qc, int mode)
-{
-    SMAD_FLAT(k_q_specmul, a, d, qc, mode);
-}
-
-template <typename T>
 void Launch<T>::q_divv(hipStream_t s, const T* rho, T* rho_old, const T* ux, const T* uy,
                        const T* uz, const T* grx, const T* gry, const T* grz, const T* f, T* divv,
                        T* lap, Dims d, const QuasiCoef& qc)
@@ -210,16 +221,6 @@ void Launch<T>::q_pvel(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, con
                        double* ph, T* ux, T* uy, T* uz, Dims d, const QuasiCoef& qc)
 {
     SMAD_FLAT(k_q_pvel, Sx, Sy, Sz, dv, ph, ux, uy, uz, d, qc);
-}
-
-template <typename T>
-void Launch<T>::q_nl(hipStream_t s, const T* mu, const T* ux, const T* uy, const T* uz,
-                     const T* gpx, const T* gpy, const T* gpz, const double* pr, const T* rho,
-                     const T* e, const T* grx, const T* gry, const T* grz, T* nl, Dims d,
-                     const QuasiCoef& qc, int press_term)
-{
-    SMAD_FLAT(k_q_nl, mu, ux, uy, uz, gpx, gpy, gpz, pr, rho, e, grx, gry, grz, nl, d.total(), qc,
-              press_term);
 }
 
 template <typename T>

@@ -106,6 +106,23 @@ PassSet<T> pass3(T* o0, int k0, int op0, T* o1, int k1, int op1, T* o2, int k2, 
     ps.n = 3;
     return ps;
 }
+template <typename T>
+PassSet<T> pass4(T* o0, int k0, int op0, T* o1, int k1, int op1, T* o2, int k2, int op2, T* o3, int k3,
+                 int op3)
+{
+    PassSet<T> ps = pass3(o0, k0, op0, o1, k1, op1, o2, k2, op2);
+    ps.out[3] = o3; ps.kind[3] = k3; ps.op[3] = op3;
+    ps.n = 4;
+    return ps;
+}
+template <typename T>
+PassSet<T> pass1(T* o0, int k0, int op0)
+{
+    PassSet<T> ps{};
+    ps.out[0] = o0; ps.kind[0] = k0; ps.op[0] = op0;
+    ps.n = 1;
+    return ps;
+}
 
 struct Prof;
 
@@ -626,13 +643,14 @@ struct AdvEngine : EngineBase {
         // psi, grad psi and Sp2 = REDFT01^3(-aLin psihat) (:847-991) with
         // shared passes: z from psihat, y from B = REDFT01_z psihat, x from
         // A = REDFT01_y B
-        const PlainCoef& cf = ac.pc;
-        Launch<T>::zmulti(s, ph.p, pass3(psi.p, REDFT01, OP_ID, gz.p, RODFT01, OP_SHIFT, sp.p, REDFT01, OP_ALIN),
-                          d, cf, twz.tw);
-        Launch<T>::multi(s, psi.p, pass2(gy.p, RODFT01, OP_SHIFT, psi.p, REDFT01, OP_ID), d, 1, cf, twy.tw);
+        const SpecCoef cf{ac.pc, 0.0};
+        Launch<T>::zpass(s, ZS_ADV, ph.p,
+                         pass3(psi.p, REDFT01, OP_ID, gz.p, RODFT01, OP_SHIFT, sp.p, REDFT01, OP_ALIN), d, cf,
+                         twz.tw);
+        Launch<T>::spass(s, psi.p, pass2(gy.p, RODFT01, OP_SHIFT, psi.p, REDFT01, OP_ID), d, 1, cf, twy.tw);
         Launch<T>::axis(s, gz.p, gz.p, d, 1, REDFT01, T(1), twy.tw);
         Launch<T>::axis(s, sp.p, sp.p, d, 1, REDFT01, T(1), twy.tw);
-        Launch<T>::multi(s, psi.p, pass2(gx.p, RODFT01, OP_SHIFT, psi.p, REDFT01, OP_ID), d, 0, cf, twx.tw);
+        Launch<T>::spass(s, psi.p, pass2(gx.p, RODFT01, OP_SHIFT, psi.p, REDFT01, OP_ID), d, 0, cf, twx.tw);
         Launch<T>::axis(s, gy.p, gy.p, d, 0, REDFT01, T(1), twx.tw);
         Launch<T>::axis(s, gz.p, gz.p, d, 0, REDFT01, T(1), twx.tw);
         Launch<T>::axis(s, sp.p, sp.p, d, 0, REDFT01, T(1), twx.tw);
@@ -749,8 +767,10 @@ struct QuasiEngine : EngineBase {
     QuasiCoef qc{};
     TwTable<T> twx, twy, twz;
     TwTable<double> tdx, tdy, tdz;
+    // grx/gry/grz: grad of the previous step's filtered rho (this step's);
+    // grx2/gry2/grz2: the next step's, computed with this step's rho
     DevBuf<T> ph, pho, nl, nlo, rhoq, rho, rho_old, divv, ux, uy, uz, psi, psi_prev, f, gpx, gpy,
-        gpz, grx, gry, grz, lap, e, mu, sx, sy, sz;
+        gpz, grx, gry, grz, grx2, gry2, grz2, lap, e, mu, sx, sy, sz;
     DevBuf<double> pr, part, red;
     int nparts = 1024;
     int nloop = 0;
@@ -787,8 +807,8 @@ struct QuasiEngine : EngineBase {
         tdy.make(p.ny, s);
         tdz.make(p.nz, s);
         for (DevBuf<T>* b : {&ph, &pho, &nl, &nlo, &rhoq, &rho, &rho_old, &divv, &ux, &uy, &uz, &psi,
-                             &psi_prev, &f, &gpx, &gpy, &gpz, &grx, &gry, &grz, &lap, &e, &mu, &sx,
-                             &sy, &sz})
+                             &psi_prev, &f, &gpx, &gpy, &gpz, &grx, &gry, &grz, &grx2, &gry2, &grz2, &lap,
+                             &e, &mu, &sx, &sy, &sz})
             b->alloc(n);
         pr.alloc(n);
         part.alloc(3 * nparts);
@@ -824,7 +844,9 @@ struct QuasiEngine : EngineBase {
         Launch<T>::axis(s, nl.p, nl.p, d, 1, REDFT10, T(1), twy.tw);
         Launch<T>::axis(s, nl.p, nl.p, d, 0, REDFT10, T(1), twx.tw);
         HIPCHK(hipMemcpyAsync(pho.p, ph.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-        zero(rhoq);
+        zero(grx);  // grad of the initial filtered rho, which is zero
+        zero(gry);
+        zero(grz);
         zero(rho_old);
         zero(ux);
         zero(uy);
@@ -847,36 +869,50 @@ struct QuasiEngine : EngineBase {
         auto mk = [&](int c) { if (prof) prof->mark(c); };
         ++nloop;
         qc.pre_switch = nloop <= p.div_switch ? 1 : 0;
-        // psi of this step from psihat (the reference carries it from the
-        // previous step's inverse transform)
-        if (!psi_valid) tr3(ph.p, psi.p, REDFT01, REDFT01, REDFT01);
+        const SpecCoef sc{qc.pc, qc.sig2};
+        // psi, f, grad psi (and lap_hat before the switch) from psihat
+        // (:1121-1256, :1411-1421) with shared passes: z from psihat, y from
+        // B = REDFT01_z psihat, x from A = REDFT01_y B
+        if (qc.pre_switch)
+            Launch<T>::zpass(s, ZS_QPSI_LAP, ph.p,
+                             pass4(psi.p, REDFT01, OP_ID, gpz.p, RODFT01, OP_SHIFT, f.p, REDFT01, OP_FQ, lap.p,
+                                   REDFT01, OP_LAPF),
+                             d, sc, twz.tw);
+        else
+            Launch<T>::zpass(s, ZS_QPSI, ph.p,
+                             pass3(psi.p, REDFT01, OP_ID, gpz.p, RODFT01, OP_SHIFT, f.p, REDFT01, OP_FQ), d, sc,
+                             twz.tw);
+        Launch<T>::spass(s, psi.p, pass2(gpy.p, RODFT01, OP_SHIFT, psi.p, REDFT01, OP_ID), d, 1, sc, twy.tw);
+        for (T* b : {gpz.p, f.p, qc.pre_switch ? lap.p : nullptr})
+            if (b) Launch<T>::axis(s, b, b, d, 1, REDFT01, T(1), twy.tw);
+        Launch<T>::spass(s, psi.p, pass2(gpx.p, RODFT01, OP_SHIFT, psi.p, REDFT01, OP_ID), d, 0, sc, twx.tw);
+        for (T* b : {gpy.p, gpz.p, f.p, qc.pre_switch ? lap.p : nullptr})
+            if (b) Launch<T>::axis(s, b, b, d, 0, REDFT01, T(1), twx.tw);
         mk(0);
         if (keep_prev) HIPCHK(hipMemcpyAsync(psi_prev.p, psi.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-        Launch<T>::q_spec(s, ph.p, rhoq.p, f.p, gpx.p, gpy.p, gpz.p, grx.p, gry.p, grz.p,
-                          qc.pre_switch ? lap.p : nullptr, d, qc);
-        mk(1);
-        tr3(f.p, f.p, REDFT01, REDFT01, REDFT01);
-        ist3(gpx.p, gpy.p, gpz.p);
-        ist3(grx.p, gry.p, grz.p);
-        mk(0);
-        Launch<T>::q_rho(s, psi.p, gpx.p, gpy.p, gpz.p, rho.p, d, qc);
-        mk(1);
-        tr3(rho.p, rhoq.p, REDFT10, REDFT10, REDFT10);
-        mk(0);
-        Launch<T>::q_specmul(s, rhoq.p, d, qc, 0);
-        mk(1);
-        tr3(rhoq.p, rho.p, REDFT01, REDFT01, REDFT01);
-        if (qc.pre_switch) tr3(lap.p, lap.p, REDFT01, REDFT01, REDFT01);
+        // density law and filter (:1322-1360): the law is formed in the
+        // forward z pass, the filter s exp(-2 sig^2 |q|^2) in the first
+        // inverse pass, whose passes also give the next step's grad rho
+        Launch<T>::q_rho_z(s, psi.p, gpx.p, gpy.p, gpz.p, rhoq.p, d, qc, twz.tw);
+        Launch<T>::axis(s, rhoq.p, rhoq.p, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(s, rhoq.p, rhoq.p, d, 0, REDFT10, T(1), twx.tw);
+        Launch<T>::zpass(s, ZS_QRHO, rhoq.p, pass2(rho.p, REDFT01, OP_FILT, grz2.p, RODFT01, OP_FILT_SHIFT), d, sc,
+                         twz.tw);
+        Launch<T>::spass(s, rho.p, pass2(gry2.p, RODFT01, OP_SHIFT, rho.p, REDFT01, OP_ID), d, 1, sc, twy.tw);
+        Launch<T>::axis(s, grz2.p, grz2.p, d, 1, REDFT01, T(1), twy.tw);
+        Launch<T>::spass(s, rho.p, pass2(grx2.p, RODFT01, OP_SHIFT, rho.p, REDFT01, OP_ID), d, 0, sc, twx.tw);
+        Launch<T>::axis(s, gry2.p, gry2.p, d, 0, REDFT01, T(1), twx.tw);
+        Launch<T>::axis(s, grz2.p, grz2.p, d, 0, REDFT01, T(1), twx.tw);
         mk(0);
         Launch<T>::q_divv(s, rho.p, rho_old.p, ux.p, uy.p, uz.p, grx.p, gry.p, grz.p, f.p, divv.p,
                           lap.p, d, qc);
         mk(1);
         if (!qc.pre_switch) {
+            // lap = REDFT01^3(-s |q|^2 REDFT10^3(rho f)) (:1386-1404)
             tr3(lap.p, lap.p, REDFT10, REDFT10, REDFT10);
-            mk(0);
-            Launch<T>::q_specmul(s, lap.p, d, qc, 1);
-            mk(1);
-            tr3(lap.p, lap.p, REDFT01, REDFT01, REDFT01);
+            Launch<T>::zpass(s, ZS_NMQ2, lap.p, pass1(lap.p, REDFT01, OP_NMQ2), d, sc, twz.tw);
+            Launch<T>::axis(s, lap.p, lap.p, d, 1, REDFT01, T(1), twy.tw);
+            Launch<T>::axis(s, lap.p, lap.p, d, 0, REDFT01, T(1), twx.tw);
             tr3(divv.p, divv.p, REDFT10, REDFT10, REDFT10);
             mk(0);
         }
@@ -894,14 +930,18 @@ struct QuasiEngine : EngineBase {
         Launch<double>::axis(s, pr.p, pr.p, d, 0, REDFT01, 1.0, tdx.tw);
         ist3(ux.p, uy.p, uz.p);
         mk(0);
-        Launch<T>::q_nl(s, mu.p, ux.p, uy.p, uz.p, gpx.p, gpy.p, gpz.p, pr.p, rho.p, e.p, grx.p,
-                        gry.p, grz.p, nl.p, d, qc, nloop > p.div_switch + 1 ? 1 : 0);
+        Launch<T>::q_nl_z(s, mu.p, ux.p, uy.p, uz.p, gpx.p, gpy.p, gpz.p, pr.p, rho.p, e.p, grx.p, gry.p,
+                          grz.p, nl.p, d, qc, nloop > p.div_switch + 1 ? 1 : 0, twz.tw);
         mk(1);
-        tr3(nl.p, nl.p, REDFT10, REDFT10, REDFT10);
+        Launch<T>::axis(s, nl.p, nl.p, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(s, nl.p, nl.p, d, 0, REDFT10, T(1), twx.tw);
         mk(0);
         if (nloop == 1) HIPCHK(hipMemcpyAsync(nlo.p, nl.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
         Launch<T>::q_update(s, ph.p, pho.p, nl.p, nlo.p, d, qc);
         mk(1);
+        std::swap(grx.p, grx2.p);
+        std::swap(gry.p, gry2.p);
+        std::swap(grz.p, grz2.p);
         psi_valid = false;
     }
 
