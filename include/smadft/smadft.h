@@ -117,6 +117,22 @@ int smad_set_field(smad_ctx *ctx, int field, const double *host_in);
  * (no-adv/cosNoAdvTrackSing.cpp:681). */
 int smad_reduce(smad_ctx *ctx, int what, double *out);
 
+/* Interface tracker and surface diagnostics of the current psi, plain model
+ * (replaces no-adv/cosNoAdvTrackMult.cpp:787-796 and 853-1022, the save
+ * block's surface computation).  For every column (i, j), scanning k from
+ * Nz-1 down, the first psi > 0.7 arms the tracker and the first psi < 0 after
+ * it is the crossing k; k2 = k + 1 when |psi(k)| > |psi(k+1)|, else k.  Each
+ * output holds nx*ny doubles, index i*ny + j:
+ *   surf   = k2 (the reference's surfPsi.dat), -1 for a column without crossing
+ *   vel    = dTpsi / |grad psi| at (i, j, k2)       (velSurf.dat)
+ *   curv_h = 2H, curv_k = K at (i, j, k2)          (curvH.dat, curvK.dat)
+ * with finite-difference gradients and spectral second derivatives as the
+ * reference.  The reference's save block reports the state one step before
+ * the save step (it runs before psi <- psi_new); call this after the
+ * penultimate step of a save block to reproduce its files.  Columns without a
+ * crossing get 0 (the reference leaves stale buffer contents there). */
+int smad_surface(smad_ctx *ctx, double *surf, double *vel, double *curv_h, double *curv_k);
+
 /* Wait for all work on the context stream. */
 int smad_sync(smad_ctx *ctx);
 

@@ -55,7 +55,10 @@ def lib():
         L.orc_adv_stokes.argtypes = [pp, dp, dp, dp, dp, dp, dp]
         L.orc_adv_stokes.restype = None
         L.orc_plain_adjoint.argtypes = [pp, dp, ctypes.c_int, dp, dp, dp, dp]
-        for f in ("orc_plain_adjoint", "orc_r2r1d_naive", "orc_r2r1d", "orc_r2r3d", "orc_plain_preloop",
+        L.orc_surface_aux.argtypes = [pp, dp, dp, dp, dp, dp]
+        L.orc_surface_track.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [dp] * 9
+        for f in ("orc_surface_aux", "orc_surface_track", "orc_plain_adjoint", "orc_r2r1d_naive",
+                  "orc_r2r1d", "orc_r2r3d", "orc_plain_preloop",
                   "orc_plain_step", "orc_l1_sums", "orc_adv_preloop", "orc_adv_step"):
             getattr(L, f).restype = None
         _LIB = L
@@ -183,3 +186,27 @@ def adv_stokes(p, sx, sy, sz):
     op = oparams(p)
     lib().orc_adv_stokes(ctypes.byref(op), *[_p(v) for v in a], *[_p(v) for v in out])
     return tuple(out)
+
+
+def surface_aux(p, psi):
+    """Real-space psi_xx, psi_yy, psi_zz, dTpsi of psi (reference order)."""
+    psi = np.ascontiguousarray(psi, dtype=np.float64)
+    out = [np.empty_like(psi) for _ in range(4)]
+    lib().orc_surface_aux(ctypes.byref(oparams(p)), _p(psi), *[_p(a) for a in out])
+    return out
+
+
+def surface_track(psi, dxx, dyy, dzz, dtp, h):
+    """Column scan + velocity + curvatures with grid spacing h (scalar or 3-tuple)."""
+    hx, hy, hz = (h, h, h) if np.isscalar(h) else h
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (psi, dxx, dyy, dzz, dtp)]
+    nx, ny, nz = arrs[0].shape
+    out = [np.empty((nx, ny)) for _ in range(4)]
+    lib().orc_surface_track(nx, ny, nz, hx, hy, hz, *[_p(a) for a in arrs], *[_p(a) for a in out])
+    return dict(zip(("surf", "vel", "curv_h", "curv_k"), out))
+
+
+def surface(p, psi):
+    """The plain model's surface diagnostics of psi (spacing 2 pi/(q0 nw))."""
+    dxx, dyy, dzz, dtp = surface_aux(p, psi)
+    return surface_track(psi, dxx, dyy, dzz, dtp, 2 * np.pi / (p.q0 * p.nw))

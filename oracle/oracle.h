@@ -46,6 +46,17 @@ void orc_plain_step(const orc_params *p, double *psi, double *psiq, double *nq, 
 /* L1 sums of no-adv/cosNoAdvTrackMult.cpp:814-842: out[0] = sum|a-b|, out[1] = sum|a| */
 void orc_l1_sums(const double *a_new, const double *b_old, long n, double *out);
 
+/* Surface diagnostics (no-adv/cosNoAdvTrackMult.cpp:787-796, 853-1022).
+ * orc_surface_aux: real-space psi_xx, psi_yy, psi_zz and dTpsi of psi from
+ *   its spectrum (the save block's iPlanPsiD** / iPlanDTpsi).
+ * orc_surface_track: the column scan, normal velocity and curvatures; outputs
+ *   nx*ny values each (index i*ny + j), surf = -1 where no crossing. */
+void orc_surface_aux(const orc_params *p, const double *psi, double *dxx, double *dyy, double *dzz,
+                     double *dt);
+void orc_surface_track(int nx, int ny, int nz, double dx, double dy, double dz, const double *psi,
+                       const double *dxx, const double *dyy, const double *dzz, const double *dtp,
+                       double *surf, double *vel, double *curv_h, double *curv_k);
+
 /* ------------------------------------------------------- advective model -- */
 /* adv/cosAdvTrackMult.cpp:824-1369.  Pre-loop N without advection (:760-777).
  * State: psi (real), nq, nq_past.  Diagnostic fields (may be NULL):

@@ -1005,6 +1005,141 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_nl_z(const T* __restrict__
     });
 }
 
+// ------------------------------------------------------------------------
+// Surface diagnostics of the plain model (no-adv/cosNoAdvTrackMult.cpp:787-796,
+// 853-1022): interface tracker, normal velocity and curvatures.
+//
+// Spectral inputs of the auxiliary fields (psihat scaled, nq = REDFT10^3 of
+// the nonlinearity, unscaled): dxx = -qx^2 psihat (likewise y, z) and
+// dTpsi = aLin psihat + s nq, each then taken to real space by REDFT01^3.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_surf_spec(const T* __restrict__ ph, const T* __restrict__ nq,
+                                                  T* __restrict__ dxx, T* __restrict__ dyy,
+                                                  T* __restrict__ dzz, T* __restrict__ dt, Dims d,
+                                                  PlainCoef cf)
+{
+    const long n = d.total();
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        const double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
+        const double v = (double)ph[id];
+        dxx[id] = T(-qx * qx * v);
+        dyy[id] = T(-qy * qy * v);
+        dzz[id] = T(-qz * qz * v);
+        const double t = qx * qx + qy * qy + qz * qz - cf.q0sq;
+        dt[id] = T((cf.eps - cf.alpha * t * t) * v + cf.scale * (double)nq[id]);
+    }
+}
+
+struct SurfCoef {
+    double tdx, tdy, tdz;  // twice the grid spacings (the central-difference denominators)
+};
+
+// Central difference along a stride with the reference's one-sided edges:
+// (f[c+1] - f[c-1]) / 2h inside, 2 (f[1] - f[0]) / 2h and 2 (f[N-1] - f[N-2]) / 2h at the ends.
+template <typename T>
+__device__ __forceinline__ double fd_at(const T* __restrict__ f, long idx, int c, int nc, long stride, double th)
+{
+    if (c > 0 && c < nc - 1) return ((double)f[idx + stride] - (double)f[idx - stride]) / th;
+    if (c == 0) return 2.0 * ((double)f[idx + stride] - (double)f[idx]) / th;
+    return 2.0 * ((double)f[idx] - (double)f[idx - stride]) / th;
+}
+
+// Finite-difference gradient component `a` (0 x, 1 y, 2 z) of psi at flat
+// index idx; an index outside the grid (reachable through the reference's
+// flat idx +- 1 reads at the top of a column) contributes 0.
+template <typename T>
+__device__ __forceinline__ double grad_at(const T* __restrict__ psi, long idx, int a, const Dims& d,
+                                          const SurfCoef& sc)
+{
+    if (idx < 0 || idx >= d.total()) return 0.0;
+    int i, j, k;
+    unflat(idx, d, i, j, k);
+    if (a == 0) return fd_at(psi, idx, i, d.nx, (long)d.ny * d.nz, sc.tdx);
+    if (a == 1) return fd_at(psi, idx, j, d.ny, (long)d.nz, sc.tdy);
+    return fd_at(psi, idx, k, d.nz, 1L, sc.tdz);
+}
+
+// One thread per (i, j) column.  Scanning k from Nz-1 down: the first
+// psi > 0.7 arms the tracker, the first psi < 0 after it is the crossing k;
+// k2 = k + 1 when |psi(k)| > |psi(k+1)|, else k.  At (i, j, k2):
+//   surf = k2, vel = dTpsi / |grad psi|,
+//   curv_h = [(gy^2+gz^2) dxx + (gx^2+gz^2) dyy + (gx^2+gy^2) dzz
+//             - 2 (gx gy dxy + gx gz dxz + gy gz dyz)] / |g|^3          (= 2H)
+//   curv_k = [gz^2 (dxx dyy - dxy^2) + gx^2 (dyy dzz - dyz^2) + gy^2 (dxx dzz - dxz^2)
+//             + 2 (gy dxy (gz dxz - gx dzz) + gx dxz (gy dyz - gz dyy)
+//                  + gz dyz (gx dxy - gy dxx))] / |g|^4
+// with g the finite-difference gradient, dxx.. the spectral second
+// derivatives, dxy the y-difference of gx at k2 and dxz, dyz the flat
+// index +-1 differences of gx, gy around k2 with the edge rule of k (both as
+// the reference).  Columns without a crossing get surf = -1 and zeros.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_surf_track(const T* __restrict__ psi, const T* __restrict__ dxx,
+                                                   const T* __restrict__ dyy, const T* __restrict__ dzz,
+                                                   const T* __restrict__ dtp, Dims d, SurfCoef sc,
+                                                   double* __restrict__ surf, double* __restrict__ vel,
+                                                   double* __restrict__ ch, double* __restrict__ ck)
+{
+    const long col = (long)blockIdx.x * WG + threadIdx.x;
+    if (col >= (long)d.nx * d.ny) return;
+    const int j = (int)(col % d.ny);
+    const long base = col * d.nz;
+    double o_s = -1.0, o_v = 0.0, o_h = 0.0, o_k = 0.0;
+    int armed = 0;
+    for (int k = d.nz - 1; k >= 0; --k) {
+        const double v = (double)psi[base + k];
+        if (v > 0.7 && !armed) armed = 1;
+        if (v < 0.0 && armed) {
+            long idx = base + k;
+            int k2 = k;
+            if (fabs(v) > fabs((double)psi[idx + 1])) {
+                idx += 1;
+                k2 = k + 1;
+            }
+            const double gx = grad_at(psi, idx, 0, d, sc), gy = grad_at(psi, idx, 1, d, sc),
+                         gz = grad_at(psi, idx, 2, d, sc);
+            const double g = sqrt(gx * gx + gy * gy + gz * gz);
+            const long ck2 = base + k2;
+            const long sy = d.nz;
+            double dxy;
+            if (j > 0 && j < d.ny - 1)
+                dxy = (grad_at(psi, ck2 + sy, 0, d, sc) - grad_at(psi, ck2 - sy, 0, d, sc)) / sc.tdy;
+            else if (j == 0)
+                dxy = 2.0 * (grad_at(psi, ck2 + sy, 0, d, sc) - grad_at(psi, ck2, 0, d, sc)) / sc.tdy;
+            else
+                dxy = 2.0 * (grad_at(psi, ck2, 0, d, sc) - grad_at(psi, ck2 - sy, 0, d, sc)) / sc.tdy;
+            double dxz, dyz;
+            if (k > 0 && k < d.nz - 1) {
+                dxz = (grad_at(psi, idx + 1, 0, d, sc) - grad_at(psi, idx - 1, 0, d, sc)) / sc.tdz;
+                dyz = (grad_at(psi, idx + 1, 1, d, sc) - grad_at(psi, idx - 1, 1, d, sc)) / sc.tdz;
+            } else if (k == 0) {
+                dxz = 2.0 * (grad_at(psi, idx + 1, 0, d, sc) - gx) / sc.tdz;
+                dyz = 2.0 * (grad_at(psi, idx + 1, 1, d, sc) - gy) / sc.tdz;
+            } else {
+                dxz = 2.0 * (gx - grad_at(psi, idx - 1, 0, d, sc)) / sc.tdz;
+                dyz = 2.0 * (gy - grad_at(psi, idx - 1, 1, d, sc)) / sc.tdz;
+            }
+            const double a = (double)dxx[idx], b = (double)dyy[idx], c = (double)dzz[idx];
+            const double g2 = g * g;
+            o_s = k2;
+            o_v = (double)dtp[idx] / g;
+            o_h = ((gy * gy + gz * gz) * a + (gx * gx + gz * gz) * b + (gx * gx + gy * gy) * c -
+                   2.0 * (gx * gy * dxy + gx * gz * dxz + gy * gz * dyz)) /
+                  (g2 * g);
+            o_k = (gz * gz * (a * b - dxy * dxy) + gx * gx * (b * c - dyz * dyz) + gy * gy * (a * c - dxz * dxz) +
+                   2.0 * (gy * dxy * (gz * dxz - gx * c) + gx * dxz * (gy * dyz - gz * b) +
+                          gz * dyz * (gx * dxy - gy * a))) /
+                  (g2 * g2);
+            break;
+        }
+    }
+    surf[col] = o_s;
+    vel[col] = o_v;
+    ch[col] = o_h;
+    ck[col] = o_k;
+}
+
 // BDF2/AB2 update (:2169-2182), C1 = 2 - dt rho_m aLin, C2 = 1.5 - dt rho_m aLin:
 //   psihat' = (C1 psihat - psihat_old/2 + dt/2 s (3 Nl - Nl_old)) / C2
 // then psihat_old = psihat, Nl_old = Nl (the next step's history).
@@ -1286,6 +1421,10 @@ struct Launch {
                       const Tw<T>& tw);
     static void spass(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, int axis, const SpecCoef& sc,
                       const Tw<T>& tw);
+    static void surf_spec(hipStream_t s, const T* ph, const T* nq, T* dxx, T* dyy, T* dzz, T* dt, Dims d,
+                          const PlainCoef& cf);
+    static void surf_track(hipStream_t s, const T* psi, const T* dxx, const T* dyy, const T* dzz, const T* dt,
+                           Dims d, const SurfCoef& sc, double* surf, double* vel, double* ch, double* ck);
     static void q_rho_z(hipStream_t s, const T* psi, const T* gx, const T* gy, const T* gz, T* out, Dims d,
                         const QuasiCoef& qc, const Tw<T>& tw);
     static void q_nl_z(hipStream_t s, const T* mu, const T* ux, const T* uy, const T* uz, const T* gpx,

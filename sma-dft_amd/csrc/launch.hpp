@@ -115,6 +115,22 @@ void Launch<T>::spass(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, 
 }
 
 template <typename T>
+void Launch<T>::surf_spec(hipStream_t s, const T* ph, const T* nq, T* dxx, T* dyy, T* dzz, T* dt, Dims d,
+                          const PlainCoef& cf)
+{
+    hipLaunchKernelGGL(k_surf_spec<T>, flat_grid(d.total()), dim3(WG), 0, s, ph, nq, dxx, dyy, dzz, dt, d, cf);
+}
+
+template <typename T>
+void Launch<T>::surf_track(hipStream_t s, const T* psi, const T* dxx, const T* dyy, const T* dzz, const T* dt,
+                           Dims d, const SurfCoef& sc, double* surf, double* vel, double* ch, double* ck)
+{
+    const long cols = (long)d.nx * d.ny;
+    hipLaunchKernelGGL(k_surf_track<T>, dim3((unsigned)((cols + WG - 1) / WG)), dim3(WG), 0, s, psi, dxx, dyy, dzz,
+                       dt, d, sc, surf, vel, ch, ck);
+}
+
+template <typename T>
 void Launch<T>::q_rho_z(hipStream_t s, const T* psi, const T* gx, const T* gy, const T* gz, T* out, Dims d,
                         const QuasiCoef& qc, const Tw<T>& tw)
 {

@@ -158,6 +158,10 @@ struct EngineBase {
     virtual int reduce(int what, double* out) = 0;
     virtual void forward_tape(int K, int ck) = 0;
     virtual void set_tape_budget(double) {}
+    virtual void surface(double*, double*, double*, double*)
+    {
+        throw ApiError{SMAD_EINVAL, "surface diagnostics are defined for the plain model"};
+    }
     double last_fwd_ms = 0.0, last_adj_ms = 0.0;
     virtual void adjoint(const double* g_psik, double* dpsi0, double* dparams) = 0;
 };
@@ -496,6 +500,42 @@ struct PlainEngine : EngineBase {
         HIPCHK(hipStreamSynchronize(s));
         psi_valid = false;
         prev_valid = false;
+    }
+
+    // ---------------------------------------------------- surface --
+    // Interface tracker, normal velocity and curvatures of the current psi
+    // (no-adv/cosNoAdvTrackMult.cpp:787-796, 853-1022).
+    DevBuf<T> sf_dxx, sf_dyy, sf_dzz, sf_dt, sf_nq;
+    DevBuf<double> sf_out;
+
+    void surface(double* surf, double* vel, double* ch, double* ck) override
+    {
+        const size_t cols = (size_t)p.nx * p.ny;
+        if (!sf_nq.p) {
+            for (DevBuf<T>* b : {&sf_dxx, &sf_dyy, &sf_dzz, &sf_dt, &sf_nq}) b->alloc(n);
+            sf_out.alloc(4 * cols);
+        }
+        materialize();
+        // nq = REDFT10^3(beta psi^3 - gamma psi^5) of this psi
+        Launch<T>::plain_z(s, psi.p, sf_nq.p, nullptr, d, cf, 1, twz.tw);
+        Launch<T>::axis(s, sf_nq.p, sf_nq.p, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(s, sf_nq.p, sf_nq.p, d, 0, REDFT10, T(1), twx.tw);
+        Launch<T>::surf_spec(s, psihat.p, sf_nq.p, sf_dxx.p, sf_dyy.p, sf_dzz.p, sf_dt.p, d, cf);
+        for (DevBuf<T>* b : {&sf_dxx, &sf_dyy, &sf_dzz, &sf_dt}) {
+            Launch<T>::axis(s, b->p, b->p, d, 2, REDFT01, T(1), twz.tw);
+            Launch<T>::axis(s, b->p, b->p, d, 1, REDFT01, T(1), twy.tw);
+            Launch<T>::axis(s, b->p, b->p, d, 0, REDFT01, T(1), twx.tw);
+        }
+        const double pi = M_PI;
+        const double dx = 2.0 * pi / (p.q0 * p.nw);  // L/N on every axis
+        SurfCoef sc{2.0 * dx, 2.0 * dx, 2.0 * dx};
+        Launch<T>::surf_track(s, psi.p, sf_dxx.p, sf_dyy.p, sf_dzz.p, sf_dt.p, d, sc, sf_out.p, sf_out.p + cols,
+                              sf_out.p + 2 * cols, sf_out.p + 3 * cols);
+        LAUNCHCHK();
+        double* outs[4] = {surf, vel, ch, ck};
+        for (int q = 0; q < 4; ++q)
+            HIPCHK(hipMemcpyAsync(outs[q], sf_out.p + q * cols, cols * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
     }
 
     void materialize()
@@ -1181,6 +1221,15 @@ int smad_forward_tape(smad_ctx* c, int nsteps, int ckpt_every)
     return guarded(c, [&] {
         c->eng->forward_tape(nsteps, ckpt_every);
         c->steps += nsteps;
+        return SMAD_OK;
+    });
+}
+
+int smad_surface(smad_ctx* c, double* surf, double* vel, double* curv_h, double* curv_k)
+{
+    return guarded(c, [&] {
+        if (!surf || !vel || !curv_h || !curv_k) throw ApiError{SMAD_EINVAL, "null output"};
+        c->eng->surface(surf, vel, curv_h, curv_k);
         return SMAD_OK;
     });
 }

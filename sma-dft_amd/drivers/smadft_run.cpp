@@ -17,10 +17,15 @@
 // plain/adv, 50 for quasi), append L1 = sum|psi'-psi| / (dt sum|psi'|) to
 // L1.dat, and stop once L1 <= --l1 (1e-6, cosNoAdvTrackMult.cpp:696) or
 // after --steps steps.  Quasi also appends mass.dat and energy.dat
-// (quasi.cpp:2223-2248).  Every --save-every L1 blocks psi.dat is rewritten
-// (one value per line, k fastest, default stream precision as the
-// reference's writers) together with psi.bin, a full-precision restart
-// (header "SMADPSI1", int32 nx ny nz, int64 steps, then nx*ny*nz doubles).
+// (quasi.cpp:2223-2248).  Every --save-every L1 blocks (10) the save block
+// runs one step before the block's end, as the reference's does: psi.dat is
+// rewritten (one value per line, k fastest, default stream precision as the
+// reference's writers) with psi.bin, a full-precision restart (header
+// "SMADPSI1", int32 nx ny nz, int64 steps, then nx*ny*nz doubles); the
+// y = Ny/2 slice is appended to psiMid.dat and, for the plain model, the
+// surface diagnostics to surfPsi.dat, velSurf.dat, curvH.dat and curvK.dat
+// (cosNoAdvTrackMult.cpp:1049-1138).  At the end psi.dat and psi.bin hold the
+// final state.
 // A non-finite field ends the run with exit code 3 instead of the
 // reference's silent loop exit.
 #include "smadft/smadft.h"
@@ -181,6 +186,14 @@ bool append(const std::string& path, double x)
     return (bool)f;
 }
 
+// append n values, one per line, with the reference writers' separator
+bool append_all(const std::string& path, const double* v, size_t n, const char* sep)
+{
+    std::ofstream f(path, std::ios_base::app);
+    for (size_t i = 0; i < n; ++i) f << v[i] << sep;
+    return (bool)f;
+}
+
 int fail(smad_ctx* c, const char* where)
 {
     std::cerr << "smadft_run: " << where << ": " << smad_last_error(c) << "\n";
@@ -241,8 +254,35 @@ int main(int argc, char** argv)
              << taken << "\n";
     }
     if (taken == 0) {
-        for (const char* f : {"L1.dat", "mass.dat", "energy.dat"}) std::ofstream(dir + f).close();
+        for (const char* f : {"L1.dat", "mass.dat", "energy.dat", "psiMid.dat", "surfPsi.dat", "velSurf.dat",
+                              "curvH.dat", "curvK.dat"})
+            std::ofstream(dir + f).close();
     }
+    const size_t cols = (size_t)p.nx * p.ny;
+    std::vector<double> sf(4 * cols), mid((size_t)p.nx * p.nz);
+    // the save block (cosNoAdvTrackMult.cpp:850-1138): psi.dat and the restart,
+    // psiMid.dat (the y = Ny/2 slice) and, for the plain model, the surface
+    // files; `at` is the step count of the state written
+    auto save_fields = [&](long at) -> int {
+        if (smad_fetch(ctx, SMAD_FIELD_PSI, psi.data()) != SMAD_OK) return fail(ctx, "fetch");
+        for (int i = 0; i < p.nx; ++i)
+            for (int k = 0; k < p.nz; ++k) mid[(size_t)i * p.nz + k] = psi[((size_t)i * p.ny + p.ny / 2) * p.nz + k];
+        bool ok = write_ascii(dir + "psi.dat", psi) && write_restart(dir + "psi.bin", p, at, psi) &&
+                  append_all(dir + "psiMid.dat", mid.data(), mid.size(), "\n");
+        if (ok && o.model == SMAD_MODEL_PLAIN) {
+            if (smad_surface(ctx, sf.data(), sf.data() + cols, sf.data() + 2 * cols, sf.data() + 3 * cols) != SMAD_OK)
+                return fail(ctx, "surface");
+            ok = append_all(dir + "surfPsi.dat", sf.data(), cols, "\n") &&
+                 append_all(dir + "velSurf.dat", sf.data() + cols, cols, "\n ") &&
+                 append_all(dir + "curvH.dat", sf.data() + 2 * cols, cols, "\n ") &&
+                 append_all(dir + "curvK.dat", sf.data() + 3 * cols, cols, "\n ");
+        }
+        if (!ok) {
+            std::cerr << "smadft_run: cannot write fields in " << dir << "\n";
+            return 1;
+        }
+        return 0;
+    };
 
     const long limit = o.steps > 0 ? o.steps : o.max_steps;
     long done = 0;
@@ -250,7 +290,14 @@ int main(int argc, char** argv)
     double l1 = INFINITY;
     while (done < limit && (o.steps > 0 || l1 > o.l1_lim)) {
         int k = (int)std::min<long>(l1_every, limit - done);
-        if (smad_step(ctx, k) != SMAD_OK) { rc = fail(ctx, "step"); break; }
+        const bool save = o.save_every && blocks + 1 == o.save_every;
+        // the reference's save block sees the state one step before the
+        // block's end (it runs before psi <- psi_new)
+        if (smad_step(ctx, save ? k - 1 : k) != SMAD_OK) { rc = fail(ctx, "step"); break; }
+        if (save) {
+            if ((rc = save_fields(taken + done + k - 1)) != 0) break;
+            if (smad_step(ctx, 1) != SMAD_OK) { rc = fail(ctx, "step"); break; }
+        }
         done += k;
         double red[3];
         int r = smad_reduce(ctx, SMAD_RED_L1, red);
@@ -271,15 +318,7 @@ int main(int argc, char** argv)
             append(dir + "mass.dat", m[0]);
             append(dir + "energy.dat", e[0]);
         }
-        if (o.save_every && ++blocks == o.save_every) {
-            blocks = 0;
-            if (smad_fetch(ctx, SMAD_FIELD_PSI, psi.data()) != SMAD_OK) { rc = fail(ctx, "fetch"); break; }
-            if (!write_ascii(dir + "psi.dat", psi) || !write_restart(dir + "psi.bin", p, taken + done, psi)) {
-                std::cerr << "smadft_run: cannot write fields in " << dir << "\n";
-                rc = 1;
-                break;
-            }
-        }
+        blocks = save ? 0 : blocks + 1;
     }
     if (rc == 0) {
         if (smad_fetch(ctx, SMAD_FIELD_PSI, psi.data()) != SMAD_OK) {

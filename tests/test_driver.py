@@ -88,3 +88,28 @@ def test_plain_restart_and_l1_stop(smadft, tmp_path):
     assert r.returncode == 0, r.stderr
     l1 = np.atleast_1d(np.loadtxt(tmp_path / "stop" / "L1.dat"))
     assert l1[-1] <= 1e-3 and (l1[:-1] > 1e-3).all()
+
+
+@pytest.mark.gpu
+def test_plain_save_block_surface_files(smadft, tmp_path):
+    """Save blocks write psiMid.dat and the surface files of the state one
+    step before the block's end, as the reference's save block does."""
+    out = tmp_path / "run"
+    n = 32
+    r = run(["plain", "--n", str(n), "--steps", "40", "--l1-every", "10", "--save-every", "2",
+             "--ic", "curved", "--amp", "1.328", "--seed", "4", "--out", str(out)])
+    assert r.returncode == 0, r.stderr
+    files = {k: np.loadtxt(out / f"{k}.dat").reshape(2, -1)
+             for k in ("surfPsi", "velSurf", "curvH", "curvK", "psiMid")}
+    p = smadft.Params.default(smadft.MODEL_PLAIN, n)
+    psi0 = smadft.ic_curved(p, 1.328, 1e-3, seed=4)
+    with smadft.Stepper(p, psi0) as s:
+        for blk, nsteps in enumerate((19, 20)):
+            s.step(nsteps)
+            sf = s.surface()
+            psi = s.fetch()
+            assert np.array_equal(files["surfPsi"][blk], sf["surf"].ravel())
+            assert (sf["surf"] >= 0).any()
+            for key, name in (("vel", "velSurf"), ("curv_h", "curvH"), ("curv_k", "curvK")):
+                assert np.allclose(files[name][blk], sf[key].ravel(), rtol=1e-5, atol=1e-6 * np.abs(sf[key]).max())
+            assert np.allclose(files["psiMid"][blk], psi[:, n // 2, :].ravel(), rtol=1e-5, atol=1e-6)

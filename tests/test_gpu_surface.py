@@ -1,0 +1,50 @@
+"""GPU surface diagnostics (smad_surface) against the oracle's restatement of
+no-adv/cosNoAdvTrackMult.cpp:787-796, 853-1022 on the same psi."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def run_state(smadft, n, prec, steps):
+    p = smadft.Params.default(smadft.MODEL_PLAIN, n)
+    p.precision = prec
+    psi0 = smadft.ic_curved(p, 1.328, 1e-3, seed=4)
+    s = smadft.Stepper(p, psi0)
+    s.step(steps)
+    return p, s
+
+
+@pytest.mark.parametrize("n,steps", [(32, 30), (64, 60)])
+def test_surface_fp64_matches_oracle(smadft, oracle, n, steps):
+    p, s = run_state(smadft, n, 64, steps)
+    with s:
+        psi = s.fetch()
+        got = s.surface()
+    ref = oracle.surface(p, psi)
+    assert (ref["surf"] >= 0).sum() > n * n // 4  # the curved interface is found
+    assert np.array_equal(got["surf"], ref["surf"])
+    for key in ("vel", "curv_h", "curv_k"):
+        scale = np.abs(ref[key]).max()
+        assert np.max(np.abs(got[key] - ref[key])) < 1e-8 * scale, key
+
+
+def test_surface_fp32_mixed_close(smadft, oracle):
+    p, s = run_state(smadft, 64, 32, 60)
+    with s:
+        psi = s.fetch()
+        got = s.surface()
+    ref = oracle.surface(p, psi)
+    same = got["surf"] == ref["surf"]
+    assert same.mean() > 0.99
+    for key in ("vel", "curv_h", "curv_k"):
+        scale = np.abs(ref[key][same]).max()
+        assert np.max(np.abs(got[key][same] - ref[key][same])) < 1e-3 * scale, key
+
+
+def test_surface_is_plain_only(smadft):
+    p = smadft.Params.default(smadft.MODEL_ADV, 16)
+    with smadft.Stepper(p, smadft.ic_slab(p, 1.328, 0.0, seed=0)) as s:
+        with pytest.raises(smadft.SmadError) as e:
+            s.surface()
+        assert e.value.code == smadft.EINVAL

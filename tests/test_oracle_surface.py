@@ -1,0 +1,126 @@
+"""Pins the oracle's surface diagnostics (no-adv/cosNoAdvTrackMult.cpp:787-796,
+853-1022) against closed forms: single cosine modes for the auxiliary
+spectral fields, spherical and planar level sets for the curvatures, and
+hand-built columns for the scan."""
+import numpy as np
+import pytest
+
+
+def plain_params(smadft, n):
+    p = smadft.Params.default(smadft.MODEL_PLAIN, n)
+    return p
+
+
+def test_aux_fields_single_mode(smadft, oracle):
+    p = plain_params(smadft, 16)
+    p.nx, p.ny, p.nz = 16, 12, 20
+    h = 2 * np.pi / (p.q0 * p.nw)
+    L = np.array([p.nx, p.ny, p.nz]) * h
+    m = (3, 2, 5)
+    idx = np.meshgrid(*[np.arange(n) + 0.5 for n in (p.nx, p.ny, p.nz)], indexing="ij")
+    q = [np.pi * mi / Li for mi, Li in zip(m, L)]
+    psi = 0.8 * np.cos(q[0] * idx[0] * h) * np.cos(q[1] * idx[1] * h) * np.cos(q[2] * idx[2] * h)
+    dxx, dyy, dzz, dt = oracle.surface_aux(p, psi)
+    assert np.allclose(dxx, -q[0] ** 2 * psi, atol=1e-12)
+    assert np.allclose(dyy, -q[1] ** 2 * psi, atol=1e-12)
+    assert np.allclose(dzz, -q[2] ** 2 * psi, atol=1e-12)
+    q2 = sum(qi ** 2 for qi in q)
+    alin = p.eps - p.alpha * (q2 - p.q0 ** 2) ** 2
+    assert np.allclose(dt, alin * psi + p.beta * psi ** 3 - p.gamma * psi ** 5, atol=1e-12)
+
+
+def radial_case(n=64, R=20.0, w=8.0, c=(31.3, 30.6, 14.2)):
+    """psi = tanh((r - R)/w) (grid units), with its analytic second
+    derivatives; level sets are spheres."""
+    x, y, z = np.meshgrid(*[np.arange(n, dtype=float)] * 3, indexing="ij")
+    X, Y, Z = x - c[0], y - c[1], z - c[2]
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    t = np.tanh((r - R) / w)
+    f1 = (1 - t ** 2) / w
+    f2 = -2 * t * (1 - t ** 2) / w ** 2
+    d2 = [f2 * (A / r) ** 2 + f1 * (1 / r - A ** 2 / r ** 3) for A in (X, Y, Z)]
+    return t, d2, (X, Y, Z, r), f1
+
+
+@pytest.mark.parametrize("c", [(31.3, 30.6, 14.2), (22.7, 39.1, 12.4)])
+def test_curvatures_of_spheres(oracle, c):
+    """The level set through a point at distance r from the centre is a
+    sphere: 2H = 2/r, K = 1/r^2.  Off the column axes every term of both
+    formulas (the mixed derivatives included) contributes."""
+    R = 20.0
+    psi, (dxx, dyy, dzz), (X, Y, Z, r), f1 = radial_case(R=R, c=c)
+    out = oracle.surface_track(psi, dxx, dyy, dzz, np.ones_like(psi), 1.0)
+    s = out["surf"]
+    ii, jj = np.nonzero(s >= 0)
+    assert len(ii) > 1000
+    k2 = s[ii, jj].astype(int)
+    rp = r[ii, jj, k2]
+    # the crossing sits next to the sphere
+    assert np.all(np.abs(rp - R) < 1.0)
+    # columns away from the sphere's rim, where the gradient turns horizontal
+    Xp, Yp = X[ii, jj, k2], Y[ii, jj, k2]
+    inner = (Xp ** 2 + Yp ** 2) < (0.7 * R) ** 2
+    # the tested points exercise the mixed terms: n_x n_y up to ~0.2
+    assert np.max(np.abs(Xp * Yp)[inner] / rp[inner] ** 2) > 0.2
+    h = out["curv_h"][ii, jj][inner]
+    kk = out["curv_k"][ii, jj][inner]
+    assert np.max(np.abs(h * rp[inner] / 2 - 1)) < 1e-2
+    assert np.max(np.abs(kk * rp[inner] ** 2 - 1)) < 2e-2
+    # vel = dTpsi / |grad psi| with dTpsi = 1 and |grad psi| = f'(r) up to O(h^2)
+    v = out["vel"][ii, jj][inner]
+    assert np.max(np.abs(v * f1[ii, jj, k2][inner] - 1)) < 1e-2
+
+
+def test_tilted_plane_has_no_curvature(oracle):
+    n = 40
+    x, y, z = np.meshgrid(*[np.arange(n, dtype=float)] * 3, indexing="ij")
+    th = 0.3
+    s = np.sin(th) * (x - 20) + np.cos(th) * (z - 20)
+    w = 4.0
+    psi = np.tanh(s / w)
+    t = psi
+    f2 = -2 * t * (1 - t ** 2) / w ** 2
+    dxx, dzz = f2 * np.sin(th) ** 2, f2 * np.cos(th) ** 2
+    out = oracle.surface_track(psi, dxx, np.zeros_like(psi), dzz, np.ones_like(psi), 1.0)
+    ii, jj = np.nonzero(out["surf"] >= 0)
+    assert len(ii) == n * n
+    # curvature scale 1/w: the residual is the O(h^2) finite-difference error
+    assert np.max(np.abs(out["curv_h"])) < 2e-2 / w
+    assert np.max(np.abs(out["curv_k"])) < 1e-6 / w ** 2
+
+
+def test_scan_rules(oracle):
+    nx, ny, nz = 3, 3, 16
+    psi = np.full((nx, ny, nz), 0.3)
+    # (0,0): arm at k=12, crossing at k=9 with |psi(9)| < |psi(10)|  -> 9
+    psi[0, 0, 12] = 0.9
+    psi[0, 0, 10] = 0.2
+    psi[0, 0, 9] = -0.1
+    # (0,1): crossing at k=9 but |psi(9)| > |psi(10)|                 -> 10
+    psi[0, 1, 12] = 0.9
+    psi[0, 1, 10] = 0.05
+    psi[0, 1, 9] = -0.3
+    # (0,2): never above 0.7                                           -> -1
+    psi[0, 2, 5] = -0.5
+    # (1,0): negative above the arming point is ignored                -> 3
+    psi[1, 0, 14] = -0.4
+    psi[1, 0, 8] = 0.8
+    psi[1, 0, 3] = -0.2
+    psi[1, 0, 4] = 0.25
+    # (1,1): armed, never negative                                     -> -1
+    psi[1, 1, 7] = 0.75
+    # (1,2): only the first crossing counts                            -> 6
+    psi[1, 2, 11] = 0.71
+    psi[1, 2, 6] = -0.2
+    psi[1, 2, 7] = 0.4
+    psi[1, 2, 2] = -0.6
+    z = np.zeros_like(psi)
+    out = oracle.surface_track(psi, z, z, z, z, 0.5)
+    assert out["surf"][0, 0] == 9
+    assert out["surf"][0, 1] == 10
+    assert out["surf"][0, 2] == -1
+    assert out["surf"][1, 0] == 3
+    assert out["surf"][1, 1] == -1
+    assert out["surf"][1, 2] == 6
+    for key in ("vel", "curv_h", "curv_k"):
+        assert out[key][0, 2] == 0 and out[key][1, 1] == 0
