@@ -35,7 +35,9 @@ typedef struct smad_ctx smad_ctx;
 enum {
     SMAD_MODEL_PLAIN = 0, /* non-conserved SH relaxation, CN/AB2 */
     SMAD_MODEL_ADV = 1,   /* + uniform-density Stokes advection */
-    SMAD_MODEL_QUASI = 2  /* quasi-incompressible, varying density, BDF2/AB2 */
+    SMAD_MODEL_QUASI = 2, /* quasi-incompressible, varying density, BDF2/AB2 */
+    SMAD_MODEL_PLAIN_CONSERVED = 3 /* conserved dynamics: the plain step with aLin and
+                                      N_hat times |q|^2 (no-adv/cosNoAdvConserv.cpp:647-697, :809) */
 };
 
 enum { SMAD_FP64 = 64, SMAD_FP32_MIXED = 32 };

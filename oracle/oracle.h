@@ -38,7 +38,9 @@ typedef struct {
  * orc_plain_preloop: nq = REDFT10^3(beta psi^3 - gamma psi^5)
  *   (no-adv/cosNoAdvTrackMult.cpp:630-678).
  * orc_plain_step: nsteps iterations of the loop body
- *   (no-adv/cosNoAdvTrackMult.cpp:729-805 and :1147).  */
+ *   (no-adv/cosNoAdvTrackMult.cpp:729-805 and :1147).
+ * model = 3 selects the conserved variant (no-adv/cosNoAdvConserv.cpp:647-697,
+ *   :809): aLin and Nq both carry a factor |q|^2.  */
 void orc_plain_preloop(const orc_params *p, const double *psi, double *nq);
 void orc_plain_step(const orc_params *p, double *psi, double *psiq, double *nq, double *nq_past,
                     int nsteps);

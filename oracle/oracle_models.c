@@ -84,6 +84,26 @@ static void nonlin(const orc_params *p, const double *psi, double *nr, long n)
     for (long i = 0; i < n; ++i) nr[i] = p->beta * pow(psi[i], 3) - p->gamma * pow(psi[i], 5);
 }
 
+/* |q|^2 of the conserved variant (no-adv/cosNoAdvConserv.cpp:647); 1 for
+ * the non-conserved plain model */
+static double cons_mq2(const orc_params *p, const grid_t *gp, int i, int j, int k)
+{
+    if (p->model != 3) return 1.0;
+    return pow(gp->qx[i], 2) + pow(gp->qy[j], 2) + pow(gp->qz[k], 2);
+}
+
+/* Nq <- |q|^2 Nq in the conserved variant (cosNoAdvConserv.cpp:690-697, :809) */
+static void cons_scale_nq(const orc_params *p, const grid_t *gp, double *nq)
+{
+    if (p->model != 3) return;
+    for (int i = 0; i < gp->nx; ++i)
+        for (int j = 0; j < gp->ny; ++j)
+            for (int k = 0; k < gp->nz; ++k) {
+                long id = ((long)i * gp->ny + j) * gp->nz + k;
+                nq[id] = cons_mq2(p, gp, i, j, k) * nq[id];
+            }
+}
+
 void orc_plain_preloop(const orc_params *p, const double *psi, double *nq)
 {
     grid_t g;
@@ -91,6 +111,7 @@ void orc_plain_preloop(const orc_params *p, const double *psi, double *nq)
     double *nr = malloc(sizeof(double) * g.n);
     nonlin(p, psi, nr, g.n);
     ct(&g, nr, nq);
+    cons_scale_nq(p, &g, nq);
     free(nr);
     grid_free(&g);
 }
@@ -107,12 +128,14 @@ void orc_plain_step(const orc_params *p, double *psi, double *psiq, double *nq, 
         nonlin(p, psi, nr, g.n);                              /* :760-766 */
         ct(&g, nr, nq);                                       /* :770 */
         ct(&g, psi, psiq);                                    /* :774 */
+        cons_scale_nq(p, &g, nq);                             /* conserved: cosNoAdvConserv.cpp:809 */
 #pragma omp parallel for collapse(2) schedule(static)
         for (int i = 0; i < g.nx; ++i)                        /* :778-800 */
             for (int j = 0; j < g.ny; ++j)
                 for (int k = 0; k < g.nz; ++k) {
                     long id = IDX(i, j, k);
-                    double al = alin_at(p, &g, i, j, k);
+                    /* conserved: aLin = |q|^2 (eps - alpha (|q|^2 - q0^2)^2), cosNoAdvConserv.cpp:649 */
+                    double al = cons_mq2(p, &g, i, j, k) * alin_at(p, &g, i, j, k);
                     double c1 = 1.0 + dtd2 * al, c2 = 1.0 - dtd2 * al;
                     psiq[id] = g.scale * (c1 * psiq[id] + dtd2 * (3.0 * nq[id] - nq_past[id])) / c2;
                 }

@@ -281,6 +281,7 @@ __global__ __launch_bounds__(WG) void k_axis(const T* in, T* out, Dims d, int ax
 struct PlainCoef {
     double qfx, qfy, qfz;  // DCT wavenumber step pi/L per axis
     double eps, alpha, q0sq, beta, gamma, dtd2, scale;
+    int conserved;  // conserved dynamics: aLin and Nq carry |q|^2 (no-adv/cosNoAdvConserv.cpp:649, 809)
 };
 
 // z pass: psi = REDFT01_z(w) (or psi read directly when from_real),
@@ -374,11 +375,14 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_plain_z_real(const T* in, T*
 
 // CN/AB2 update of one spectral value, FP64 arithmetic:
 //   psihat' = (C1 psihat + dtd2 s (3 Nq - Nq_past)) / C2,
-//   C1 = 1 + dtd2 aLin, C2 = 1 - dtd2 aLin, aLin = eps - alpha (|q|^2 - q0^2)^2.
+//   C1 = 1 + dtd2 aLin, C2 = 1 - dtd2 aLin, aLin = eps - alpha (|q|^2 - q0^2)^2
+//   (times |q|^2 in the conserved variant, whose Nq also carries |q|^2).
+template <bool CONS>
 __device__ __forceinline__ double cn_ab2(const PlainCoef& cf, double mq2, double ph, double nq, double np)
 {
     double t = mq2 - cf.q0sq;
     double al = cf.eps - cf.alpha * t * t;
+    if constexpr (CONS) al = mq2 * al;
     double c1 = 1.0 + cf.dtd2 * al, c2 = 1.0 - cf.dtd2 * al;
     return (c1 * ph + cf.dtd2 * cf.scale * (3.0 * nq - np)) / c2;
 }
@@ -386,10 +390,10 @@ __device__ __forceinline__ double cn_ab2(const PlainCoef& cf, double mq2, double
 // x pass: Nq = REDFT10_x(w); CN/AB2 update of psihat with the history nhat
 // (in place: nhat holds Nq_past on entry, Nq on exit); w = REDFT01_x(psihat').
 // psihat is the pre-scaled spectral field.
-template <typename T, int N>
-__global__ __launch_bounds__(WG) void k_plain_x(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
-                                                T* __restrict__ nhat, Dims d, PlainCoef cf,
-                                                int inverse, Tw<T> tw)
+template <typename T, int N, bool CONS>
+__device__ __forceinline__ void plain_x_body(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
+                                             T* __restrict__ nhat, const Dims& d, const PlainCoef& cf,
+                                             int inverse, const Tw<T>& tw)
 {
     constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
     constexpr int W = V16<T>::W;
@@ -424,7 +428,8 @@ __global__ __launch_bounds__(WG) void k_plain_x(const T* a_in, T* w_out, const T
                 double qz = (k0 + c0 + t) * cf.qfz;
                 double mq2 = qx * qx + qy * qy + qz * qz;
                 T nqt = lds[(c0 + t) * LS + PR(p)];
-                double r = cn_ab2(cf, mq2, (double)ph.v[t], (double)nqt, (double)np.v[t]);
+                if constexpr (CONS) nqt = T(mq2 * (double)nqt);
+                double r = cn_ab2<CONS>(cf, mq2, (double)ph.v[t], (double)nqt, (double)np.v[t]);
                 nq.v[t] = nqt;
                 out.v[t] = T(r);
                 lds[(c0 + t) * LS + PR(p)] = out.v[t];
@@ -438,7 +443,8 @@ __global__ __launch_bounds__(WG) void k_plain_x(const T* a_in, T* w_out, const T
                 double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
                 double mq2 = qx * qx + qy * qy + qz * qz;
                 T nqt = lds[c * LS + PR(p)];
-                double r = cn_ab2(cf, mq2, (double)ph_in[gi], (double)nqt, (double)nhat[gi]);
+                if constexpr (CONS) nqt = T(mq2 * (double)nqt);
+                double r = cn_ab2<CONS>(cf, mq2, (double)ph_in[gi], (double)nqt, (double)nhat[gi]);
                 ph_out[gi] = T(r);
                 nhat[gi] = nqt;
                 lds[c * LS + PR(p)] = T(r);
@@ -452,6 +458,21 @@ __global__ __launch_bounds__(WG) void k_plain_x(const T* a_in, T* w_out, const T
     dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
     __syncthreads();
     store_rows<T, N, NL, LS>(g, w_out, lds, Mk{}, T(1));
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(WG) void k_plain_x(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
+                                                T* __restrict__ nhat, Dims d, PlainCoef cf, int inverse, Tw<T> tw)
+{
+    plain_x_body<T, N, false>(a_in, w_out, ph_in, ph_out, nhat, d, cf, inverse, tw);
+}
+// conserved variant: aLin and Nq times |q|^2
+template <typename T, int N>
+__global__ __launch_bounds__(WG) void k_plain_x_cons(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
+                                                     T* __restrict__ nhat, Dims d, PlainCoef cf, int inverse,
+                                                     Tw<T> tw)
+{
+    plain_x_body<T, N, true>(a_in, w_out, ph_in, ph_out, nhat, d, cf, inverse, tw);
 }
 
 // ------------------------------------------------------------------------
@@ -1005,6 +1026,20 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_nl_z(const T* __restrict__
     });
 }
 
+// a <- |q|^2 a on the spectral grid (the conserved variant's N_hat^0,
+// no-adv/cosNoAdvConserv.cpp:690-697)
+template <typename T>
+__global__ __launch_bounds__(WG) void k_mul_mq2(T* __restrict__ a, Dims d, PlainCoef cf)
+{
+    const long n = d.total();
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        const double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
+        a[id] = T((qx * qx + qy * qy + qz * qz) * (double)a[id]);
+    }
+}
+
 // ------------------------------------------------------------------------
 // Surface diagnostics of the plain model (no-adv/cosNoAdvTrackMult.cpp:787-796,
 // 853-1022): interface tracker, normal velocity and curvatures.
@@ -1421,6 +1456,7 @@ struct Launch {
                       const Tw<T>& tw);
     static void spass(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, int axis, const SpecCoef& sc,
                       const Tw<T>& tw);
+    static void mul_mq2(hipStream_t s, T* a, Dims d, const PlainCoef& cf);
     static void surf_spec(hipStream_t s, const T* ph, const T* nq, T* dxx, T* dyy, T* dzz, T* dt, Dims d,
                           const PlainCoef& cf);
     static void surf_track(hipStream_t s, const T* psi, const T* dxx, const T* dyy, const T* dzz, const T* dt,

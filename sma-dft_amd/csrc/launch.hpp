@@ -58,7 +58,11 @@ void Launch<T>::plain_x(hipStream_t s, const T* a_in, T* w_out, const T* ph_in, 
                         T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse)
 {
     dim3 grid(num_tiles(d, 0, Lines<T>::NL));
-    SMAD_DISPATCH(d.nx, k_plain_x, grid, a_in, w_out, ph_in, ph_out, nhat, d, cf, inverse, tw);
+    if (cf.conserved) {
+        SMAD_DISPATCH(d.nx, k_plain_x_cons, grid, a_in, w_out, ph_in, ph_out, nhat, d, cf, inverse, tw);
+    } else {
+        SMAD_DISPATCH(d.nx, k_plain_x, grid, a_in, w_out, ph_in, ph_out, nhat, d, cf, inverse, tw);
+    }
 }
 
 inline dim3 flat_grid(long n)
@@ -112,6 +116,12 @@ void Launch<T>::spass(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, 
         throw std::logic_error("spass: outputs must be (OP_SHIFT, OP_ID)");
     dim3 grid(num_tiles(d, axis, Lines<T>::NL)), block(WG);
     SMAD_DISPATCH_OPS(axis_len(d, axis), k_multi, (OP_SHIFT, OP_ID), grid, block, in, ps, d, axis, sc, tw);
+}
+
+template <typename T>
+void Launch<T>::mul_mq2(hipStream_t s, T* a, Dims d, const PlainCoef& cf)
+{
+    hipLaunchKernelGGL(k_mul_mq2<T>, flat_grid(d.total()), dim3(WG), 0, s, a, d, cf);
 }
 
 template <typename T>

@@ -260,6 +260,7 @@ struct PlainEngine : EngineBase {
         cf.gamma = p.gamma;
         cf.dtd2 = p.dt / 2;
         cf.scale = 0.125 / ((double)p.nx * p.ny * p.nz);
+        cf.conserved = p.model == SMAD_MODEL_PLAIN_CONSERVED ? 1 : 0;
         twx.make(p.nx, s);
         twy.make(p.ny, s);
         twz.make(p.nz, s);
@@ -286,6 +287,7 @@ struct PlainEngine : EngineBase {
         Launch<T>::plain_z(s, psi.p, w.p, nullptr, d, cf, 1, twz.tw);
         Launch<T>::axis(s, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
         Launch<T>::axis(s, w.p, nhat.p, d, 0, REDFT10, T(1), twx.tw);
+        if (cf.conserved) Launch<T>::mul_mq2(s, nhat.p, d, cf);
         ct3(psi.p, psihat.p, T(cf.scale));
         Launch<T>::axis(s, psihat.p, w.p, d, 0, REDFT01, T(1), twx.tw);
         Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
@@ -397,6 +399,7 @@ struct PlainEngine : EngineBase {
 
     void forward_tape(int K, int ck) override
     {
+        if (cf.conserved) throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
         if (!fresh) throw ApiError{SMAD_ESTATE, "forward_tape must start from the pre-loop state (after init or set_field(PSI))"};
         if (K < 1 || ck < 1) throw ApiError{SMAD_EINVAL, "K and ckpt_every must be >= 1"};
         ck = std::min(ck, K);
@@ -510,6 +513,7 @@ struct PlainEngine : EngineBase {
 
     void surface(double* surf, double* vel, double* ch, double* ck) override
     {
+        if (cf.conserved) throw ApiError{SMAD_EINVAL, "surface diagnostics are defined for the plain model"};
         const size_t cols = (size_t)p.nx * p.ny;
         if (!sf_nq.p) {
             for (DevBuf<T>* b : {&sf_dxx, &sf_dyy, &sf_dzz, &sf_dt, &sf_nq}) b->alloc(n);
@@ -1158,11 +1162,12 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
         g_init_error = "smad_init: precision must be 64 or 32";
         return SMAD_EINVAL;
     }
-    if (p.model != SMAD_MODEL_PLAIN && p.model != SMAD_MODEL_ADV && p.model != SMAD_MODEL_QUASI) {
+    if (p.model != SMAD_MODEL_PLAIN && p.model != SMAD_MODEL_ADV && p.model != SMAD_MODEL_QUASI &&
+        p.model != SMAD_MODEL_PLAIN_CONSERVED) {
         g_init_error = "smad_init: unknown model";
         return SMAD_EINVAL;
     }
-    if (p.model != SMAD_MODEL_PLAIN && !(p.nu > 0)) {
+    if ((p.model == SMAD_MODEL_ADV || p.model == SMAD_MODEL_QUASI) && !(p.nu > 0)) {
         g_init_error = "smad_init: nu must be positive";
         return SMAD_EINVAL;
     }
@@ -1176,7 +1181,7 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
     try {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         const bool f64 = p.precision == SMAD_FP64;
-        if (p.model == SMAD_MODEL_PLAIN) {
+        if (p.model == SMAD_MODEL_PLAIN || p.model == SMAD_MODEL_PLAIN_CONSERVED) {
             if (f64) c->eng.reset(new PlainEngine<double>(p, c->stream));
             else c->eng.reset(new PlainEngine<float>(p, c->stream));
         } else if (p.model == SMAD_MODEL_ADV) {

@@ -12,6 +12,7 @@
 //   smadft_run plain [opts]
 //   smadft_run adv   <nu> <ep_arg> <aE> <load> [opts]
 //   smadft_run quasi <nu> <ep_arg> <aE> [opts]
+//   smadft_run conserved <ep_arg> <aE> <N> [opts]   (no-adv/cosNoAdvConserv.cpp:137-163)
 //
 // Loop control follows the reference: step `--l1-every` steps (100 for
 // plain/adv, 50 for quasi), append L1 = sum|psi'-psi| / (dt sum|psi'|) to
@@ -68,6 +69,7 @@ struct Opts {
     std::cerr << "usage: smadft_run plain [opts]\n"
                  "       smadft_run adv <nu> <ep_arg> <aE> <load> [opts]\n"
                  "       smadft_run quasi <nu> <ep_arg> <aE> [opts]\n"
+                 "       smadft_run conserved <ep_arg> <aE> <N> [opts]\n"
                  "opts: --n N | --nx/--ny/--nz N, --fp32, --steps K, --l1 LIM, --l1-every S,\n"
                  "      --save-every B, --max-steps K, --ic focal|slab|curved, --amp A,\n"
                  "      --noise X, --seed S, --ae A, --eps E, --dt DT, --out DIR,\n"
@@ -106,6 +108,14 @@ Opts parse(int argc, char** argv)
         o.nu = num(argv[2], "nu");
         o.ep_arg = num(argv[3], "ep_arg");
         o.ae = num(argv[4], "aE");
+        pos = 5;
+    } else if (m == "conserved") {
+        // no-adv/cosNoAdvConserv.cpp:137-156: ep_arg (eps = -0.01 ep_arg), aE, N
+        o.model = SMAD_MODEL_PLAIN_CONSERVED;
+        if (argc < 5) usage("conserved needs <ep_arg> <aE> <N>");
+        o.ep_arg = num(argv[2], "ep_arg");
+        o.ae = num(argv[3], "aE");
+        o.n = (int)num(argv[4], "N");
         pos = 5;
     } else {
         usage(("unknown model " + m).c_str());
@@ -215,7 +225,7 @@ int main(int argc, char** argv)
         p.nu = o.nu;
         p.lambda = -2.0 * o.nu / 3.0;  // quasi.cpp:248
     }
-    if (!std::isnan(o.ep_arg)) p.eps = -0.001 * o.ep_arg;
+    if (!std::isnan(o.ep_arg)) p.eps = (o.model == SMAD_MODEL_PLAIN_CONSERVED ? -0.01 : -0.001) * o.ep_arg;
     if (!std::isnan(o.eps)) p.eps = o.eps;
     if (!std::isnan(o.dt)) p.dt = o.dt;
     const int l1_every = o.l1_every ? o.l1_every : (o.model == SMAD_MODEL_QUASI ? 50 : 100);

@@ -1,0 +1,72 @@
+"""Pins the oracle's conserved plain variant (no-adv/cosNoAdvConserv.cpp:647-697,
+:780-812): aLin and Nq carry |q|^2.  Closed forms: single linear modes decay
+by the CN factor of |q|^2 L; the mean of psi (the q = 0 mode) is conserved
+exactly; the first step from the pre-loop equals a scipy restatement of the
+scheme (Euler in N on the first step)."""
+import numpy as np
+import pytest
+import scipy.fft as sf
+
+from test_oracle import cos_mode
+
+
+def cparams(smadft, shape, **kw):
+    p = smadft.Params.default(smadft.MODEL_PLAIN_CONSERVED, shape[0])
+    p.nx, p.ny, p.nz = shape
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def q2_grid(p):
+    dx = 2 * np.pi / (p.q0 * p.nw)
+    q = [np.pi * np.arange(n) / (n * dx) for n in (p.nx, p.ny, p.nz)]
+    return q[0][:, None, None] ** 2 + q[1][None, :, None] ** 2 + q[2][None, None, :] ** 2
+
+
+def test_defaults_follow_the_reference(smadft):
+    p = smadft.Params.default(smadft.MODEL_PLAIN_CONSERVED, 64)
+    assert p.nw == 16 and p.dt == 0.0005
+
+
+@pytest.mark.parametrize("mode", [(1, 2, 3), (5, 0, 7), (3, 3, 3)])
+def test_conserved_linear_mode_decay(smadft, oracle, mode):
+    shape = (16, 8, 24)
+    p = cparams(smadft, shape, beta=0.0, gamma=0.0, eps=-0.3, dt=0.01)
+    psi0 = cos_mode(shape, *mode)
+    o = oracle.PlainOracle(p, psi0)
+    nst = 30
+    o.step(nst)
+    dx = 2 * np.pi / (p.q0 * p.nw)
+    q2 = sum((np.pi * m / (n * dx)) ** 2 for m, n in zip(mode, shape))
+    lin = q2 * (p.eps - p.alpha * (q2 - p.q0 ** 2) ** 2)
+    g = (1 + p.dt / 2 * lin) / (1 - p.dt / 2 * lin)
+    err = np.linalg.norm(o.psi - g ** nst * psi0) / np.linalg.norm(psi0)
+    assert err < 1e-13
+
+
+def test_conserved_mean_is_exact(smadft, oracle):
+    shape = (16, 16, 16)
+    p = cparams(smadft, shape)
+    psi0 = 0.3 + smadft.ic_curved(p, 1.328, 1e-3, seed=2)
+    o = oracle.PlainOracle(p, psi0)
+    o.step(40)
+    assert abs(o.psi.mean() - psi0.mean()) < 1e-13
+    assert np.linalg.norm(o.psi - psi0) > 1e-4  # it did evolve
+
+
+def test_conserved_first_step_scipy(smadft, oracle):
+    shape = (16, 12, 20)
+    p = cparams(smadft, shape, dt=0.002)
+    psi0 = 0.2 + smadft.ic_curved(p, 1.328, 1e-2, seed=5)
+    o = oracle.PlainOracle(p, psi0)
+    o.step(1)
+    s = 1.0 / (8 * np.prod(shape))
+    q2 = q2_grid(p)
+    L = q2 * (p.eps - p.alpha * (q2 - p.q0 ** 2) ** 2)
+    psiq = sf.dctn(psi0, type=2)
+    nq = q2 * sf.dctn(p.beta * psi0 ** 3 - p.gamma * psi0 ** 5, type=2)
+    # first step: Nq_past = Nq (pre-loop), so 3 Nq - Nq_past = 2 Nq
+    new = s * ((1 + p.dt / 2 * L) * psiq + p.dt / 2 * (3 * nq - nq)) / (1 - p.dt / 2 * L)
+    ref = sf.dctn(new, type=3)
+    assert np.max(np.abs(o.psi - ref)) < 1e-12 * np.max(np.abs(ref))
