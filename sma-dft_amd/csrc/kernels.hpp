@@ -1336,7 +1336,11 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restr
     constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
+    // FP64 only: in FP32 the FP64 multiply of all packed values of a thread
+    // in flight costs more registers (169 vs 78) than the LDS pass it saves
+    constexpr bool FUSED = OpFused<N, TPL>::value && sizeof(T) == 8;
     __shared__ T lds[NL * LS];
+    __shared__ T tl[FUSED ? NL * LS : 1];  // psi tape at Makhoul positions
     __shared__ T ct[N + 1];
     load_cosq<T, N>(ct, tw);
     __syncthreads();
@@ -1345,25 +1349,50 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restr
         V16<T> v = *reinterpret_cast<const V16<T>*>(w + gi);
 #pragma unroll
         for (int t = 0; t < W; ++t) lds[c * LS + PR(p + t)] = v.v[t];
-    });
-    lsync<true>();
-    dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
-    double sums[2] = {0.0, 0.0};
-    const double b3 = 3.0 * cf.beta, g5 = 5.0 * cf.gamma;
-    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
-        V16<T> ps = *reinterpret_cast<const V16<T>*>(psi_tape + gi);
+        if constexpr (FUSED) {
+            V16<T> ps = *reinterpret_cast<const V16<T>*>(psi_tape + gi);
 #pragma unroll
-        for (int t = 0; t < W; ++t) {
-            int q = c * LS + PR(mk_pos<N>(p + t));
-            double r = (double)lds[q];
-            double x = (double)ps.v[t], x2 = x * x;
-            sums[0] += r * x2 * x;
-            sums[1] -= r * x2 * x2 * x;
-            lds[q] = T((b3 * x2 - g5 * x2 * x2) * r);
+            for (int t = 0; t < W; ++t) tl[c * LS + PR(mk_pos<N>(p + t))] = ps.v[t];
         }
     });
     lsync<true>();
-    dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    double sums[2] = {0.0, 0.0};
+    const double b3 = 3.0 * cf.beta, g5 = 5.0 * cf.gamma;
+    if constexpr (FUSED) {
+        // r = REDFT01_z(w) in registers, g = (3 beta psi^2 - 5 gamma psi^4) r
+        // with psi from the tape, then REDFT10_z(g)
+        const cpx<T>* tz = reinterpret_cast<const cpx<T>*>(tl + l * LS);
+        auto op = [&](cpx<T>& v, int m) {
+            const cpx<T> x = tz[PC(m)];
+            double r = (double)v.re, xv = (double)x.re, x2 = xv * xv;
+            sums[0] += r * x2 * xv;
+            sums[1] -= r * x2 * x2 * xv;
+            v.re = T((b3 * x2 - g5 * x2 * x2) * r);
+            r = (double)v.im;
+            xv = (double)x.im;
+            x2 = xv * xv;
+            sums[0] += r * x2 * xv;
+            sums[1] -= r * x2 * x2 * xv;
+            v.im = T((b3 * x2 - g5 * x2 * x2) * r);
+        };
+        dct3_map_dct2<T, N, TPL, true, true>(lds + l * LS, u, ct, op, false, [] {});
+    } else {
+        dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
+        zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+            V16<T> ps = *reinterpret_cast<const V16<T>*>(psi_tape + gi);
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                int q = c * LS + PR(mk_pos<N>(p + t));
+                double r = (double)lds[q];
+                double x = (double)ps.v[t], x2 = x * x;
+                sums[0] += r * x2 * x;
+                sums[1] -= r * x2 * x2 * x;
+                lds[q] = T((b3 * x2 - g5 * x2 * x2) * r);
+            }
+        });
+        lsync<true>();
+        dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    }
     zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v;
 #pragma unroll

@@ -628,7 +628,9 @@ struct OpFused {
     static constexpr bool value = I::size >= 2 && SplitFused<N, TPL>::value;
 };
 
-template <typename T, int N, int TPL, bool WAVE, typename Op, typename Mid>
+// The map may depend on the sample: with MAPC, op(v, m) receives packed
+// value m (samples 2m and 2m+1 of the Makhoul-ordered line) and updates it.
+template <typename T, int N, int TPL, bool WAVE, bool MAPC = false, typename Op, typename Mid>
 __device__ __forceinline__ void dct3_map_dct2(T* lr, int u, const T* ct, Op&& op, bool spill, Mid&& mid)
 {
     constexpr int M = N / 2;
@@ -651,12 +653,18 @@ __device__ __forceinline__ void dct3_map_dct2(T* lr, int u, const T* ct, Op&& op
         mid();
     }
 #pragma unroll
-    for (int p = 0; p < Bm::PER; ++p)
+    for (int p = 0; p < Bm::PER; ++p) {
+        const int b = u + p * TPL;
 #pragma unroll
         for (int r = 0; r < RM; ++r) {
-            v[p][r].re = op(v[p][r].re);
-            v[p][r].im = op(v[p][r].im);
+            if constexpr (MAPC) {
+                if (Bm::live(b)) op(v[p][r], b + r * Bm::NB);
+            } else {
+                v[p][r].re = op(v[p][r].re);
+                v[p][r].im = op(v[p][r].im);
+            }
         }
+    }
     bf_compute<T, N, RM, TPL, false, true>(v, u, 1, ct);
     bf_store<T, N, RM, TPL>(z, u, 1, v);
     lsync<WAVE>();
