@@ -36,8 +36,11 @@ enum {
     SMAD_MODEL_PLAIN = 0, /* non-conserved SH relaxation, CN/AB2 */
     SMAD_MODEL_ADV = 1,   /* + uniform-density Stokes advection */
     SMAD_MODEL_QUASI = 2, /* quasi-incompressible, varying density, BDF2/AB2 */
-    SMAD_MODEL_PLAIN_CONSERVED = 3 /* conserved dynamics: the plain step with aLin and
-                                      N_hat times |q|^2 (no-adv/cosNoAdvConserv.cpp:647-697, :809) */
+    SMAD_MODEL_PLAIN_CONSERVED = 3, /* conserved dynamics: the plain step with aLin and
+                                       N_hat times |q|^2 (no-adv/cosNoAdvConserv.cpp:647-697, :809) */
+    SMAD_MODEL_ADV_CONSERVED = 4    /* conserved advective step: C1, C2 with |q|^2 aLin and
+                                       N_hat = |q|^2 CT(beta psi^3 - gamma psi^5) - CT(u . grad psi)
+                                       (adv/cosAdvConSingle.cpp:741-742, :1290-1369) */
 };
 
 enum { SMAD_FP64 = 64, SMAD_FP32_MIXED = 32 };

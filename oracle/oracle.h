@@ -60,6 +60,9 @@ void orc_surface_track(int nx, int ny, int nz, double dx, double dy, double dz, 
                        double *surf, double *vel, double *curv_h, double *curv_k);
 
 /* ------------------------------------------------------- advective model -- */
+/* model = 4 selects the conserved advective variant (adv/cosAdvConSingle.cpp):
+ * C1, C2 with |q|^2 aLin, Nq = |q|^2 CT(beta psi^3 - gamma psi^5) - CT(u . grad psi);
+ * the pre-loop history is CT(beta psi^3 - gamma psi^5) without |q|^2, as there. */
 /* adv/cosAdvTrackMult.cpp:824-1369.  Pre-loop N without advection (:760-777).
  * State: psi (real), nq, nq_past.  Diagnostic fields (may be NULL):
  * grad[3*n] (real grad psi), vel[3*n] (real u). */

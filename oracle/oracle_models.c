@@ -276,19 +276,39 @@ void orc_adv_step(const orc_params *p, double *psi, double *nq, double *nq_past,
         orc_r2r3d(vz, vz, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_REDFT01, ORC_RODFT01);
 
         /* Nr :1290-1303, Nq :1307 */
+        if (p->model != 4) {
 #pragma omp parallel for schedule(static)
-        for (long id = 0; id < n; ++id)
-            nr[id] = p->beta * pow(psi[id], 3) - p->gamma * pow(psi[id], 5) - vx[id] * gx[id] -
-                     vy[id] * gy[id] - vz[id] * gz[id];
-        ct(&g, nr, nq);
+            for (long id = 0; id < n; ++id)
+                nr[id] = p->beta * pow(psi[id], 3) - p->gamma * pow(psi[id], 5) - vx[id] * gx[id] -
+                         vy[id] * gy[id] - vz[id] * gz[id];
+            ct(&g, nr, nq);
+        } else {
+            /* conserved variant (adv/cosAdvConSingle.cpp:1290-1310, 1369):
+             * Nq = |q|^2 CT(beta psi^3 - gamma psi^5) - CT(u . grad psi) */
+            for (long id = 0; id < n; ++id) {
+                nr[id] = p->beta * pow(psi[id], 3) - p->gamma * pow(psi[id], 5);
+                sx[id] = vx[id] * gx[id] + vy[id] * gy[id] + vz[id] * gz[id];
+            }
+            ct(&g, nr, nq);
+            ct(&g, sx, sx);
+            for (int i = 0; i < g.nx; ++i)
+                for (int j = 0; j < g.ny; ++j)
+                    for (int k = 0; k < g.nz; ++k) {
+                        long id = IDX(i, j, k);
+                        double mq2 = pow(g.qx[i], 2) + pow(g.qy[j], 2) + pow(g.qz[k], 2);
+                        nq[id] = mq2 * nq[id] - sx[id];
+                    }
+        }
 
-        /* CN/AB2 :1355-1365 with pre-scaled psiq */
+        /* CN/AB2 :1355-1365 with pre-scaled psiq; the conserved variant's
+         * C1, C2 carry |q|^2 aLin (cosAdvConSingle.cpp:741-742) */
 #pragma omp parallel for collapse(2) schedule(static)
         for (int i = 0; i < g.nx; ++i)
             for (int j = 0; j < g.ny; ++j)
                 for (int k = 0; k < g.nz; ++k) {
                     long id = IDX(i, j, k);
                     double al = alin_at(p, &g, i, j, k);
+                    if (p->model == 4) al = (pow(g.qx[i], 2) + pow(g.qy[j], 2) + pow(g.qz[k], 2)) * al;
                     double c1 = 1.0 + dtd2 * al, c2 = 1.0 - dtd2 * al;
                     psiq[id] = (c1 * psiq[id] + dtd2 * sc * (3.0 * nq[id] - nq_past[id])) / c2;
                 }

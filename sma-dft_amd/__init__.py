@@ -20,7 +20,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsmadft.so")
 
-MODEL_PLAIN, MODEL_ADV, MODEL_QUASI, MODEL_PLAIN_CONSERVED = 0, 1, 2, 3
+MODEL_PLAIN, MODEL_ADV, MODEL_QUASI, MODEL_PLAIN_CONSERVED, MODEL_ADV_CONSERVED = 0, 1, 2, 3, 4
 FP64, FP32_MIXED = 64, 32
 
 FIELD_PSI, FIELD_PSI_HAT, FIELD_N_HAT = 0, 1, 2

@@ -138,8 +138,8 @@ void smad_default_params(int model, int n, smad_params* o)
     // derivatives) diverges at dt = 0.05 for a persistent smectic (nw = 8);
     // 0.02 is stable
     if (model == SMAD_MODEL_ADV) p.dt = 0.02;
-    if (model == SMAD_MODEL_PLAIN_CONSERVED) {
-        // no-adv/cosNoAdvConserv.cpp:156-163: nw = 16, dt = 5e-4
+    if (model == SMAD_MODEL_PLAIN_CONSERVED || model == SMAD_MODEL_ADV_CONSERVED) {
+        // no-adv/cosNoAdvConserv.cpp:156-163, adv/cosAdvConSingle.cpp:169-180: nw = 16, dt = 5e-4
         p.nw = 16;
         p.dt = 0.0005;
     }

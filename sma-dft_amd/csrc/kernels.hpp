@@ -392,8 +392,8 @@ __device__ __forceinline__ double cn_ab2(const PlainCoef& cf, double mq2, double
 // psihat is the pre-scaled spectral field.
 template <typename T, int N, bool CONS>
 __device__ __forceinline__ void plain_x_body(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
-                                             T* __restrict__ nhat, const Dims& d, const PlainCoef& cf,
-                                             int inverse, const Tw<T>& tw)
+                                             T* __restrict__ nhat, const T* __restrict__ sub, const Dims& d,
+                                             const PlainCoef& cf, int inverse, const Tw<T>& tw)
 {
     constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
     constexpr int W = V16<T>::W;
@@ -421,6 +421,9 @@ __device__ __forceinline__ void plain_x_body(const T* a_in, T* w_out, const T* p
         rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
             V16<T> ph = *reinterpret_cast<const V16<T>*>(ph_in + gi);
             V16<T> np = *reinterpret_cast<const V16<T>*>(nhat + gi);
+            V16<T> sb{};
+            if constexpr (CONS)
+                if (sub) sb = *reinterpret_cast<const V16<T>*>(sub + gi);
             V16<T> nq, out;
             const double qx = p * cf.qfx;
 #pragma unroll
@@ -428,7 +431,7 @@ __device__ __forceinline__ void plain_x_body(const T* a_in, T* w_out, const T* p
                 double qz = (k0 + c0 + t) * cf.qfz;
                 double mq2 = qx * qx + qy * qy + qz * qz;
                 T nqt = lds[(c0 + t) * LS + PR(p)];
-                if constexpr (CONS) nqt = T(mq2 * (double)nqt);
+                if constexpr (CONS) nqt = T(mq2 * (double)nqt - (double)sb.v[t]);
                 double r = cn_ab2<CONS>(cf, mq2, (double)ph.v[t], (double)nqt, (double)np.v[t]);
                 nq.v[t] = nqt;
                 out.v[t] = T(r);
@@ -443,7 +446,7 @@ __device__ __forceinline__ void plain_x_body(const T* a_in, T* w_out, const T* p
                 double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
                 double mq2 = qx * qx + qy * qy + qz * qz;
                 T nqt = lds[c * LS + PR(p)];
-                if constexpr (CONS) nqt = T(mq2 * (double)nqt);
+                if constexpr (CONS) nqt = T(mq2 * (double)nqt - (sub ? (double)sub[gi] : 0.0));
                 double r = cn_ab2<CONS>(cf, mq2, (double)ph_in[gi], (double)nqt, (double)nhat[gi]);
                 ph_out[gi] = T(r);
                 nhat[gi] = nqt;
@@ -464,15 +467,16 @@ template <typename T, int N>
 __global__ __launch_bounds__(WG) void k_plain_x(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
                                                 T* __restrict__ nhat, Dims d, PlainCoef cf, int inverse, Tw<T> tw)
 {
-    plain_x_body<T, N, false>(a_in, w_out, ph_in, ph_out, nhat, d, cf, inverse, tw);
+    plain_x_body<T, N, false>(a_in, w_out, ph_in, ph_out, nhat, nullptr, d, cf, inverse, tw);
 }
-// conserved variant: aLin and Nq times |q|^2
+// conserved variants: aLin times |q|^2 and Nq = |q|^2 REDFT10_x(a_in) - sub
+// (sub = the advection term's spectrum, or null)
 template <typename T, int N>
 __global__ __launch_bounds__(WG) void k_plain_x_cons(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
-                                                     T* __restrict__ nhat, Dims d, PlainCoef cf, int inverse,
-                                                     Tw<T> tw)
+                                                     T* __restrict__ nhat, const T* __restrict__ sub, Dims d,
+                                                     PlainCoef cf, int inverse, Tw<T> tw)
 {
-    plain_x_body<T, N, true>(a_in, w_out, ph_in, ph_out, nhat, d, cf, inverse, tw);
+    plain_x_body<T, N, true>(a_in, w_out, ph_in, ph_out, nhat, sub, d, cf, inverse, tw);
 }
 
 // ------------------------------------------------------------------------
@@ -770,12 +774,14 @@ __global__ __launch_bounds__(WG) void k_multi(const T* in, PassSet<T> ps, Dims d
 
 // REDFT10_z of Nr = beta psi^3 - gamma psi^5 - u . grad psi (:1290-1307),
 // the nonlinear term formed while the z lines are loaded.
+// part 0: the whole Nr; 1: beta psi^3 - gamma psi^5 only; 2: u . grad psi only
+// (the conserved variant transforms the two separately, adv/cosAdvConSingle.cpp:1290-1310)
 template <typename T, int N>
 __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_nl_z(const T* __restrict__ psi, const T* __restrict__ ux,
                                                              const T* __restrict__ uy, const T* __restrict__ uz,
                                                              const T* __restrict__ gx, const T* __restrict__ gy,
                                                              const T* __restrict__ gz, T* __restrict__ out,
-                                                             Dims d, AdvCoef ac, Tw<T> tw)
+                                                             Dims d, AdvCoef ac, int part, Tw<T> tw)
 {
     constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
@@ -796,8 +802,12 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_nl_z(const T* __restrict
 #pragma unroll
         for (int t = 0; t < W; ++t) {
             double x = (double)a.v[t], x2 = x * x, x3 = x2 * x;
-            double v = ac.pc.beta * x3 - ac.pc.gamma * x3 * x2 - (double)vx.v[t] * (double)hx.v[t] -
-                       (double)vy.v[t] * (double)hy.v[t] - (double)vz.v[t] * (double)hz.v[t];
+            double th = ac.pc.beta * x3 - ac.pc.gamma * x3 * x2;
+            double adv = (double)vx.v[t] * (double)hx.v[t] + (double)vy.v[t] * (double)hy.v[t] +
+                         (double)vz.v[t] * (double)hz.v[t];
+            double v = part == 0 ? ac.pc.beta * x3 - ac.pc.gamma * x3 * x2 - (double)vx.v[t] * (double)hx.v[t] -
+                                       (double)vy.v[t] * (double)hy.v[t] - (double)vz.v[t] * (double)hz.v[t]
+                                 : (part == 1 ? th : adv);
             lds[c * LS + PR(mk_pos<N>(p + t))] = T(v);
         }
     });
@@ -1474,7 +1484,8 @@ struct Launch {
     static void plain_z(hipStream_t s, const T* in, T* out, T* psi_out, Dims d,
                         const PlainCoef& cf, int from_real, const Tw<T>& tw);
     static void plain_x(hipStream_t s, const T* a_in, T* w_out, const T* ph_in, T* ph_out,
-                        T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse = 1);
+                        T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse = 1,
+                        const T* sub = nullptr);
     static void adv_force(hipStream_t s, const T* psi, const T* sp2, const T* gx, const T* gy,
                           const T* gz, T* sx, T* sy, T* sz, Dims d, const AdvCoef& ac);
     static void adv_stokes(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, T* ux, T* uy,
@@ -1498,7 +1509,7 @@ struct Launch {
                        const Tw<T>& tw);
     static void adv_nl_z(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
                          const T* gx, const T* gy, const T* gz, T* out, Dims d, const AdvCoef& ac,
-                         const Tw<T>& tw);
+                         const Tw<T>& tw, int part = 0);
     static void adj_x(hipStream_t s, T* w, T* lbuf, const T* ph, Dims d, const PlainCoef& cf,
                       int mode, double* acc, const Tw<T>& tw);
     static void adj_z(hipStream_t s, T* w, const T* psi_tape, Dims d, const PlainCoef& cf,

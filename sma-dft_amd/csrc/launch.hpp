@@ -55,11 +55,11 @@ void Launch<T>::plain_z(hipStream_t s, const T* in, T* out, T* psi_out, Dims d,
 
 template <typename T>
 void Launch<T>::plain_x(hipStream_t s, const T* a_in, T* w_out, const T* ph_in, T* ph_out,
-                        T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse)
+                        T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse, const T* sub)
 {
     dim3 grid(num_tiles(d, 0, Lines<T>::NL));
     if (cf.conserved) {
-        SMAD_DISPATCH(d.nx, k_plain_x_cons, grid, a_in, w_out, ph_in, ph_out, nhat, d, cf, inverse, tw);
+        SMAD_DISPATCH(d.nx, k_plain_x_cons, grid, a_in, w_out, ph_in, ph_out, nhat, sub, d, cf, inverse, tw);
     } else {
         SMAD_DISPATCH(d.nx, k_plain_x, grid, a_in, w_out, ph_in, ph_out, nhat, d, cf, inverse, tw);
     }
@@ -162,10 +162,10 @@ void Launch<T>::q_nl_z(hipStream_t s, const T* mu, const T* ux, const T* uy, con
 template <typename T>
 void Launch<T>::adv_nl_z(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
                          const T* gx, const T* gy, const T* gz, T* out, Dims d, const AdvCoef& ac,
-                         const Tw<T>& tw)
+                         const Tw<T>& tw, int part)
 {
     dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
-    SMAD_DISPATCH_Z(d.nz, k_adv_nl_z, grid, psi, ux, uy, uz, gx, gy, gz, out, d, ac, tw);
+    SMAD_DISPATCH_Z(d.nz, k_adv_nl_z, grid, psi, ux, uy, uz, gx, gy, gz, out, d, ac, part, tw);
 }
 
 template <typename T>

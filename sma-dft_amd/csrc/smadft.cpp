@@ -637,6 +637,7 @@ struct AdvEngine : EngineBase {
         cf.gamma = p.gamma;
         cf.dtd2 = p.dt / 2;
         cf.scale = 0.125 / ((double)p.nx * p.ny * p.nz);
+        cf.conserved = p.model == SMAD_MODEL_ADV_CONSERVED ? 1 : 0;
         ac.nu = p.nu;
         twx.make(p.nx, s);
         twy.make(p.ny, s);
@@ -712,11 +713,24 @@ struct AdvEngine : EngineBase {
         tr3(uy.p, uy.p, REDFT01, RODFT01, REDFT01);
         tr3(uz.p, uz.p, REDFT01, REDFT01, RODFT01);
         mk(0);
-        Launch<T>::adv_nl_z(s, psi.p, ux.p, uy.p, uz.p, gx.p, gy.p, gz.p, nr.p, d, ac, twz.tw);
-        mk(4);
-        Launch<T>::axis(s, nr.p, nr.p, d, 1, REDFT10, T(1), twy.tw);
-        mk(0);
-        Launch<T>::plain_x(s, nr.p, nr.p, ph.p, ph.p, nhat.p, d, ac.pc, twx.tw, 0);
+        if (!ac.pc.conserved) {
+            Launch<T>::adv_nl_z(s, psi.p, ux.p, uy.p, uz.p, gx.p, gy.p, gz.p, nr.p, d, ac, twz.tw);
+            mk(4);
+            Launch<T>::axis(s, nr.p, nr.p, d, 1, REDFT10, T(1), twy.tw);
+            mk(0);
+            Launch<T>::plain_x(s, nr.p, nr.p, ph.p, ph.p, nhat.p, d, ac.pc, twx.tw, 0);
+        } else {
+            // conserved variant (adv/cosAdvConSingle.cpp:1290-1369): the
+            // thermodynamic part carries |q|^2, the advection term does not
+            Launch<T>::adv_nl_z(s, psi.p, ux.p, uy.p, uz.p, gx.p, gy.p, gz.p, nr.p, d, ac, twz.tw, 1);
+            Launch<T>::adv_nl_z(s, psi.p, ux.p, uy.p, uz.p, gx.p, gy.p, gz.p, sp.p, d, ac, twz.tw, 2);
+            mk(4);
+            Launch<T>::axis(s, nr.p, nr.p, d, 1, REDFT10, T(1), twy.tw);
+            Launch<T>::axis(s, sp.p, sp.p, d, 1, REDFT10, T(1), twy.tw);
+            Launch<T>::axis(s, sp.p, sp.p, d, 0, REDFT10, T(1), twx.tw);
+            mk(0);
+            Launch<T>::plain_x(s, nr.p, nr.p, ph.p, ph.p, nhat.p, d, ac.pc, twx.tw, 0, sp.p);
+        }
         mk(5);
     }
 
@@ -1163,11 +1177,12 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
         return SMAD_EINVAL;
     }
     if (p.model != SMAD_MODEL_PLAIN && p.model != SMAD_MODEL_ADV && p.model != SMAD_MODEL_QUASI &&
-        p.model != SMAD_MODEL_PLAIN_CONSERVED) {
+        p.model != SMAD_MODEL_PLAIN_CONSERVED && p.model != SMAD_MODEL_ADV_CONSERVED) {
         g_init_error = "smad_init: unknown model";
         return SMAD_EINVAL;
     }
-    if ((p.model == SMAD_MODEL_ADV || p.model == SMAD_MODEL_QUASI) && !(p.nu > 0)) {
+    if ((p.model == SMAD_MODEL_ADV || p.model == SMAD_MODEL_QUASI || p.model == SMAD_MODEL_ADV_CONSERVED) &&
+        !(p.nu > 0)) {
         g_init_error = "smad_init: nu must be positive";
         return SMAD_EINVAL;
     }
@@ -1184,7 +1199,7 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
         if (p.model == SMAD_MODEL_PLAIN || p.model == SMAD_MODEL_PLAIN_CONSERVED) {
             if (f64) c->eng.reset(new PlainEngine<double>(p, c->stream));
             else c->eng.reset(new PlainEngine<float>(p, c->stream));
-        } else if (p.model == SMAD_MODEL_ADV) {
+        } else if (p.model == SMAD_MODEL_ADV || p.model == SMAD_MODEL_ADV_CONSERVED) {
             if (f64) c->eng.reset(new AdvEngine<double>(p, c->stream));
             else c->eng.reset(new AdvEngine<float>(p, c->stream));
         } else {

@@ -13,6 +13,7 @@
 //   smadft_run adv   <nu> <ep_arg> <aE> <load> [opts]
 //   smadft_run quasi <nu> <ep_arg> <aE> [opts]
 //   smadft_run conserved <ep_arg> <aE> <N> [opts]   (no-adv/cosNoAdvConserv.cpp:137-163)
+//   smadft_run advcons <nu> <ep_arg> <aE> <N> <load> [opts]   (adv/cosAdvConSingle.cpp:126-180)
 //
 // Loop control follows the reference: step `--l1-every` steps (100 for
 // plain/adv, 50 for quasi), append L1 = sum|psi'-psi| / (dt sum|psi'|) to
@@ -70,6 +71,7 @@ struct Opts {
                  "       smadft_run adv <nu> <ep_arg> <aE> <load> [opts]\n"
                  "       smadft_run quasi <nu> <ep_arg> <aE> [opts]\n"
                  "       smadft_run conserved <ep_arg> <aE> <N> [opts]\n"
+                 "       smadft_run advcons <nu> <ep_arg> <aE> <N> <load> [opts]\n"
                  "opts: --n N | --nx/--ny/--nz N, --fp32, --steps K, --l1 LIM, --l1-every S,\n"
                  "      --save-every B, --max-steps K, --ic focal|slab|curved, --amp A,\n"
                  "      --noise X, --seed S, --ae A, --eps E, --dt DT, --out DIR,\n"
@@ -109,6 +111,16 @@ Opts parse(int argc, char** argv)
         o.ep_arg = num(argv[3], "ep_arg");
         o.ae = num(argv[4], "aE");
         pos = 5;
+    } else if (m == "advcons") {
+        // adv/cosAdvConSingle.cpp:126-175: nu, ep_arg (eps = -0.01 ep_arg), aE, N, load
+        o.model = SMAD_MODEL_ADV_CONSERVED;
+        if (argc < 7) usage("advcons needs <nu> <ep_arg> <aE> <N> <load>");
+        o.nu = num(argv[2], "nu");
+        o.ep_arg = num(argv[3], "ep_arg");
+        o.ae = num(argv[4], "aE");
+        o.n = (int)num(argv[5], "N");
+        o.load = (int)num(argv[6], "load");
+        pos = 7;
     } else if (m == "conserved") {
         // no-adv/cosNoAdvConserv.cpp:137-156: ep_arg (eps = -0.01 ep_arg), aE, N
         o.model = SMAD_MODEL_PLAIN_CONSERVED;
@@ -225,7 +237,8 @@ int main(int argc, char** argv)
         p.nu = o.nu;
         p.lambda = -2.0 * o.nu / 3.0;  // quasi.cpp:248
     }
-    if (!std::isnan(o.ep_arg)) p.eps = (o.model == SMAD_MODEL_PLAIN_CONSERVED ? -0.01 : -0.001) * o.ep_arg;
+    const bool cons = o.model == SMAD_MODEL_PLAIN_CONSERVED || o.model == SMAD_MODEL_ADV_CONSERVED;
+    if (!std::isnan(o.ep_arg)) p.eps = (cons ? -0.01 : -0.001) * o.ep_arg;
     if (!std::isnan(o.eps)) p.eps = o.eps;
     if (!std::isnan(o.dt)) p.dt = o.dt;
     const int l1_every = o.l1_every ? o.l1_every : (o.model == SMAD_MODEL_QUASI ? 50 : 100);

@@ -47,3 +47,32 @@ def test_conserved_has_no_adjoint_or_surface(smadft):
             with pytest.raises(smadft.SmadError) as e:
                 call()
             assert e.value.code == smadft.EINVAL
+
+
+@pytest.mark.parametrize("shape,nsteps", [((32, 32, 32), 30), ((16, 32, 64), 20)])
+def test_adv_conserved_fp64_matches_oracle(smadft, oracle, shape, nsteps):
+    p = smadft.Params.default(smadft.MODEL_ADV_CONSERVED, shape[0])
+    p.nx, p.ny, p.nz = shape
+    psi0 = smadft.ic_curved(p, 1.328, 1e-3, seed=2)
+    o = oracle.AdvOracle(p, psi0)
+    o.step(nsteps)
+    with smadft.Stepper(p, psi0) as s:
+        s.step(nsteps)
+        psi = s.fetch()
+        vel = [s.fetch(f) for f in (smadft.FIELD_VEL_X, smadft.FIELD_VEL_Y, smadft.FIELD_VEL_Z)]
+    assert rel_l2(psi, o.psi) < 1e-10
+    for c in range(3):
+        assert rel_l2(vel[c], o.vel[c]) < 1e-9
+    assert np.abs(o.vel).max() > 1e-8
+
+
+def test_adv_conserved_fp32_mixed(smadft, oracle):
+    p = smadft.Params.default(smadft.MODEL_ADV_CONSERVED, 32)
+    p.precision = 32
+    psi0 = smadft.ic_curved(p, 1.328, 1e-3, seed=2)
+    o = oracle.AdvOracle(p, psi0)
+    o.step(30)
+    with smadft.Stepper(p, psi0) as s:
+        s.step(30)
+        psi = s.fetch()
+    assert rel_l2(psi, o.psi) < 1e-5

@@ -70,3 +70,64 @@ def test_conserved_first_step_scipy(smadft, oracle):
     new = s * ((1 + p.dt / 2 * L) * psiq + p.dt / 2 * (3 * nq - nq)) / (1 - p.dt / 2 * L)
     ref = sf.dctn(new, type=3)
     assert np.max(np.abs(o.psi - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+# ---------------------------------------------------------------- advective --
+
+def acparams(smadft, shape, model=None, **kw):
+    p = smadft.Params.default(smadft.MODEL_ADV_CONSERVED if model is None else model, shape[0])
+    p.nx, p.ny, p.nz = shape
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def test_adv_conserved_defaults(smadft):
+    p = smadft.Params.default(smadft.MODEL_ADV_CONSERVED, 64)
+    assert p.nw == 16 and p.dt == 0.0005 and p.nu > 0
+
+
+def test_adv_conserved_planar_layers_follow_the_conserved_plain_step(smadft, oracle):
+    """Planar layers have no flow, so the conserved advective step is the
+    conserved plain step, except for the pre-loop history, which the
+    advective variant leaves without |q|^2 (adv/cosAdvConSingle.cpp:755-790)."""
+    shape = (16, 8, 32)
+    p = acparams(smadft, shape, dt=0.002)
+    dz = 2 * np.pi / (p.q0 * p.nw)
+    z = np.arange(shape[2]) * dz
+    psi0 = np.broadcast_to(1.2 * np.cos(p.q0 * z) * np.exp(-((z - z.mean()) / 8) ** 2), shape).copy()
+    a = oracle.AdvOracle(p, psi0)
+    a.step(12)
+    assert np.abs(a.vel).max() < 1e-12
+    pc = acparams(smadft, shape, model=smadft.MODEL_PLAIN_CONSERVED, dt=0.002, nw=p.nw)
+    pl = oracle.PlainOracle(pc, psi0)
+    pl.nq[...] = sf.dctn(p.beta * psi0 ** 3 - p.gamma * psi0 ** 5, type=2)  # unscaled history
+    pl.step(12)
+    assert np.max(np.abs(a.psi - pl.psi)) < 1e-12 * np.max(np.abs(psi0))
+
+
+def test_adv_conserved_first_step_scipy(smadft, oracle):
+    """One step from the pre-loop, restated with scipy transforms and the
+    flow and gradient of the non-conserved advective step from the same psi
+    (the force does not change between the variants)."""
+    shape = (16, 16, 16)
+    pc = acparams(smadft, shape)
+    psi0 = smadft.ic_curved(pc, 1.328, 1e-2, seed=2)
+    a = oracle.AdvOracle(pc, psi0)
+    a.step(1)
+    pn = acparams(smadft, shape, model=smadft.MODEL_ADV, dt=pc.dt, nw=pc.nw, eps=pc.eps)
+    b = oracle.AdvOracle(pn, psi0)
+    b.step(1)
+    assert np.allclose(a.vel, b.vel, rtol=0, atol=1e-14 * np.abs(b.vel).max())
+    assert np.abs(b.vel).max() > 1e-8
+    s = 1.0 / (8 * np.prod(shape))
+    q2 = q2_grid(pc)
+    L = q2 * (pc.eps - pc.alpha * (q2 - pc.q0 ** 2) ** 2)
+    th = pc.beta * psi0 ** 3 - pc.gamma * psi0 ** 5
+    adv = (b.vel * b.grad).sum(axis=0)
+    nq = q2 * sf.dctn(th, type=2) - sf.dctn(adv, type=2)
+    nq_past = sf.dctn(th, type=2)
+    psiq = s * sf.dctn(psi0, type=2)
+    new = ((1 + pc.dt / 2 * L) * psiq + pc.dt / 2 * s * (3 * nq - nq_past)) / (1 - pc.dt / 2 * L)
+    ref = sf.dctn(new, type=3)
+    assert np.max(np.abs(a.psi - ref)) < 1e-12 * np.max(np.abs(ref))
