@@ -190,7 +190,7 @@ def main():
                 "dpsi0": float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64)),
                 "dtheta": [float(abs(a - b) / abs(b)) for a, b in zip(p32g, p64g)]}
 
-    # advective (256^3, config 2) and quasi-incompressible (384^3, config 3)
+    # advective (256^3) and quasi-incompressible (384^3)
     # models: steps/s in both precisions and per-field FP32-mixed error
     models = {}
     if args.model_steps > 0 and args.only_fp is None:

@@ -1295,8 +1295,9 @@ __global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__
         double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
         double t = qx * qx + qy * qy + qz * qz - cf.q0sq;
         double al = cf.eps - cf.alpha * t * t;
-        double h = cf.dtd2 / (1.0 - cf.dtd2 * al);  // dt/2 / C2
-        double a = (1.0 + cf.dtd2 * al) * (h / cf.dtd2);
+        const double inv = 1.0 / (1.0 - cf.dtd2 * al);  // 1 / C2
+        double h = cf.dtd2 * inv;                        // dt/2 / C2
+        double a = (1.0 + cf.dtd2 * al) * inv;           // C1 / C2
         double lold = (double)lb;
         double lnew = a * lold + (double)lds[c * LS + PR(p)];
         double wgt = (p == 0 ? 0.5 : 1.0) * (j == 0 ? 0.5 : 1.0) * (k0 + c == 0 ? 0.5 : 1.0);
