@@ -113,3 +113,36 @@ def test_plain_save_block_surface_files(smadft, tmp_path):
             for key, name in (("vel", "velSurf"), ("curv_h", "curvH"), ("curv_k", "curvK")):
                 assert np.allclose(files[name][blk], sf[key].ravel(), rtol=1e-5, atol=1e-6 * np.abs(sf[key]).max())
             assert np.allclose(files["psiMid"][blk], psi[:, n // 2, :].ravel(), rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("argv,model,l1_every,n", [
+    (["adv", "1", "700", "8", "0"], 1, 100, 16),
+    (["quasi", "1", "700", "8"], 2, 50, 16),
+    (["conserved", "70", "8", "16"], 3, 100, 16),
+    (["advcons", "1", "70", "8", "16", "0"], 4, 100, 16),
+])
+def test_model_modes_match_library(smadft, tmp_path, argv, model, l1_every, n):
+    """The positional argv of each reference main and its default cadence:
+    two L1 blocks, L1.dat against the library driven directly."""
+    out = tmp_path / "run"
+    extra = [] if "--n" in argv or model in (3, 4) else ["--n", str(n)]
+    r = run(argv + extra + ["--steps", str(2 * l1_every), "--ic", "slab", "--amp", "1.328",
+                            "--seed", "2", "--out", str(out)])
+    assert r.returncode == 0, r.stderr
+    l1 = np.loadtxt(out / "L1.dat")
+    assert l1.shape == (2,)
+    p = smadft.Params.default(model, n)
+    if model in (1, 2, 4):
+        p.nu = 1.0
+        p.lambda_ = -2.0 / 3.0
+    p.eps = (-0.01 if model in (3, 4) else -0.001) * float(argv[2 if model in (1, 2, 4) else 1])
+    psi0 = smadft.ic_slab(p, 1.328, 1e-3, seed=2)
+    with smadft.Stepper(p, psi0) as s:
+        ref = []
+        for _ in range(2):
+            s.step(l1_every)
+            ref.append(s.l1())
+    assert np.allclose(l1, ref, rtol=1e-5)
+    if model == 2:
+        assert np.loadtxt(out / "mass.dat").shape == (2,)
