@@ -56,7 +56,7 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--acc-steps", type=int, default=1000)
     ap.add_argument("--prof-steps", type=int, default=50)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-ckpt", type=int, default=20)
     ap.add_argument("--adj-reps", type=int, default=3)
@@ -139,9 +139,15 @@ def main():
                 with smadft.Stepper(p, ic) as s:
                     s.step(args.acc_steps)
                     out[prec] = s.fetch()
-            err = float(np.linalg.norm(out[32] - out[64]) / np.linalg.norm(out[64]))
-            acc[label] = {"rel_l2_fp32_vs_fp64": err, "norm_fp64": float(np.linalg.norm(out[64]))}
-            log(f"accuracy {label}: rel L2 = {err:.3e}")
+            diff = float(np.linalg.norm(out[32] - out[64]))
+            err = diff / float(np.linalg.norm(out[64]))
+            acc[label] = {"rel_l2_fp32_vs_fp64": err, "norm_fp64": float(np.linalg.norm(out[64])),
+                          "norm_initial": float(np.linalg.norm(ic)),
+                          "l2_diff_over_initial_norm": diff / float(np.linalg.norm(ic))}
+            log(f"accuracy {label}: rel L2 = {err:.3e} (difference / |psi0| = {diff / np.linalg.norm(ic):.3e})")
+        acc["note"] = ("amplitude 0.6 is below the smectic's metastability barrier at eps = -0.7: psi "
+                       "melts to ~1e-10, so its relative error compares two vanishing fields; "
+                       "amplitude 1.328 (the reference's) keeps the smectic")
 
     # adjoint / forward time ratio: K steps with checkpoints every ckpt,
     # (taped forward + adjoint) over the plain forward of the same K steps.
@@ -240,6 +246,12 @@ def main():
     if not args.no_cpu and args.cpu_steps > 0:
         import oracle
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        model = "unknown CPU"
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
         o = oracle.PlainOracle(p64, psi0)
         o.step(1)
         t0 = time.perf_counter()
@@ -247,7 +259,8 @@ def main():
         dt = time.perf_counter() - t0
         cpu = {"value": args.cpu_steps / dt, "unit": "steps/s", "cores": cores, "kind": "port",
                "sample": f"{args.cpu_steps} steps of the FP64 oracle plain step at {n}^3 "
-                         f"(3 r2r transforms/step, OpenMP over lines) after 1 warm step"}
+                         f"(3 r2r transforms/step, OpenMP over lines) after 1 warm step; "
+                         f"{cores} threads of {os.cpu_count()} ({model})"}
         log(f"cpu baseline: {cpu}")
 
     ms64 = res.get(64)
