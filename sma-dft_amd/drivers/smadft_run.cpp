@@ -26,13 +26,16 @@
 // "SMADPSI1", int32 nx ny nz, int64 steps, then nx*ny*nz doubles); the
 // y = Ny/2 slice is appended to psiMid.dat and, for the plain model, the
 // surface diagnostics to surfPsi.dat, velSurf.dat, curvH.dat and curvK.dat
-// (cosNoAdvTrackMult.cpp:1049-1138).  At the end psi.dat and psi.bin hold the
-// final state.
+// (cosNoAdvTrackMult.cpp:1049-1138); --track-every B writes the surface files
+// every B blocks instead (the tracking cadence of the 256^3 relaxation, every
+// 100 steps).  At the end psi.dat and psi.bin hold the final state, and the
+// wall time and steps/s of the loop are printed.
 // A non-finite field ends the run with exit code 3 instead of the
 // reference's silent loop exit.
 #include "smadft/smadft.h"
 
 #include <cerrno>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -54,6 +57,7 @@ struct Opts {
     double l1_lim = 1e-6;
     int l1_every = 0;        // 0 = model default
     int save_every = 10;     // L1 blocks between field saves; 0 = never
+    int track_every = -1;    // L1 blocks between surface outputs; -1 = with the saves
     long max_steps = 1000000;
     std::string ic = "";     // focal | slab | curved | file
     double amp = 0, noise = 1e-3, ae = -1;
@@ -73,7 +77,7 @@ struct Opts {
                  "       smadft_run conserved <ep_arg> <aE> <N> [opts]\n"
                  "       smadft_run advcons <nu> <ep_arg> <aE> <N> <load> [opts]\n"
                  "opts: --n N | --nx/--ny/--nz N, --fp32, --steps K, --l1 LIM, --l1-every S,\n"
-                 "      --save-every B, --max-steps K, --ic focal|slab|curved, --amp A,\n"
+                 "      --save-every B, --track-every B, --max-steps K, --ic focal|slab|curved, --amp A,\n"
                  "      --noise X, --seed S, --ae A, --eps E, --dt DT, --out DIR,\n"
                  "      --restart psi.bin\n";
     std::exit(2);
@@ -147,6 +151,7 @@ Opts parse(int argc, char** argv)
         else if (a == "--l1") o.l1_lim = num(val(), "--l1");
         else if (a == "--l1-every") o.l1_every = (int)num(val(), "--l1-every");
         else if (a == "--save-every") o.save_every = (int)num(val(), "--save-every");
+        else if (a == "--track-every") o.track_every = (int)num(val(), "--track-every");
         else if (a == "--max-steps") o.max_steps = (long)num(val(), "--max-steps");
         else if (a == "--ic") o.ic = val();
         else if (a == "--amp") o.amp = num(val(), "--amp");
@@ -160,7 +165,8 @@ Opts parse(int argc, char** argv)
         else usage(("unknown option " + a).c_str());
     }
     if (o.n < 2) usage("grid must be >= 2");
-    if (o.l1_every < 0 || o.save_every < 0) usage("cadences must be >= 0");
+    if (o.l1_every < 0 || o.save_every < 0 || o.track_every < -1) usage("cadences must be >= 0");
+    if (o.track_every == -1) o.track_every = o.save_every;
     if (o.steps == 0 || o.steps < -1) usage("--steps must be positive");
     return o;
 }
@@ -284,41 +290,47 @@ int main(int argc, char** argv)
     const size_t cols = (size_t)p.nx * p.ny;
     std::vector<double> sf(4 * cols), mid((size_t)p.nx * p.nz);
     // the save block (cosNoAdvTrackMult.cpp:850-1138): psi.dat and the restart,
-    // psiMid.dat (the y = Ny/2 slice) and, for the plain model, the surface
-    // files; `at` is the step count of the state written
+    // psiMid.dat (the y = Ny/2 slice); `at` is the step count of the state written
     auto save_fields = [&](long at) -> int {
         if (smad_fetch(ctx, SMAD_FIELD_PSI, psi.data()) != SMAD_OK) return fail(ctx, "fetch");
         for (int i = 0; i < p.nx; ++i)
             for (int k = 0; k < p.nz; ++k) mid[(size_t)i * p.nz + k] = psi[((size_t)i * p.ny + p.ny / 2) * p.nz + k];
-        bool ok = write_ascii(dir + "psi.dat", psi) && write_restart(dir + "psi.bin", p, at, psi) &&
-                  append_all(dir + "psiMid.dat", mid.data(), mid.size(), "\n");
-        if (ok && o.model == SMAD_MODEL_PLAIN) {
-            if (smad_surface(ctx, sf.data(), sf.data() + cols, sf.data() + 2 * cols, sf.data() + 3 * cols) != SMAD_OK)
-                return fail(ctx, "surface");
-            ok = append_all(dir + "surfPsi.dat", sf.data(), cols, "\n") &&
-                 append_all(dir + "velSurf.dat", sf.data() + cols, cols, "\n ") &&
-                 append_all(dir + "curvH.dat", sf.data() + 2 * cols, cols, "\n ") &&
-                 append_all(dir + "curvK.dat", sf.data() + 3 * cols, cols, "\n ");
-        }
-        if (!ok) {
+        if (!write_ascii(dir + "psi.dat", psi) || !write_restart(dir + "psi.bin", p, at, psi) ||
+            !append_all(dir + "psiMid.dat", mid.data(), mid.size(), "\n")) {
             std::cerr << "smadft_run: cannot write fields in " << dir << "\n";
+            return 1;
+        }
+        return 0;
+    };
+    // the surface files of the plain model (cosNoAdvTrackMult.cpp:1062-1090)
+    auto track = [&]() -> int {
+        if (smad_surface(ctx, sf.data(), sf.data() + cols, sf.data() + 2 * cols, sf.data() + 3 * cols) != SMAD_OK)
+            return fail(ctx, "surface");
+        if (!append_all(dir + "surfPsi.dat", sf.data(), cols, "\n") ||
+            !append_all(dir + "velSurf.dat", sf.data() + cols, cols, "\n ") ||
+            !append_all(dir + "curvH.dat", sf.data() + 2 * cols, cols, "\n ") ||
+            !append_all(dir + "curvK.dat", sf.data() + 3 * cols, cols, "\n ")) {
+            std::cerr << "smadft_run: cannot write surface files in " << dir << "\n";
             return 1;
         }
         return 0;
     };
 
     const long limit = o.steps > 0 ? o.steps : o.max_steps;
-    long done = 0;
-    int blocks = 0, rc = 0;
+    long done = 0, blk = 0;
+    int rc = 0;
     double l1 = INFINITY;
+    const auto t0 = std::chrono::steady_clock::now();
     while (done < limit && (o.steps > 0 || l1 > o.l1_lim)) {
         int k = (int)std::min<long>(l1_every, limit - done);
-        const bool save = o.save_every && blocks + 1 == o.save_every;
+        const bool save = o.save_every && (blk + 1) % o.save_every == 0;
+        const bool trk = o.model == SMAD_MODEL_PLAIN && o.track_every && (blk + 1) % o.track_every == 0;
         // the reference's save block sees the state one step before the
         // block's end (it runs before psi <- psi_new)
-        if (smad_step(ctx, save ? k - 1 : k) != SMAD_OK) { rc = fail(ctx, "step"); break; }
-        if (save) {
-            if ((rc = save_fields(taken + done + k - 1)) != 0) break;
+        if (smad_step(ctx, save || trk ? k - 1 : k) != SMAD_OK) { rc = fail(ctx, "step"); break; }
+        if (save || trk) {
+            if (save && (rc = save_fields(taken + done + k - 1)) != 0) break;
+            if (trk && (rc = track()) != 0) break;
             if (smad_step(ctx, 1) != SMAD_OK) { rc = fail(ctx, "step"); break; }
         }
         done += k;
@@ -341,8 +353,9 @@ int main(int argc, char** argv)
             append(dir + "mass.dat", m[0]);
             append(dir + "energy.dat", e[0]);
         }
-        blocks = save ? 0 : blocks + 1;
+        ++blk;
     }
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (rc == 0) {
         if (smad_fetch(ctx, SMAD_FIELD_PSI, psi.data()) != SMAD_OK) {
             rc = fail(ctx, "fetch");
@@ -351,7 +364,8 @@ int main(int argc, char** argv)
             rc = 1;
         }
     }
-    std::cout << "steps " << taken + done << " L1 " << l1 << "\n";
+    std::cout << "steps " << taken + done << " L1 " << l1 << " wall_s " << wall << " steps_per_s "
+              << (wall > 0 ? done / wall : 0.0) << "\n";
     smad_destroy(ctx);
     return rc;
 }
