@@ -1349,7 +1349,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restr
     constexpr int W = V16<T>::W;
     // FP64 only: in FP32 the FP64 multiply of all packed values of a thread
     // in flight costs more registers (169 vs 78) than the LDS pass it saves
-    constexpr bool FUSED = OpFused<N, TPL>::value && sizeof(T) == 8;
+    constexpr bool FUSED = OpFused<N, TPL>::value && sizeof(T) == 8 && N <= 512;  // two tiles must fit LDS
     __shared__ T lds[NL * LS];
     __shared__ T tl[FUSED ? NL * LS : 1];  // psi tape at Makhoul positions
     __shared__ T ct[N + 1];

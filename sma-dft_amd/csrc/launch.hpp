@@ -20,6 +20,7 @@ namespace smad {
     case 256: hipLaunchKernelGGL((KERNEL<T, 256>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
     case 384: hipLaunchKernelGGL((KERNEL<T, 384>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
     case 512: hipLaunchKernelGGL((KERNEL<T, 512>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 1024: hipLaunchKernelGGL((KERNEL<T, 1024>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
     default: break;                                                                    \
     }
 #define SMAD_DISPATCH(n, KERNEL, GRID, ...) SMAD_DISPATCH_B(n, KERNEL, GRID, dim3(WG), __VA_ARGS__)
@@ -81,6 +82,7 @@ inline dim3 flat_grid(long n)
     case 256: hipLaunchKernelGGL((KERNEL<T, 256, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
     case 384: hipLaunchKernelGGL((KERNEL<T, 384, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
     case 512: hipLaunchKernelGGL((KERNEL<T, 512, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 1024: hipLaunchKernelGGL((KERNEL<T, 1024, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
     default: break;                                                                                     \
     }
 

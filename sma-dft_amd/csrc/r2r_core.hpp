@@ -375,6 +375,7 @@ template <> struct Plan<64> { using inv = RList<8, 8>; using fwd = RList<8, 8>; 
 template <> struct Plan<128> { using inv = RList<4, 4, 8>; using fwd = RList<8, 4, 4>; };
 template <> struct Plan<192> { using inv = RList<3, 8, 8>; using fwd = RList<8, 8, 3>; };
 template <> struct Plan<256> { using inv = RList<4, 8, 8>; using fwd = RList<8, 8, 4>; };
+template <> struct Plan<512> { using inv = RList<8, 8, 8>; using fwd = RList<8, 8, 8>; };
 
 // Stages of a radix list, starting at span Ns.
 template <typename T, int N, int TPL, bool INV, bool WAVE, class L>

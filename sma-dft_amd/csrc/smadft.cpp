@@ -1095,7 +1095,7 @@ struct QuasiEngine : EngineBase {
 
 bool length_ok(int n)
 {
-    return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 384 || n == 512;
+    return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 384 || n == 512 || n == 1024;
 }
 
 }  // namespace
@@ -1169,7 +1169,7 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
     *out = nullptr;
     const smad_params& p = *params;
     if (!length_ok(p.nx) || !length_ok(p.ny) || !length_ok(p.nz)) {
-        g_init_error = "smad_init: each grid size must be one of 16, 32, 64, 128, 256, 384, 512";
+        g_init_error = "smad_init: each grid size must be one of 16, 32, 64, 128, 256, 384, 512, 1024";
         return SMAD_EINVAL;
     }
     if (p.precision != SMAD_FP64 && p.precision != SMAD_FP32_MIXED) {

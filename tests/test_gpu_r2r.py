@@ -6,7 +6,8 @@ from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(16, 16, 16), (32, 16, 64), (64, 32, 16), (16, 128, 32), (384, 16, 16), (16, 16, 256)]
+SHAPES = [(16, 16, 16), (32, 16, 64), (64, 32, 16), (16, 128, 32), (384, 16, 16), (16, 16, 256),
+          (1024, 16, 16), (16, 16, 1024)]
 COMBOS = [(0, 0, 0), (1, 1, 1), (2, 0, 0), (0, 2, 0), (0, 0, 2), (3, 1, 1), (1, 3, 1), (1, 1, 3)]
 
 
@@ -20,7 +21,7 @@ def test_r2r3d_fp64(smadft, oracle, shape, kinds):
     assert rel_l2(y, ref) < 1e-14
 
 
-@pytest.mark.parametrize("shape", [(32, 16, 64), (16, 384, 16), (128, 32, 32)])
+@pytest.mark.parametrize("shape", [(32, 16, 64), (16, 384, 16), (128, 32, 32), (16, 1024, 32)])
 @pytest.mark.parametrize("kinds", COMBOS)
 def test_r2r3d_fp32(smadft, oracle, shape, kinds):
     rng = np.random.default_rng(7)
