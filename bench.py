@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--quasi-grid", type=int, default=384)
     ap.add_argument("--quasi-acc", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--large-grid", type=int, default=0,
+                    help="steps of the plain and quasi models on the 1024x512x512 grid (0 = skip)")
     ap.add_argument("--only-fp", choices=["64", "32"], default=None,
                     help="time one precision only (for profiler runs)")
     args = ap.parse_args()
@@ -241,6 +243,22 @@ def main():
                 log(f"{label} fp32 errors: {entry['fp32_rel_l2_vs_fp64']}")
             models[label] = entry
 
+    # the reference programs' 1024x512x512 grid, resident on one GPU
+    large = {}
+    if args.large_grid > 0:
+        for label, model in (("plain", smadft.MODEL_PLAIN), ("quasi", smadft.MODEL_QUASI)):
+            for prec in (64, 32):
+                pm = smadft.Params.default(model, 512)
+                pm.nx, pm.ny, pm.nz = 1024, 512, 512
+                pm.precision = prec
+                ic = smadft.ic_slab(pm, 1.34164 if model == smadft.MODEL_QUASI else 1.328, 1e-3, seed=1)
+                with smadft.Stepper(pm, ic) as s:
+                    s.step(2)
+                    ms = s.time_steps(args.large_grid)
+                sps = 1e3 * args.large_grid / ms
+                large[f"{label}_fp{prec}_steps_per_s"] = sps
+                log(f"large grid 1024x512x512 {label} fp{prec}: {sps:.2f} steps/s")
+
     # CPU baseline: the FP64 oracle on the host cores, bounded sample
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
@@ -292,6 +310,7 @@ def main():
         "adjoint_fwd_ratio_hbm_tape": adj["fp64"]["hbm_tape"]["ratio"] if adj and "fp64" in adj else None,
         "adjoint": adj,
         "models": models,
+        "large_grid_1024x512x512": large or None,
         "kernels": kernels,
         "kernels_fp32": kernels32,
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved,
