@@ -69,7 +69,8 @@ def main():
     ap.add_argument("--quasi-acc", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--large-grid", type=int, default=0,
-                    help="steps of the plain and quasi models on the 1024x512x512 grid (0 = skip)")
+                    help="steps of the plain and quasi models on 1024x512x512 and of the plain "
+                         "model on 1024^3 (0 = skip)")
     ap.add_argument("--only-fp", choices=["64", "32"], default=None,
                     help="time one precision only (for profiler runs)")
     args = ap.parse_args()
@@ -258,6 +259,17 @@ def main():
                 sps = 1e3 * args.large_grid / ms
                 large[f"{label}_fp{prec}_steps_per_s"] = sps
                 log(f"large grid 1024x512x512 {label} fp{prec}: {sps:.2f} steps/s")
+        # the 1024^3 programs: the plain step (5 fields, 43 GB in FP64)
+        for prec in (64, 32):
+            pm = smadft.Params.default(smadft.MODEL_PLAIN, 1024)
+            pm.precision = prec
+            ic = smadft.ic_slab(pm, 1.328, 1e-3, seed=1)
+            with smadft.Stepper(pm, ic) as s:
+                s.step(2)
+                ms = s.time_steps(args.large_grid)
+            sps = 1e3 * args.large_grid / ms
+            large[f"plain_1024cube_fp{prec}_steps_per_s"] = sps
+            log(f"large grid 1024^3 plain fp{prec}: {sps:.2f} steps/s")
 
     # CPU baseline: the FP64 oracle on the host cores, bounded sample
     cpu = None
@@ -310,7 +322,7 @@ def main():
         "adjoint_fwd_ratio_hbm_tape": adj["fp64"]["hbm_tape"]["ratio"] if adj and "fp64" in adj else None,
         "adjoint": adj,
         "models": models,
-        "large_grid_1024x512x512": large or None,
+        "large_grids": large or None,
         "kernels": kernels,
         "kernels_fp32": kernels32,
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved,
