@@ -40,12 +40,21 @@ struct Lines {
 #ifndef SMAD_ZTPL32
 #define SMAD_ZTPL32 16
 #endif
-template <typename T>
+// Lines of 512 points and more take twice the threads per line (half the
+// registers per thread, half the lines per tile); z lines are contiguous, so
+// the narrower tile costs no coalescing.
+constexpr int zlong(int n) { return n >= 512 ? 2 : 1; }
+template <typename T, int N = 256>
 struct LinesZ {
     static constexpr int WGZ = sizeof(T) == 8 ? SMAD_ZWG64 : SMAD_ZWG32;
-    static constexpr int TPL = sizeof(T) == 8 ? SMAD_ZTPL64 : SMAD_ZTPL32;
+    static constexpr int TPL = (sizeof(T) == 8 ? SMAD_ZTPL64 : SMAD_ZTPL32) * zlong(N);
     static constexpr int NL = WGZ / TPL;
 };
+template <typename T>
+inline int linesz_nl(int n)
+{
+    return (sizeof(T) == 8 ? SMAD_ZWG64 : SMAD_ZWG32) / ((sizeof(T) == 8 ? SMAD_ZTPL64 : SMAD_ZTPL32) * zlong(n));
+}
 
 struct Dims {
     int nx, ny, nz;
@@ -225,7 +234,7 @@ template <typename T, int N>
 __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_axis_z(const T* in, T* out, Dims d, int kind, T scale,
                                                Tw<T> tw)
 {
-    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     __shared__ T lds[NL * LS];
     __shared__ T ct[N + 1];
@@ -291,7 +300,7 @@ template <typename T, int N, bool from_real>
 __device__ __forceinline__ void plain_z_body(const T* in, T* out, T* __restrict__ psi_out, Dims d,
                                              const PlainCoef& cf, const Tw<T>& tw)
 {
-    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
     __shared__ T lds[NL * LS];
@@ -664,7 +673,7 @@ template <typename T, int N, int OP>
 __device__ __forceinline__ void zmulti_one(const T* in, T* out, int kind, const Dims& d, const SpecCoef& sc,
                                            T* lds, const T* ct, int l, int u)
 {
-    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
     zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
@@ -704,7 +713,7 @@ template <typename T, int N, int... OPS>
 __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_zmulti(const T* in, PassSet<T> ps, Dims d, SpecCoef sc,
                                                            Tw<T> tw)
 {
-    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     __shared__ T lds[NL * LS];
     __shared__ T ct[N + 1];
     load_cosq<T, N>(ct, tw);
@@ -783,7 +792,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_nl_z(const T* __restrict
                                                              const T* __restrict__ gz, T* __restrict__ out,
                                                              Dims d, AdvCoef ac, int part, Tw<T> tw)
 {
-    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
     __shared__ T lds[NL * LS];
@@ -947,7 +956,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_rho_z(const T* __restrict_
                                                             const T* __restrict__ gy, const T* __restrict__ gz,
                                                             T* __restrict__ out, Dims d, QuasiCoef qc, Tw<T> tw)
 {
-    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
     __shared__ T lds[NL * LS];
@@ -990,7 +999,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_nl_z(const T* __restrict__
                                                            const T* __restrict__ grz, T* __restrict__ out, Dims d,
                                                            QuasiCoef qc, int press_term, Tw<T> tw)
 {
-    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
     __shared__ T lds[NL * LS];
@@ -1344,7 +1353,7 @@ template <typename T, int N>
 __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restrict__ psi_tape, Dims d,
                                               PlainCoef cf, double* __restrict__ acc, Tw<T> tw)
 {
-    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
     // FP64 only: in FP32 the FP64 multiply of all packed values of a thread

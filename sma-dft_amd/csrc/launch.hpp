@@ -34,7 +34,7 @@ void Launch<T>::axis(hipStream_t s, const T* in, T* out, Dims d, int axis, int k
                      const Tw<T>& tw)
 {
     if (axis == 2) {
-        dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+        dim3 grid(num_tiles(d, 2, linesz_nl<T>(d.nz)));
         SMAD_DISPATCH_Z(d.nz, k_axis_z, grid, in, out, d, kind, scale, tw);
     } else {
         dim3 grid(num_tiles(d, axis, Lines<T>::NL));
@@ -46,7 +46,7 @@ template <typename T>
 void Launch<T>::plain_z(hipStream_t s, const T* in, T* out, T* psi_out, Dims d,
                         const PlainCoef& cf, int from_real, const Tw<T>& tw)
 {
-    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+    dim3 grid(num_tiles(d, 2, linesz_nl<T>(d.nz)));
     if (from_real) {
         SMAD_DISPATCH_Z(d.nz, k_plain_z_real, grid, in, out, psi_out, d, cf, tw);
     } else {
@@ -98,7 +98,7 @@ void Launch<T>::zpass(hipStream_t s, int zs, const T* in, const PassSet<T>& ps, 
     if (zs < 0 || zs > ZS_NMQ2) throw std::logic_error("zpass: unknown op set");
     for (int o = 0; o < 4; ++o)
         if ((o < ps.n ? ps.op[o] : -1) != ops[zs][o]) throw std::logic_error("zpass: outputs do not match the op set");
-    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL)), block(LinesZ<T>::WGZ);
+    dim3 grid(num_tiles(d, 2, linesz_nl<T>(d.nz))), block(LinesZ<T>::WGZ);
     switch (zs) {
     case ZS_ADV: SMAD_DISPATCH_OPS(d.nz, k_zmulti, (OP_ID, OP_SHIFT, OP_ALIN), grid, block, in, ps, d, sc, tw); break;
     case ZS_QPSI: SMAD_DISPATCH_OPS(d.nz, k_zmulti, (OP_ID, OP_SHIFT, OP_FQ), grid, block, in, ps, d, sc, tw); break;
@@ -146,7 +146,7 @@ template <typename T>
 void Launch<T>::q_rho_z(hipStream_t s, const T* psi, const T* gx, const T* gy, const T* gz, T* out, Dims d,
                         const QuasiCoef& qc, const Tw<T>& tw)
 {
-    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+    dim3 grid(num_tiles(d, 2, linesz_nl<T>(d.nz)));
     SMAD_DISPATCH_Z(d.nz, k_q_rho_z, grid, psi, gx, gy, gz, out, d, qc, tw);
 }
 
@@ -156,7 +156,7 @@ void Launch<T>::q_nl_z(hipStream_t s, const T* mu, const T* ux, const T* uy, con
                        const T* gry, const T* grz, T* out, Dims d, const QuasiCoef& qc, int press_term,
                        const Tw<T>& tw)
 {
-    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+    dim3 grid(num_tiles(d, 2, linesz_nl<T>(d.nz)));
     SMAD_DISPATCH_Z(d.nz, k_q_nl_z, grid, mu, ux, uy, uz, gpx, gpy, gpz, pr, rho, e, grx, gry, grz, out, d, qc,
                     press_term, tw);
 }
@@ -166,7 +166,7 @@ void Launch<T>::adv_nl_z(hipStream_t s, const T* psi, const T* ux, const T* uy, 
                          const T* gx, const T* gy, const T* gz, T* out, Dims d, const AdvCoef& ac,
                          const Tw<T>& tw, int part)
 {
-    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+    dim3 grid(num_tiles(d, 2, linesz_nl<T>(d.nz)));
     SMAD_DISPATCH_Z(d.nz, k_adv_nl_z, grid, psi, ux, uy, uz, gx, gy, gz, out, d, ac, part, tw);
 }
 
@@ -198,7 +198,7 @@ template <typename T>
 void Launch<T>::adj_z(hipStream_t s, T* w, const T* psi_tape, Dims d, const PlainCoef& cf,
                       double* acc, const Tw<T>& tw)
 {
-    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+    dim3 grid(num_tiles(d, 2, linesz_nl<T>(d.nz)));
     SMAD_DISPATCH_Z(d.nz, k_adj_z, grid, w, psi_tape, d, cf, acc, tw);
 }
 

@@ -150,10 +150,14 @@ __device__ __forceinline__ constexpr int PR(int q) { return q + ((q >> 4) << 1);
 // reals per line in LDS: the padded length rounded up to 2 (mod 32), so
 // consecutive lines of a tile start 8 B (FP32) / 16 B (FP64) apart modulo
 // the 128 B bank width and the column-wise tile loads stay conflict-free.
+// 30 (mod 32) does the same with the opposite sign; lines of 1024 points take
+// whichever is shorter, so that two FP64 z tiles of eight lines fit in LDS.
 template <int N>
 struct LineLS {
     static constexpr int raw = N + 2 * ((N - 1) >> 4);
-    static constexpr int value = raw + ((2 - raw % 32) + 32) % 32;
+    static constexpr int up2 = raw + ((2 - raw % 32) + 32) % 32;
+    static constexpr int up30 = raw + ((30 - raw % 32) + 32) % 32;
+    static constexpr int value = (N >= 1024 && up30 < up2) ? up30 : up2;
 };
 
 // ------------------------------------------------------------ small DFTs --

@@ -416,7 +416,7 @@ struct PlainEngine : EngineBase {
         if (!end_ph.p) {
             for (DevBuf<T>* b : {&end_ph, &end_nh, &lbuf, &gbuf}) b->alloc(n);
             acc_x.alloc(2 * (size_t)num_tiles(d, 0, Lines<T>::NL));
-            acc_z.alloc(2 * (size_t)num_tiles(d, 2, LinesZ<T>::NL));
+            acc_z.alloc(2 * (size_t)num_tiles(d, 2, linesz_nl<T>(d.nz)));
             gsum.alloc(4);
         }
         EvPair ev;

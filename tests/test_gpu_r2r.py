@@ -7,7 +7,7 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(16, 16, 16), (32, 16, 64), (64, 32, 16), (16, 128, 32), (384, 16, 16), (16, 16, 256),
-          (1024, 16, 16), (16, 16, 1024)]
+          (1024, 16, 16), (16, 16, 1024), (16, 32, 512)]
 COMBOS = [(0, 0, 0), (1, 1, 1), (2, 0, 0), (0, 2, 0), (0, 0, 2), (3, 1, 1), (1, 3, 1), (1, 1, 3)]
 
 
