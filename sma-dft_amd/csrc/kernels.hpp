@@ -505,24 +505,6 @@ __device__ __forceinline__ void unflat(long id, const Dims& d, int& i, int& j, i
     i = (int)(r / d.ny);
 }
 
-// S = mu grad psi, mu = Sp2 - beta psi^3 + gamma psi^5 (:996-1015); S
-// overwrites grad psi's buffers only through separate outputs.
-template <typename T>
-__global__ __launch_bounds__(WG) void k_adv_force(const T* __restrict__ psi, const T* __restrict__ sp2,
-                                                  const T* __restrict__ gx, const T* __restrict__ gy,
-                                                  const T* __restrict__ gz, T* __restrict__ sx,
-                                                  T* __restrict__ sy, T* __restrict__ sz, long n,
-                                                  AdvCoef ac)
-{
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        double x = (double)psi[id], x2 = x * x, x3 = x2 * x;
-        double mu = (double)sp2[id] - ac.pc.beta * x3 + ac.pc.gamma * x3 * x2;
-        sx[id] = T(mu * (double)gx[id]);
-        sy[id] = T(mu * (double)gy[id]);
-        sz[id] = T(mu * (double)gz[id]);
-    }
-}
-
 // Stokes projection with the reference's cross-index reads (:1103-1274):
 //  ux[i,j,k] = CM1x (Sx - CM2x (qsx Sx + qy Sy[i+1,j-1,k] + qz Sz[i+1,j,k-1])),  i <= Nx-2
 //  uy[i,j,k] = CM1y (Sy - CM2y (qx Sx[i-1,j+1,k] + qsy Sy + qz Sz[i,j+1,k-1])),  j <= Ny-2, k <= Nz-2
@@ -783,6 +765,60 @@ __global__ __launch_bounds__(WG) void k_multi(const T* in, PassSet<T> ps, Dims d
 
 // REDFT10_z of Nr = beta psi^3 - gamma psi^5 - u . grad psi (:1290-1307),
 // the nonlinear term formed while the z lines are loaded.
+// Forward z passes of S = mu grad psi, mu = Sp2 - beta psi^3 + gamma psi^5
+// (adv/cosAdvTrackMult.cpp:996-1027): the force is formed while the z lines
+// are loaded, one output per component (z kinds REDFT10, REDFT10, RODFT10);
+// psi and Sp2 are re-read from cache for the second and third.
+template <typename T, int N>
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_force_z(const T* __restrict__ psi, const T* __restrict__ sp2,
+                                                                const T* __restrict__ gx, const T* __restrict__ gy,
+                                                                const T* __restrict__ gz, T* __restrict__ sx,
+                                                                T* __restrict__ sy, T* __restrict__ sz, Dims d,
+                                                                AdvCoef ac, Tw<T> tw)
+{
+    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int WGZ = LinesZ<T>::WGZ;
+    constexpr int W = V16<T>::W;
+    __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    __syncthreads();
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    auto component = [&](const T* __restrict__ g, T* __restrict__ out, int kind) {
+        InMap<T, N> im{kind};
+        zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+            const V16<T> a = *reinterpret_cast<const V16<T>*>(psi + gi);
+            const V16<T> b = *reinterpret_cast<const V16<T>*>(sp2 + gi);
+            const V16<T> h = *reinterpret_cast<const V16<T>*>(g + gi);
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                double x = (double)a.v[t], x2 = x * x, x3 = x2 * x;
+                double mu = (double)b.v[t] - ac.pc.beta * x3 + ac.pc.gamma * x3 * x2;
+                int q;
+                T sg = im(p + t, q);
+                lds[c * LS + PR(q)] = sg * T(mu * (double)h.v[t]);
+            }
+        });
+        lsync<true>();
+        r2r_line<T, N, TPL, true>(lds + l * LS, u, kind, ct);
+        OutMap<T, N> om{kind};
+        zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+            V16<T> v;
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                int q;
+                T sg = om(p + t, q);
+                v.v[t] = sg * lds[c * LS + PR(q)];
+            }
+            *reinterpret_cast<V16<T>*>(out + gi) = v;
+        });
+        lsync<true>();
+    };
+    component(gx, sx, REDFT10);
+    component(gy, sy, REDFT10);
+    component(gz, sz, RODFT10);
+}
+
 // part 0: the whole Nr; 1: beta psi^3 - gamma psi^5 only; 2: u . grad psi only
 // (the conserved variant transforms the two separately, adv/cosAdvConSingle.cpp:1290-1310)
 template <typename T, int N>
@@ -1496,8 +1532,6 @@ struct Launch {
     static void plain_x(hipStream_t s, const T* a_in, T* w_out, const T* ph_in, T* ph_out,
                         T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse = 1,
                         const T* sub = nullptr);
-    static void adv_force(hipStream_t s, const T* psi, const T* sp2, const T* gx, const T* gy,
-                          const T* gz, T* sx, T* sy, T* sz, Dims d, const AdvCoef& ac);
     static void adv_stokes(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, T* ux, T* uy,
                            T* uz, Dims d, const AdvCoef& ac);
     // z pass with one output per op of the set `zs` (ZS_*); strided pass
@@ -1517,6 +1551,8 @@ struct Launch {
                        const T* gpy, const T* gpz, const double* pr, const T* rho, const T* e, const T* grx,
                        const T* gry, const T* grz, T* out, Dims d, const QuasiCoef& qc, int press_term,
                        const Tw<T>& tw);
+    static void adv_force_z(hipStream_t s, const T* psi, const T* sp2, const T* gx, const T* gy, const T* gz,
+                            T* sx, T* sy, T* sz, Dims d, const AdvCoef& ac, const Tw<T>& tw);
     static void adv_nl_z(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
                          const T* gx, const T* gy, const T* gz, T* out, Dims d, const AdvCoef& ac,
                          const Tw<T>& tw, int part = 0);

@@ -162,20 +162,20 @@ void Launch<T>::q_nl_z(hipStream_t s, const T* mu, const T* ux, const T* uy, con
 }
 
 template <typename T>
+void Launch<T>::adv_force_z(hipStream_t s, const T* psi, const T* sp2, const T* gx, const T* gy, const T* gz,
+                            T* sx, T* sy, T* sz, Dims d, const AdvCoef& ac, const Tw<T>& tw)
+{
+    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+    SMAD_DISPATCH_Z(d.nz, k_adv_force_z, grid, psi, sp2, gx, gy, gz, sx, sy, sz, d, ac, tw);
+}
+
+template <typename T>
 void Launch<T>::adv_nl_z(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
                          const T* gx, const T* gy, const T* gz, T* out, Dims d, const AdvCoef& ac,
                          const Tw<T>& tw, int part)
 {
     dim3 grid(num_tiles(d, 2, linesz_nl<T>(d.nz)));
     SMAD_DISPATCH_Z(d.nz, k_adv_nl_z, grid, psi, ux, uy, uz, gx, gy, gz, out, d, ac, part, tw);
-}
-
-template <typename T>
-void Launch<T>::adv_force(hipStream_t s, const T* psi, const T* sp2, const T* gx, const T* gy,
-                          const T* gz, T* sx, T* sy, T* sz, Dims d, const AdvCoef& ac)
-{
-    hipLaunchKernelGGL(k_adv_force<T>, flat_grid(d.total()), dim3(WG), 0, s, psi, sp2, gx, gy, gz,
-                       sx, sy, sz, d.total(), ac);
 }
 
 template <typename T>

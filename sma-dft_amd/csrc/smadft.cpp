@@ -651,7 +651,7 @@ struct AdvEngine : EngineBase {
     const char* kernel_class(int c) const override
     {
         static const char* names[] = {"axis passes (r2r)", "inverse passes: psi, grad psi, Sp2",
-                                      "adv_force", "adv_stokes", "adv_nl + REDFT10_z",
+                                      "adv_force + S forward z passes", "adv_stokes", "adv_nl + REDFT10_z",
                                       "plain_x: REDFT10_x + CN/AB2 update"};
         return (c >= 0 && c < 6) ? names[c] : nullptr;
     }
@@ -701,11 +701,15 @@ struct AdvEngine : EngineBase {
         Launch<T>::axis(s, sp.p, sp.p, d, 0, REDFT01, T(1), twx.tw);
         mk(1);
         if (keep_prev) HIPCHK(hipMemcpyAsync(psi_prev.p, psi.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-        Launch<T>::adv_force(s, psi.p, sp.p, gx.p, gy.p, gz.p, sx.p, sy.p, sz.p, d, ac);
+        // S = mu grad psi formed in its forward z passes (:996-1027)
+        Launch<T>::adv_force_z(s, psi.p, sp.p, gx.p, gy.p, gz.p, sx.p, sy.p, sz.p, d, ac, twz.tw);
         mk(2);
-        tr3(sx.p, sx.p, RODFT10, REDFT10, REDFT10);
-        tr3(sy.p, sy.p, REDFT10, RODFT10, REDFT10);
-        tr3(sz.p, sz.p, REDFT10, REDFT10, RODFT10);
+        Launch<T>::axis(s, sx.p, sx.p, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(s, sx.p, sx.p, d, 0, RODFT10, T(1), twx.tw);
+        Launch<T>::axis(s, sy.p, sy.p, d, 1, RODFT10, T(1), twy.tw);
+        Launch<T>::axis(s, sy.p, sy.p, d, 0, REDFT10, T(1), twx.tw);
+        Launch<T>::axis(s, sz.p, sz.p, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(s, sz.p, sz.p, d, 0, REDFT10, T(1), twx.tw);
         mk(0);
         Launch<T>::adv_stokes(s, sx.p, sy.p, sz.p, ux.p, uy.p, uz.p, d, ac);
         mk(3);
