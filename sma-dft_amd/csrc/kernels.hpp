@@ -505,52 +505,6 @@ __device__ __forceinline__ void unflat(long id, const Dims& d, int& i, int& j, i
     i = (int)(r / d.ny);
 }
 
-// Stokes projection with the reference's cross-index reads (:1103-1274):
-//  ux[i,j,k] = CM1x (Sx - CM2x (qsx Sx + qy Sy[i+1,j-1,k] + qz Sz[i+1,j,k-1])),  i <= Nx-2
-//  uy[i,j,k] = CM1y (Sy - CM2y (qx Sx[i-1,j+1,k] + qsy Sy + qz Sz[i,j+1,k-1])),  j <= Ny-2, k <= Nz-2
-//  uz[i,j,k] = CM1z (Sz - CM2z (qx Sx[i-1,j,k+1] + qy Sy[i,j-1,k+1] + qsz Sz)),  j <= Ny-2, k <= Nz-2
-// CM1 = s/(nu |q|^2), CM2 = qs/|q|^2 with the DST wavenumber on the own axis;
-// out-of-range neighbours read as 0, all other entries are 0.
-template <typename T>
-__global__ __launch_bounds__(WG) void k_adv_stokes(const T* __restrict__ Sx, const T* __restrict__ Sy,
-                                                   const T* __restrict__ Sz, T* __restrict__ ux,
-                                                   T* __restrict__ uy, T* __restrict__ uz, Dims d,
-                                                   AdvCoef ac)
-{
-    const PlainCoef& cf = ac.pc;
-    long n = d.total();
-    const long sx = (long)d.ny * d.nz, sy = d.nz;
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        int i, j, k;
-        unflat(id, d, i, j, k);
-        double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
-        double qsx = (i + 1) * cf.qfx, qsy = (j + 1) * cf.qfy, qsz = (k + 1) * cf.qfz;
-        double vx = 0.0, vy = 0.0, vz = 0.0;
-        if (i < d.nx - 1) {
-            double syv = j != 0 ? (double)Sy[id + sx - sy] : 0.0;
-            double szv = k != 0 ? (double)Sz[id + sx - 1] : 0.0;
-            double mq2 = qsx * qsx + qy * qy + qz * qz;
-            double dot = qsx * (double)Sx[id] + qy * syv + qz * szv;
-            vx = cf.scale / (ac.nu * mq2) * ((double)Sx[id] - qsx / mq2 * dot);
-        }
-        if (j < d.ny - 1 && k < d.nz - 1) {
-            double sxv = i != 0 ? (double)Sx[id - sx + sy] : 0.0;
-            double szv = k != 0 ? (double)Sz[id + sy - 1] : 0.0;
-            double mq2 = qx * qx + qsy * qsy + qz * qz;
-            double dot = qx * sxv + qsy * (double)Sy[id] + qz * szv;
-            vy = cf.scale / (ac.nu * mq2) * ((double)Sy[id] - qsy / mq2 * dot);
-            sxv = i != 0 ? (double)Sx[id - sx + 1] : 0.0;
-            double syv = j != 0 ? (double)Sy[id - sy + 1] : 0.0;
-            mq2 = qx * qx + qy * qy + qsz * qsz;
-            dot = qx * sxv + qy * syv + qsz * (double)Sz[id];
-            vz = cf.scale / (ac.nu * mq2) * ((double)Sz[id] - qsz / mq2 * dot);
-        }
-        ux[id] = T(vx);
-        uy[id] = T(vy);
-        uz[id] = T(vz);
-    }
-}
-
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* out);
 
@@ -817,6 +771,91 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_force_z(const T* __restr
     component(gx, sx, REDFT10);
     component(gy, sy, REDFT10);
     component(gz, sz, RODFT10);
+}
+
+// Stokes projection with the reference's cross-index reads (:1103-1274):
+//  ux[i,j,k] = CM1x (Sx - CM2x (qsx Sx + qy Sy[i+1,j-1,k] + qz Sz[i+1,j,k-1])),  i <= Nx-2
+//  uy[i,j,k] = CM1y (Sy - CM2y (qx Sx[i-1,j+1,k] + qsy Sy + qz Sz[i,j+1,k-1])),  j <= Ny-2, k <= Nz-2
+//  uz[i,j,k] = CM1z (Sz - CM2z (qx Sx[i-1,j,k+1] + qy Sy[i,j-1,k+1] + qsz Sz)),  j <= Ny-2, k <= Nz-2
+// CM1 = s/(nu |q|^2), CM2 = qs/|q|^2 with the DST wavenumber on the own axis;
+// out-of-range neighbours read as 0, all other entries are 0.  Each
+// component is formed while the lines of its inverse z pass (:1276-1286) are
+// loaded, from the own line of S and the neighbour lines (from cache); z
+// kinds REDFT01, REDFT01, RODFT01.
+template <typename T, int N>
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_stokes_z(const T* __restrict__ Sx, const T* __restrict__ Sy,
+                                                                 const T* __restrict__ Sz, T* __restrict__ ux,
+                                                                 T* __restrict__ uy, T* __restrict__ uz, Dims d,
+                                                                 AdvCoef ac, Tw<T> tw)
+{
+    constexpr int NL = LinesZ<T>::NL, TPL = LinesZ<T>::TPL, LS = LineLS<N>::value;
+    constexpr int WGZ = LinesZ<T>::WGZ;
+    constexpr int W = V16<T>::W;
+    __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    __syncthreads();
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    const PlainCoef& cf = ac.pc;
+    const long sx = (long)d.ny * d.nz, sy = d.nz;
+    // comp 0: ux, 1: uy, 2: uz; value of element k of line (i, j) at global gi
+    auto vel = [&](int comp, int i, int j, int k, long gi) -> double {
+        const double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
+        if (comp == 0) {
+            if (i >= d.nx - 1) return 0.0;
+            const double qsx = (i + 1) * cf.qfx;
+            const double syv = j != 0 ? (double)Sy[gi + sx - sy] : 0.0;
+            const double szv = k != 0 ? (double)Sz[gi + sx - 1] : 0.0;
+            const double mq2 = qsx * qsx + qy * qy + qz * qz;
+            const double dot = qsx * (double)Sx[gi] + qy * syv + qz * szv;
+            return cf.scale / (ac.nu * mq2) * ((double)Sx[gi] - qsx / mq2 * dot);
+        }
+        if (j >= d.ny - 1 || k >= d.nz - 1) return 0.0;
+        if (comp == 1) {
+            const double qsy = (j + 1) * cf.qfy;
+            const double sxv = i != 0 ? (double)Sx[gi - sx + sy] : 0.0;
+            const double szv = k != 0 ? (double)Sz[gi + sy - 1] : 0.0;
+            const double mq2 = qx * qx + qsy * qsy + qz * qz;
+            const double dot = qx * sxv + qsy * (double)Sy[gi] + qz * szv;
+            return cf.scale / (ac.nu * mq2) * ((double)Sy[gi] - qsy / mq2 * dot);
+        }
+        const double qsz = (k + 1) * cf.qfz;
+        const double sxv = i != 0 ? (double)Sx[gi - sx + 1] : 0.0;
+        const double syv = j != 0 ? (double)Sy[gi - sy + 1] : 0.0;
+        const double mq2 = qx * qx + qy * qy + qsz * qsz;
+        const double dot = qx * sxv + qy * syv + qsz * (double)Sz[gi];
+        return cf.scale / (ac.nu * mq2) * ((double)Sz[gi] - qsz / mq2 * dot);
+    };
+    auto component = [&](int comp, T* __restrict__ out, int kind) {
+        InMap<T, N> im{kind};
+        zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+            const long L = gi / N;
+            const int i = (int)(L / d.ny), j = (int)(L - (long)i * d.ny);
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                int q;
+                T sg = im(p + t, q);
+                lds[c * LS + PR(q)] = sg * T(vel(comp, i, j, p + t, gi + t));
+            }
+        });
+        lsync<true>();
+        r2r_line<T, N, TPL, true>(lds + l * LS, u, kind, ct);
+        OutMap<T, N> om{kind};
+        zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+            V16<T> v;
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                int q;
+                T sg = om(p + t, q);
+                v.v[t] = sg * lds[c * LS + PR(q)];
+            }
+            *reinterpret_cast<V16<T>*>(out + gi) = v;
+        });
+        lsync<true>();
+    };
+    component(0, ux, REDFT01);
+    component(1, uy, REDFT01);
+    component(2, uz, RODFT01);
 }
 
 // part 0: the whole Nr; 1: beta psi^3 - gamma psi^5 only; 2: u . grad psi only
@@ -1532,8 +1571,6 @@ struct Launch {
     static void plain_x(hipStream_t s, const T* a_in, T* w_out, const T* ph_in, T* ph_out,
                         T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse = 1,
                         const T* sub = nullptr);
-    static void adv_stokes(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, T* ux, T* uy,
-                           T* uz, Dims d, const AdvCoef& ac);
     // z pass with one output per op of the set `zs` (ZS_*); strided pass
     // with the outputs (OP_SHIFT, OP_ID)
     static void zpass(hipStream_t s, int zs, const T* in, const PassSet<T>& ps, Dims d, const SpecCoef& sc,
@@ -1553,6 +1590,8 @@ struct Launch {
                        const Tw<T>& tw);
     static void adv_force_z(hipStream_t s, const T* psi, const T* sp2, const T* gx, const T* gy, const T* gz,
                             T* sx, T* sy, T* sz, Dims d, const AdvCoef& ac, const Tw<T>& tw);
+    static void adv_stokes_z(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, T* ux, T* uy, T* uz, Dims d,
+                             const AdvCoef& ac, const Tw<T>& tw);
     static void adv_nl_z(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
                          const T* gx, const T* gy, const T* gz, T* out, Dims d, const AdvCoef& ac,
                          const Tw<T>& tw, int part = 0);

@@ -170,20 +170,20 @@ void Launch<T>::adv_force_z(hipStream_t s, const T* psi, const T* sp2, const T* 
 }
 
 template <typename T>
+void Launch<T>::adv_stokes_z(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, T* ux, T* uy, T* uz, Dims d,
+                             const AdvCoef& ac, const Tw<T>& tw)
+{
+    dim3 grid(num_tiles(d, 2, LinesZ<T>::NL));
+    SMAD_DISPATCH_Z(d.nz, k_adv_stokes_z, grid, Sx, Sy, Sz, ux, uy, uz, d, ac, tw);
+}
+
+template <typename T>
 void Launch<T>::adv_nl_z(hipStream_t s, const T* psi, const T* ux, const T* uy, const T* uz,
                          const T* gx, const T* gy, const T* gz, T* out, Dims d, const AdvCoef& ac,
                          const Tw<T>& tw, int part)
 {
     dim3 grid(num_tiles(d, 2, linesz_nl<T>(d.nz)));
     SMAD_DISPATCH_Z(d.nz, k_adv_nl_z, grid, psi, ux, uy, uz, gx, gy, gz, out, d, ac, part, tw);
-}
-
-template <typename T>
-void Launch<T>::adv_stokes(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, T* ux, T* uy,
-                           T* uz, Dims d, const AdvCoef& ac)
-{
-    hipLaunchKernelGGL(k_adv_stokes<T>, flat_grid(d.total()), dim3(WG), 0, s, Sx, Sy, Sz, ux, uy,
-                       uz, d, ac);
 }
 
 template <typename T>

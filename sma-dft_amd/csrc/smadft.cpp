@@ -651,7 +651,7 @@ struct AdvEngine : EngineBase {
     const char* kernel_class(int c) const override
     {
         static const char* names[] = {"axis passes (r2r)", "inverse passes: psi, grad psi, Sp2",
-                                      "adv_force + S forward z passes", "adv_stokes", "adv_nl + REDFT10_z",
+                                      "adv_force + S forward z passes", "adv_stokes + u inverse z passes", "adv_nl + REDFT10_z",
                                       "plain_x: REDFT10_x + CN/AB2 update"};
         return (c >= 0 && c < 6) ? names[c] : nullptr;
     }
@@ -711,11 +711,15 @@ struct AdvEngine : EngineBase {
         Launch<T>::axis(s, sz.p, sz.p, d, 1, REDFT10, T(1), twy.tw);
         Launch<T>::axis(s, sz.p, sz.p, d, 0, REDFT10, T(1), twx.tw);
         mk(0);
-        Launch<T>::adv_stokes(s, sx.p, sy.p, sz.p, ux.p, uy.p, uz.p, d, ac);
+        // Stokes projection formed in the velocity's inverse z passes (:1103-1286)
+        Launch<T>::adv_stokes_z(s, sx.p, sy.p, sz.p, ux.p, uy.p, uz.p, d, ac, twz.tw);
         mk(3);
-        tr3(ux.p, ux.p, RODFT01, REDFT01, REDFT01);
-        tr3(uy.p, uy.p, REDFT01, RODFT01, REDFT01);
-        tr3(uz.p, uz.p, REDFT01, REDFT01, RODFT01);
+        Launch<T>::axis(s, ux.p, ux.p, d, 1, REDFT01, T(1), twy.tw);
+        Launch<T>::axis(s, ux.p, ux.p, d, 0, RODFT01, T(1), twx.tw);
+        Launch<T>::axis(s, uy.p, uy.p, d, 1, RODFT01, T(1), twy.tw);
+        Launch<T>::axis(s, uy.p, uy.p, d, 0, REDFT01, T(1), twx.tw);
+        Launch<T>::axis(s, uz.p, uz.p, d, 1, REDFT01, T(1), twy.tw);
+        Launch<T>::axis(s, uz.p, uz.p, d, 0, REDFT01, T(1), twx.tw);
         mk(0);
         if (!ac.pc.conserved) {
             Launch<T>::adv_nl_z(s, psi.p, ux.p, uy.p, uz.p, gx.p, gy.p, gz.p, nr.p, d, ac, twz.tw);
