@@ -1269,27 +1269,67 @@ __global__ __launch_bounds__(WG) void k_surf_track(const T* __restrict__ psi, co
     ck[col] = o_k;
 }
 
-// BDF2/AB2 update (:2169-2182), C1 = 2 - dt rho_m aLin, C2 = 1.5 - dt rho_m aLin:
-//   psihat' = (C1 psihat - psihat_old/2 + dt/2 s (3 Nl - Nl_old)) / C2
-// then psihat_old = psihat, Nl_old = Nl (the next step's history).
-template <typename T>
-__global__ __launch_bounds__(WG) void k_q_update(T* __restrict__ ph, T* __restrict__ pho,
-                                                 const T* __restrict__ nl, T* __restrict__ nlo,
-                                                 Dims d, QuasiCoef qc)
+// x pass of the quasi step: Nl = REDFT10_x(nl_zy) (nl_zy already through its
+// z and y passes), then the BDF2/AB2 update (:2169-2182) on the tile,
+// C1 = 2 - dt rho_m aLin, C2 = 1.5 - dt rho_m aLin:
+//   psihat' = (C1 psihat - psihat_old/2 + dt/2 s (3 Nl - Nl_old)) / C2,
+// psihat_old <- psihat, Nl_old <- Nl (Nl_old = Nl on the first step,
+// :2159-2161).  Nl itself is not stored.
+template <typename T, int N>
+__global__ __launch_bounds__(WG) void k_q_x(const T* __restrict__ nl_zy, T* __restrict__ ph, T* __restrict__ pho,
+                                            T* __restrict__ nlo, Dims d, QuasiCoef qc, int first, Tw<T> tw)
 {
+    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
+    constexpr int W = V16<T>::W;
+    __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    TileGeo g = tile_geo(d, 0, blockIdx.x, NL);
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    struct Mk {
+        __device__ T operator()(int p, int& q) const { q = mk_pos<N>(p); return T(1); }
+    };
+    load_rows<T, N, NL, LS>(g, nl_zy, lds, Mk{});
+    __syncthreads();
+    dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    __syncthreads();
     const PlainCoef& cf = qc.pc;
-    const long n = d.total();
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        int i, j, k;
-        unflat(id, d, i, j, k);
-        double t = cf.q0sq - mq2_at(cf, i, j, k);
+    const int nct = (d.nz + NL - 1) / NL;
+    const int j = blockIdx.x / nct;
+    const int k0 = (blockIdx.x % nct) * NL;
+    auto upd = [&](int p, int c, double p0, double po, double nv, double no) {
+        double t = cf.q0sq - mq2_at(cf, p, j, k0 + c);
         double al = cf.eps - cf.alpha * t * t;
         double c1 = 2.0 - qc.dt * qc.rho_m * al, c2 = 1.5 - qc.dt * qc.rho_m * al;
-        double p0 = (double)ph[id], nv = (double)nl[id];
-        double r = (c1 * p0 - 0.5 * (double)pho[id] + cf.dtd2 * cf.scale * (3.0 * nv - (double)nlo[id])) / c2;
-        pho[id] = ph[id];
-        ph[id] = T(r);
-        nlo[id] = nl[id];
+        return (c1 * p0 - 0.5 * po + cf.dtd2 * cf.scale * (3.0 * nv - no)) / c2;
+    };
+    if (g.ncol == NL) {
+        rows_vec<T, N, NL, 1>(g, [&](int p, int c0, long gi) {
+            V16<T> a = *reinterpret_cast<const V16<T>*>(ph + gi);
+            V16<T> b = *reinterpret_cast<const V16<T>*>(pho + gi);
+            V16<T> o = *reinterpret_cast<const V16<T>*>(nlo + gi);
+            V16<T> na, nq;
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                T nv = lds[(c0 + t) * LS + PR(p)];
+                nq.v[t] = nv;
+                na.v[t] = T(upd(p, c0 + t, (double)a.v[t], (double)b.v[t], (double)nv,
+                                first ? (double)nv : (double)o.v[t]));
+            }
+            *reinterpret_cast<V16<T>*>(pho + gi) = a;
+            *reinterpret_cast<V16<T>*>(ph + gi) = na;
+            *reinterpret_cast<V16<T>*>(nlo + gi) = nq;
+        });
+    } else {
+        tile_for<N, NL>(g, 0, [&](int p, int c, long gi, bool ok) {
+            if (!ok) return;
+            T nv = lds[c * LS + PR(p)];
+            T a = ph[gi];
+            T r = T(upd(p, c, (double)a, (double)pho[gi], (double)nv, first ? (double)nv : (double)nlo[gi]));
+            pho[gi] = a;
+            ph[gi] = r;
+            nlo[gi] = nv;
+        });
     }
 }
 
@@ -1609,8 +1649,8 @@ struct Launch {
                         const T* grz, T* e, T* mu, T* sx, T* sy, T* sz, Dims d, const QuasiCoef& qc);
     static void q_pvel(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, const T* dv,
                        double* ph, T* ux, T* uy, T* uz, Dims d, const QuasiCoef& qc);
-    static void q_update(hipStream_t s, T* ph, T* pho, const T* nl, T* nlo, Dims d,
-                         const QuasiCoef& qc);
+    static void q_x(hipStream_t s, const T* nl_zy, T* ph, T* pho, T* nlo, Dims d, const QuasiCoef& qc, int first,
+                    const Tw<T>& tw);
     static void l1(hipStream_t s, const T* a, const T* b, long n, double* part, int nparts,
                    double* out);
 };

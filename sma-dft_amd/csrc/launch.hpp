@@ -252,10 +252,11 @@ void Launch<T>::q_pvel(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, con
 }
 
 template <typename T>
-void Launch<T>::q_update(hipStream_t s, T* ph, T* pho, const T* nl, T* nlo, Dims d,
-                         const QuasiCoef& qc)
+void Launch<T>::q_x(hipStream_t s, const T* nl_zy, T* ph, T* pho, T* nlo, Dims d, const QuasiCoef& qc, int first,
+                    const Tw<T>& tw)
 {
-    SMAD_FLAT(k_q_update, ph, pho, nl, nlo, d, qc);
+    dim3 grid(num_tiles(d, 0, Lines<T>::NL));
+    SMAD_DISPATCH(d.nx, k_q_x, grid, nl_zy, ph, pho, nlo, d, qc, first, tw);
 }
 
 }  // namespace smad

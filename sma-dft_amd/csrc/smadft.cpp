@@ -910,6 +910,7 @@ struct QuasiEngine : EngineBase {
         Launch<T>::axis(s, nl.p, nl.p, d, 1, REDFT10, T(1), twy.tw);
         Launch<T>::axis(s, nl.p, nl.p, d, 0, REDFT10, T(1), twx.tw);
         HIPCHK(hipMemcpyAsync(pho.p, ph.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(nlo.p, nl.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
         zero(grx);  // grad of the initial filtered rho, which is zero
         zero(gry);
         zero(grz);
@@ -1000,10 +1001,10 @@ struct QuasiEngine : EngineBase {
                           grz.p, nl.p, d, qc, nloop > p.div_switch + 1 ? 1 : 0, twz.tw);
         mk(1);
         Launch<T>::axis(s, nl.p, nl.p, d, 1, REDFT10, T(1), twy.tw);
-        Launch<T>::axis(s, nl.p, nl.p, d, 0, REDFT10, T(1), twx.tw);
         mk(0);
-        if (nloop == 1) HIPCHK(hipMemcpyAsync(nlo.p, nl.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-        Launch<T>::q_update(s, ph.p, pho.p, nl.p, nlo.p, d, qc);
+        // REDFT10_x of Nl and the BDF2/AB2 update (:2128-2182) in one pass;
+        // nlo receives this step's Nl hat (Nl_old = Nl on the first step)
+        Launch<T>::q_x(s, nl.p, ph.p, pho.p, nlo.p, d, qc, nloop == 1 ? 1 : 0, twx.tw);
         mk(1);
         std::swap(grx.p, grx2.p);
         std::swap(gry.p, gry2.p);
@@ -1032,7 +1033,7 @@ struct QuasiEngine : EngineBase {
         switch (field) {
         case SMAD_FIELD_PSI: materialize(); return download(s, psi.p, n, out);
         case SMAD_FIELD_PSI_HAT: return download(s, ph.p, n, out);
-        case SMAD_FIELD_N_HAT: return download(s, nl.p, n, out);
+        case SMAD_FIELD_N_HAT: return download(s, nlo.p, n, out);  // this step's Nl hat
         case SMAD_FIELD_GRAD_X: return download(s, gpx.p, n, out);
         case SMAD_FIELD_GRAD_Y: return download(s, gpy.p, n, out);
         case SMAD_FIELD_GRAD_Z: return download(s, gpz.p, n, out);
