@@ -40,20 +40,22 @@ struct Lines {
 #ifndef SMAD_ZTPL32
 #define SMAD_ZTPL32 16
 #endif
-// Lines of 512 points and more take twice the threads per line (half the
-// registers per thread, half the lines per tile); z lines are contiguous, so
-// the narrower tile costs no coalescing.
-constexpr int zlong(int n) { return n >= 512 ? 2 : 1; }
+// Long lines take twice the threads per line (half the registers per
+// thread, half the lines per tile): from 384 points in FP32 and from 512 in
+// FP64 (at 384 the FP64 pass is no faster). z lines are contiguous, so the
+// narrower tile costs no coalescing.
+template <typename T>
+constexpr int zlong(int n) { return n >= (sizeof(T) == 8 ? 512 : 384) ? 2 : 1; }
 template <typename T, int N = 256>
 struct LinesZ {
     static constexpr int WGZ = sizeof(T) == 8 ? SMAD_ZWG64 : SMAD_ZWG32;
-    static constexpr int TPL = (sizeof(T) == 8 ? SMAD_ZTPL64 : SMAD_ZTPL32) * zlong(N);
+    static constexpr int TPL = (sizeof(T) == 8 ? SMAD_ZTPL64 : SMAD_ZTPL32) * zlong<T>(N);
     static constexpr int NL = WGZ / TPL;
 };
 template <typename T>
 inline int linesz_nl(int n)
 {
-    return (sizeof(T) == 8 ? SMAD_ZWG64 : SMAD_ZWG32) / ((sizeof(T) == 8 ? SMAD_ZTPL64 : SMAD_ZTPL32) * zlong(n));
+    return (sizeof(T) == 8 ? SMAD_ZWG64 : SMAD_ZWG32) / ((sizeof(T) == 8 ? SMAD_ZTPL64 : SMAD_ZTPL32) * zlong<T>(n));
 }
 
 struct Dims {
