@@ -116,15 +116,17 @@ struct alignas(16) V16 {
 
 // Iterate a strided tile (axis x or y) in 16-byte groups: f(p, c0, gi) for
 // columns c0 .. c0+W-1 of row p.  A wavefront covers 64 groups, i.e. whole
-// 128 B row segments.  Requires a full tile (ncol == NL).
+// 128 B row segments.  Requires a full tile (ncol == NL).  e0 is zero (an
+// opaque_zero() where several transforms follow one another), as in
+// tile_for, load_rows and store_rows.
 template <typename T, int N, int NL, int UNROLL = 8, typename F>
-__device__ __forceinline__ void rows_vec(const TileGeo& g, F&& f)
+__device__ __forceinline__ void rows_vec(const TileGeo& g, F&& f, int e0 = 0)
 {
     constexpr int W = V16<T>::W;
     constexpr int GPR = NL / W;
     constexpr int TOT = N * GPR;
 #pragma unroll UNROLL
-    for (int e = threadIdx.x; e < TOT; e += WG) {
+    for (int e = threadIdx.x + e0; e < TOT; e += WG) {
         int p = e / GPR, c0 = (e - p * GPR) * W;
         f(p, c0, g.base + (long)p * g.stride + c0);
     }
@@ -132,10 +134,10 @@ __device__ __forceinline__ void rows_vec(const TileGeo& g, F&& f)
 
 // Scalar iteration of any tile (partial tiles, small grids).
 template <int N, int NL, typename F>
-__device__ __forceinline__ void tile_for(const TileGeo& g, int axis, F&& f)
+__device__ __forceinline__ void tile_for(const TileGeo& g, int axis, F&& f, int e0 = 0)
 {
     constexpr int TOT = N * NL;
-    for (int e = threadIdx.x; e < TOT; e += WG) {
+    for (int e = threadIdx.x + e0; e < TOT; e += WG) {
         int p, c;
         if (axis == 2) { c = e / N; p = e - c * N; }
         else { p = e / NL; c = e - p * NL; }
@@ -168,7 +170,7 @@ __device__ __forceinline__ void load_rows(const TileGeo& g, const T* src, T* lds
 
 // Store a strided tile from LDS: dst[(p, c)] = scale * sgn(p) * lds[c*LS + pos(p)].
 template <typename T, int N, int NL, int LS, typename Pos>
-__device__ __forceinline__ void store_rows(const TileGeo& g, T* dst, const T* lds, Pos pos, T scale)
+__device__ __forceinline__ void store_rows(const TileGeo& g, T* dst, const T* lds, Pos pos, T scale, int e0 = 0)
 {
     if (g.ncol == NL) {
         rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
@@ -178,21 +180,22 @@ __device__ __forceinline__ void store_rows(const TileGeo& g, T* dst, const T* ld
 #pragma unroll
             for (int t = 0; t < V16<T>::W; ++t) v.v[t] = sg * lds[(c0 + t) * LS + PR(q)];
             *reinterpret_cast<V16<T>*>(dst + gi) = v;
-        });
+        }, e0);
     } else {
         tile_for<N, NL>(g, 1, [&](int p, int c, long gi, bool ok) {
             int q;
             T sg = scale * pos(p, q);
             if (ok) dst[gi] = sg * lds[c * LS + PR(q)];
-        });
+        }, e0);
     }
 }
 
 // z lines are contiguous: each wavefront loads, transforms and stores only
 // its own NL/4 lines, so the z passes need no workgroup barrier after the
-// twiddle table is in LDS.  f(p, c, gi) per 16-byte group of line c.
+// twiddle table is in LDS.  f(p, c, gi) per 16-byte group of line c; e0 is
+// zero (an opaque_zero() where several transforms follow one another).
 template <typename T, int N, int NL, int WGK, typename F>
-__device__ __forceinline__ void zlines_vec(const Dims& d, F&& f)
+__device__ __forceinline__ void zlines_vec(const Dims& d, F&& f, int e0 = 0)
 {
     constexpr int W = V16<T>::W;
     constexpr int LPW = NL / (WGK / 64);  // lines per wavefront
@@ -201,7 +204,7 @@ __device__ __forceinline__ void zlines_vec(const Dims& d, F&& f)
     const long nlines = (long)d.nx * d.ny;
     const long l0 = (long)blockIdx.x * NL + wv * LPW;
 #pragma unroll 4
-    for (int e = lane; e < TOT; e += 64) {
+    for (int e = lane + e0; e < TOT; e += 64) {
         int lc = (e * W) / N;
         int p = e * W - lc * N;
         if (l0 + lc < nlines) f(p, wv * LPW + lc, (l0 + lc) * N + p);
@@ -469,9 +472,10 @@ __device__ __forceinline__ void plain_x_body(const T* a_in, T* w_out, const T* p
     }
     if (!inverse) return;
     __syncthreads();
-    dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    const int e0 = opaque_zero_if<N>();
+    dct3_line<T, N, TPL, true>(lds + l * LS, u + e0, ct);
     __syncthreads();
-    store_rows<T, N, NL, LS>(g, w_out, lds, Mk{}, T(1));
+    store_rows<T, N, NL, LS>(g, w_out, lds, Mk{}, T(1), e0);
 }
 
 template <typename T, int N>
@@ -614,6 +618,7 @@ __device__ __forceinline__ void zmulti_one(const T* in, T* out, int kind, const 
     constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
+    const int e0 = opaque_zero_if<N>();
     zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
         int i = 0, j = 0;
@@ -628,9 +633,9 @@ __device__ __forceinline__ void zmulti_one(const T* in, T* out, int kind, const 
             T y = pass_in<T, N, OP>(kind, sc.pc.qfz, sc, p + t, v.v[t], i, j, p + t, q);
             lds[c * LS + PR(q)] = y;
         }
-    });
+    }, e0);
     lsync<true>();
-    r2r_line<T, N, TPL, true>(lds + l * LS, u, kind, ct);
+    r2r_line<T, N, TPL, true>(lds + l * LS, u + e0, kind, ct);
     OutMap<T, N> om{kind};
     zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v;
@@ -641,7 +646,7 @@ __device__ __forceinline__ void zmulti_one(const T* in, T* out, int kind, const 
             v.v[t] = sg * lds[c * LS + PR(q)];
         }
         *reinterpret_cast<V16<T>*>(out + gi) = v;
-    });
+    }, e0);
     lsync<true>();
 }
 
@@ -675,6 +680,7 @@ __device__ __forceinline__ void multi_one(const T* in, T* out, int kind, int axi
         else { i = outer; j = p; }
     };
     __syncthreads();
+    const int e0 = opaque_zero_if<N>();
     if (g.ncol == NL) {
         rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
             V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
@@ -685,19 +691,19 @@ __device__ __forceinline__ void multi_one(const T* in, T* out, int kind, int axi
                 T y = pass_in<T, N, OP>(kind, qf, sc, p, v.v[t], i, j, k, q);
                 lds[(c0 + t) * LS + PR(q)] = y;
             }
-        });
+        }, e0);
     } else {
         tile_for<N, NL>(g, 1, [&](int p, int c, long gi, bool ok) {
             int i = 0, j = 0, k = 0, q;
             if constexpr (needs_ijk<OP>()) ijk(p, c, i, j, k);
             T y = pass_in<T, N, OP>(kind, qf, sc, p, ok ? in[gi] : T(0), i, j, k, q);
             lds[c * LS + PR(q)] = ok ? y : T(0);
-        });
+        }, e0);
     }
     __syncthreads();
-    r2r_line<T, N, TPL, true>(lds + l * LS, u, kind, ct);
+    r2r_line<T, N, TPL, true>(lds + l * LS, u + e0, kind, ct);
     __syncthreads();
-    store_rows<T, N, NL, LS>(g, out, lds, OutMap<T, N>{kind}, T(1));
+    store_rows<T, N, NL, LS>(g, out, lds, OutMap<T, N>{kind}, T(1), e0);
 }
 
 // Strided axes (0 = x, 1 = y), one output per op in OPS; an output may
@@ -741,6 +747,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_force_z(const T* __restr
     __syncthreads();
     const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
     auto component = [&](const T* __restrict__ g, T* __restrict__ out, int kind) {
+        const int e0 = opaque_zero();  // 384 points: 149 -> 90 VGPRs
         InMap<T, N> im{kind};
         zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
             const V16<T> a = *reinterpret_cast<const V16<T>*>(psi + gi);
@@ -754,9 +761,9 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_force_z(const T* __restr
                 T sg = im(p + t, q);
                 lds[c * LS + PR(q)] = sg * T(mu * (double)h.v[t]);
             }
-        });
+        }, e0);
         lsync<true>();
-        r2r_line<T, N, TPL, true>(lds + l * LS, u, kind, ct);
+        r2r_line<T, N, TPL, true>(lds + l * LS, u + e0, kind, ct);
         OutMap<T, N> om{kind};
         zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
             V16<T> v;
@@ -767,7 +774,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_force_z(const T* __restr
                 v.v[t] = sg * lds[c * LS + PR(q)];
             }
             *reinterpret_cast<V16<T>*>(out + gi) = v;
-        });
+        }, e0);
         lsync<true>();
     };
     component(gx, sx, REDFT10);
@@ -800,48 +807,96 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_stokes_z(const T* __rest
     const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
     const PlainCoef& cf = ac.pc;
     const long sx = (long)d.ny * d.nz, sy = d.nz;
-    // comp 0: ux, 1: uy, 2: uz; value of element k of line (i, j) at global gi
-    auto vel = [&](int comp, int i, int j, int k, long gi) -> double {
-        const double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
+    const double cs = cf.scale / ac.nu;
+    using VT = V16<T>;
+    auto vld = [](const T* __restrict__ ptr) { return *reinterpret_cast<const VT*>(ptr); };
+    VT zero;
+#pragma unroll
+    for (int t = 0; t < W; ++t) zero.v[t] = T(0);
+    // comp 0: ux, 1: uy, 2: uz; the W elements k = p .. p+W-1 of line (i, j)
+    // from global gi.  Own and same-k neighbours are 16-byte loads; the k-1
+    // and k+1 neighbours are the loaded group shifted by one element plus
+    // one scalar read across the group edge.
+    auto vel = [&](int comp, int i, int j, int p, long gi, double* v) {
         if (comp == 0) {
-            if (i >= d.nx - 1) return 0.0;
-            const double qsx = (i + 1) * cf.qfx;
-            const double syv = j != 0 ? (double)Sy[gi + sx - sy] : 0.0;
-            const double szv = k != 0 ? (double)Sz[gi + sx - 1] : 0.0;
-            const double mq2 = qsx * qsx + qy * qy + qz * qz;
-            const double dot = qsx * (double)Sx[gi] + qy * syv + qz * szv;
-            return cf.scale / (ac.nu * mq2) * ((double)Sx[gi] - qsx / mq2 * dot);
+            if (i >= d.nx - 1) {
+#pragma unroll
+                for (int t = 0; t < W; ++t) v[t] = 0.0;
+                return;
+            }
+            const VT a = vld(Sx + gi);
+            const VT b = j != 0 ? vld(Sy + gi + sx - sy) : zero;
+            const VT cz = vld(Sz + gi + sx);
+            const T cprev = p != 0 ? Sz[gi + sx - 1] : T(0);
+            const double qsx = (i + 1) * cf.qfx, qy = j * cf.qfy;
+            const double base = qsx * qsx + qy * qy;
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                const double qz = (p + t) * cf.qfz;
+                const double inv = 1.0 / (base + qz * qz);
+                const double szv = (double)(t == 0 ? cprev : cz.v[t - 1]);
+                const double dot = qsx * (double)a.v[t] + qy * (double)b.v[t] + qz * szv;
+                v[t] = cs * inv * ((double)a.v[t] - qsx * inv * dot);
+            }
+            return;
         }
-        if (j >= d.ny - 1 || k >= d.nz - 1) return 0.0;
+        if (j >= d.ny - 1) {
+#pragma unroll
+            for (int t = 0; t < W; ++t) v[t] = 0.0;
+            return;
+        }
         if (comp == 1) {
-            const double qsy = (j + 1) * cf.qfy;
-            const double sxv = i != 0 ? (double)Sx[gi - sx + sy] : 0.0;
-            const double szv = k != 0 ? (double)Sz[gi + sy - 1] : 0.0;
-            const double mq2 = qx * qx + qsy * qsy + qz * qz;
-            const double dot = qx * sxv + qsy * (double)Sy[gi] + qz * szv;
-            return cf.scale / (ac.nu * mq2) * ((double)Sy[gi] - qsy / mq2 * dot);
+            const VT a = vld(Sy + gi);
+            const VT b = i != 0 ? vld(Sx + gi - sx + sy) : zero;
+            const VT cz = vld(Sz + gi + sy);
+            const T cprev = p != 0 ? Sz[gi + sy - 1] : T(0);
+            const double qx = i * cf.qfx, qsy = (j + 1) * cf.qfy;
+            const double base = qx * qx + qsy * qsy;
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                const double qz = (p + t) * cf.qfz;
+                const double inv = 1.0 / (base + qz * qz);
+                const double szv = (double)(t == 0 ? cprev : cz.v[t - 1]);
+                const double dot = qx * (double)b.v[t] + qsy * (double)a.v[t] + qz * szv;
+                v[t] = p + t < d.nz - 1 ? cs * inv * ((double)a.v[t] - qsy * inv * dot) : 0.0;
+            }
+            return;
         }
-        const double qsz = (k + 1) * cf.qfz;
-        const double sxv = i != 0 ? (double)Sx[gi - sx + 1] : 0.0;
-        const double syv = j != 0 ? (double)Sy[gi - sy + 1] : 0.0;
-        const double mq2 = qx * qx + qy * qy + qsz * qsz;
-        const double dot = qx * sxv + qy * syv + qsz * (double)Sz[gi];
-        return cf.scale / (ac.nu * mq2) * ((double)Sz[gi] - qsz / mq2 * dot);
+        const VT a = vld(Sz + gi);
+        const bool more = p + W < N;
+        const VT bx = i != 0 ? vld(Sx + gi - sx) : zero;
+        const T bnext = i != 0 && more ? Sx[gi - sx + W] : T(0);
+        const VT cy = j != 0 ? vld(Sy + gi - sy) : zero;
+        const T cnext = j != 0 && more ? Sy[gi - sy + W] : T(0);
+        const double qx = i * cf.qfx, qy = j * cf.qfy;
+        const double base = qx * qx + qy * qy;
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            const double qsz = (p + t + 1) * cf.qfz;
+            const double inv = 1.0 / (base + qsz * qsz);
+            const double sxv = (double)(t < W - 1 ? bx.v[t + 1] : bnext);
+            const double syv = (double)(t < W - 1 ? cy.v[t + 1] : cnext);
+            const double dot = qx * sxv + qy * syv + qsz * (double)a.v[t];
+            v[t] = p + t < d.nz - 1 ? cs * inv * ((double)a.v[t] - qsz * inv * dot) : 0.0;
+        }
     };
     auto component = [&](int comp, T* __restrict__ out, int kind) {
+        const int e0 = opaque_zero();  // 384 points: 149 -> 90 VGPRs
         InMap<T, N> im{kind};
         zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
             const long L = gi / N;
             const int i = (int)(L / d.ny), j = (int)(L - (long)i * d.ny);
+            double v[W];
+            vel(comp, i, j, p, gi, v);
 #pragma unroll
             for (int t = 0; t < W; ++t) {
                 int q;
                 T sg = im(p + t, q);
-                lds[c * LS + PR(q)] = sg * T(vel(comp, i, j, p + t, gi + t));
+                lds[c * LS + PR(q)] = sg * T(v[t]);
             }
-        });
+        }, e0);
         lsync<true>();
-        r2r_line<T, N, TPL, true>(lds + l * LS, u, kind, ct);
+        r2r_line<T, N, TPL, true>(lds + l * LS, u + e0, kind, ct);
         OutMap<T, N> om{kind};
         zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
             V16<T> v;
@@ -852,7 +907,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_stokes_z(const T* __rest
                 v.v[t] = sg * lds[c * LS + PR(q)];
             }
             *reinterpret_cast<V16<T>*>(out + gi) = v;
-        });
+        }, e0);
         lsync<true>();
     };
     component(0, ux, REDFT01);
@@ -1459,9 +1514,10 @@ __global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__
     }
     block_accumulate<2>(sums, acc);
     __syncthreads();
-    dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    const int e0 = opaque_zero_if<N>();
+    dct3_line<T, N, TPL, true>(lds + l * LS, u + e0, ct);
     __syncthreads();
-    store_rows<T, N, NL, LS>(g, w, lds, Mk{}, T(1));
+    store_rows<T, N, NL, LS>(g, w, lds, Mk{}, T(1), e0);
 }
 
 // z pass: r = REDFT01_z(w); g = (3 beta psi^2 - 5 gamma psi^4) r with psi^n

@@ -126,6 +126,32 @@ __device__ __forceinline__ void load_cosq(T* dst, const Tw<T>& tw)
     for (int j = threadIdx.x; j <= N; j += blockDim.x) dst[j] = tw.cosq[j];
 }
 
+// A zero the compiler cannot see through (and a compiler-level memory
+// barrier).  A kernel that runs several transforms one after another adds it
+// to the lane and line indices of each: the index and twiddle arithmetic of
+// the first transform is otherwise kept live in registers across the later
+// ones, which costs occupancy (three back-to-back 256-point FP32 z
+// transforms: 134 VGPRs with the arithmetic shared, 75 without).
+__device__ __forceinline__ int opaque_zero()
+{
+    int z;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z) : : "memory");
+    return z;
+}
+
+// opaque_zero() for 256-point lines, 0 otherwise, for kernels whose register
+// count stays under the LDS cap either way at other sizes.  At 256 points the
+// LDS tile leaves room for 8 waves/SIMD and the registers decide the
+// occupancy; at 384 points the LDS caps it at 4 and recomputing the index
+// arithmetic is measurably slower (quasi step 384^3: 1.5-2 %); below 256
+// points the lines are short and the counts mixed.
+template <int N>
+__device__ __forceinline__ int opaque_zero_if()
+{
+    if constexpr (N == 256) return opaque_zero();
+    else return 0;
+}
+
 // Synchronisation of the threads that share a line.  Lines never straddle a
 // wavefront (TPL divides 64), so inside the transforms a wavefront-scope
 // barrier with LDS fences suffices; WAVE = false falls back to the
