@@ -1550,6 +1550,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restr
     });
     lsync<true>();
     double sums[2] = {0.0, 0.0};
+    int e0 = 0;
     const double b3 = 3.0 * cf.beta, g5 = 5.0 * cf.gamma;
     if constexpr (FUSED) {
         // r = REDFT01_z(w) in registers, g = (3 beta psi^2 - 5 gamma psi^4) r
@@ -1584,14 +1585,15 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restr
             }
         });
         lsync<true>();
-        dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
+        e0 = opaque_zero_if<N>();
+        dct2_line<T, N, TPL, true>(lds + l * LS, u + e0, ct);
     }
     zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v;
 #pragma unroll
         for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + PR(p + t)];
         *reinterpret_cast<V16<T>*>(w + gi) = v;
-    });
+    }, e0);
     block_accumulate<2, WGZ>(sums, acc);
 }
 
