@@ -235,10 +235,10 @@ struct OutMap {
 // ------------------------------------------------------------------------
 // Generic single-axis r2r pass: out = scale * R2R_kind(in) along `axis`.
 // in may equal out.
-template <typename T, int N>
-__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_axis_z(const T* in, T* out, Dims d, int kind, T scale,
-                                               Tw<T> tw)
+template <typename T, int N, int KIND>
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_axis_z(const T* in, T* out, Dims d, T scale, Tw<T> tw)
 {
+    constexpr int kind = KIND;
     constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     __shared__ T lds[NL * LS];
@@ -273,10 +273,13 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_axis_z(const T* in, T* out, 
     }
 }
 
-template <typename T, int N>
-__global__ __launch_bounds__(WG) void k_axis(const T* in, T* out, Dims d, int axis, int kind,
-                                             T scale, Tw<T> tw)
+// The kind is a template parameter: a kernel that carries both transform
+// directions holds the registers of the larger (FP64 256-point y pass: 74
+// VGPRs for both kinds, 64 for REDFT10 alone, 72 for REDFT01 alone).
+template <typename T, int N, int KIND>
+__global__ __launch_bounds__(WG) void k_axis(const T* in, T* out, Dims d, int axis, T scale, Tw<T> tw)
 {
+    constexpr int kind = KIND;
     constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
     __shared__ T lds[NL * LS];
     __shared__ T ct[N + 1];

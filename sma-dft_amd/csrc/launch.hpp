@@ -27,19 +27,44 @@ namespace smad {
 #define SMAD_DISPATCH_Z(n, KERNEL, GRID, ...) \
     SMAD_DISPATCH_B(n, KERNEL, GRID, dim3(LinesZ<T>::WGZ), __VA_ARGS__)
 
+#define SMAD_UNPAREN(...) __VA_ARGS__
+#define SMAD_DISPATCH_OPS(n, KERNEL, OPS, GRID, BLOCK, ...)                                              \
+    switch (n) {                                                                                        \
+    case 16: hipLaunchKernelGGL((KERNEL<T, 16, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
+    case 32: hipLaunchKernelGGL((KERNEL<T, 32, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
+    case 64: hipLaunchKernelGGL((KERNEL<T, 64, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
+    case 128: hipLaunchKernelGGL((KERNEL<T, 128, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 256: hipLaunchKernelGGL((KERNEL<T, 256, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 384: hipLaunchKernelGGL((KERNEL<T, 384, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 512: hipLaunchKernelGGL((KERNEL<T, 512, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 1024: hipLaunchKernelGGL((KERNEL<T, 1024, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    default: break;                                                                                     \
+    }
+
 inline int axis_len(const Dims& d, int axis) { return axis == 0 ? d.nx : axis == 1 ? d.ny : d.nz; }
 
 template <typename T>
 void Launch<T>::axis(hipStream_t s, const T* in, T* out, Dims d, int axis, int kind, T scale,
                      const Tw<T>& tw)
 {
-    if (axis == 2) {
-        dim3 grid(num_tiles(d, 2, linesz_nl<T>(d.nz)));
-        SMAD_DISPATCH_Z(d.nz, k_axis_z, grid, in, out, d, kind, scale, tw);
-    } else {
-        dim3 grid(num_tiles(d, axis, Lines<T>::NL));
-        SMAD_DISPATCH(axis_len(d, axis), k_axis, grid, in, out, d, axis, kind, scale, tw);
+    // one kernel instance per kind
+#define SMAD_AXIS_KIND(K)                                                                                    \
+    case K:                                                                                                  \
+        if (axis == 2) {                                                                                     \
+            SMAD_DISPATCH_OPS(d.nz, k_axis_z, (K), gz, dim3(LinesZ<T>::WGZ), in, out, d, scale, tw);          \
+        } else {                                                                                             \
+            SMAD_DISPATCH_OPS(axis_len(d, axis), k_axis, (K), gs, dim3(WG), in, out, d, axis, scale, tw);     \
+        }                                                                                                    \
+        break;
+    const dim3 gz(num_tiles(d, 2, linesz_nl<T>(d.nz))), gs(num_tiles(d, axis, Lines<T>::NL));
+    switch (kind) {
+        SMAD_AXIS_KIND(REDFT10)
+        SMAD_AXIS_KIND(REDFT01)
+        SMAD_AXIS_KIND(RODFT10)
+        SMAD_AXIS_KIND(RODFT01)
+    default: break;
     }
+#undef SMAD_AXIS_KIND
 }
 
 template <typename T>
@@ -71,20 +96,6 @@ inline dim3 flat_grid(long n)
     long b = (n + WG - 1) / WG;
     return dim3((unsigned)std::min(b, 8192L));
 }
-
-#define SMAD_UNPAREN(...) __VA_ARGS__
-#define SMAD_DISPATCH_OPS(n, KERNEL, OPS, GRID, BLOCK, ...)                                              \
-    switch (n) {                                                                                        \
-    case 16: hipLaunchKernelGGL((KERNEL<T, 16, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
-    case 32: hipLaunchKernelGGL((KERNEL<T, 32, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
-    case 64: hipLaunchKernelGGL((KERNEL<T, 64, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
-    case 128: hipLaunchKernelGGL((KERNEL<T, 128, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
-    case 256: hipLaunchKernelGGL((KERNEL<T, 256, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
-    case 384: hipLaunchKernelGGL((KERNEL<T, 384, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
-    case 512: hipLaunchKernelGGL((KERNEL<T, 512, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
-    case 1024: hipLaunchKernelGGL((KERNEL<T, 1024, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
-    default: break;                                                                                     \
-    }
 
 template <typename T>
 void Launch<T>::zpass(hipStream_t s, int zs, const T* in, const PassSet<T>& ps, Dims d, const SpecCoef& sc,
