@@ -537,6 +537,15 @@ enum PassOp : int {
     OP_NMQ2 = 7,        // -s |q|^2 x                              (quasi laplacian of rho f)
 };
 
+// Every multi-output pass runs an inverse transform: RODFT01 after a DST
+// shift, REDFT01 otherwise.  The kind is fixed by the op at compile time (a
+// kernel holding both directions carries the registers of the larger); the
+// launchers check the kinds the caller passes against it.  With the kinds
+// known, the outputs' transforms would share their twiddle arithmetic in
+// registers (FP64 384-point z pass: 158 VGPRs), so each output's transform
+// starts from an opaque_zero() at every size (102).
+constexpr int op_kind(int op) { return op == OP_SHIFT || op == OP_FILT_SHIFT ? RODFT01 : REDFT01; }
+
 // Op sets of the z passes with several outputs.
 enum ZSet : int {
     ZS_ADV = 0,       // (OP_ID, OP_SHIFT, OP_ALIN): psi, grad_z psi, Sp2 of the advective step
@@ -615,13 +624,14 @@ constexpr bool needs_ijk()
 
 // One output of a z pass: load with op OP, transform, store.
 template <typename T, int N, int OP>
-__device__ __forceinline__ void zmulti_one(const T* in, T* out, int kind, const Dims& d, const SpecCoef& sc,
+__device__ __forceinline__ void zmulti_one(const T* in, T* out, const Dims& d, const SpecCoef& sc,
                                            T* lds, const T* ct, int l, int u)
 {
+    constexpr int kind = op_kind(OP);
     constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
-    const int e0 = opaque_zero_if<N>();
+    const int e0 = opaque_zero();
     zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
         int i = 0, j = 0;
@@ -666,15 +676,16 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_zmulti(const T* in, PassSet<
     __syncthreads();
     const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
     int o = 0;
-    ((zmulti_one<T, N, OPS>(in, ps.out[o], ps.kind[o], d, sc, lds, ct, l, u), ++o), ...);
+    ((zmulti_one<T, N, OPS>(in, ps.out[o], d, sc, lds, ct, l, u), ++o), ...);
 }
 
 // One output of a strided pass.
 template <typename T, int N, int OP>
-__device__ __forceinline__ void multi_one(const T* in, T* out, int kind, int axis, const TileGeo& g, int outer,
+__device__ __forceinline__ void multi_one(const T* in, T* out, int axis, const TileGeo& g, int outer,
                                           int k0, double qf, const SpecCoef& sc, T* lds, const T* ct, int l,
                                           int u)
 {
+    constexpr int kind = op_kind(OP);
     constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
     constexpr int W = V16<T>::W;
     auto ijk = [&](int p, int c, int& i, int& j, int& k) {
@@ -683,7 +694,7 @@ __device__ __forceinline__ void multi_one(const T* in, T* out, int kind, int axi
         else { i = outer; j = p; }
     };
     __syncthreads();
-    const int e0 = opaque_zero_if<N>();
+    const int e0 = opaque_zero();
     if (g.ncol == NL) {
         rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
             V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
@@ -725,7 +736,7 @@ __global__ __launch_bounds__(WG) void k_multi(const T* in, PassSet<T> ps, Dims d
     const int outer = blockIdx.x / nct, k0 = (blockIdx.x % nct) * NL;
     const double qf = axis == 0 ? sc.pc.qfx : sc.pc.qfy;
     int o = 0;
-    ((multi_one<T, N, OPS>(in, ps.out[o], ps.kind[o], axis, g, outer, k0, qf, sc, lds, ct, l, u), ++o), ...);
+    ((multi_one<T, N, OPS>(in, ps.out[o], axis, g, outer, k0, qf, sc, lds, ct, l, u), ++o), ...);
 }
 
 // REDFT10_z of Nr = beta psi^3 - gamma psi^5 - u . grad psi (:1290-1307),

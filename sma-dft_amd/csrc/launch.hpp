@@ -109,6 +109,8 @@ void Launch<T>::zpass(hipStream_t s, int zs, const T* in, const PassSet<T>& ps, 
     if (zs < 0 || zs > ZS_NMQ2) throw std::logic_error("zpass: unknown op set");
     for (int o = 0; o < 4; ++o)
         if ((o < ps.n ? ps.op[o] : -1) != ops[zs][o]) throw std::logic_error("zpass: outputs do not match the op set");
+    for (int o = 0; o < ps.n; ++o)
+        if (ps.kind[o] != op_kind(ps.op[o])) throw std::logic_error("zpass: output kind does not match its op");
     dim3 grid(num_tiles(d, 2, linesz_nl<T>(d.nz))), block(LinesZ<T>::WGZ);
     switch (zs) {
     case ZS_ADV: SMAD_DISPATCH_OPS(d.nz, k_zmulti, (OP_ID, OP_SHIFT, OP_ALIN), grid, block, in, ps, d, sc, tw); break;
@@ -127,6 +129,8 @@ void Launch<T>::spass(hipStream_t s, const T* in, const PassSet<T>& ps, Dims d, 
 {
     if (ps.n != 2 || ps.op[0] != OP_SHIFT || ps.op[1] != OP_ID)
         throw std::logic_error("spass: outputs must be (OP_SHIFT, OP_ID)");
+    if (ps.kind[0] != op_kind(OP_SHIFT) || ps.kind[1] != op_kind(OP_ID))
+        throw std::logic_error("spass: output kinds must be (RODFT01, REDFT01)");
     dim3 grid(num_tiles(d, axis, Lines<T>::NL)), block(WG);
     SMAD_DISPATCH_OPS(axis_len(d, axis), k_multi, (OP_SHIFT, OP_ID), grid, block, in, ps, d, axis, sc, tw);
 }
