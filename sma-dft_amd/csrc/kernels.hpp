@@ -1543,9 +1543,10 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restr
     constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
-    // FP64 only: in FP32 the FP64 multiply of all packed values of a thread
-    // in flight costs more registers (169 vs 78) than the LDS pass it saves
-    constexpr bool FUSED = OpFused<N, TPL>::value && sizeof(T) == 8 && N <= 512;  // two tiles must fit LDS
+    // In FP32 the weight and the gradient terms are formed in FP32 (each term
+    // is converted to FP64 before it is summed): forming them in FP64 kept
+    // the FP64 copies of all the thread's values in flight (169 VGPRs).
+    constexpr bool FUSED = OpFused<N, TPL>::value && N <= 512;  // two tiles must fit LDS
     __shared__ T lds[NL * LS];
     __shared__ T tl[FUSED ? NL * LS : 1];  // psi tape at Makhoul positions
     __shared__ T ct[N + 1];
@@ -1572,6 +1573,20 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restr
         const cpx<T>* tz = reinterpret_cast<const cpx<T>*>(tl + l * LS);
         auto op = [&](cpx<T>& v, int m) {
             const cpx<T> x = tz[PC(m)];
+            if constexpr (sizeof(T) == 4) {
+                const float b3f = float(b3), g5f = float(g5);
+                float r = v.re, x2 = x.re * x.re, x3 = x2 * x.re;
+                sums[0] += (double)(r * x3);
+                sums[1] -= (double)(r * x3 * x2);
+                v.re = (b3f * x2 - g5f * x2 * x2) * r;
+                r = v.im;
+                x2 = x.im * x.im;
+                x3 = x2 * x.im;
+                sums[0] += (double)(r * x3);
+                sums[1] -= (double)(r * x3 * x2);
+                v.im = (b3f * x2 - g5f * x2 * x2) * r;
+                return;
+            }
             double r = (double)v.re, xv = (double)x.re, x2 = xv * xv;
             sums[0] += r * x2 * xv;
             sums[1] -= r * x2 * x2 * xv;
