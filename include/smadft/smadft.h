@@ -138,6 +138,18 @@ int smad_reduce(smad_ctx *ctx, int what, double *out);
  * crossing get 0 (the reference leaves stale buffer contents there). */
 int smad_surface(smad_ctx *ctx, double *surf, double *vel, double *curv_h, double *curv_k);
 
+/* The same diagnostics with the |psi|-envelope column scan of
+ * no-adv/cosNoAdvTrackBump.cpp:929-1031 (replaces that program's save-block
+ * surface computation): from the first |psi| > 0.6 below the top of a column
+ * (at k), walk up the flat index to the first |psi| <= 0.1 (k2); when |psi|
+ * there exceeds |psi| at the next point, surf = k2 + 1 and the derivatives
+ * are taken at that next point, else surf = k2 and they are taken at the
+ * |psi| > 0.6 point.  The z-difference edge rule follows k, the y difference
+ * of grad_x psi is taken at k2, as the reference.  A walk longer than nz
+ * points, or one that would leave the grid, reports no surface (-1 and
+ * zeros; the reference walks on through memory). */
+int smad_surface_envelope(smad_ctx *ctx, double *surf, double *vel, double *curv_h, double *curv_k);
+
 /* Wait for all work on the context stream. */
 int smad_sync(smad_ctx *ctx);
 

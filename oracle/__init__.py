@@ -57,7 +57,9 @@ def lib():
         L.orc_plain_adjoint.argtypes = [pp, dp, ctypes.c_int, dp, dp, dp, dp]
         L.orc_surface_aux.argtypes = [pp, dp, dp, dp, dp, dp]
         L.orc_surface_track.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [dp] * 9
-        for f in ("orc_surface_aux", "orc_surface_track", "orc_plain_adjoint", "orc_r2r1d_naive",
+        L.orc_surface_track_env.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [dp] * 9
+        for f in ("orc_surface_aux", "orc_surface_track", "orc_surface_track_env", "orc_plain_adjoint",
+                  "orc_r2r1d_naive",
                   "orc_r2r1d", "orc_r2r3d", "orc_plain_preloop",
                   "orc_plain_step", "orc_l1_sums", "orc_adv_preloop", "orc_adv_step"):
             getattr(L, f).restype = None
@@ -196,17 +198,19 @@ def surface_aux(p, psi):
     return out
 
 
-def surface_track(psi, dxx, dyy, dzz, dtp, h):
-    """Column scan + velocity + curvatures with grid spacing h (scalar or 3-tuple)."""
+def surface_track(psi, dxx, dyy, dzz, dtp, h, envelope=False):
+    """Column scan + velocity + curvatures with grid spacing h (scalar or
+    3-tuple); envelope=True takes the |psi|-envelope scan."""
     hx, hy, hz = (h, h, h) if np.isscalar(h) else h
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (psi, dxx, dyy, dzz, dtp)]
     nx, ny, nz = arrs[0].shape
     out = [np.empty((nx, ny)) for _ in range(4)]
-    lib().orc_surface_track(nx, ny, nz, hx, hy, hz, *[_p(a) for a in arrs], *[_p(a) for a in out])
+    f = lib().orc_surface_track_env if envelope else lib().orc_surface_track
+    f(nx, ny, nz, hx, hy, hz, *[_p(a) for a in arrs], *[_p(a) for a in out])
     return dict(zip(("surf", "vel", "curv_h", "curv_k"), out))
 
 
-def surface(p, psi):
+def surface(p, psi, envelope=False):
     """The plain model's surface diagnostics of psi (spacing 2 pi/(q0 nw))."""
     dxx, dyy, dzz, dtp = surface_aux(p, psi)
-    return surface_track(psi, dxx, dyy, dzz, dtp, 2 * np.pi / (p.q0 * p.nw))
+    return surface_track(psi, dxx, dyy, dzz, dtp, 2 * np.pi / (p.q0 * p.nw), envelope)

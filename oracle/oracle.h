@@ -52,12 +52,17 @@ void orc_l1_sums(const double *a_new, const double *b_old, long n, double *out);
  * orc_surface_aux: real-space psi_xx, psi_yy, psi_zz and dTpsi of psi from
  *   its spectrum (the save block's iPlanPsiD** / iPlanDTpsi).
  * orc_surface_track: the column scan, normal velocity and curvatures; outputs
- *   nx*ny values each (index i*ny + j), surf = -1 where no crossing. */
+ *   nx*ny values each (index i*ny + j), surf = -1 where no crossing.
+ * orc_surface_track_env: the same with the |psi|-envelope column scan of
+ *   no-adv/cosNoAdvTrackBump.cpp:929-959. */
 void orc_surface_aux(const orc_params *p, const double *psi, double *dxx, double *dyy, double *dzz,
                      double *dt);
 void orc_surface_track(int nx, int ny, int nz, double dx, double dy, double dz, const double *psi,
                        const double *dxx, const double *dyy, const double *dzz, const double *dtp,
                        double *surf, double *vel, double *curv_h, double *curv_k);
+void orc_surface_track_env(int nx, int ny, int nz, double dx, double dy, double dz, const double *psi,
+                           const double *dxx, const double *dyy, const double *dzz, const double *dtp,
+                           double *surf, double *vel, double *curv_h, double *curv_k);
 
 /* ------------------------------------------------------- advective model -- */
 /* model = 4 selects the conserved advective variant (adv/cosAdvConSingle.cpp):

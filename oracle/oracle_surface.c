@@ -3,7 +3,8 @@
  *
  * Surface diagnostics of the plain model, restated in the reference's order
  * (no-adv/cosNoAdvTrackMult.cpp:787-796 for the auxiliary spectral fields,
- * :853-1022 for the gradients, the column scan and the curvatures): the
+ * :853-1022 for the gradients, the column scan and the curvatures;
+ * no-adv/cosNoAdvTrackBump.cpp:929-1031 for the |psi|-envelope scan): the
  * finite-difference gradients are built as whole 3D arrays first, as the
  * reference does, and the scan reads them.
  *
@@ -15,7 +16,12 @@
  *     finite differences, analytic second derivatives fed in); a tilted plane
  *     gives 0 and 0; vel = dTpsi/|grad psi| against the analytic gradient;
  *   - the scan: arming at psi > 0.7, the crossing at psi < 0, the k + 1 rule
- *     and columns without a crossing on hand-built columns.
+ *     and columns without a crossing on hand-built columns;
+ *   - the |psi|-envelope scan (no-adv/cosNoAdvTrackBump.cpp:929-1031): the
+ *     surface index against an independent vectorised formulation on
+ *     random columns, hand-built columns for the swap rule and the point the
+ *     derivatives are taken at, and (curv_h/2)^2 = curv_k on spherical
+ *     layers (every level set is a sphere).
  * A column without a crossing reports surf = -1 and zeros; the reference
  * leaves stale buffer contents there.
  */
@@ -70,15 +76,53 @@ static double fd(const double *f, long id, int c, int nc, long stride, double th
     return 2 * (f[id] - f[id - stride]) / th;
 }
 
-void orc_surface_track(int nx, int ny, int nz, double dx, double dy, double dz, const double *psi,
-                       const double *dxx, const double *dyy, const double *dzz, const double *dtp,
-                       double *surf, double *vel, double *curv_h, double *curv_k)
+/* value of a whole-grid array at a flat index, 0 outside the grid (the
+ * reference's flat reads at the top of a column run on into the next one) */
+static double at_flat(const double *f, long id, long n) { return id >= 0 && id < n ? f[id] : 0.0; }
+
+/* normal velocity and curvatures at flat index `index` of column c (row j):
+ * the edge rule of the z differences follows k, the y difference of gx is
+ * taken at k2 (cosNoAdvTrackMult.cpp:960-1018, cosNoAdvTrackBump.cpp:963-1028) */
+static void diagnostics(long c, int j, int k, int k2, long index, int ny, int nz, long n, const double *gx,
+                        const double *gy, const double *gz, const double *dxx, const double *dyy,
+                        const double *dzz, const double *dtp, double tdy, double tdz, double *vel,
+                        double *curv_h, double *curv_k)
+{
+    const long sy = nz;
+    double X = at_flat(gx, index, n), Y = at_flat(gy, index, n), Z = at_flat(gz, index, n);
+    double g = sqrt(X * X + Y * Y + Z * Z);
+    vel[c] = dtp[index] / g;
+    double pxy, pxz, pyz;
+    long at = c * nz + k2;
+    if (j > 0 && j < ny - 1) pxy = (at_flat(gx, at + sy, n) - at_flat(gx, at - sy, n)) / tdy;
+    else if (j == 0) pxy = 2 * (at_flat(gx, at + sy, n) - at_flat(gx, at, n)) / tdy;
+    else pxy = 2 * (at_flat(gx, at, n) - at_flat(gx, at - sy, n)) / tdy;
+    if (k > 0 && k < nz - 1) {
+        pxz = (at_flat(gx, index + 1, n) - at_flat(gx, index - 1, n)) / tdz;
+        pyz = (at_flat(gy, index + 1, n) - at_flat(gy, index - 1, n)) / tdz;
+    } else if (k == 0) {
+        pxz = 2 * (at_flat(gx, index + 1, n) - X) / tdz;
+        pyz = 2 * (at_flat(gy, index + 1, n) - Y) / tdz;
+    } else {
+        pxz = 2 * (X - at_flat(gx, index - 1, n)) / tdz;
+        pyz = 2 * (Y - at_flat(gy, index - 1, n)) / tdz;
+    }
+    double A = dxx[index], B = dyy[index], C = dzz[index];
+    curv_h[c] = ((Y * Y + Z * Z) * A + (X * X + Z * Z) * B + (X * X + Y * Y) * C -
+                 2 * (X * Y * pxy + X * Z * pxz + Y * Z * pyz)) /
+                pow(g, 3);
+    curv_k[c] = (Z * Z * (A * B - pxy * pxy) + X * X * (B * C - pyz * pyz) + Y * Y * (A * C - pxz * pxz) +
+                 2 * (Y * pxy * (Z * pxz - X * C) + X * pxz * (Y * pyz - Z * B) + Z * pyz * (X * pxy - Y * A))) /
+                pow(g, 4);
+}
+
+static void track(int envelope, int nx, int ny, int nz, double dx, double dy, double dz, const double *psi,
+                  const double *dxx, const double *dyy, const double *dzz, const double *dtp, double *surf,
+                  double *vel, double *curv_h, double *curv_k)
 {
     const long n = (long)nx * ny * nz, sx = (long)ny * nz, sy = nz;
     const double tdx = 2 * dx, tdy = 2 * dy, tdz = 2 * dz;
-    /* gradient arrays, one guard value past the end for the flat index + 1 read */
-    double *gx = calloc(n + 1, sizeof(double)), *gy = calloc(n + 1, sizeof(double)),
-           *gz = calloc(n + 1, sizeof(double));
+    double *gx = malloc(sizeof(double) * n), *gy = malloc(sizeof(double) * n), *gz = malloc(sizeof(double) * n);
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -95,46 +139,63 @@ void orc_surface_track(int nx, int ny, int nz, double dx, double dy, double dz, 
             int track = 0;
             for (int k = nz - 1; k > -1; --k) {
                 long index = c * nz + k;
-                if (psi[index] > 0.7 && track == 0) track = 1;
-                if (psi[index] < 0.0 && track == 1) {
-                    int k2 = k;
-                    if (fabs(psi[index]) > fabs(psi[index + 1])) {
-                        index = index + 1;
-                        k2 = k + 1;
+                if (!envelope) {
+                    /* cosNoAdvTrackMult.cpp:930-958 */
+                    if (psi[index] > 0.7 && track == 0) track = 1;
+                    if (psi[index] < 0.0 && track == 1) {
+                        int k2 = k;
+                        if (fabs(psi[index]) > fabs(psi[index + 1])) {
+                            index = index + 1;
+                            k2 = k + 1;
+                        }
+                        surf[c] = k2;
+                        diagnostics(c, j, k, k2, index, ny, nz, n, gx, gy, gz, dxx, dyy, dzz, dtp, tdy, tdz, vel,
+                                    curv_h, curv_k);
+                        break;
                     }
+                } else if (fabs(psi[index]) > 0.6) {
+                    /* cosNoAdvTrackBump.cpp:929-959: from the first |psi| > 0.6
+                     * below the top, walk up (flat index) to |psi| <= 0.1; the
+                     * next point replaces it when smaller in magnitude, and the
+                     * derivatives are then taken there, else at the |psi| > 0.6
+                     * point.  A walk longer than nz points or one that would
+                     * leave the grid reports no surface (the reference walks on
+                     * through memory). */
+                    long i3 = index;
+                    int k2 = k, steps = 0;
+                    while (fabs(at_flat(psi, i3, n)) > 0.1 && steps <= nz) {
+                        ++i3;
+                        ++k2;
+                        ++steps;
+                    }
+                    if (steps > nz) break;
+                    if (fabs(at_flat(psi, i3, n)) > fabs(at_flat(psi, i3 + 1, n))) {
+                        index = i3 + 1;
+                        ++k2;
+                    }
+                    if (index >= n) break;
                     surf[c] = k2;
-                    double g = sqrt(gx[index] * gx[index] + gy[index] * gy[index] + gz[index] * gz[index]);
-                    vel[c] = dtp[index] / g;
-                    double pxy, pxz, pyz;
-                    long at = c * nz + k2;
-                    if (j > 0 && j < ny - 1) pxy = (gx[at + sy] - gx[at - sy]) / tdy;
-                    else if (j == 0) pxy = 2 * (gx[at + sy] - gx[at]) / tdy;
-                    else pxy = 2 * (gx[at] - gx[at - sy]) / tdy;
-                    if (k > 0 && k < nz - 1) {
-                        pxz = (gx[index + 1] - gx[index - 1]) / tdz;
-                        pyz = (gy[index + 1] - gy[index - 1]) / tdz;
-                    } else if (k == 0) {
-                        pxz = 2 * (gx[index + 1] - gx[index]) / tdz;
-                        pyz = 2 * (gy[index + 1] - gy[index]) / tdz;
-                    } else {
-                        pxz = 2 * (gx[index] - gx[index - 1]) / tdz;
-                        pyz = 2 * (gy[index] - gy[index - 1]) / tdz;
-                    }
-                    double X = gx[index], Y = gy[index], Z = gz[index];
-                    double A = dxx[index], B = dyy[index], C = dzz[index];
-                    curv_h[c] = ((Y * Y + Z * Z) * A + (X * X + Z * Z) * B + (X * X + Y * Y) * C -
-                                 2 * (X * Y * pxy + X * Z * pxz + Y * Z * pyz)) /
-                                pow(g, 3);
-                    curv_k[c] = (Z * Z * (A * B - pxy * pxy) + X * X * (B * C - pyz * pyz) +
-                                 Y * Y * (A * C - pxz * pxz) +
-                                 2 * (Y * pxy * (Z * pxz - X * C) + X * pxz * (Y * pyz - Z * B) +
-                                      Z * pyz * (X * pxy - Y * A))) /
-                                pow(g, 4);
-                    track = 2;
+                    diagnostics(c, j, k, k2, index, ny, nz, n, gx, gy, gz, dxx, dyy, dzz, dtp, tdy, tdz, vel,
+                                curv_h, curv_k);
+                    break;
                 }
             }
         }
     free(gx);
     free(gy);
     free(gz);
+}
+
+void orc_surface_track(int nx, int ny, int nz, double dx, double dy, double dz, const double *psi,
+                       const double *dxx, const double *dyy, const double *dzz, const double *dtp,
+                       double *surf, double *vel, double *curv_h, double *curv_k)
+{
+    track(0, nx, ny, nz, dx, dy, dz, psi, dxx, dyy, dzz, dtp, surf, vel, curv_h, curv_k);
+}
+
+void orc_surface_track_env(int nx, int ny, int nz, double dx, double dy, double dz, const double *psi,
+                           const double *dxx, const double *dyy, const double *dzz, const double *dtp,
+                           double *surf, double *vel, double *curv_h, double *curv_k)
+{
+    track(1, nx, ny, nz, dx, dy, dz, psi, dxx, dyy, dzz, dtp, surf, vel, curv_h, curv_k);
 }

@@ -39,6 +39,7 @@ EXPORTS = (
     "smad_last_error", "smad_ic_focal_conic", "smad_ic_slab", "smad_ic_curved",
     "smad_ic_add_noise", "smad_default_params", "smad_r2r_3d", "smad_time_steps",
     "smad_profile_steps", "smad_kernel_class_name", "smad_forward_tape", "smad_adjoint", "smad_adjoint_timing", "smad_set_tape_budget", "smad_surface",
+    "smad_surface_envelope",
 )
 
 REDFT10, REDFT01, RODFT10, RODFT01 = 0, 1, 2, 3
@@ -134,6 +135,7 @@ def lib() -> ctypes.CDLL:
         L.smad_adjoint.argtypes = [vp, dp, dp, dp]
         L.smad_set_tape_budget.argtypes = [vp, ctypes.c_double]
         L.smad_surface.argtypes = [vp, dp, dp, dp, dp]
+        L.smad_surface_envelope.argtypes = [vp, dp, dp, dp, dp]
         L.smad_adjoint_timing.argtypes = [vp, dp, dp]
         L.smad_profile_steps.argtypes = [vp, ctypes.c_int, dp, P(ctypes.c_int), ctypes.c_int,
                                          P(ctypes.c_int)]
@@ -253,12 +255,16 @@ class Stepper:
         ckpt_every steps for a following :meth:`adjoint`."""
         self._check(lib().smad_forward_tape(self._h, int(nsteps), int(ckpt_every)))
 
-    def surface(self):
+    def surface(self, envelope: bool = False):
         """Interface tracker of the current psi (plain model): dict of
-        (nx, ny) arrays surf (crossing k, -1 if none), vel, curv_h, curv_k."""
+        (nx, ny) arrays surf (surface k, -1 if none), vel, curv_h, curv_k.
+        envelope=False: the crossing scan (psi > 0.7 then psi < 0);
+        envelope=True: the |psi|-envelope scan (|psi| > 0.6, walk up to
+        |psi| <= 0.1)."""
         p = self.params
         out = [np.empty((p.nx, p.ny)) for _ in range(4)]
-        self._check(lib().smad_surface(self._h, *[_dptr(a) for a in out]))
+        f = lib().smad_surface_envelope if envelope else lib().smad_surface
+        self._check(f(self._h, *[_dptr(a) for a in out]))
         return dict(zip(("surf", "vel", "curv_h", "curv_k"), out))
 
     def set_tape_budget(self, gib: float):

@@ -1234,6 +1234,8 @@ __global__ __launch_bounds__(WG) void k_surf_spec(const T* __restrict__ ph, cons
 
 struct SurfCoef {
     double tdx, tdy, tdz;  // twice the grid spacings (the central-difference denominators)
+    int envelope;          // 0: crossing scan (no-adv/cosNoAdvTrackMult.cpp:930-958),
+                           // 1: |psi| envelope (no-adv/cosNoAdvTrackBump.cpp:929-959)
 };
 
 // Central difference along a stride with the reference's one-sided edges:
@@ -1261,9 +1263,22 @@ __device__ __forceinline__ double grad_at(const T* __restrict__ psi, long idx, i
     return fd_at(psi, idx, k, d.nz, 1L, sc.tdz);
 }
 
-// One thread per (i, j) column.  Scanning k from Nz-1 down: the first
-// psi > 0.7 arms the tracker, the first psi < 0 after it is the crossing k;
-// k2 = k + 1 when |psi(k)| > |psi(k+1)|, else k.  At (i, j, k2):
+// psi at a flat index, 0 outside the grid
+template <typename T>
+__device__ __forceinline__ double psi_flat(const T* __restrict__ psi, long idx, const Dims& d)
+{
+    return idx >= 0 && idx < d.total() ? (double)psi[idx] : 0.0;
+}
+
+// One thread per (i, j) column.  Crossing scan: scanning k from Nz-1 down,
+// the first psi > 0.7 arms the tracker, the first psi < 0 after it is the
+// crossing k; k2 = k + 1 when |psi(k)| > |psi(k+1)|, else k; the point is
+// (i, j, k2).  Envelope scan: from the first |psi| > 0.6 below the top (at k),
+// walk up the flat index to the first |psi| <= 0.1 (k2); when |psi| there is
+// larger than at the next point, k2 + 1 and the next point, else k2 and the
+// |psi| > 0.6 point at k; a walk longer than Nz points or one that leaves the
+// grid reports no surface (the reference walks on through memory).  At the
+// point:
 //   surf = k2, vel = dTpsi / |grad psi|,
 //   curv_h = [(gy^2+gz^2) dxx + (gx^2+gz^2) dyy + (gx^2+gy^2) dzz
 //             - 2 (gx gy dxy + gx gz dxz + gy gz dyz)] / |g|^3          (= 2H)
@@ -1272,8 +1287,8 @@ __device__ __forceinline__ double grad_at(const T* __restrict__ psi, long idx, i
 //                  + gz dyz (gx dxy - gy dxx))] / |g|^4
 // with g the finite-difference gradient, dxx.. the spectral second
 // derivatives, dxy the y-difference of gx at k2 and dxz, dyz the flat
-// index +-1 differences of gx, gy around k2 with the edge rule of k (both as
-// the reference).  Columns without a crossing get surf = -1 and zeros.
+// index +-1 differences of gx, gy around the point with the edge rule of k
+// (both as the reference).  Columns without a surface get surf = -1 and zeros.
 template <typename T>
 __global__ __launch_bounds__(WG) void k_surf_track(const T* __restrict__ psi, const T* __restrict__ dxx,
                                                    const T* __restrict__ dyy, const T* __restrict__ dzz,
@@ -1286,53 +1301,80 @@ __global__ __launch_bounds__(WG) void k_surf_track(const T* __restrict__ psi, co
     const int j = (int)(col % d.ny);
     const long base = col * d.nz;
     double o_s = -1.0, o_v = 0.0, o_h = 0.0, o_k = 0.0;
-    int armed = 0;
-    for (int k = d.nz - 1; k >= 0; --k) {
-        const double v = (double)psi[base + k];
-        if (v > 0.7 && !armed) armed = 1;
-        if (v < 0.0 && armed) {
-            long idx = base + k;
-            int k2 = k;
-            if (fabs(v) > fabs((double)psi[idx + 1])) {
-                idx += 1;
-                k2 = k + 1;
+    long idx = -1;  // the point the derivatives are taken at
+    int k = -1, k2 = -1;
+    if (!sc.envelope) {
+        int armed = 0;
+        for (int kk = d.nz - 1; kk >= 0; --kk) {
+            const double v = (double)psi[base + kk];
+            if (v > 0.7 && !armed) armed = 1;
+            if (v < 0.0 && armed) {
+                k = k2 = kk;
+                idx = base + kk;
+                if (fabs(v) > fabs((double)psi[idx + 1])) {
+                    idx += 1;
+                    k2 = kk + 1;
+                }
+                break;
             }
-            const double gx = grad_at(psi, idx, 0, d, sc), gy = grad_at(psi, idx, 1, d, sc),
-                         gz = grad_at(psi, idx, 2, d, sc);
-            const double g = sqrt(gx * gx + gy * gy + gz * gz);
-            const long ck2 = base + k2;
-            const long sy = d.nz;
-            double dxy;
-            if (j > 0 && j < d.ny - 1)
-                dxy = (grad_at(psi, ck2 + sy, 0, d, sc) - grad_at(psi, ck2 - sy, 0, d, sc)) / sc.tdy;
-            else if (j == 0)
-                dxy = 2.0 * (grad_at(psi, ck2 + sy, 0, d, sc) - grad_at(psi, ck2, 0, d, sc)) / sc.tdy;
-            else
-                dxy = 2.0 * (grad_at(psi, ck2, 0, d, sc) - grad_at(psi, ck2 - sy, 0, d, sc)) / sc.tdy;
-            double dxz, dyz;
-            if (k > 0 && k < d.nz - 1) {
-                dxz = (grad_at(psi, idx + 1, 0, d, sc) - grad_at(psi, idx - 1, 0, d, sc)) / sc.tdz;
-                dyz = (grad_at(psi, idx + 1, 1, d, sc) - grad_at(psi, idx - 1, 1, d, sc)) / sc.tdz;
-            } else if (k == 0) {
-                dxz = 2.0 * (grad_at(psi, idx + 1, 0, d, sc) - gx) / sc.tdz;
-                dyz = 2.0 * (grad_at(psi, idx + 1, 1, d, sc) - gy) / sc.tdz;
-            } else {
-                dxz = 2.0 * (gx - grad_at(psi, idx - 1, 0, d, sc)) / sc.tdz;
-                dyz = 2.0 * (gy - grad_at(psi, idx - 1, 1, d, sc)) / sc.tdz;
-            }
-            const double a = (double)dxx[idx], b = (double)dyy[idx], c = (double)dzz[idx];
-            const double g2 = g * g;
-            o_s = k2;
-            o_v = (double)dtp[idx] / g;
-            o_h = ((gy * gy + gz * gz) * a + (gx * gx + gz * gz) * b + (gx * gx + gy * gy) * c -
-                   2.0 * (gx * gy * dxy + gx * gz * dxz + gy * gz * dyz)) /
-                  (g2 * g);
-            o_k = (gz * gz * (a * b - dxy * dxy) + gx * gx * (b * c - dyz * dyz) + gy * gy * (a * c - dxz * dxz) +
-                   2.0 * (gy * dxy * (gz * dxz - gx * c) + gx * dxz * (gy * dyz - gz * b) +
-                          gz * dyz * (gx * dxy - gy * a))) /
-                  (g2 * g2);
-            break;
         }
+    } else {
+        for (int kk = d.nz - 1; kk >= 0; --kk) {
+            if (fabs((double)psi[base + kk]) > 0.6) {
+                long i3 = base + kk;
+                int steps = 0;
+                while (fabs(psi_flat(psi, i3, d)) > 0.1 && steps <= d.nz) {
+                    ++i3;
+                    ++steps;
+                }
+                if (steps > d.nz) break;
+                k = kk;
+                k2 = kk + steps;
+                idx = base + kk;
+                if (fabs(psi_flat(psi, i3, d)) > fabs(psi_flat(psi, i3 + 1, d))) {
+                    idx = i3 + 1;
+                    ++k2;
+                }
+                if (idx >= d.total()) idx = -1;
+                break;
+            }
+        }
+    }
+    if (idx >= 0) {
+        const double gx = grad_at(psi, idx, 0, d, sc), gy = grad_at(psi, idx, 1, d, sc),
+                     gz = grad_at(psi, idx, 2, d, sc);
+        const double g = sqrt(gx * gx + gy * gy + gz * gz);
+        const long ck2 = base + k2;
+        const long sy = d.nz;
+        double dxy;
+        if (j > 0 && j < d.ny - 1)
+            dxy = (grad_at(psi, ck2 + sy, 0, d, sc) - grad_at(psi, ck2 - sy, 0, d, sc)) / sc.tdy;
+        else if (j == 0)
+            dxy = 2.0 * (grad_at(psi, ck2 + sy, 0, d, sc) - grad_at(psi, ck2, 0, d, sc)) / sc.tdy;
+        else
+            dxy = 2.0 * (grad_at(psi, ck2, 0, d, sc) - grad_at(psi, ck2 - sy, 0, d, sc)) / sc.tdy;
+        double dxz, dyz;
+        if (k > 0 && k < d.nz - 1) {
+            dxz = (grad_at(psi, idx + 1, 0, d, sc) - grad_at(psi, idx - 1, 0, d, sc)) / sc.tdz;
+            dyz = (grad_at(psi, idx + 1, 1, d, sc) - grad_at(psi, idx - 1, 1, d, sc)) / sc.tdz;
+        } else if (k == 0) {
+            dxz = 2.0 * (grad_at(psi, idx + 1, 0, d, sc) - gx) / sc.tdz;
+            dyz = 2.0 * (grad_at(psi, idx + 1, 1, d, sc) - gy) / sc.tdz;
+        } else {
+            dxz = 2.0 * (gx - grad_at(psi, idx - 1, 0, d, sc)) / sc.tdz;
+            dyz = 2.0 * (gy - grad_at(psi, idx - 1, 1, d, sc)) / sc.tdz;
+        }
+        const double a = (double)dxx[idx], b = (double)dyy[idx], c = (double)dzz[idx];
+        const double g2 = g * g;
+        o_s = k2;
+        o_v = (double)dtp[idx] / g;
+        o_h = ((gy * gy + gz * gz) * a + (gx * gx + gz * gz) * b + (gx * gx + gy * gy) * c -
+               2.0 * (gx * gy * dxy + gx * gz * dxz + gy * gz * dyz)) /
+              (g2 * g);
+        o_k = (gz * gz * (a * b - dxy * dxy) + gx * gx * (b * c - dyz * dyz) + gy * gy * (a * c - dxz * dxz) +
+               2.0 * (gy * dxy * (gz * dxz - gx * c) + gx * dxz * (gy * dyz - gz * b) +
+                      gz * dyz * (gx * dxy - gy * a))) /
+              (g2 * g2);
     }
     surf[col] = o_s;
     vel[col] = o_v;

@@ -158,7 +158,7 @@ struct EngineBase {
     virtual int reduce(int what, double* out) = 0;
     virtual void forward_tape(int K, int ck) = 0;
     virtual void set_tape_budget(double) {}
-    virtual void surface(double*, double*, double*, double*)
+    virtual void surface(double*, double*, double*, double*, int)
     {
         throw ApiError{SMAD_EINVAL, "surface diagnostics are defined for the plain model"};
     }
@@ -507,11 +507,12 @@ struct PlainEngine : EngineBase {
 
     // ---------------------------------------------------- surface --
     // Interface tracker, normal velocity and curvatures of the current psi
-    // (no-adv/cosNoAdvTrackMult.cpp:787-796, 853-1022).
+    // (no-adv/cosNoAdvTrackMult.cpp:787-796, 853-1022; with envelope, the
+    // |psi|-envelope scan of no-adv/cosNoAdvTrackBump.cpp:929-1031).
     DevBuf<T> sf_dxx, sf_dyy, sf_dzz, sf_dt, sf_nq;
     DevBuf<double> sf_out;
 
-    void surface(double* surf, double* vel, double* ch, double* ck) override
+    void surface(double* surf, double* vel, double* ch, double* ck, int envelope) override
     {
         if (cf.conserved) throw ApiError{SMAD_EINVAL, "surface diagnostics are defined for the plain model"};
         const size_t cols = (size_t)p.nx * p.ny;
@@ -532,7 +533,7 @@ struct PlainEngine : EngineBase {
         }
         const double pi = M_PI;
         const double dx = 2.0 * pi / (p.q0 * p.nw);  // L/N on every axis
-        SurfCoef sc{2.0 * dx, 2.0 * dx, 2.0 * dx};
+        SurfCoef sc{2.0 * dx, 2.0 * dx, 2.0 * dx, envelope ? 1 : 0};
         Launch<T>::surf_track(s, psi.p, sf_dxx.p, sf_dyy.p, sf_dzz.p, sf_dt.p, d, sc, sf_out.p, sf_out.p + cols,
                               sf_out.p + 2 * cols, sf_out.p + 3 * cols);
         LAUNCHCHK();
@@ -1258,7 +1259,16 @@ int smad_surface(smad_ctx* c, double* surf, double* vel, double* curv_h, double*
 {
     return guarded(c, [&] {
         if (!surf || !vel || !curv_h || !curv_k) throw ApiError{SMAD_EINVAL, "null output"};
-        c->eng->surface(surf, vel, curv_h, curv_k);
+        c->eng->surface(surf, vel, curv_h, curv_k, 0);
+        return SMAD_OK;
+    });
+}
+
+int smad_surface_envelope(smad_ctx* c, double* surf, double* vel, double* curv_h, double* curv_k)
+{
+    return guarded(c, [&] {
+        if (!surf || !vel || !curv_h || !curv_k) throw ApiError{SMAD_EINVAL, "null output"};
+        c->eng->surface(surf, vel, curv_h, curv_k, 1);
         return SMAD_OK;
     });
 }

@@ -28,7 +28,8 @@
 // surface diagnostics to surfPsi.dat, velSurf.dat, curvH.dat and curvK.dat
 // (cosNoAdvTrackMult.cpp:1049-1138); --track-every B writes the surface files
 // every B blocks instead (the tracking cadence of the 256^3 relaxation, every
-// 100 steps).  At the end psi.dat and psi.bin hold the final state, and the
+// 100 steps); --tracker envelope takes the |psi|-envelope column scan of
+// cosNoAdvTrackBump.cpp:929-1031 instead of the crossing scan.  At the end psi.dat and psi.bin hold the final state, and the
 // wall time and steps/s of the loop are printed.
 // A non-finite field ends the run with exit code 3 instead of the
 // reference's silent loop exit.
@@ -58,6 +59,7 @@ struct Opts {
     int l1_every = 0;        // 0 = model default
     int save_every = 10;     // L1 blocks between field saves; 0 = never
     int track_every = -1;    // L1 blocks between surface outputs; -1 = with the saves
+    int envelope = 0;        // --tracker envelope: the |psi|-envelope column scan
     long max_steps = 1000000;
     std::string ic = "";     // focal | slab | curved | file
     double amp = 0, noise = 1e-3, ae = -1;
@@ -77,7 +79,8 @@ struct Opts {
                  "       smadft_run conserved <ep_arg> <aE> <N> [opts]\n"
                  "       smadft_run advcons <nu> <ep_arg> <aE> <N> <load> [opts]\n"
                  "opts: --n N | --nx/--ny/--nz N, --fp32, --steps K, --l1 LIM, --l1-every S,\n"
-                 "      --save-every B, --track-every B, --max-steps K, --ic focal|slab|curved, --amp A,\n"
+                 "      --save-every B, --track-every B, --tracker crossing|envelope, --max-steps K,\n"
+                 "      --ic focal|slab|curved, --amp A,\n"
                  "      --noise X, --seed S, --ae A, --eps E, --dt DT, --out DIR,\n"
                  "      --restart psi.bin\n";
     std::exit(2);
@@ -152,6 +155,12 @@ Opts parse(int argc, char** argv)
         else if (a == "--l1-every") o.l1_every = (int)num(val(), "--l1-every");
         else if (a == "--save-every") o.save_every = (int)num(val(), "--save-every");
         else if (a == "--track-every") o.track_every = (int)num(val(), "--track-every");
+        else if (a == "--tracker") {
+            const std::string t = val();
+            if (t == "crossing") o.envelope = 0;
+            else if (t == "envelope") o.envelope = 1;
+            else usage("--tracker must be crossing or envelope");
+        }
         else if (a == "--max-steps") o.max_steps = (long)num(val(), "--max-steps");
         else if (a == "--ic") o.ic = val();
         else if (a == "--amp") o.amp = num(val(), "--amp");
@@ -304,7 +313,8 @@ int main(int argc, char** argv)
     };
     // the surface files of the plain model (cosNoAdvTrackMult.cpp:1062-1090)
     auto track = [&]() -> int {
-        if (smad_surface(ctx, sf.data(), sf.data() + cols, sf.data() + 2 * cols, sf.data() + 3 * cols) != SMAD_OK)
+        auto surface = o.envelope ? smad_surface_envelope : smad_surface;
+        if (surface(ctx, sf.data(), sf.data() + cols, sf.data() + 2 * cols, sf.data() + 3 * cols) != SMAD_OK)
             return fail(ctx, "surface");
         if (!append_all(dir + "surfPsi.dat", sf.data(), cols, "\n") ||
             !append_all(dir + "velSurf.dat", sf.data() + cols, cols, "\n ") ||

@@ -23,7 +23,7 @@ def run(args, **kw):
 
 @pytest.mark.parametrize("args", [[], ["nosuch"], ["adv", "1", "0.7"], ["quasi", "1"],
                                   ["plain", "--bogus"], ["plain", "--n"], ["plain", "--steps", "x"],
-                                  ["plain", "--steps", "0"]])
+                                  ["plain", "--steps", "0"], ["plain", "--tracker", "top"]])
 def test_usage_errors(args):
     r = run(args)
     assert r.returncode == 2
@@ -91,13 +91,14 @@ def test_plain_restart_and_l1_stop(smadft, tmp_path):
 
 
 @pytest.mark.gpu
-def test_plain_save_block_surface_files(smadft, tmp_path):
+@pytest.mark.parametrize("tracker", ["crossing", "envelope"])
+def test_plain_save_block_surface_files(smadft, tmp_path, tracker):
     """Save blocks write psiMid.dat and the surface files of the state one
     step before the block's end, as the reference's save block does."""
     out = tmp_path / "run"
     n = 32
     r = run(["plain", "--n", str(n), "--steps", "40", "--l1-every", "10", "--save-every", "2",
-             "--ic", "curved", "--amp", "1.328", "--seed", "4", "--out", str(out)])
+             "--ic", "curved", "--amp", "1.328", "--seed", "4", "--tracker", tracker, "--out", str(out)])
     assert r.returncode == 0, r.stderr
     files = {k: np.loadtxt(out / f"{k}.dat").reshape(2, -1)
              for k in ("surfPsi", "velSurf", "curvH", "curvK", "psiMid")}
@@ -106,7 +107,7 @@ def test_plain_save_block_surface_files(smadft, tmp_path):
     with smadft.Stepper(p, psi0) as s:
         for blk, nsteps in enumerate((19, 20)):
             s.step(nsteps)
-            sf = s.surface()
+            sf = s.surface(envelope=tracker == "envelope")
             psi = s.fetch()
             assert np.array_equal(files["surfPsi"][blk], sf["surf"].ravel())
             assert (sf["surf"] >= 0).any()

@@ -1,7 +1,8 @@
 """Pins the oracle's surface diagnostics (no-adv/cosNoAdvTrackMult.cpp:787-796,
-853-1022) against closed forms: single cosine modes for the auxiliary
-spectral fields, spherical and planar level sets for the curvatures, and
-hand-built columns for the scan."""
+853-1022; the |psi|-envelope scan of no-adv/cosNoAdvTrackBump.cpp:929-1031)
+against closed forms: single cosine modes for the auxiliary spectral fields,
+spherical and planar level sets for the curvatures, hand-built columns for
+the scans and an independent vectorised formulation of the envelope index."""
 import numpy as np
 import pytest
 
@@ -124,3 +125,122 @@ def test_scan_rules(oracle):
     assert out["surf"][1, 2] == 6
     for key in ("vel", "curv_h", "curv_k"):
         assert out[key][0, 2] == 0 and out[key][1, 1] == 0
+
+
+# ---------------------------------------------------------------------------
+# |psi|-envelope scan (no-adv/cosNoAdvTrackBump.cpp:929-1031): the first
+# |psi| > 0.6 below the top, a walk up the flat index to |psi| <= 0.1, the
+# next point taken when smaller in magnitude; the derivatives are taken at the
+# |psi| > 0.6 point unless that swap happened.
+
+
+def columns_grid(profile, nxy=3):
+    """A grid whose columns all equal `profile`: the x and y differences vanish
+    and |grad psi| = |d psi/dz|."""
+    return np.broadcast_to(np.asarray(profile, float), (nxy, nxy, len(profile))).copy()
+
+
+def dz_at(profile, k, h):
+    """The z difference at k with the one-sided top and bottom rule."""
+    f, n = np.asarray(profile, float), len(profile)
+    if 0 < k < n - 1:
+        return (f[k + 1] - f[k - 1]) / (2 * h)
+    return (f[1] - f[0]) / h if k == 0 else (f[-1] - f[-2]) / h
+
+
+@pytest.mark.parametrize("profile,surf,k_at", [
+    # walk 12 -> 14 (0.05 <= 0.1); |0.05| > |0.02|: the next point, derivatives there
+    ({12: 0.9, 13: 0.3, 14: 0.05, 15: 0.02}, 15, 15),
+    # no swap (|0.05| < |0.08|): surface at 14, derivatives at the 0.6 point 12
+    ({12: 0.9, 13: 0.3, 14: 0.05, 15: 0.08}, 14, 12),
+    # negative values count by magnitude; the first from the top wins
+    ({4: 0.95, 10: -0.7, 11: -0.2, 12: 0.05, 13: 0.0}, 13, 13),
+    # |psi| > 0.1 up to the top: the walk runs into the next column's k = 0
+    # (flat index), where psi = 0 stops it; no swap (0 > |psi(k=1)| is false)
+    ({1: 0.04, 12: 0.9, 13: 0.5, 14: 0.5, 15: 0.5}, 16, 12),
+])
+def test_envelope_scan_rules(oracle, profile, surf, k_at):
+    nz, h = 16, 0.5
+    prof = np.zeros(nz)
+    for k, v in profile.items():
+        prof[k] = v
+    psi = columns_grid(prof)
+    dtp = np.broadcast_to(np.arange(nz, dtype=float) + 1, psi.shape).copy()  # marks the point used
+    z = np.zeros_like(psi)
+    out = oracle.surface_track(psi, z, z, z, dtp, h, envelope=True)
+    assert np.all(out["surf"] == surf)
+    g = abs(dz_at(prof, k_at, h))
+    assert np.allclose(out["vel"], (k_at + 1) / g, rtol=1e-14)
+
+
+def test_envelope_without_large_values(oracle):
+    psi = columns_grid(np.full(12, 0.55))
+    z = np.zeros_like(psi)
+    out = oracle.surface_track(psi, z, z, z, z, 1.0, envelope=True)
+    assert np.all(out["surf"] == -1)
+    assert not out["vel"].any() and not out["curv_h"].any() and not out["curv_k"].any()
+
+
+def test_envelope_surface_index_vectorised(oracle):
+    """The surface index against an independent formulation: numpy masks over
+    the flat |psi| array instead of the per-column loop."""
+    rng = np.random.default_rng(7)
+    nx, ny, nz = 6, 5, 24
+    # smooth columns: a random envelope height, layers below it, noise above
+    top = rng.integers(6, 20, size=(nx, ny))
+    k = np.arange(nz)
+    psi = np.where(k[None, None, :] <= top[..., None],
+                   np.cos(0.9 * (k[None, None, :] - top[..., None])) * rng.uniform(0.7, 1.0, (nx, ny, 1)),
+                   rng.uniform(-0.3, 0.3, (nx, ny, nz)) * (rng.random((nx, ny, nz)) < 0.5))
+    z = np.zeros_like(psi)
+    out = oracle.surface_track(psi, z, z, z, np.ones_like(psi), 1.0, envelope=True)
+    a = np.abs(psi).ravel()
+    n = a.size
+    expect = np.full((nx, ny), -1.0)
+    for i in range(nx):
+        for j in range(ny):
+            c = i * ny + j
+            big = np.nonzero(a[c * nz:(c + 1) * nz] > 0.6)[0]
+            if big.size == 0:
+                continue
+            p0 = c * nz + big.max()
+            tail = np.append(a[p0:], 0.0)  # zero past the end of the grid
+            p = p0 + int(np.argmax(tail <= 0.1))
+            if p - p0 > nz:
+                continue
+            nxt = a[p + 1] if p + 1 < n else 0.0
+            cur = a[p] if p < n else 0.0
+            expect[i, j] = (p + (1 if cur > nxt else 0)) - c * nz
+    assert (expect >= 0).sum() > 20
+    assert np.array_equal(out["surf"], expect)
+
+
+def test_envelope_curvatures_of_spherical_level_sets(oracle):
+    """psi = (1 - tanh((r - R)/w))/2: every level set is a sphere, so at any
+    point with a nonzero gradient (curv_h/2)^2 = curv_k, 2H = -2/r for this
+    inward-increasing field, and the point lies between the 0.1 and 0.6
+    levels."""
+    n, R, w, c = 64, 18.0, 4.0, (31.3, 30.6, 12.2)
+    x, y, zz = np.meshgrid(*[np.arange(n, dtype=float)] * 3, indexing="ij")
+    X, Y, Z = x - c[0], y - c[1], zz - c[2]
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    t = np.tanh((r - R) / w)
+    psi = 0.5 * (1 - t)
+    f1 = -0.5 * (1 - t ** 2) / w
+    f2 = t * (1 - t ** 2) / w ** 2
+    d2 = [f2 * (A / r) ** 2 + f1 * (1 / r - A ** 2 / r ** 3) for A in (X, Y, Z)]
+    out = oracle.surface_track(psi, *d2, np.ones_like(psi), 1.0, envelope=True)
+    ii, jj = np.nonzero(out["surf"] >= 0)
+    k2 = out["surf"][ii, jj].astype(int)
+    inner = (X[ii, jj, 0] ** 2 + Y[ii, jj, 0] ** 2) < (0.6 * R) ** 2
+    assert inner.sum() > 300
+    # the surface index sits just above the 0.1 level
+    assert np.all(psi[ii, jj, k2][inner] <= 0.1 + 1e-12)
+    h = out["curv_h"][ii, jj][inner]
+    kk = out["curv_k"][ii, jj][inner]
+    assert np.all(h < 0)
+    assert np.max(np.abs((h / 2) ** 2 / kk - 1)) < 3e-2
+    # 1/r between the radii of the 0.6 level and two grid steps past the 0.1
+    # level (the walk's end and the swap to the next point)
+    rmin, rmax = R + w * np.arctanh(1 - 2 * 0.6) - 1.0, R + w * np.arctanh(1 - 2 * 0.1) + 2.5
+    assert np.all(-h / 2 > 1 / rmax) and np.all(-h / 2 < 1 / rmin)
