@@ -17,7 +17,8 @@ def adv_params(smadft, shape, prec=64, **kw):
     return p
 
 
-@pytest.mark.parametrize("shape,nsteps", [((32, 32, 32), 40), ((16, 32, 64), 25)])
+@pytest.mark.parametrize("shape,nsteps", [((32, 32, 32), 40), ((16, 32, 64), 25), ((384, 16, 32), 20),
+                                          ((16, 384, 32), 20), ((32, 16, 384), 20)])
 def test_adv_fp64_matches_oracle(smadft, oracle, shape, nsteps):
     p = adv_params(smadft, shape)
     psi0 = smadft.ic_curved(p, 1.328, 1e-3, seed=2)
@@ -35,8 +36,9 @@ def test_adv_fp64_matches_oracle(smadft, oracle, shape, nsteps):
     assert np.abs(o.vel).max() > 1e-8  # the flow is not trivially zero
 
 
-def test_adv_fp32_mixed_error(smadft, oracle):
-    p = adv_params(smadft, (32, 32, 32), prec=32)
+@pytest.mark.parametrize("shape", [(32, 32, 32), (32, 16, 384)])
+def test_adv_fp32_mixed_error(smadft, oracle, shape):
+    p = adv_params(smadft, shape, prec=32)
     psi0 = smadft.ic_curved(p, 1.328, 1e-3, seed=2)
     o = oracle.AdvOracle(p, psi0)
     o.step(40)

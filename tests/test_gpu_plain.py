@@ -19,7 +19,8 @@ def params(smadft, shape, prec=64, **kw):
 
 
 @pytest.mark.parametrize("shape,nsteps", [((32, 32, 32), 100), ((16, 32, 64), 60), ((64, 64, 64), 400),
-                                          ((1024, 16, 32), 30), ((32, 16, 1024), 30), ((16, 16, 512), 30)])
+                                          ((1024, 16, 32), 30), ((32, 16, 1024), 30), ((16, 16, 512), 30),
+                                          ((384, 16, 32), 30), ((32, 384, 16), 30), ((16, 32, 384), 30)])
 def test_plain_fp64_matches_oracle(smadft, oracle, shape, nsteps):
     p = params(smadft, shape)
     psi0 = smadft.ic_slab(p, 0.6, 1e-3, seed=0)
@@ -44,7 +45,7 @@ def test_plain_fp64_1000_steps_32(smadft, oracle):
     assert rel_l2(psi, o.psi) < 1e-10
 
 
-@pytest.mark.parametrize("shape", [(1024, 16, 32), (16, 32, 1024)])
+@pytest.mark.parametrize("shape", [(1024, 16, 32), (16, 32, 1024), (384, 16, 32), (16, 32, 384)])
 def test_plain_fp32_mixed_1024_lines(smadft, oracle, shape):
     p = params(smadft, shape, prec=32)
     psi0 = smadft.ic_slab(p, 1.328, 1e-3, seed=0)

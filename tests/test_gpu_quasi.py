@@ -17,7 +17,8 @@ def qparams(smadft, shape, prec=64, **kw):
     return p
 
 
-@pytest.mark.parametrize("shape,nsteps,switch", [((32, 32, 32), 14, 5), ((16, 32, 64), 9, 3)])
+@pytest.mark.parametrize("shape,nsteps,switch", [((32, 32, 32), 14, 5), ((16, 32, 64), 9, 3),
+                                                 ((384, 16, 32), 6, 2), ((32, 16, 384), 6, 2)])
 def test_quasi_fp64_matches_oracle(smadft, oracle, shape, nsteps, switch):
     p = qparams(smadft, shape, div_switch=switch)
     psi0 = smadft.ic_curved(p, 1.34164, 1e-3, seed=3)
@@ -44,8 +45,9 @@ def test_quasi_fp64_matches_oracle(smadft, oracle, shape, nsteps, switch):
     assert energy == pytest.approx(q.sums[3], rel=1e-9)
 
 
-def test_quasi_fp32_mixed_error(smadft, oracle):
-    p = qparams(smadft, (32, 32, 32), prec=32, div_switch=5)
+@pytest.mark.parametrize("shape", [(32, 32, 32), (32, 16, 384)])
+def test_quasi_fp32_mixed_error(smadft, oracle, shape):
+    p = qparams(smadft, shape, prec=32, div_switch=5)
     psi0 = smadft.ic_curved(p, 1.34164, 1e-3, seed=3)
     q = oracle.QuasiOracle(p, psi0)
     q.step(14)
