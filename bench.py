@@ -60,8 +60,8 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-ckpt", type=int, default=20)
     ap.add_argument("--adj-reps", type=int, default=3)
-    ap.add_argument("--adj-tape-gib", type=float, default=96.0,
-                    help="HBM budget for the hbm_tape adjoint leg")
+    ap.add_argument("--adj-tape-gib", type=float, default=-1.0,
+                    help="HBM budget (GiB) for the hbm_tape adjoint leg; < 0: the library default")
     ap.add_argument("--model-steps", type=int, default=10)
     ap.add_argument("--adv-grid", type=int, default=256)
     ap.add_argument("--adv-acc", type=int, default=200)
@@ -154,13 +154,16 @@ def main():
 
     # adjoint / forward time ratio: K steps with checkpoints every ckpt,
     # (taped forward + adjoint) over the plain forward of the same K steps.
-    # "strict": only the last segment is taped, every other one is
-    # recomputed from its checkpoint; "hbm_tape": segments are taped from the
-    # end while --adj-tape-gib of HBM allows (all of them at 256^3, K=200)
+    # "hbm_tape": the library's default tape budget (a quarter of the free
+    # HBM, or --adj-tape-gib when given), which tapes every segment at
+    # 256^3, K = 200; "strict": budget 0, only the last segment is taped and
+    # every other one is recomputed from its checkpoint
     adj = None
     if args.adj_steps > 0:
         adj = {"K": args.adj_steps, "ckpt_every": args.adj_ckpt,
-               "timing": f"best of {args.adj_reps} runs per leg", "tape_gib": args.adj_tape_gib}
+               "timing": f"best of {args.adj_reps} runs per leg",
+               "tape_budget": ("library default (a quarter of the free HBM)" if args.adj_tape_gib < 0
+                               else f"{args.adj_tape_gib} GiB")}
         grads = {}
         for prec in precs:
             p = smadft.Params.default(smadft.MODEL_PLAIN, n)
@@ -175,8 +178,10 @@ def main():
                     s.set_field(smadft.FIELD_PSI, ic)
                     fwd_ms = min(fwd_ms, s.time_steps(args.adj_steps))
                 entry = {"forward_ms": fwd_ms}
-                for mode, gib in (("strict", 0.0), ("hbm_tape", args.adj_tape_gib)):
-                    s.set_tape_budget(gib)
+                # the default-budget leg first: a context keeps the last budget set
+                for mode, gib in (("hbm_tape", args.adj_tape_gib), ("strict", 0.0)):
+                    if gib >= 0:
+                        s.set_tape_budget(gib)
                     tape_ms = adj_ms = float("inf")
                     for _ in range(args.adj_reps):
                         s.set_field(smadft.FIELD_PSI, ic)
@@ -329,8 +334,8 @@ def main():
             "accuracy_after_steps": args.acc_steps,
             "accuracy": acc,
         },
-        "adjoint_fwd_ratio": adj["fp64"]["strict"]["ratio"] if adj and "fp64" in adj else None,
-        "adjoint_fwd_ratio_hbm_tape": adj["fp64"]["hbm_tape"]["ratio"] if adj and "fp64" in adj else None,
+        "adjoint_fwd_ratio": adj["fp64"]["hbm_tape"]["ratio"] if adj and "fp64" in adj else None,
+        "adjoint_fwd_ratio_strict": adj["fp64"]["strict"]["ratio"] if adj and "fp64" in adj else None,
         "adjoint": adj,
         "models": models,
         "large_grids": large or None,

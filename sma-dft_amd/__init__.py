@@ -269,7 +269,8 @@ class Stepper:
 
     def set_tape_budget(self, gib: float):
         """HBM (GiB) the next forward_tape may spend on tape beyond one
-        segment; 0 = strict checkpointing."""
+        segment; 0 = strict checkpointing.  Without a call: a quarter of
+        the free device memory."""
         self._check(lib().smad_set_tape_budget(self._h, float(gib)))
 
     def adjoint(self, dl_dpsik: np.ndarray):

@@ -353,7 +353,10 @@ struct PlainEngine : EngineBase {
         std::vector<std::unique_ptr<DevBuf<T>>> psi, ph;  // ph[m-1] holds slot m
     };
     int tape_K = 0, tape_ck = 0, tape_nseg = 0, tape_first = 0;  // segments >= tape_first are taped
-    double tape_budget = 0.0;                                    // bytes beyond one segment
+    // bytes of tape beyond one segment; < 0 (the default): a quarter of the
+    // device memory that is free or already held by tapes when forward_tape
+    // runs, i.e. on a 288 GB GPU every segment of a 256^3 K = 200 run
+    double tape_budget = -1.0;
     std::vector<std::unique_ptr<DevBuf<T>>> ck_ph, ck_nh, ck_w;
     std::vector<std::unique_ptr<Tape>> tapes;  // per segment (taped ones), plus scratch
     std::unique_ptr<Tape> scratch;
@@ -362,6 +365,17 @@ struct PlainEngine : EngineBase {
     bool fresh = true;
 
     void set_tape_budget(double bytes) override { tape_budget = bytes > 0 ? bytes : 0.0; }
+
+    double held_tape_bytes() const
+    {
+        double b = 0.0;
+        for (const auto& t : tapes)
+            if (t)
+                for (const auto* v : {&t->psi, &t->ph})
+                    for (const auto& e : *v)
+                        if (e) b += (double)n * sizeof(T);
+        return b;
+    }
 
     void copy_dd(T* dst, const T* src)
     {
@@ -405,7 +419,13 @@ struct PlainEngine : EngineBase {
         ck = std::min(ck, K);
         const int nseg = (K + ck - 1) / ck;
         const double seg_bytes = 2.0 * ck * n * sizeof(T);
-        const int ntaped = std::min(nseg, 1 + (int)(tape_budget / seg_bytes));
+        double budget = tape_budget;
+        if (budget < 0.0) {
+            size_t fr = 0, tot = 0;
+            HIPCHK(hipMemGetInfo(&fr, &tot));
+            budget = 0.25 * ((double)fr + held_tape_bytes());
+        }
+        const int ntaped = std::min(nseg, 1 + (int)std::min(budget / seg_bytes, 1e9));
         const int first = nseg - ntaped;
         ensure(ck_ph, nseg, n);
         ensure(ck_nh, nseg, n);
