@@ -216,6 +216,27 @@ enum { SMAD_REDFT10 = 0, SMAD_REDFT01 = 1, SMAD_RODFT10 = 2, SMAD_RODFT01 = 3 };
 int smad_r2r_3d(int nx, int ny, int nz, const int kinds[3], int precision, const double *in,
                 double *out);
 
+/* The transform of smad_r2r_3d on caller-owned device buffers, allocating
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that
+ * forbid hipMalloc).  in_dev and out_dev are FP64 device buffers of
+ * nx*ny*nz values, 16-byte aligned (in_dev may equal out_dev); stream is a
+ * hipStream_t (NULL: the default stream).  The call enqueues the work and
+ * returns; precision selects the device arithmetic as in smad_r2r_3d, with
+ * the same kernels and twiddle tables, so the results are identical.
+ * The library's device memory for this entry point is one fixed array per
+ * GPU, held until the process ends: the twiddle tables (written by the first
+ * call on each device, which therefore must not be captured into a graph)
+ * and, for precision 32, an FP32 staging area of at most
+ * SMAD_NOALLOC_MAX_F32_POINTS points (larger grids are rejected with
+ * SMAD_EINVAL).  Every FP32 call on a GPU reuses that staging area: calls
+ * that overlap on one GPU (different streams, different threads) race.
+ * The error message of a failed call is in smad_noalloc_last_error(). */
+#define SMAD_NOALLOC_MAX_F32_POINTS (1L << 24) /* 256^3; 64 MiB of staging */
+int smad_r2r_3d_noalloc(int nx, int ny, int nz, const int kinds[3], int precision,
+                        const double *in_dev, double *out_dev, void *stream);
+const char *smad_noalloc_last_error(void);
+
 /* ----------------------------------------------------------------------
  * Initial-condition builders (host only; no GPU needed).  All fill
  * nx*ny*nz doubles in the field layout above.  Noise is SplitMix64 of

@@ -39,7 +39,7 @@ EXPORTS = (
     "smad_last_error", "smad_ic_focal_conic", "smad_ic_slab", "smad_ic_curved",
     "smad_ic_add_noise", "smad_default_params", "smad_r2r_3d", "smad_time_steps",
     "smad_profile_steps", "smad_kernel_class_name", "smad_forward_tape", "smad_adjoint", "smad_adjoint_timing", "smad_set_tape_budget", "smad_surface",
-    "smad_surface_envelope",
+    "smad_surface_envelope", "smad_r2r_3d_noalloc", "smad_noalloc_last_error",
 )
 
 REDFT10, REDFT01, RODFT10, RODFT01 = 0, 1, 2, 3
@@ -143,6 +143,10 @@ def lib() -> ctypes.CDLL:
         L.smad_kernel_class_name.restype = ctypes.c_char_p
         L.smad_r2r_3d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int),
                                   ctypes.c_int, dp, dp]
+        L.smad_r2r_3d_noalloc.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int),
+                                          ctypes.c_int, vp, vp, vp]
+        L.smad_noalloc_last_error.argtypes = []
+        L.smad_noalloc_last_error.restype = ctypes.c_char_p
         for fn in ("smad_ic_focal_conic", "smad_ic_slab", "smad_ic_curved", "smad_ic_add_noise",
                    "smad_default_params"):
             getattr(L, fn).restype = None
@@ -186,6 +190,19 @@ def add_noise(p: Params, psi: np.ndarray, amp=1e-3, seed=0) -> np.ndarray:
     out = np.ascontiguousarray(psi, dtype=np.float64).copy()
     lib().smad_ic_add_noise(ctypes.byref(p.c()), amp, seed, _dptr(out))
     return out
+
+
+def r2r_3d_noalloc(in_ptr: int, out_ptr: int, shape, kinds, precision: int = FP64, stream: int = 0) -> None:
+    """smad_r2r_3d_noalloc on device buffers given by address (FP64,
+    nx*ny*nz values each): enqueues the transform on `stream` (a
+    hipStream_t handle, 0 = default stream) and returns."""
+    nx, ny, nz = shape
+    k = (ctypes.c_int * 3)(*kinds)
+    L = lib()
+    rc = L.smad_r2r_3d_noalloc(nx, ny, nz, k, precision, ctypes.c_void_p(in_ptr), ctypes.c_void_p(out_ptr),
+                               ctypes.c_void_p(stream))
+    if rc != OK:
+        raise SmadError(rc, L.smad_noalloc_last_error().decode())
 
 
 def r2r_3d(x: np.ndarray, kinds, precision: int = FP64) -> np.ndarray:

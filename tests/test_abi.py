@@ -88,3 +88,21 @@ def test_focal_conic_ic_formula(smadft):
             elif abs(k) < b_e:
                 v = 1.328 * np.cos(p.q0 * k * dz)
         assert psi[i, j, k] == pytest.approx(v, abs=1e-14)
+
+
+def test_noalloc_rejects_bad_arguments_before_any_device_call(smadft):
+    """smad_r2r_3d_noalloc validates on the host: no device is touched for
+    a rejected call (this runs without a GPU)."""
+    fake = 1 << 20  # aligned, never dereferenced
+    with pytest.raises(smadft.SmadError) as e:  # FP32 staging holds at most 2^24 points
+        smadft.r2r_3d_noalloc(fake, fake, (512, 256, 256), (0, 0, 0), smadft.FP32_MIXED)
+    assert e.value.code == smadft.EINVAL and "scratch area" in str(e.value)
+    with pytest.raises(smadft.SmadError) as e:
+        smadft.r2r_3d_noalloc(fake, fake, (20, 16, 16), (0, 0, 0))
+    assert e.value.code == smadft.EINVAL
+    with pytest.raises(smadft.SmadError) as e:
+        smadft.r2r_3d_noalloc(fake + 8, fake, (16, 16, 16), (0, 0, 0))
+    assert e.value.code == smadft.EINVAL and "aligned" in str(e.value)
+    with pytest.raises(smadft.SmadError) as e:
+        smadft.r2r_3d_noalloc(fake, fake, (16, 16, 16), (0, 4, 0))
+    assert e.value.code == smadft.EINVAL
