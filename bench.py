@@ -71,6 +71,10 @@ def main():
     ap.add_argument("--large-grid", type=int, default=0,
                     help="steps of the plain and quasi models on 1024x512x512 and of the plain "
                          "model on 1024^3 (0 = skip)")
+    ap.add_argument("--device-warmup-s", type=float, default=1.0,
+                    help="seconds of untimed steps before each timed plain leg")
+    ap.add_argument("--copy-gib", type=float, default=1.0,
+                    help="buffer size of the measured device-copy bandwidth (0 = skip)")
     ap.add_argument("--only-fp", choices=["64", "32"], default=None,
                     help="time one precision only (for profiler runs)")
     args = ap.parse_args()
@@ -79,6 +83,28 @@ def main():
         print("bench.py: this build runs on exactly one GPU (the grid is resident on one "
               "device; multi-GPU sharding is design-only). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    # measured copy ceiling: device-to-device copy of a large buffer, best
+    # of several, read + write bytes over device time (torch initialises the
+    # device here, before the library does)
+    copy_gbps = None
+    if args.copy_gib > 0:
+        import torch
+        nb = int(args.copy_gib * (1 << 30)) // 16 * 16
+        src = torch.empty(nb // 8, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        best = float("inf")
+        for _ in range(6):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            dst.copy_(src)
+            e1.record()
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        copy_gbps = 2 * nb / (best * 1e-3) / 1e9
+        log(f"measured device copy: {copy_gbps:.0f} GB/s ({nb / 2**30:.2f} GiB, best of 6)")
+        del src, dst
+        torch.cuda.empty_cache()
 
     smadft = importlib.import_module("sma-dft_amd")
     n = args.grid
@@ -90,6 +116,11 @@ def main():
         p = smadft.Params.default(smadft.MODEL_PLAIN, n)
         p.precision = prec
         s = smadft.Stepper(p, psi0)
+        # untimed device warm-up (clocks settle under load), then the W
+        # warmup steps, then exactly K timed steps
+        warm_ms = 0.0
+        while warm_ms < 1000.0 * args.device_warmup_s:
+            warm_ms += s.time_steps(100)
         s.step(args.warmup)
         s.sync()
         ms = s.time_steps(args.steps)
@@ -128,6 +159,7 @@ def main():
         for kd in kernels32:
             log(f"kernel fp32 {kd}")
     traffic = traffic_from_profiles(key, n, f"fp{prec_main}")
+
 
     # FP32-mixed error against FP64 after acc_steps, on the workload IC and on
     # the reference-amplitude IC (1.328, where the smectic persists)
@@ -344,8 +376,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "alg_bytes_per_launch": alg_bytes[key],
-                     "avg_launch_us": 1e3 * avg_ms},
+                     "avg_launch_us": 1e3 * avg_ms,
+                     "measured_copy_GBps": copy_gbps,
+                     "frac_of_measured_copy": (achieved / copy_gbps) if copy_gbps else None},
         "step_alg_GBps": (18 * F) / (ms64 / args.steps * 1e-3) / 1e9 if ms64 else None,
+        "device_warmup_s": args.device_warmup_s,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
