@@ -52,6 +52,8 @@ def lib():
         L.orc_quasi_step.argtypes = [pp, ctypes.POINTER(QState), ctypes.c_int, dp]
         L.orc_quasi_preloop.restype = None
         L.orc_quasi_step.restype = None
+        L.orc_quasi_project.argtypes = [pp, dp, dp, dp, dp, dp, dp, dp, dp]
+        L.orc_quasi_project.restype = None
         L.orc_adv_stokes.argtypes = [pp, dp, dp, dp, dp, dp, dp]
         L.orc_adv_stokes.restype = None
         L.orc_plain_adjoint.argtypes = [pp, dp, ctypes.c_int, dp, dp, dp, dp]
@@ -93,7 +95,11 @@ def r2r3d(x, kinds):
 class QState(ctypes.Structure):
     _fields_ = [(nm, ctypes.POINTER(ctypes.c_double)) for nm in
                 ("psi", "psiq", "psiq_old", "nl", "rhoq", "rho", "rho_old", "divv", "vel", "p",
-                 "energy", "grad_psi")] + [("nloop", ctypes.c_int)]
+                 "energy", "grad_psi")] + [("nloop", ctypes.c_int),
+                                           ("mu", ctypes.POINTER(ctypes.c_double))]
+
+_QARRAYS = ("psi", "psiq", "psiq_old", "nl", "rhoq", "rho", "rho_old", "divv", "vel", "p", "energy",
+            "grad_psi", "mu")
 
 
 class QuasiOracle:
@@ -103,7 +109,7 @@ class QuasiOracle:
         self.op = oparams(p)
         sh = tuple(np.shape(psi0))
         self.arrays = {}
-        for nm, _ in QState._fields_[:-1]:
+        for nm in _QARRAYS:
             shape = (3,) + sh if nm in ("vel", "grad_psi") else sh
             self.arrays[nm] = np.zeros(shape)
         self.arrays["psi"][...] = psi0
@@ -187,6 +193,15 @@ def adv_stokes(p, sx, sy, sz):
     out = [np.empty_like(a[0]) for _ in range(3)]
     op = oparams(p)
     lib().orc_adv_stokes(ctypes.byref(op), *[_p(v) for v in a], *[_p(v) for v in out])
+    return tuple(out)
+
+
+def quasi_project(p, sx, sy, sz, divv_hat):
+    """Quasi pressure and velocity projection alone (spectral in, spectral
+    out): returns (p_hat, ux_hat, uy_hat, uz_hat)."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (sx, sy, sz, divv_hat)]
+    out = [np.empty_like(a[0]) for _ in range(4)]
+    lib().orc_quasi_project(ctypes.byref(oparams(p)), *[_p(v) for v in a], *[_p(v) for v in out])
     return tuple(out)
 
 

@@ -83,8 +83,13 @@ void orc_adv_stokes(const orc_params *p, const double *sx, const double *sy, con
 typedef struct {
     double *psi, *psiq, *psiq_old, *nl, *rhoq, *rho, *rho_old, *divv, *vel, *p, *energy, *grad_psi;
     int nloop;
+    double *mu; /* optional (may be NULL): mu of the last step */
 } orc_quasi_state;
 void orc_quasi_preloop(const orc_params *p, orc_quasi_state *st);
+/* pressure and velocity projection alone (quasi/quasi.cpp:1510-2109),
+ * spectral in and out; see oracle_models.c */
+void orc_quasi_project(const orc_params *p, const double *Sx, const double *Sy, const double *Sz,
+                       const double *divv, double *pr, double *vx, double *vy, double *vz);
 /* sums (may be NULL): sum|psi-psi_old|, sum|psi|, mass sum(rho) dx^3, energy sum(e) dx^3
  * for the last step taken */
 void orc_quasi_step(const orc_params *p, orc_quasi_state *st, int nsteps, double *sums);

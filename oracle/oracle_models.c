@@ -460,6 +460,67 @@ static double mq2c_at(const grid_t *gp, int i, int j, int k)
     return pow(gp->qx[i], 2) + pow(gp->qy[j], 2) + pow(gp->qz[k], 2);
 }
 
+/* Pressure and velocity projection of quasi/quasi.cpp:1510-2109, spectral in
+ * and out: S_hat in the bases (STx, STy, STz), divv_hat = CT(divv) unscaled;
+ * p_hat (scaled, p = iCT(p_hat)) and u_hat (scaled, u = iST(u_hat)).
+ *   p_hat = s (-(q . S_hat)/|q|^2 + (lambda + 2 nu) divv_hat),  p_hat[0,0,0] = 0
+ *   u_hat = s/(nu |q|^2) (S_hat - q (q . S_hat)/|q|^2) + s q divv_hat/|q|^2
+ * with the reference's shifted reads of S_hat and divv_hat and its
+ * truncations (u_x at i = nx-1, u_y and u_z at j = ny-1 or k = nz-1 are 0). */
+void orc_quasi_project(const orc_params *p, const double *Sx, const double *Sy, const double *Sz,
+                       const double *divv, double *pr, double *vx, double *vy, double *vz)
+{
+    grid_t g;
+    grid_make(p, &g);
+    const long n = g.n;
+    const double sc = g.scale, nu = p->nu;
+    for (int i = 0; i < g.nx; ++i)                                          /* :1510-1592 */
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                long id = IDX(i, j, k);
+                double sxv = i > 0 ? Sx[IDX(i - 1, j, k)] : 0.0;
+                double syv = j > 0 ? Sy[IDX(i, j - 1, k)] : 0.0;
+                double szv = k > 0 ? Sz[IDX(i, j, k - 1)] : 0.0;
+                if (i + j + k != 0)
+                    pr[id] = sc * (-(g.qx[i] * sxv + g.qy[j] * syv + g.qz[k] * szv) / mq2c_at(&g, i, j, k) +
+                                   (p->lambda + 2 * nu) * divv[id]);
+                else
+                    pr[id] = 0.0;
+            }
+    memset(vx, 0, sizeof(double) * n);                                     /* :1599-2109 */
+    memset(vy, 0, sizeof(double) * n);
+    memset(vz, 0, sizeof(double) * n);
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                long id = IDX(i, j, k);
+                if (i < g.nx - 1) {
+                    double syv = (j != 0) ? Sy[IDX(i + 1, j - 1, k)] : 0.0;
+                    double szv = (k != 0) ? Sz[IDX(i + 1, j, k - 1)] : 0.0;
+                    double mq2 = pow(g.sx[i], 2) + pow(g.qy[j], 2) + pow(g.qz[k], 2);
+                    double cm1 = sc / (nu * mq2), cm2 = g.sx[i] / mq2;
+                    double dot = g.sx[i] * Sx[id] + g.qy[j] * syv + g.qz[k] * szv;
+                    vx[id] = cm1 * (Sx[id] - cm2 * dot) + cm1 * (nu * g.sx[i] * divv[IDX(i + 1, j, k)]);
+                }
+                if (j < g.ny - 1 && k < g.nz - 1) {
+                    double sxv = (i != 0) ? Sx[IDX(i - 1, j + 1, k)] : 0.0;
+                    double szv = (k != 0) ? Sz[IDX(i, j + 1, k - 1)] : 0.0;
+                    double mq2 = pow(g.qx[i], 2) + pow(g.sy[j], 2) + pow(g.qz[k], 2);
+                    double cm1 = sc / (nu * mq2), cm2 = g.sy[j] / mq2;
+                    double dot = g.qx[i] * sxv + g.sy[j] * Sy[id] + g.qz[k] * szv;
+                    vy[id] = cm1 * (Sy[id] - cm2 * dot) + cm1 * (nu * g.sy[j] * divv[IDX(i, j + 1, k)]);
+                    sxv = (i != 0) ? Sx[IDX(i - 1, j, k + 1)] : 0.0;
+                    double syv = (j != 0) ? Sy[IDX(i, j - 1, k + 1)] : 0.0;
+                    mq2 = pow(g.qx[i], 2) + pow(g.qy[j], 2) + pow(g.sz[k], 2);
+                    cm1 = sc / (nu * mq2);
+                    cm2 = g.sz[k] / mq2;
+                    dot = g.qx[i] * sxv + g.qy[j] * syv + g.sz[k] * Sz[id];
+                    vz[id] = cm1 * (Sz[id] - cm2 * dot) + cm1 * (nu * g.sz[k] * divv[IDX(i, j, k + 1)]);
+                }
+            }
+    grid_free(&g);
+}
+
 void orc_quasi_preloop(const orc_params *p, orc_quasi_state *st)
 {
     grid_t g;
@@ -486,7 +547,7 @@ void orc_quasi_step(const orc_params *p, orc_quasi_state *st, int nsteps, double
     grid_t g;
     grid_make(p, &g);
     const long n = g.n;
-    const double dt = p->dt, dtd2 = dt / 2, sc = g.scale, nu = p->nu, q02 = p->q0 * p->q0;
+    const double dt = p->dt, dtd2 = dt / 2, sc = g.scale, q02 = p->q0 * p->q0;
     const double sig2 = p->filter_sigma * p->filter_sigma;
     double *nl_old = malloc(sizeof(double) * n), *f = malloc(sizeof(double) * n);
     double *gpx = malloc(sizeof(double) * n), *gpy = malloc(sizeof(double) * n), *gpz = malloc(sizeof(double) * n);
@@ -564,54 +625,13 @@ void orc_quasi_step(const orc_params *p, orc_quasi_state *st, int nsteps, double
             Sy[id] = mu[id] * gpy[id] + e[id] * gry[id];
             Sz[id] = mu[id] * gpz[id] + e[id] * grz[id];
         }
+        if (st->mu) memcpy(st->mu, mu, sizeof(double) * n);
         orc_r2r3d(Sx, Sx, g.nx, g.ny, g.nz, ORC_RODFT10, ORC_REDFT10, ORC_REDFT10);  /* :1494-1504 */
         orc_r2r3d(Sy, Sy, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_RODFT10, ORC_REDFT10);
         orc_r2r3d(Sz, Sz, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_REDFT10, ORC_RODFT10);
-        /* pressure :1510-1592 */
-        for (int i = 0; i < g.nx; ++i)
-            for (int j = 0; j < g.ny; ++j)
-                for (int k = 0; k < g.nz; ++k) {
-                    long id = IDX(i, j, k);
-                    double sxv = i > 0 ? Sx[IDX(i - 1, j, k)] : 0.0;
-                    double syv = j > 0 ? Sy[IDX(i, j - 1, k)] : 0.0;
-                    double szv = k > 0 ? Sz[IDX(i, j, k - 1)] : 0.0;
-                    if (i + j + k != 0)
-                        pr[id] = sc * (-(g.qx[i] * sxv + g.qy[j] * syv + g.qz[k] * szv) / mq2c_at(&g, i, j, k) +
-                                       (p->lambda + 2 * nu) * st->divv[id]);
-                    else
-                        pr[id] = 0.0;
-                }
+        /* pressure :1510-1592 and velocity :1599-2109, spectral */
+        orc_quasi_project(p, Sx, Sy, Sz, st->divv, pr, vx, vy, vz);
         ict(&g, pr, pr);
-        /* velocity: solenoidal projection + irrotational part :1599-2109 */
-        memset(vx, 0, sizeof(double) * 3 * n);
-        for (int i = 0; i < g.nx; ++i)
-            for (int j = 0; j < g.ny; ++j)
-                for (int k = 0; k < g.nz; ++k) {
-                    long id = IDX(i, j, k);
-                    if (i < g.nx - 1) {
-                        double syv = (j != 0) ? Sy[IDX(i + 1, j - 1, k)] : 0.0;
-                        double szv = (k != 0) ? Sz[IDX(i + 1, j, k - 1)] : 0.0;
-                        double mq2 = pow(g.sx[i], 2) + pow(g.qy[j], 2) + pow(g.qz[k], 2);
-                        double cm1 = sc / (nu * mq2), cm2 = g.sx[i] / mq2;
-                        double dot = g.sx[i] * Sx[id] + g.qy[j] * syv + g.qz[k] * szv;
-                        vx[id] = cm1 * (Sx[id] - cm2 * dot) + cm1 * (nu * g.sx[i] * st->divv[IDX(i + 1, j, k)]);
-                    }
-                    if (j < g.ny - 1 && k < g.nz - 1) {
-                        double sxv = (i != 0) ? Sx[IDX(i - 1, j + 1, k)] : 0.0;
-                        double szv = (k != 0) ? Sz[IDX(i, j + 1, k - 1)] : 0.0;
-                        double mq2 = pow(g.qx[i], 2) + pow(g.sy[j], 2) + pow(g.qz[k], 2);
-                        double cm1 = sc / (nu * mq2), cm2 = g.sy[j] / mq2;
-                        double dot = g.qx[i] * sxv + g.sy[j] * Sy[id] + g.qz[k] * szv;
-                        vy[id] = cm1 * (Sy[id] - cm2 * dot) + cm1 * (nu * g.sy[j] * st->divv[IDX(i, j + 1, k)]);
-                        sxv = (i != 0) ? Sx[IDX(i - 1, j, k + 1)] : 0.0;
-                        double syv = (j != 0) ? Sy[IDX(i, j - 1, k + 1)] : 0.0;
-                        mq2 = pow(g.qx[i], 2) + pow(g.qy[j], 2) + pow(g.sz[k], 2);
-                        cm1 = sc / (nu * mq2);
-                        cm2 = g.sz[k] / mq2;
-                        dot = g.qx[i] * sxv + g.qy[j] * syv + g.sz[k] * Sz[id];
-                        vz[id] = cm1 * (Sz[id] - cm2 * dot) + cm1 * (nu * g.sz[k] * st->divv[IDX(i, j, k + 1)]);
-                    }
-                }
         ist3(&g, vx, vy, vz);                                                 /* :2113-2123 */
         /* Nl :2128-2154 */
         for (long id = 0; id < n; ++id) {

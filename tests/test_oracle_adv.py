@@ -92,3 +92,106 @@ def test_single_force_mode_stokes_solution(smadft, oracle):
     assert np.allclose(got, u, rtol=1e-13, atol=0)
     # no other mode is excited
     assert np.count_nonzero(ux) == 1 and np.count_nonzero(uy) == 1 and np.count_nonzero(uz) == 1
+
+
+# ---------------------------------------------------------------------------
+# Closed-form pins of the gradient, mu, the force and the advection term
+# (adv/cosAdvTrackMult.cpp:866-1303).  Everything but the Stokes projection
+# (pinned above by the single force mode and the divergence test) is
+# computed here from closed forms and scipy's transforms.
+
+import forms  # noqa: E402
+
+
+@pytest.mark.parametrize("modes", [
+    [(0, 0, 0, 0.3), (3, 0, 0, 0.5), (15, 0, 0, 0.2)],   # constant + mid + top x mode
+    [(0, 0, 0, -0.4), (0, 5, 0, 0.7), (0, 7, 0, 0.1)],   # y, incl. the top mode of ny = 8
+    [(0, 0, 0, 0.2), (0, 0, 9, 0.6), (0, 0, 31, 0.3)],   # z, incl. the top mode
+    [(2, 3, 4, 0.8), (15, 7, 31, 0.25)],                # products of the three axes
+])
+def test_gradient_of_cosine_modes_is_the_analytic_derivative(smadft, oracle, modes):
+    """d/dx cos(q_a x) = -q_a sin(q_a x) exactly: the DST-shift gradient maps
+    DCT mode a to DST mode a-1 with the factor -qs(a-1) = -q_a.  The
+    constant mode must give no gradient, which fails if the top DST mode
+    (which has no DCT partner, :877-902) were fed from a wrapped index."""
+    p = adv_params(smadft, (16, 8, 32))
+    psi0 = np.zeros(p.shape)
+    grad = [np.zeros(p.shape) for _ in range(3)]
+    for a, b, c, amp in modes:
+        f, g = forms.mode(p, a, b, c)
+        psi0 += amp * f
+        for ax in range(3):
+            grad[ax] += amp * g[ax]
+    o = oracle.AdvOracle(p, psi0)
+    o.step(1)  # the gradient of the step is that of psi^0
+    gmax = max(np.abs(g).max() for g in grad)
+    for ax in range(3):
+        assert np.abs(o.grad[ax] - grad[ax]).max() < 1e-12 * max(gmax, 1.0), ax
+
+
+def first_step_closed_form(p, amps_modes, advection_sign=-1.0, mu_sign=1.0):
+    """psi^1, u and S of the first advective step for psi^0 = sum of cosine
+    modes, from closed forms: Sp2 = -L(q) psi per mode (mu's linear part),
+    mu = Sp2 - beta psi^3 + gamma psi^5, S = mu grad psi, the projection of
+    the (pinned) oracle Stokes solve, N = beta psi^3 - gamma psi^5 - u . grad
+    psi and the CN/AB2 update with the pre-loop history
+    N_hat^{-1} = CT(beta psi^3 - gamma psi^5) (first step Euler in N)."""
+    import oracle as orc
+    psi = np.zeros(p.shape)
+    grad = [np.zeros(p.shape) for _ in range(3)]
+    sp2 = np.zeros(p.shape)
+    for amp, (a, b, c) in amps_modes:
+        f, g = forms.mode(p, a, b, c)
+        psi += amp * f
+        sp2 += -forms.alin(p, forms.mq2(p, a, b, c)) * amp * f
+        for ax in range(3):
+            grad[ax] += amp * g[ax]
+    mu = mu_sign * (sp2 - p.beta * psi ** 3 + p.gamma * psi ** 5)
+    S = [mu * g for g in grad]
+    Sh = [forms.r2r(S[ax], forms.ST[ax]) for ax in range(3)]
+    uh = orc.adv_stokes(p, *Sh)
+    u = [forms.r2r(uh[ax], forms.IST[ax]) for ax in range(3)]
+    nl0 = p.beta * psi ** 3 - p.gamma * psi ** 5
+    nr = nl0 + advection_sign * sum(u[ax] * grad[ax] for ax in range(3))
+    s = forms.scale(p)
+    L = forms.alin(p, forms.mq2_grid(p))
+    c1, c2 = 1 + p.dt / 2 * L, 1 - p.dt / 2 * L
+    ph = s * forms.ct(psi)
+    ph1 = (c1 * ph + p.dt / 2 * s * (3 * forms.ct(nr) - forms.ct(nl0))) / c2
+    return forms.ict(ph1), u, S
+
+
+def test_first_step_closed_form_mu_force_and_advection_sign(smadft, oracle):
+    p = adv_params(smadft, (16, 16, 16), dt=0.02)
+    am = [(1.1, (2, 1, 3)), (0.45, (1, 3, 2)), (-0.3, (3, 2, 1))]
+    psi0 = sum(a * forms.mode(p, *m)[0] for a, m in am)
+    o = oracle.AdvOracle(p, psi0)
+    o.step(1)
+    psi1, u, S = first_step_closed_form(p, am)
+    umax = max(np.abs(v).max() for v in u)
+    assert umax > 1e-6
+    for ax in range(3):
+        assert np.abs(o.vel[ax] - u[ax]).max() < 1e-11 * umax, ax
+    tol = 1e-13 * np.abs(psi1).max()
+    assert np.abs(o.psi - psi1).max() < tol
+    # the checks are sharp: the wrong advection sign, or mu with its
+    # nonlinear terms' signs reversed, moves psi^1 far beyond the tolerance
+    flipped, _, _ = first_step_closed_form(p, am, advection_sign=+1.0)
+    assert np.abs(flipped - psi1).max() > 1e4 * tol
+    _, u_wrong, _ = first_step_closed_form(p, am, mu_sign=-1.0)
+    assert max(np.abs(u_wrong[ax] - u[ax]).max() for ax in range(3)) > 1.0 * umax
+
+
+def test_stokes_flow_dissipates(smadft, oracle):
+    """The flow does positive work against the force, sum u . S > 0
+    (u = P S/(nu |q|^2)): advection lowers the free energy, as the model's
+    -u . grad psi with the force mu grad psi must."""
+    p = adv_params(smadft, (16, 16, 16))
+    am = [(1.2, (2, 1, 3)), (0.5, (1, 3, 2))]
+    psi0 = sum(a * forms.mode(p, *m)[0] for a, m in am)
+    o = oracle.AdvOracle(p, psi0)
+    o.step(1)
+    _, _, S = first_step_closed_form(p, am)
+    work = sum(float(np.sum(o.vel[ax] * S[ax])) for ax in range(3))
+    scale_ = sum(float(np.sum(np.abs(o.vel[ax] * S[ax]))) for ax in range(3))
+    assert work > 0.2 * scale_
