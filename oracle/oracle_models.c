@@ -5,17 +5,29 @@
  * reference's index shifts, truncations and scale placement.  The product
  * never links this file; tests compare the HIP path against it.
  *
- * Pinning (tests/test_oracle.py):
+ * Pinning (tests/test_oracle*.py; each check fails for a plausible mistake,
+ * shown by planting such mistakes in this file):
  *   - plain update: linear single mode decays as G(q)^n with
  *     G = (1 + dt L/2)/(1 - dt L/2) when beta = gamma = 0; uniform psi follows
  *     the scalar AB2/CN recurrence (first step Euler in N).
- *   - advective gradient: single cosine modes give the analytic derivative;
- *     planar layers psi(z) give u == 0; single-mode Stokes solve matches
- *     (I - q q^T/|q|^2) F / (nu |q|^2) for an x-only force.
- *   - the cross-index reads of the velocity projection
- *     (adv/cosAdvTrackMult.cpp:1103-1274) are pinned only by the single-mode
- *     and divergence checks (parity partially unpinned: no compiled reference
- *     can be built here, FFTW and MPI are absent).
+ *   - advective: single cosine modes give the analytic derivative (constant
+ *     and top modes included); the first step of a three-mode psi equals its
+ *     closed form (Sp2 = -L(q) psi per mode, mu, S = mu grad psi, the Stokes
+ *     solve, N = beta psi^3 - gamma psi^5 - u . grad psi, CN/AB2 with the
+ *     pre-loop history), which fixes the sign of Sp2 in mu and of the
+ *     advection term; planar layers give u == 0; a single force mode gives
+ *     (I - q q^T/|q|^2) F / (nu |q|^2); the spectral divergence vanishes on
+ *     every untruncated mode; the flow does positive work against S.
+ *   - quasi: uniform psi follows the scalar BDF2/AB2 recurrence (C1, C2,
+ *     rho_m, mu, the density law, e, mass, energy); a linear single mode
+ *     follows it across the switch (both lap branches); the two lap branches
+ *     coincide for uniform rho; the density law and Gaussian filter equal a
+ *     closed form (q0 != 1); e and mu of cosine modes; the projection of a
+ *     single force and divergence mode equals the compressible Stokes
+ *     solution; divv, S = mu grad psi + e grad rho and the Nl coupling term
+ *     match their forms for a manufactured previous density.
+ *   - the truncated top modes of the projections (j = ny-1, k = nz-1,
+ *     adv/cosAdvTrackMult.cpp:1202-1203) follow the reference by reading.
  */
 #include "oracle.h"
 

@@ -180,6 +180,17 @@ def test_first_step_closed_form_mu_force_and_advection_sign(smadft, oracle):
     assert np.abs(flipped - psi1).max() > 1e4 * tol
     _, u_wrong, _ = first_step_closed_form(p, am, mu_sign=-1.0)
     assert max(np.abs(u_wrong[ax] - u[ax]).max() for ax in range(3)) > 1.0 * umax
+    # mu's local part g(psi) = -beta psi^3 + gamma psi^5 adds g grad psi =
+    # grad G(psi) to S, a gradient the projection removes: only Sp2 (the
+    # nonlocal part) drives the flow, so the local terms' signs are not
+    # observable through u (and the reference's choice cannot be pinned
+    # there); the projection must show exactly that
+    import oracle as orc
+    psi = sum(a * forms.mode(p, *m)[0] for a, m in am)
+    grad = [sum(a * forms.mode(p, *m)[1][ax] for a, m in am) for ax in range(3)]
+    G = [(p.beta * psi ** 3) * g for g in grad]
+    uG = orc.adv_stokes(p, *[forms.r2r(G[ax], forms.ST[ax]) for ax in range(3)])
+    assert max(np.abs(v).max() for v in uG) < 1e-13 * umax
 
 
 def test_stokes_flow_dissipates(smadft, oracle):

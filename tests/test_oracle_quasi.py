@@ -185,7 +185,7 @@ def test_lap_branches_agree_for_uniform_density(smadft, oracle):
 def test_density_law_and_gaussian_filter(smadft, oracle):
     """rho = iCT(s CT(kp sqrt(q0^2 psi^2 + |grad psi|^2) + rho0) exp(-2 sigma^2 |q|^2)),
     with psi and grad psi in closed form and scipy's transforms."""
-    p = qparams(smadft, (16, 16, 16), kp=0.3, rho0=0.9)
+    p = qparams(smadft, (16, 16, 16), kp=0.3, rho0=0.9, q0=1.3)  # q0 != 1 pins its factor
     am = [(1.0, (2, 1, 3)), (0.4, (1, 2, 1)), (0.2, (0, 0, 0))]
     psi = np.zeros(p.shape)
     grad = [np.zeros(p.shape) for _ in range(3)]
