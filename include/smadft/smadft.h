@@ -59,6 +59,9 @@ typedef struct {
     double filter_sigma;    /* Gaussian filter width of rho (quasi), reference 1.57 */
     int div_switch;         /* quasi: step after which divv and grad^2(rho f) are used */
     int precision;          /* SMAD_FP64 or SMAD_FP32_MIXED */
+    int load;               /* quasi: 1 = the reference's restart start (load == 1,
+                               quasi/quasi.cpp:1365-1369): divv and grad^2(rho f)
+                               from the first step, with rho_old = rho at that step */
 } smad_params;
 
 enum {
@@ -84,7 +87,25 @@ enum {
     SMAD_FIELD_PRESSURE = 9, /* real pressure (quasi), FP64 storage */
     SMAD_FIELD_RHO = 10,     /* filtered density of the last step (quasi) */
     SMAD_FIELD_ENERGY = 11,  /* free-energy density e of the last step (quasi) */
-    SMAD_FIELD_DIVV_HAT = 12 /* REDFT10^3 of the imposed divergence (quasi, unscaled) */
+    SMAD_FIELD_DIVV_HAT = 12, /* REDFT10^3 of the imposed divergence (quasi, unscaled) */
+    /* restart state (fetch and set; see smad_state_fields) */
+    SMAD_FIELD_PSI_HAT_OLD = 13, /* quasi: psi_hat of the step before (BDF2 history) */
+    SMAD_FIELD_RHO_OLD = 14,     /* quasi: filtered density of the last step (divv's rho_old) */
+    SMAD_FIELD_GRAD_RHO_X = 15,  /* quasi: grad rho the next step uses (of the last filtered rho) */
+    SMAD_FIELD_GRAD_RHO_Y = 16,
+    SMAD_FIELD_GRAD_RHO_Z = 17,
+    SMAD_FIELD_SPEC_XY = 18,     /* plain: REDFT01_y REDFT01_x(psi_hat), the carried state */
+    /* quasi save-block outputs of the last step (fetch only; quasi/quasi.cpp:2379-2487) */
+    SMAD_FIELD_VSOL_X = 19,      /* solenoidal part of the velocity */
+    SMAD_FIELD_VSOL_Y = 20,
+    SMAD_FIELD_VSOL_Z = 21,
+    SMAD_FIELD_FPROJ_X = 22,     /* projected force s iST(S_hat - CM2 (q . S_hat)) */
+    SMAD_FIELD_FPROJ_Y = 23,
+    SMAD_FIELD_FPROJ_Z = 24,
+    SMAD_FIELD_FORCE_X = 25,     /* the force S = mu grad psi + e grad rho */
+    SMAD_FIELD_FORCE_Y = 26,
+    SMAD_FIELD_FORCE_Z = 27,
+    SMAD_FIELD_DIVV = 28         /* the imposed divergence in real space */
 };
 
 /* Reductions (FP64 on the device, deterministic order). */
@@ -117,22 +138,46 @@ int smad_fetch(smad_ctx *ctx, int field, double *host_out);
  * load == 1 path does (no-adv/cosNoAdvTrackMult.cpp:517-537). */
 int smad_set_field(smad_ctx *ctx, int field, const double *host_in);
 
+/* The fields that make up the model's full restart state, in the order to
+ * set them back: fields[0..count-1], count returned (at most max are
+ * written).  Fetching each with smad_fetch and the step count with
+ * smad_steps_taken, then setting each with smad_set_field on a fresh
+ * context of the same parameters and calling smad_set_steps_taken,
+ * continues the run exactly as if it had not been interrupted (FP32 storage
+ * is converted exactly).  The reference's own restart keeps psi only and
+ * re-runs the pre-loop (no-adv/cosNoAdvTrackMult.cpp:517-537); that is
+ * smad_set_field(SMAD_FIELD_PSI).  Plain: psi_hat, N_hat and the carried
+ * xy-transformed psi_hat; advective: psi_hat, N_hat; quasi: psi_hat, its
+ * previous value, N_hat, rho_old, grad rho and u. */
+int smad_state_fields(const smad_ctx *ctx, int *fields, int max);
+
+/* Sets the step count (smad_steps_taken) and, for the quasi model, the loop
+ * counter its divv switch follows; used to restore a full restart state. */
+int smad_set_steps_taken(smad_ctx *ctx, long steps);
+
 /* Global FP64 reduction (synchronises).  Returns SMAD_ENONFINITE when psi
  * holds NaN/Inf -- the reference would silently stop its loop instead
  * (no-adv/cosNoAdvTrackSing.cpp:681). */
 int smad_reduce(smad_ctx *ctx, int what, double *out);
 
-/* Interface tracker and surface diagnostics of the current psi, plain model
- * (replaces no-adv/cosNoAdvTrackMult.cpp:787-796 and 853-1022, the save
- * block's surface computation).  For every column (i, j), scanning k from
- * Nz-1 down, the first psi > 0.7 arms the tracker and the first psi < 0 after
+/* Interface tracker and surface diagnostics of the current psi, plain and
+ * advective models (replaces no-adv/cosNoAdvTrackMult.cpp:787-796 and
+ * 853-1022, and adv/cosAdvTrackMult.cpp:1311-1347 and 1412-1500, the save
+ * blocks' surface computations).  In the advective model dTpsi =
+ * REDFT01^3(aLin psi_hat + s N_hat) carries the advection term of the
+ * current psi's N_hat, and the gradient is its spectral grad psi, as the
+ * reference; the call computes u, grad psi and N_hat of the current psi
+ * (smad_fetch of the GRAD and VEL fields then returns those, which the
+ * reference's save block sees) and leaves the time stepping unchanged.
+ * For every column (i, j), scanning k from Nz-1 down, the first psi > 0.7 arms the tracker and the first psi < 0 after
  * it is the crossing k; k2 = k + 1 when |psi(k)| > |psi(k+1)|, else k.  Each
  * output holds nx*ny doubles, index i*ny + j:
  *   surf   = k2 (the reference's surfPsi.dat), -1 for a column without crossing
  *   vel    = dTpsi / |grad psi| at (i, j, k2)       (velSurf.dat)
  *   curv_h = 2H, curv_k = K at (i, j, k2)          (curvH.dat, curvK.dat)
- * with finite-difference gradients and spectral second derivatives as the
- * reference.  The reference's save block reports the state one step before
+ * with finite-difference (plain) or spectral (advective) gradients and
+ * spectral second derivatives as the reference.  The reference's save block
+ * reports the state one step before
  * the save step (it runs before psi <- psi_new); call this after the
  * penultimate step of a save block to reproduce its files.  Columns without a
  * crossing get 0 (the reference leaves stale buffer contents there). */
@@ -261,9 +306,42 @@ void smad_ic_curved(const smad_params *p, double amp, double noise, unsigned lon
 /* Adds SplitMix64 noise U[-amp, amp) to psi. */
 void smad_ic_add_noise(const smad_params *p, double amp, unsigned long long seed, double *psi);
 
-/* Reference parameter set for the given model with the reference's
- * constants (q0 = 1, alpha = 1, beta = 2, gamma = 1, ...). */
+/* A planar smectic slab |x - Lx/2| < Lx/4 whose layers' normal is tilted
+ * from z by t ~ U[0, max_tilt_deg] degrees at an azimuth ~ U[0, 2 pi), both
+ * drawn from the seed: amp cos(q0 n . r); *tilt_deg (may be null) receives
+ * t.  Plus noise. */
+void smad_ic_tilted_slab(const smad_params *p, double amp, double max_tilt_deg, double noise,
+                         unsigned long long seed, double *psi, double *tilt_deg);
+
+/* The slab of layers amp cos(q0 k dz) with two bumped interfaces: each of
+ * x = Lx/4 and x = 3Lx/4 carries one seeded Gaussian bump out of the slab
+ * (height 0.5-1.5 layer spacings, width Ly/16-Ly/8).  Plus noise. */
+void smad_ic_two_bumps(const smad_params *p, double amp, double noise, unsigned long long seed,
+                       double *psi);
+
+/* The reference program's own constants for the given model (q0 = 1,
+ * alpha = 1, beta = 2, gamma = 1; plain: nw = 8, dt = 0.1, eps = -0.7;
+ * advective and conserved: nw = 16, dt = 5e-4 (adv/cosAdvTrackMult.cpp:
+ * 181-182); quasi: nw = 8, dt = 1e-3, rho0 = 1, kp = 0, rho_m = 0.5,
+ * lambda = -2 nu/3, div_switch = 200 (quasi/quasi.cpp:204-252)).  eps is
+ * set from the programs' argv by the caller (-0.01 ep_arg advective and
+ * conserved, -0.001 ep_arg quasi). */
 void smad_default_params(int model, int n, smad_params *out);
+
+/* The benchmark configurations: 0 plain 128^3, 1 plain 256^3, 2 advective
+ * 256^3 (nw = 8, nu = 1, dt = 0.02), 3 quasi 384^3 (rho0 = 0.9,
+ * kp = 0.1/1.34164, rho_m = 0.5, dt = 1e-3), 4 plain 256^3 (the adjoint
+ * case); eps = -0.7, alpha = 1, beta = 2, gamma = 1, q0 = 1.  Configuration
+ * 2 specifies dt = 0.05 and 3 dt = 0.02; both diverge in these schemes
+ * (explicit advection; the quasi model's explicit linear operator), so the
+ * presets take the stable steps.  Returns SMAD_EINVAL for another number. */
+int smad_config_params(int config, smad_params *out);
+
+/* The configuration's initial condition on the grid of p: 0 slab (amp 0.6,
+ * seed 0), 1 and 4 curved layers (amp 0.6, seed 1), 2 two bumped interfaces
+ * (amp 1.328, seed 2), 3 tilted slab (amp 1.34164, tilt <= 15 deg, seed 3);
+ * noise 1e-3. */
+int smad_config_ic(int config, const smad_params *p, double *psi);
 
 #ifdef __cplusplus
 }

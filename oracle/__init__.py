@@ -24,7 +24,7 @@ class OParams(ctypes.Structure):
         ("dt", ctypes.c_double), ("nu", ctypes.c_double), ("lambda_", ctypes.c_double),
         ("rho0", ctypes.c_double), ("kp", ctypes.c_double), ("rho_m", ctypes.c_double),
         ("filter_sigma", ctypes.c_double), ("div_switch", ctypes.c_int),
-        ("precision", ctypes.c_int),
+        ("precision", ctypes.c_int), ("load", ctypes.c_int),
     ]
 
 
@@ -60,6 +60,10 @@ def lib():
         L.orc_surface_aux.argtypes = [pp, dp, dp, dp, dp, dp]
         L.orc_surface_track.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [dp] * 9
         L.orc_surface_track_env.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [dp] * 9
+        L.orc_surface_track_grad.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [dp] * 12
+        L.orc_surface_track_grad.restype = None
+        L.orc_adv_surface.argtypes = [pp, dp, dp, dp, dp, dp]
+        L.orc_adv_surface.restype = None
         for f in ("orc_surface_aux", "orc_surface_track", "orc_surface_track_env", "orc_plain_adjoint",
                   "orc_r2r1d_naive",
                   "orc_r2r1d", "orc_r2r3d", "orc_plain_preloop",
@@ -222,6 +226,24 @@ def surface_track(psi, dxx, dyy, dzz, dtp, h, envelope=False):
     out = [np.empty((nx, ny)) for _ in range(4)]
     f = lib().orc_surface_track_env if envelope else lib().orc_surface_track
     f(nx, ny, nz, hx, hy, hz, *[_p(a) for a in arrs], *[_p(a) for a in out])
+    return dict(zip(("surf", "vel", "curv_h", "curv_k"), out))
+
+
+def adv_surface(p, psi):
+    """The advective save block's surface diagnostics of real psi."""
+    psi = np.ascontiguousarray(psi, dtype=np.float64)
+    out = [np.empty((p.nx, p.ny)) for _ in range(4)]
+    lib().orc_adv_surface(ctypes.byref(oparams(p)), _p(psi), *[_p(a) for a in out])
+    return dict(zip(("surf", "vel", "curv_h", "curv_k"), out))
+
+
+def surface_track_grad(psi, grad, dxx, dyy, dzz, dtp, h):
+    """Crossing scan with given gradient arrays (3, nx, ny, nz)."""
+    hx, hy, hz = (h, h, h) if np.isscalar(h) else h
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (psi, grad[0], grad[1], grad[2], dxx, dyy, dzz, dtp)]
+    nx, ny, nz = arrs[0].shape
+    out = [np.empty((nx, ny)) for _ in range(4)]
+    lib().orc_surface_track_grad(nx, ny, nz, hx, hy, hz, *[_p(a) for a in arrs], *[_p(a) for a in out])
     return dict(zip(("surf", "vel", "curv_h", "curv_k"), out))
 
 

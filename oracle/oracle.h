@@ -31,6 +31,7 @@ typedef struct {
     double nu, lambda, rho0, kp, rho_m, filter_sigma;
     int div_switch;
     int precision;
+    int load; /* quasi: the reference's load == 1 start (quasi/quasi.cpp:1365-1369) */
 } orc_params;
 
 /* ------------------------------------------------------------ plain model -- */
@@ -60,6 +61,12 @@ void orc_surface_aux(const orc_params *p, const double *psi, double *dxx, double
 void orc_surface_track(int nx, int ny, int nz, double dx, double dy, double dz, const double *psi,
                        const double *dxx, const double *dyy, const double *dzz, const double *dtp,
                        double *surf, double *vel, double *curv_h, double *curv_k);
+/* the crossing scan with given gradient arrays (the advective program's
+ * spectral grad psi, adv/cosAdvTrackMult.cpp:1412-1500) */
+void orc_surface_track_grad(int nx, int ny, int nz, double dx, double dy, double dz, const double *psi,
+                            const double *gx, const double *gy, const double *gz, const double *dxx,
+                            const double *dyy, const double *dzz, const double *dtp, double *surf, double *vel,
+                            double *curv_h, double *curv_k);
 void orc_surface_track_env(int nx, int ny, int nz, double dx, double dy, double dz, const double *psi,
                            const double *dxx, const double *dyy, const double *dzz, const double *dtp,
                            double *surf, double *vel, double *curv_h, double *curv_k);
@@ -74,6 +81,10 @@ void orc_surface_track_env(int nx, int ny, int nz, double dx, double dy, double 
 void orc_adv_preloop(const orc_params *p, const double *psi, double *nq);
 void orc_adv_step(const orc_params *p, double *psi, double *nq, double *nq_past, double *grad,
                   double *vel, int nsteps);
+/* the save block's surface diagnostics for real psi (adv/cosAdvTrackMult.cpp:
+ * 1311-1347, 1412-1500); outputs as orc_surface_track */
+void orc_adv_surface(const orc_params *p, const double *psi, double *surf, double *vel, double *curv_h,
+                     double *curv_k);
 /* the velocity projection alone (adv/cosAdvTrackMult.cpp:1103-1274) */
 void orc_adv_stokes(const orc_params *p, const double *sx, const double *sy, const double *sz,
                     double *vx, double *vy, double *vz);

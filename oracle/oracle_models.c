@@ -222,6 +222,78 @@ void orc_adv_preloop(const orc_params *p, const double *psi, double *nq)
     orc_plain_preloop(p, psi, nq);
 }
 
+/* The front half of the advective step from real psi (the state at its
+ * start), adv/cosAdvTrackMult.cpp:831-1286: zero fill (:831-837), psiq =
+ * s CT(psi) and psi <- iCT(psiq) (:847-963), the DST-shift gradient
+ * (:866-975), Sp2 = iCT(-aLin psiq) (:980-991), S = mu grad psi
+ * (:996-1027), the Stokes velocity (:1103-1286).  Work arrays: psiq, g*,
+ * v*, sp2, s* (n each). */
+static void adv_front(const orc_params *p, const grid_t *gp, double *psi, double *psiq, double *gx, double *gy,
+                      double *gz, double *vx, double *vy, double *vz, double *sp2, double *sx, double *sy,
+                      double *sz)
+{
+    grid_t g = *gp;
+    const long n = g.n;
+    const double sc = g.scale;
+    memset(gx, 0, sizeof(double) * n); memset(gy, 0, sizeof(double) * n); memset(gz, 0, sizeof(double) * n);
+    memset(vx, 0, sizeof(double) * n); memset(vy, 0, sizeof(double) * n); memset(vz, 0, sizeof(double) * n);
+    ct(&g, psi, psiq);                                   /* :847-860 */
+    for (long i = 0; i < n; ++i) psiq[i] = sc * psiq[i];
+
+    /* DST-shift gradient :866-959 (x: i <= Nx-2, the last plane stays 0) */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                long id = IDX(i, j, k);
+                if (i < g.nx - 1) gx[id] = -g.sx[i] * psiq[IDX(i + 1, j, k)];
+                if (j < g.ny - 1) gy[id] = -g.sy[j] * psiq[IDX(i, j + 1, k)];
+                if (k < g.nz - 1) gz[id] = -g.sz[k] * psiq[IDX(i, j, k + 1)];
+            }
+    ict(&g, psiq, psi);                                  /* :961-963 */
+    orc_r2r3d(gx, gx, g.nx, g.ny, g.nz, ORC_RODFT01, ORC_REDFT01, ORC_REDFT01); /* :965-967 */
+    orc_r2r3d(gy, gy, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_RODFT01, ORC_REDFT01); /* :969-971 */
+    orc_r2r3d(gz, gz, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_REDFT01, ORC_RODFT01); /* :973-975 */
+
+    /* Sp2 = iCT(-aLin psiq) :980-991 */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                long id = IDX(i, j, k);
+                sp2[id] = -alin_at(p, &g, i, j, k) * psiq[id];
+            }
+    ict(&g, sp2, sp2);
+
+    /* S = mu grad psi :996-1015 */
+#pragma omp parallel for schedule(static)
+    for (long id = 0; id < n; ++id) {
+        double mu = sp2[id] - p->beta * pow(psi[id], 3) + p->gamma * pow(psi[id], 5);
+        sx[id] = mu * gx[id];
+        sy[id] = mu * gy[id];
+        sz[id] = mu * gz[id];
+    }
+    orc_r2r3d(sx, sx, g.nx, g.ny, g.nz, ORC_RODFT10, ORC_REDFT10, ORC_REDFT10); /* :1017-1027 */
+    orc_r2r3d(sy, sy, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_RODFT10, ORC_REDFT10);
+    orc_r2r3d(sz, sz, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_REDFT10, ORC_RODFT10);
+
+    /* Stokes projection :1103-1274 */
+    orc_adv_stokes(p, sx, sy, sz, vx, vy, vz);
+    orc_r2r3d(vx, vx, g.nx, g.ny, g.nz, ORC_RODFT01, ORC_REDFT01, ORC_REDFT01); /* :1276-1286 */
+    orc_r2r3d(vy, vy, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_RODFT01, ORC_REDFT01);
+    orc_r2r3d(vz, vz, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_REDFT01, ORC_RODFT01);
+}
+
+/* Nr = beta psi^3 - gamma psi^5 - u . grad psi (:1290-1303) */
+static void adv_nr(const orc_params *p, long n, const double *psi, const double *gx, const double *gy,
+                   const double *gz, const double *vx, const double *vy, const double *vz, double *nr)
+{
+#pragma omp parallel for schedule(static)
+    for (long id = 0; id < n; ++id)
+        nr[id] = p->beta * pow(psi[id], 3) - p->gamma * pow(psi[id], 5) - vx[id] * gx[id] - vy[id] * gy[id] -
+                 vz[id] * gz[id];
+}
+
 void orc_adv_step(const orc_params *p, double *psi, double *nq, double *nq_past, double *grad,
                   double *vel, int nsteps)
 {
@@ -237,62 +309,12 @@ void orc_adv_step(const orc_params *p, double *psi, double *nq, double *nq_past,
     double *nr = malloc(sizeof(double) * n);
 
     for (int s = 0; s < nsteps; ++s) {
-        /* :831-837 zero fill */
-        memset(gx, 0, sizeof(double) * n); memset(gy, 0, sizeof(double) * n); memset(gz, 0, sizeof(double) * n);
-        memset(vx, 0, sizeof(double) * n); memset(vy, 0, sizeof(double) * n); memset(vz, 0, sizeof(double) * n);
         memcpy(nq_past, nq, sizeof(double) * n);             /* :842 */
-        ct(&g, psi, psiq);                                   /* :847-860 */
-        for (long i = 0; i < n; ++i) psiq[i] = sc * psiq[i];
-
-        /* DST-shift gradient :866-959 (x: i <= Nx-2, the last plane stays 0) */
-#pragma omp parallel for collapse(2) schedule(static)
-        for (int i = 0; i < g.nx; ++i)
-            for (int j = 0; j < g.ny; ++j)
-                for (int k = 0; k < g.nz; ++k) {
-                    long id = IDX(i, j, k);
-                    if (i < g.nx - 1) gx[id] = -g.sx[i] * psiq[IDX(i + 1, j, k)];
-                    if (j < g.ny - 1) gy[id] = -g.sy[j] * psiq[IDX(i, j + 1, k)];
-                    if (k < g.nz - 1) gz[id] = -g.sz[k] * psiq[IDX(i, j, k + 1)];
-                }
-        ict(&g, psiq, psi);                                  /* :961-963 */
-        orc_r2r3d(gx, gx, g.nx, g.ny, g.nz, ORC_RODFT01, ORC_REDFT01, ORC_REDFT01); /* :965-967 */
-        orc_r2r3d(gy, gy, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_RODFT01, ORC_REDFT01); /* :969-971 */
-        orc_r2r3d(gz, gz, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_REDFT01, ORC_RODFT01); /* :973-975 */
-
-        /* Sp2 = iCT(-aLin psiq) :980-991 */
-#pragma omp parallel for collapse(2) schedule(static)
-        for (int i = 0; i < g.nx; ++i)
-            for (int j = 0; j < g.ny; ++j)
-                for (int k = 0; k < g.nz; ++k) {
-                    long id = IDX(i, j, k);
-                    sp2[id] = -alin_at(p, &g, i, j, k) * psiq[id];
-                }
-        ict(&g, sp2, sp2);
-
-        /* S = mu grad psi :996-1015 */
-#pragma omp parallel for schedule(static)
-        for (long id = 0; id < n; ++id) {
-            double mu = sp2[id] - p->beta * pow(psi[id], 3) + p->gamma * pow(psi[id], 5);
-            sx[id] = mu * gx[id];
-            sy[id] = mu * gy[id];
-            sz[id] = mu * gz[id];
-        }
-        orc_r2r3d(sx, sx, g.nx, g.ny, g.nz, ORC_RODFT10, ORC_REDFT10, ORC_REDFT10); /* :1017-1027 */
-        orc_r2r3d(sy, sy, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_RODFT10, ORC_REDFT10);
-        orc_r2r3d(sz, sz, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_REDFT10, ORC_RODFT10);
-
-        /* Stokes projection :1103-1274 */
-        orc_adv_stokes(p, sx, sy, sz, vx, vy, vz);
-        orc_r2r3d(vx, vx, g.nx, g.ny, g.nz, ORC_RODFT01, ORC_REDFT01, ORC_REDFT01); /* :1276-1286 */
-        orc_r2r3d(vy, vy, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_RODFT01, ORC_REDFT01);
-        orc_r2r3d(vz, vz, g.nx, g.ny, g.nz, ORC_REDFT01, ORC_REDFT01, ORC_RODFT01);
+        adv_front(p, &g, psi, psiq, gx, gy, gz, vx, vy, vz, sp2, sx, sy, sz);
 
         /* Nr :1290-1303, Nq :1307 */
         if (p->model != 4) {
-#pragma omp parallel for schedule(static)
-            for (long id = 0; id < n; ++id)
-                nr[id] = p->beta * pow(psi[id], 3) - p->gamma * pow(psi[id], 5) - vx[id] * gx[id] -
-                         vy[id] * gy[id] - vz[id] * gz[id];
+            adv_nr(p, n, psi, gx, gy, gz, vx, vy, vz, nr);
             ct(&g, nr, nq);
         } else {
             /* conserved variant (adv/cosAdvConSingle.cpp:1290-1310, 1369):
@@ -330,6 +352,46 @@ void orc_adv_step(const orc_params *p, double *psi, double *nq, double *nq_past,
     if (vel) { memcpy(vel, vx, sizeof(double) * n); memcpy(vel + n, vy, sizeof(double) * n); memcpy(vel + 2 * n, vz, sizeof(double) * n); }
     free(psiq); free(gx); free(gy); free(gz); free(vx); free(vy); free(vz);
     free(sp2); free(sx); free(sy); free(sz); free(nr);
+    grid_free(&g);
+}
+
+/* The advective save block for real psi (the state it sees):
+ * adv/cosAdvTrackMult.cpp:1311-1347 (dTpsi = iCT(aLin psiq + s Nq) with
+ * this psi's Nq, advection included; psi_xx = iCT(-qx^2 psiq), ...) and
+ * :1412-1500 (the crossing scan with the spectral gradient). */
+void orc_adv_surface(const orc_params *p, const double *psi_in, double *surf, double *vel, double *curv_h,
+                     double *curv_k)
+{
+    grid_t g;
+    grid_make(p, &g);
+    const long n = g.n;
+    double *psi = malloc(sizeof(double) * n), *psiq = malloc(sizeof(double) * n);
+    double *gx = malloc(sizeof(double) * n), *gy = malloc(sizeof(double) * n), *gz = malloc(sizeof(double) * n);
+    double *vx = malloc(sizeof(double) * n), *vy = malloc(sizeof(double) * n), *vz = malloc(sizeof(double) * n);
+    double *sp2 = malloc(sizeof(double) * n), *sx = malloc(sizeof(double) * n), *sy = malloc(sizeof(double) * n);
+    double *sz = malloc(sizeof(double) * n), *nr = malloc(sizeof(double) * n), *nq = malloc(sizeof(double) * n);
+    memcpy(psi, psi_in, sizeof(double) * n);
+    adv_front(p, &g, psi, psiq, gx, gy, gz, vx, vy, vz, sp2, sx, sy, sz);
+    adv_nr(p, n, psi, gx, gy, gz, vx, vy, vz, nr);
+    ct(&g, nr, nq);                                          /* :1307 */
+    double *dxx = sp2, *dyy = sx, *dzz = sy, *dtp = sz;      /* reuse work arrays */
+    for (int i = 0; i < g.nx; ++i)                           /* :1311-1329 */
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                long id = IDX(i, j, k);
+                dtp[id] = alin_at(p, &g, i, j, k) * psiq[id] + g.scale * nq[id];
+                dxx[id] = -g.qx[i] * g.qx[i] * psiq[id];
+                dyy[id] = -g.qy[j] * g.qy[j] * psiq[id];
+                dzz[id] = -g.qz[k] * g.qz[k] * psiq[id];
+            }
+    ict(&g, dxx, dxx);                                       /* :1331-1345 */
+    ict(&g, dyy, dyy);
+    ict(&g, dzz, dzz);
+    ict(&g, dtp, dtp);
+    orc_surface_track_grad(g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, psi, gx, gy, gz, dxx, dyy, dzz, dtp, surf, vel,
+                           curv_h, curv_k);
+    free(psi); free(psiq); free(gx); free(gy); free(gz); free(vx); free(vy); free(vz);
+    free(sp2); free(sx); free(sy); free(sz); free(nr); free(nq);
     grid_free(&g);
 }
 
@@ -599,8 +661,10 @@ void orc_quasi_step(const orc_params *p, orc_quasi_state *st, int nsteps, double
                     st->rhoq[id] = sc * st->rhoq[id] * exp(-sig2 * mq2c_at(&g, i, j, k) * 2);
                 }
         ict(&g, st->rhoq, rho);
-        /* divv and lap(rho f) :1365-1438 */
-        if (nloop > p->div_switch) {
+        /* divv and lap(rho f) :1365-1438; load == 1 starts with rho_old = rho
+         * and uses both from the first step */
+        if (p->load == 1 && nloop == 1) memcpy(st->rho_old, rho, sizeof(double) * n);
+        if (nloop > p->div_switch || p->load == 1) {
             for (long id = 0; id < n; ++id) {
                 st->divv[id] = -((rho[id] - st->rho_old[id]) / dt + vx[id] * grx[id] + vy[id] * gry[id] +
                                  vz[id] * grz[id]) / (0.5 * (rho[id] + st->rho_old[id]));

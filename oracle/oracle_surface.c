@@ -116,9 +116,11 @@ static void diagnostics(long c, int j, int k, int k2, long index, int ny, int nz
                 pow(g, 4);
 }
 
+/* g_in: gradient arrays to use (the advective program's spectral grad psi),
+ * or NULL for the finite differences of psi */
 static void track(int envelope, int nx, int ny, int nz, double dx, double dy, double dz, const double *psi,
-                  const double *dxx, const double *dyy, const double *dzz, const double *dtp, double *surf,
-                  double *vel, double *curv_h, double *curv_k)
+                  const double *const *g_in, const double *dxx, const double *dyy, const double *dzz,
+                  const double *dtp, double *surf, double *vel, double *curv_h, double *curv_k)
 {
     const long n = (long)nx * ny * nz, sx = (long)ny * nz, sy = nz;
     const double tdx = 2 * dx, tdy = 2 * dy, tdz = 2 * dz;
@@ -127,9 +129,9 @@ static void track(int envelope, int nx, int ny, int nz, double dx, double dy, do
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
                 long id = ((long)i * ny + j) * nz + k;
-                gx[id] = fd(psi, id, i, nx, sx, tdx);
-                gy[id] = fd(psi, id, j, ny, sy, tdy);
-                gz[id] = fd(psi, id, k, nz, 1, tdz);
+                gx[id] = g_in ? g_in[0][id] : fd(psi, id, i, nx, sx, tdx);
+                gy[id] = g_in ? g_in[1][id] : fd(psi, id, j, ny, sy, tdy);
+                gz[id] = g_in ? g_in[2][id] : fd(psi, id, k, nz, 1, tdz);
             }
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j) {
@@ -190,12 +192,21 @@ void orc_surface_track(int nx, int ny, int nz, double dx, double dy, double dz, 
                        const double *dxx, const double *dyy, const double *dzz, const double *dtp,
                        double *surf, double *vel, double *curv_h, double *curv_k)
 {
-    track(0, nx, ny, nz, dx, dy, dz, psi, dxx, dyy, dzz, dtp, surf, vel, curv_h, curv_k);
+    track(0, nx, ny, nz, dx, dy, dz, psi, NULL, dxx, dyy, dzz, dtp, surf, vel, curv_h, curv_k);
 }
 
 void orc_surface_track_env(int nx, int ny, int nz, double dx, double dy, double dz, const double *psi,
                            const double *dxx, const double *dyy, const double *dzz, const double *dtp,
                            double *surf, double *vel, double *curv_h, double *curv_k)
 {
-    track(1, nx, ny, nz, dx, dy, dz, psi, dxx, dyy, dzz, dtp, surf, vel, curv_h, curv_k);
+    track(1, nx, ny, nz, dx, dy, dz, psi, NULL, dxx, dyy, dzz, dtp, surf, vel, curv_h, curv_k);
+}
+
+void orc_surface_track_grad(int nx, int ny, int nz, double dx, double dy, double dz, const double *psi,
+                            const double *gx, const double *gy, const double *gz, const double *dxx,
+                            const double *dyy, const double *dzz, const double *dtp, double *surf, double *vel,
+                            double *curv_h, double *curv_k)
+{
+    const double *g[3] = {gx, gy, gz};
+    track(0, nx, ny, nz, dx, dy, dz, psi, g, dxx, dyy, dzz, dtp, surf, vel, curv_h, curv_k);
 }

@@ -27,6 +27,13 @@ FIELD_PSI, FIELD_PSI_HAT, FIELD_N_HAT = 0, 1, 2
 FIELD_GRAD_X, FIELD_GRAD_Y, FIELD_GRAD_Z = 3, 4, 5
 FIELD_VEL_X, FIELD_VEL_Y, FIELD_VEL_Z = 6, 7, 8
 FIELD_PRESSURE, FIELD_RHO, FIELD_ENERGY, FIELD_DIVV_HAT = 9, 10, 11, 12
+FIELD_PSI_HAT_OLD, FIELD_RHO_OLD = 13, 14
+FIELD_GRAD_RHO_X, FIELD_GRAD_RHO_Y, FIELD_GRAD_RHO_Z = 15, 16, 17
+FIELD_SPEC_XY = 18
+FIELD_VSOL_X, FIELD_VSOL_Y, FIELD_VSOL_Z = 19, 20, 21
+FIELD_FPROJ_X, FIELD_FPROJ_Y, FIELD_FPROJ_Z = 22, 23, 24
+FIELD_FORCE_X, FIELD_FORCE_Y, FIELD_FORCE_Z = 25, 26, 27
+FIELD_DIVV = 28
 
 RED_L1, RED_MASS, RED_ENERGY = 0, 1, 2
 
@@ -40,6 +47,8 @@ EXPORTS = (
     "smad_ic_add_noise", "smad_default_params", "smad_r2r_3d", "smad_time_steps",
     "smad_profile_steps", "smad_kernel_class_name", "smad_forward_tape", "smad_adjoint", "smad_adjoint_timing", "smad_set_tape_budget", "smad_surface",
     "smad_surface_envelope", "smad_r2r_3d_noalloc", "smad_noalloc_last_error",
+    "smad_state_fields", "smad_set_steps_taken", "smad_ic_tilted_slab", "smad_ic_two_bumps",
+    "smad_config_params", "smad_config_ic",
 )
 
 REDFT10, REDFT01, RODFT10, RODFT01 = 0, 1, 2, 3
@@ -55,7 +64,7 @@ class CParams(ctypes.Structure):
         ("dt", ctypes.c_double), ("nu", ctypes.c_double), ("lambda_", ctypes.c_double),
         ("rho0", ctypes.c_double), ("kp", ctypes.c_double), ("rho_m", ctypes.c_double),
         ("filter_sigma", ctypes.c_double), ("div_switch", ctypes.c_int),
-        ("precision", ctypes.c_int),
+        ("precision", ctypes.c_int), ("load", ctypes.c_int),
     ]
 
 
@@ -82,6 +91,7 @@ class Params:
     filter_sigma: float = 1.57
     div_switch: int = 200
     precision: int = FP64
+    load: int = 0
 
     def c(self) -> CParams:
         return CParams(**{f.name: getattr(self, f.name) for f in dc_fields(self)})
@@ -92,9 +102,23 @@ class Params:
 
     @classmethod
     def default(cls, model: int, n: int) -> "Params":
+        """The reference program's own constants for the model."""
         cp = CParams()
         lib().smad_default_params(model, n, ctypes.byref(cp))
         return cls(**{f.name: getattr(cp, f.name) for f in dc_fields(cls)})
+
+    @classmethod
+    def config(cls, config: int, n: int | None = None) -> "Params":
+        """The benchmark configuration's parameters (0-4), optionally on an
+        n^3 grid instead of its own."""
+        cp = CParams()
+        rc = lib().smad_config_params(config, ctypes.byref(cp))
+        if rc != OK:
+            raise SmadError(rc, f"unknown configuration {config}")
+        p = cls(**{f.name: getattr(cp, f.name) for f in dc_fields(cls)})
+        if n is not None:
+            p.nx = p.ny = p.nz = n
+        return p
 
 
 _LIB = None
@@ -130,6 +154,11 @@ def lib() -> ctypes.CDLL:
         L.smad_ic_curved.argtypes = [P(CParams), ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, dp]
         L.smad_ic_add_noise.argtypes = [P(CParams), ctypes.c_double, ctypes.c_ulonglong, dp]
         L.smad_default_params.argtypes = [ctypes.c_int, ctypes.c_int, P(CParams)]
+        L.smad_config_params.argtypes = [ctypes.c_int, P(CParams)]
+        L.smad_config_ic.argtypes = [ctypes.c_int, P(CParams), dp]
+        L.smad_ic_two_bumps.argtypes = [P(CParams), ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, dp]
+        L.smad_ic_tilted_slab.argtypes = [P(CParams), ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_ulonglong, dp, dp]
         L.smad_time_steps.argtypes = [vp, ctypes.c_int, dp]
         L.smad_forward_tape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.smad_adjoint.argtypes = [vp, dp, dp, dp]
@@ -143,12 +172,14 @@ def lib() -> ctypes.CDLL:
         L.smad_kernel_class_name.restype = ctypes.c_char_p
         L.smad_r2r_3d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int),
                                   ctypes.c_int, dp, dp]
+        L.smad_state_fields.argtypes = [vp, P(ctypes.c_int), ctypes.c_int]
+        L.smad_set_steps_taken.argtypes = [vp, ctypes.c_long]
         L.smad_r2r_3d_noalloc.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int),
                                           ctypes.c_int, vp, vp, vp]
         L.smad_noalloc_last_error.argtypes = []
         L.smad_noalloc_last_error.restype = ctypes.c_char_p
         for fn in ("smad_ic_focal_conic", "smad_ic_slab", "smad_ic_curved", "smad_ic_add_noise",
-                   "smad_default_params"):
+                   "smad_default_params", "smad_ic_two_bumps", "smad_ic_tilted_slab"):
             getattr(L, fn).restype = None
         _LIB = L
     return _LIB
@@ -175,6 +206,29 @@ def ic_slab(p: Params, amp=0.6, noise=1e-3, seed=0) -> np.ndarray:
 def ic_curved(p: Params, amp=0.6, noise=1e-3, seed=1) -> np.ndarray:
     out = np.empty(p.shape, dtype=np.float64)
     lib().smad_ic_curved(ctypes.byref(p.c()), amp, noise, seed, _dptr(out))
+    return out
+
+
+def ic_two_bumps(p: Params, amp=1.328, noise=1e-3, seed=2) -> np.ndarray:
+    out = np.empty(p.shape, dtype=np.float64)
+    lib().smad_ic_two_bumps(ctypes.byref(p.c()), amp, noise, seed, _dptr(out))
+    return out
+
+
+def ic_tilted_slab(p: Params, amp=1.34164, max_tilt_deg=15.0, noise=1e-3, seed=3):
+    """Returns (psi, tilt in degrees)."""
+    out = np.empty(p.shape, dtype=np.float64)
+    t = ctypes.c_double()
+    lib().smad_ic_tilted_slab(ctypes.byref(p.c()), amp, max_tilt_deg, noise, seed, _dptr(out), ctypes.byref(t))
+    return out, t.value
+
+
+def config_ic(config: int, p: Params) -> np.ndarray:
+    """The benchmark configuration's initial condition on the grid of p."""
+    out = np.empty(p.shape, dtype=np.float64)
+    rc = lib().smad_config_ic(config, ctypes.byref(p.c()), _dptr(out))
+    if rc != OK:
+        raise SmadError(rc, f"unknown configuration {config}")
     return out
 
 
@@ -326,6 +380,25 @@ class Stepper:
 
     def sync(self):
         self._check(lib().smad_sync(self._h))
+
+    def state_fields(self):
+        """The field ids of the model's full restart state, in restore order."""
+        buf = (ctypes.c_int * 32)()
+        nf = lib().smad_state_fields(self._h, buf, 32)
+        if nf < 0:
+            raise SmadError(nf, "smad_state_fields")
+        return [buf[i] for i in range(nf)]
+
+    def save_state(self):
+        """Full restart state: (step count, {field id: array})."""
+        return self.steps_taken, {f: self.fetch(f) for f in self.state_fields()}
+
+    def load_state(self, state):
+        """Restore a state from :meth:`save_state` (same parameters)."""
+        steps, arrays = state
+        for f in self.state_fields():
+            self.set_field(f, arrays[f])
+        self._check(lib().smad_set_steps_taken(self._h, int(steps)))
 
     @property
     def stream_ptr(self) -> int:

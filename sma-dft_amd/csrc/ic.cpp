@@ -114,6 +114,13 @@ void smad_ic_curved(const smad_params* p, double amp, double noise, unsigned lon
     if (noise != 0.0) smad_ic_add_noise(p, noise, seed, psi);
 }
 
+// The reference programs' own constants for each model (their argv-free
+// defaults): plain no-adv/cosNoAdvTrackMult.cpp:130-172; advective
+// adv/cosAdvTrackMult.cpp:163-183 (Nw = 16, dt = 5e-4; eps = -0.01 ep_arg
+// from argv); quasi quasi/quasi.cpp:230-252 (Nw = 8, dt = 1e-3, rho_0 = 1,
+// kp = 0, rho_m = rho_s/2 = 0.5, lambda = -2 nu/3, divvSwitch = 200);
+// conserved no-adv/cosNoAdvConserv.cpp:156-163 and adv/cosAdvConSingle.cpp:
+// 169-180 (Nw = 16, dt = 5e-4).
 void smad_default_params(int model, int n, smad_params* o)
 {
     smad_params p{};
@@ -134,25 +141,120 @@ void smad_default_params(int model, int n, smad_params* o)
     p.filter_sigma = 1.57;
     p.div_switch = 200;
     p.precision = SMAD_FP64;
-    // explicit u . grad psi with u driven by mu grad psi (mu holds fourth
-    // derivatives) diverges at dt = 0.05 for a persistent smectic (nw = 8);
-    // 0.02 is stable
-    if (model == SMAD_MODEL_ADV) p.dt = 0.02;
-    if (model == SMAD_MODEL_PLAIN_CONSERVED || model == SMAD_MODEL_ADV_CONSERVED) {
-        // no-adv/cosNoAdvConserv.cpp:156-163, adv/cosAdvConSingle.cpp:169-180: nw = 16, dt = 5e-4
+    p.load = 0;
+    if (model == SMAD_MODEL_ADV || model == SMAD_MODEL_PLAIN_CONSERVED || model == SMAD_MODEL_ADV_CONSERVED) {
         p.nw = 16;
         p.dt = 0.0005;
     }
-    if (model == SMAD_MODEL_QUASI) {
-        // the reference's quasi time step (quasi/quasi.cpp:251); the
-        // stabilised BDF2/AB2 scheme with the full linear operator in the
-        // explicit term diverges at dt = 0.02 for nw = 8
-        p.dt = 0.001;
+    if (model == SMAD_MODEL_QUASI) p.dt = 0.001;
+    *o = p;
+}
+
+// The benchmark configurations (0-4): see the header.
+int smad_config_params(int config, smad_params* o)
+{
+    if (!o || config < 0 || config > 4) return SMAD_EINVAL;
+    smad_params p{};
+    switch (config) {
+    case 0: smad_default_params(SMAD_MODEL_PLAIN, 128, &p); break;
+    case 1:
+    case 4: smad_default_params(SMAD_MODEL_PLAIN, 256, &p); break;
+    case 2:
+        smad_default_params(SMAD_MODEL_ADV, 256, &p);
+        p.nw = 8;     // dx = 2 pi/8 as the plain configurations
+        p.dt = 0.02;  // the specified 0.05 diverges (explicit u . grad psi)
+        break;
+    case 3:
+        smad_default_params(SMAD_MODEL_QUASI, 384, &p);
+        // rho_s/rho_iso = 1.0/0.9 through the reference's density law
+        // rho = kp |psi| q0 + rho0 at the smectic amplitude 1.34164
         p.rho0 = 0.9;
         p.kp = 0.1 / 1.34164;
         p.rho_m = (p.kp * 1.34164 + p.rho0) / 2;
+        // the specified 0.02 diverges; the reference's own step
+        p.dt = 0.001;
+        break;
     }
     *o = p;
+    return SMAD_OK;
+}
+
+// Two bumped interfaces: the slab |x - Lx/2| < Lx/4 of layers amp cos(q0 k dz)
+// whose lower and upper interfaces x = Lx/4 - h1(y, z) and x = 3Lx/4 + h2(y, z)
+// each carry one Gaussian bump h = H exp(-((y - yc)^2 + (z - zc)^2)/(2 w^2)),
+// bulging out of the slab, with H in [0.5, 1.5] layer spacings (2 pi/q0),
+// w in [Ly/16, Ly/8] and the centre (yc, zc) in the middle half of the
+// (y, z) box, all drawn from the seed; plus noise.
+void smad_ic_two_bumps(const smad_params* p, double amp, double noise, unsigned long long seed, double* psi)
+{
+    const Box b = box_of(p);
+    const double lam = 2.0 * M_PI / p->q0;
+    double H[2], w[2], yc[2], zc[2];
+    for (int m = 0; m < 2; ++m) {
+        auto u = [&](int q) { return 0.5 * (noise_at(seed ^ 0xB0B5ULL, (uint64_t)(4 * m + q), 1.0) + 1.0); };
+        H[m] = (0.5 + u(0)) * lam;
+        w[m] = b.ly * (1.0 + u(1)) / 16.0;
+        yc[m] = b.ly * (0.25 + 0.5 * u(2));
+        zc[m] = b.lz * (0.25 + 0.5 * u(3));
+    }
+    for (int j = 0; j < p->ny; ++j)
+        for (int k = 0; k < p->nz; ++k) {
+            const double y = j * b.dy, z = k * b.dz;
+            double h[2];
+            for (int m = 0; m < 2; ++m) {
+                const double r2 = (y - yc[m]) * (y - yc[m]) + (z - zc[m]) * (z - zc[m]);
+                h[m] = H[m] * std::exp(-r2 / (2.0 * w[m] * w[m]));
+            }
+            const double lo = b.lx / 4 - h[0], hi = 3 * b.lx / 4 + h[1];
+            for (int i = 0; i < p->nx; ++i) {
+                const double x = i * b.dx;
+                psi[lin(p, i, j, k)] = (x > lo && x < hi) ? amp * std::cos(p->q0 * k * b.dz) : 0.0;
+            }
+        }
+    if (noise != 0.0) smad_ic_add_noise(p, noise, seed, psi);
+}
+
+// A planar smectic slab |x - Lx/2| < Lx/4 with tilted layers:
+// amp cos(q0 n . r), r = (i dx, j dy, k dz), n = (sin t cos f, sin t sin f, cos t),
+// the tilt t uniform in [0, max_tilt_deg] degrees and the azimuth f uniform in
+// [0, 2 pi), both drawn from the seed; plus noise.  *tilt_deg (if not null)
+// receives t.
+void smad_ic_tilted_slab(const smad_params* p, double amp, double max_tilt_deg, double noise,
+                         unsigned long long seed, double* psi, double* tilt_deg)
+{
+    const Box b = box_of(p);
+    const double ut = 0.5 * (noise_at(seed ^ 0x711ULL, 0, 1.0) + 1.0);
+    const double uf = 0.5 * (noise_at(seed ^ 0x711ULL, 1, 1.0) + 1.0);
+    const double t = ut * max_tilt_deg * M_PI / 180.0, f = uf * 2.0 * M_PI;
+    const double nx = std::sin(t) * std::cos(f), ny = std::sin(t) * std::sin(f), nz = std::cos(t);
+    for (int i = 0; i < p->nx; ++i) {
+        const double x = i * b.dx;
+        const bool in = std::fabs(x - b.lx / 2) < b.lx / 4;
+        for (int j = 0; j < p->ny; ++j)
+            for (int k = 0; k < p->nz; ++k)
+                psi[lin(p, i, j, k)] =
+                    in ? amp * std::cos(p->q0 * (nx * x + ny * j * b.dy + nz * k * b.dz)) : 0.0;
+    }
+    if (tilt_deg) *tilt_deg = t * 180.0 / M_PI;
+    if (noise != 0.0) smad_ic_add_noise(p, noise, seed, psi);
+}
+
+// The initial condition of each benchmark configuration, on the grid of p
+// (which may be smaller than the configuration's own): 0 the slab
+// (amp 0.6, seed 0); 1 and 4 the curved layers (amp 0.6, seed 1); 2 two
+// bumped interfaces (amp 1.328, seed 2); 3 the tilted slab (amp 1.34164,
+// tilt up to 15 degrees, seed 3); noise 1e-3 each.
+int smad_config_ic(int config, const smad_params* p, double* psi)
+{
+    if (!p || !psi) return SMAD_EINVAL;
+    switch (config) {
+    case 0: smad_ic_slab(p, 0.6, 1e-3, 0, psi); return SMAD_OK;
+    case 1:
+    case 4: smad_ic_curved(p, 0.6, 1e-3, 1, psi); return SMAD_OK;
+    case 2: smad_ic_two_bumps(p, 1.328, 1e-3, 2, psi); return SMAD_OK;
+    case 3: smad_ic_tilted_slab(p, 1.34164, 15.0, 1e-3, 3, psi, nullptr); return SMAD_OK;
+    default: return SMAD_EINVAL;
+    }
 }
 
 }  // extern "C"

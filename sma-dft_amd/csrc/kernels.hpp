@@ -1095,6 +1095,53 @@ __global__ __launch_bounds__(WG) void k_q_pvel(const T* __restrict__ Sx, const T
     }
 }
 
+// Save-block outputs of the projection (quasi/quasi.cpp:1719-1725 and the
+// analogous y, z forms), spectral, component `comp` of:
+//   what 0: the solenoidal velocity  vsol = CM1 (S - CM2 (q . S))
+//   what 1: the projected force      f    = s (S - CM2 (q . S))
+// with the truncations and shifted reads of k_q_pvel; FP64 arithmetic.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_q_proj_out(const T* __restrict__ Sx, const T* __restrict__ Sy,
+                                                   const T* __restrict__ Sz, T* __restrict__ out, int comp,
+                                                   int what, Dims d, QuasiCoef qc)
+{
+    const PlainCoef& cf = qc.pc;
+    const long n = d.total(), sx = (long)d.ny * d.nz, sy = d.nz;
+    const double nu = qc.nu, sc = cf.scale;
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
+        double v = 0.0;
+        if (comp == 0 && i < d.nx - 1) {
+            double qsx = (i + 1) * cf.qfx;
+            double s2 = j != 0 ? (double)Sy[id + sx - sy] : 0.0;
+            double s3 = k != 0 ? (double)Sz[id + sx - 1] : 0.0;
+            double mq2 = qsx * qsx + qy * qy + qz * qz;
+            double dot = qsx * (double)Sx[id] + qy * s2 + qz * s3;
+            double pr = (double)Sx[id] - qsx / mq2 * dot;
+            v = what == 0 ? sc / (nu * mq2) * pr : sc * pr;
+        } else if (comp == 1 && j < d.ny - 1 && k < d.nz - 1) {
+            double qsy = (j + 1) * cf.qfy;
+            double s1 = i != 0 ? (double)Sx[id - sx + sy] : 0.0;
+            double s3 = k != 0 ? (double)Sz[id + sy - 1] : 0.0;
+            double mq2 = qx * qx + qsy * qsy + qz * qz;
+            double dot = qx * s1 + qsy * (double)Sy[id] + qz * s3;
+            double pr = (double)Sy[id] - qsy / mq2 * dot;
+            v = what == 0 ? sc / (nu * mq2) * pr : sc * pr;
+        } else if (comp == 2 && j < d.ny - 1 && k < d.nz - 1) {
+            double qsz = (k + 1) * cf.qfz;
+            double s1 = i != 0 ? (double)Sx[id - sx + 1] : 0.0;
+            double s2 = j != 0 ? (double)Sy[id - sy + 1] : 0.0;
+            double mq2 = qx * qx + qy * qy + qsz * qsz;
+            double dot = qx * s1 + qy * s2 + qsz * (double)Sz[id];
+            double pr = (double)Sz[id] - qsz / mq2 * dot;
+            v = what == 0 ? sc / (nu * mq2) * pr : sc * pr;
+        }
+        out[id] = T(v);
+    }
+}
+
 // REDFT10_z of rho = kp sqrt(q0^2 psi^2 + |grad psi|^2) + rho0 (:1322-1344),
 // the density law formed while the z lines are loaded.
 template <typename T, int N>
@@ -1193,6 +1240,13 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_nl_z(const T* __restrict__
 
 // a <- |q|^2 a on the spectral grid (the conserved variant's N_hat^0,
 // no-adv/cosNoAdvConserv.cpp:690-697)
+// b = f a (FP64 arithmetic)
+template <typename T>
+__global__ __launch_bounds__(WG) void k_scale_copy(const T* __restrict__ a, T* __restrict__ b, long n, double f)
+{
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) b[id] = T(f * (double)a[id]);
+}
+
 template <typename T>
 __global__ __launch_bounds__(WG) void k_mul_mq2(T* __restrict__ a, Dims d, PlainCoef cf)
 {
@@ -1248,14 +1302,24 @@ __device__ __forceinline__ double fd_at(const T* __restrict__ f, long idx, int c
     return 2.0 * ((double)f[idx] - (double)f[idx - stride]) / th;
 }
 
-// Finite-difference gradient component `a` (0 x, 1 y, 2 z) of psi at flat
-// index idx; an index outside the grid (reachable through the reference's
-// flat idx +- 1 reads at the top of a column) contributes 0.
+// Gradient arrays the tracker reads instead of differencing psi (the
+// advective program's spectral grad psi, adv/cosAdvTrackMult.cpp:1431-1435);
+// all null for the finite-difference gradient of the plain programs.
+template <typename T>
+struct GradSrc {
+    const T* g[3];
+};
+
+// Gradient component `a` (0 x, 1 y, 2 z) of psi at flat index idx: from the
+// arrays of gs when given, else the finite difference; an index outside the
+// grid (reachable through the reference's flat idx +- 1 reads at the top of
+// a column) contributes 0.
 template <typename T>
 __device__ __forceinline__ double grad_at(const T* __restrict__ psi, long idx, int a, const Dims& d,
-                                          const SurfCoef& sc)
+                                          const SurfCoef& sc, const GradSrc<T>& gs)
 {
     if (idx < 0 || idx >= d.total()) return 0.0;
+    if (gs.g[0]) return (double)gs.g[a][idx];
     int i, j, k;
     unflat(idx, d, i, j, k);
     if (a == 0) return fd_at(psi, idx, i, d.nx, (long)d.ny * d.nz, sc.tdx);
@@ -1285,14 +1349,14 @@ __device__ __forceinline__ double psi_flat(const T* __restrict__ psi, long idx, 
 //   curv_k = [gz^2 (dxx dyy - dxy^2) + gx^2 (dyy dzz - dyz^2) + gy^2 (dxx dzz - dxz^2)
 //             + 2 (gy dxy (gz dxz - gx dzz) + gx dxz (gy dyz - gz dyy)
 //                  + gz dyz (gx dxy - gy dxx))] / |g|^4
-// with g the finite-difference gradient, dxx.. the spectral second
+// with g the gradient (finite differences of psi, or the arrays of gs), dxx.. the spectral second
 // derivatives, dxy the y-difference of gx at k2 and dxz, dyz the flat
 // index +-1 differences of gx, gy around the point with the edge rule of k
 // (both as the reference).  Columns without a surface get surf = -1 and zeros.
 template <typename T>
 __global__ __launch_bounds__(WG) void k_surf_track(const T* __restrict__ psi, const T* __restrict__ dxx,
                                                    const T* __restrict__ dyy, const T* __restrict__ dzz,
-                                                   const T* __restrict__ dtp, Dims d, SurfCoef sc,
+                                                   const T* __restrict__ dtp, GradSrc<T> gs, Dims d, SurfCoef sc,
                                                    double* __restrict__ surf, double* __restrict__ vel,
                                                    double* __restrict__ ch, double* __restrict__ ck)
 {
@@ -1341,28 +1405,28 @@ __global__ __launch_bounds__(WG) void k_surf_track(const T* __restrict__ psi, co
         }
     }
     if (idx >= 0) {
-        const double gx = grad_at(psi, idx, 0, d, sc), gy = grad_at(psi, idx, 1, d, sc),
-                     gz = grad_at(psi, idx, 2, d, sc);
+        const double gx = grad_at(psi, idx, 0, d, sc, gs), gy = grad_at(psi, idx, 1, d, sc, gs),
+                     gz = grad_at(psi, idx, 2, d, sc, gs);
         const double g = sqrt(gx * gx + gy * gy + gz * gz);
         const long ck2 = base + k2;
         const long sy = d.nz;
         double dxy;
         if (j > 0 && j < d.ny - 1)
-            dxy = (grad_at(psi, ck2 + sy, 0, d, sc) - grad_at(psi, ck2 - sy, 0, d, sc)) / sc.tdy;
+            dxy = (grad_at(psi, ck2 + sy, 0, d, sc, gs) - grad_at(psi, ck2 - sy, 0, d, sc, gs)) / sc.tdy;
         else if (j == 0)
-            dxy = 2.0 * (grad_at(psi, ck2 + sy, 0, d, sc) - grad_at(psi, ck2, 0, d, sc)) / sc.tdy;
+            dxy = 2.0 * (grad_at(psi, ck2 + sy, 0, d, sc, gs) - grad_at(psi, ck2, 0, d, sc, gs)) / sc.tdy;
         else
-            dxy = 2.0 * (grad_at(psi, ck2, 0, d, sc) - grad_at(psi, ck2 - sy, 0, d, sc)) / sc.tdy;
+            dxy = 2.0 * (grad_at(psi, ck2, 0, d, sc, gs) - grad_at(psi, ck2 - sy, 0, d, sc, gs)) / sc.tdy;
         double dxz, dyz;
         if (k > 0 && k < d.nz - 1) {
-            dxz = (grad_at(psi, idx + 1, 0, d, sc) - grad_at(psi, idx - 1, 0, d, sc)) / sc.tdz;
-            dyz = (grad_at(psi, idx + 1, 1, d, sc) - grad_at(psi, idx - 1, 1, d, sc)) / sc.tdz;
+            dxz = (grad_at(psi, idx + 1, 0, d, sc, gs) - grad_at(psi, idx - 1, 0, d, sc, gs)) / sc.tdz;
+            dyz = (grad_at(psi, idx + 1, 1, d, sc, gs) - grad_at(psi, idx - 1, 1, d, sc, gs)) / sc.tdz;
         } else if (k == 0) {
-            dxz = 2.0 * (grad_at(psi, idx + 1, 0, d, sc) - gx) / sc.tdz;
-            dyz = 2.0 * (grad_at(psi, idx + 1, 1, d, sc) - gy) / sc.tdz;
+            dxz = 2.0 * (grad_at(psi, idx + 1, 0, d, sc, gs) - gx) / sc.tdz;
+            dyz = 2.0 * (grad_at(psi, idx + 1, 1, d, sc, gs) - gy) / sc.tdz;
         } else {
-            dxz = 2.0 * (gx - grad_at(psi, idx - 1, 0, d, sc)) / sc.tdz;
-            dyz = 2.0 * (gy - grad_at(psi, idx - 1, 1, d, sc)) / sc.tdz;
+            dxz = 2.0 * (gx - grad_at(psi, idx - 1, 0, d, sc, gs)) / sc.tdz;
+            dyz = 2.0 * (gy - grad_at(psi, idx - 1, 1, d, sc, gs)) / sc.tdz;
         }
         const double a = (double)dxx[idx], b = (double)dyy[idx], c = (double)dzz[idx];
         const double g2 = g * g;
@@ -1782,7 +1846,8 @@ struct Launch {
     static void surf_spec(hipStream_t s, const T* ph, const T* nq, T* dxx, T* dyy, T* dzz, T* dt, Dims d,
                           const PlainCoef& cf);
     static void surf_track(hipStream_t s, const T* psi, const T* dxx, const T* dyy, const T* dzz, const T* dt,
-                           Dims d, const SurfCoef& sc, double* surf, double* vel, double* ch, double* ck);
+                           const GradSrc<T>& gs, Dims d, const SurfCoef& sc, double* surf, double* vel, double* ch,
+                           double* ck);
     static void q_rho_z(hipStream_t s, const T* psi, const T* gx, const T* gy, const T* gz, T* out, Dims d,
                         const QuasiCoef& qc, const Tw<T>& tw);
     static void q_nl_z(hipStream_t s, const T* mu, const T* ux, const T* uy, const T* uz, const T* gpx,
@@ -1810,6 +1875,9 @@ struct Launch {
                         const T* grz, T* e, T* mu, T* sx, T* sy, T* sz, Dims d, const QuasiCoef& qc);
     static void q_pvel(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, const T* dv,
                        double* ph, T* ux, T* uy, T* uz, Dims d, const QuasiCoef& qc);
+    static void q_proj_out(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, T* out, int comp, int what, Dims d,
+                           const QuasiCoef& qc);
+    static void scale_copy(hipStream_t s, const T* a, T* b, long n, double f);
     static void q_x(hipStream_t s, const T* nl_zy, T* ph, T* pho, T* nlo, Dims d, const QuasiCoef& qc, int first,
                     const Tw<T>& tw);
     static void l1(hipStream_t s, const T* a, const T* b, long n, double* part, int nparts,

@@ -142,6 +142,19 @@ void Launch<T>::mul_mq2(hipStream_t s, T* a, Dims d, const PlainCoef& cf)
 }
 
 template <typename T>
+void Launch<T>::q_proj_out(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, T* out, int comp, int what, Dims d,
+                           const QuasiCoef& qc)
+{
+    hipLaunchKernelGGL(k_q_proj_out<T>, flat_grid(d.total()), dim3(WG), 0, s, Sx, Sy, Sz, out, comp, what, d, qc);
+}
+
+template <typename T>
+void Launch<T>::scale_copy(hipStream_t s, const T* a, T* b, long n, double f)
+{
+    hipLaunchKernelGGL(k_scale_copy<T>, flat_grid(n), dim3(WG), 0, s, a, b, n, f);
+}
+
+template <typename T>
 void Launch<T>::surf_spec(hipStream_t s, const T* ph, const T* nq, T* dxx, T* dyy, T* dzz, T* dt, Dims d,
                           const PlainCoef& cf)
 {
@@ -150,11 +163,12 @@ void Launch<T>::surf_spec(hipStream_t s, const T* ph, const T* nq, T* dxx, T* dy
 
 template <typename T>
 void Launch<T>::surf_track(hipStream_t s, const T* psi, const T* dxx, const T* dyy, const T* dzz, const T* dt,
-                           Dims d, const SurfCoef& sc, double* surf, double* vel, double* ch, double* ck)
+                           const GradSrc<T>& gs, Dims d, const SurfCoef& sc, double* surf, double* vel, double* ch,
+                           double* ck)
 {
     const long cols = (long)d.nx * d.ny;
     hipLaunchKernelGGL(k_surf_track<T>, dim3((unsigned)((cols + WG - 1) / WG)), dim3(WG), 0, s, psi, dxx, dyy, dzz,
-                       dt, d, sc, surf, vel, ch, ck);
+                       dt, gs, d, sc, surf, vel, ch, ck);
 }
 
 template <typename T>

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <string>
 #include <vector>
@@ -160,11 +161,26 @@ struct EngineBase {
     virtual void set_tape_budget(double) {}
     virtual void surface(double*, double*, double*, double*, int)
     {
-        throw ApiError{SMAD_EINVAL, "surface diagnostics are defined for the plain model"};
+        throw ApiError{SMAD_EINVAL, "surface diagnostics are defined for the plain and advective models"};
     }
     double last_fwd_ms = 0.0, last_adj_ms = 0.0;
-    virtual void adjoint(const double* g_psik, double* dpsi0, double* dparams) = 0;
+    // returns SMAD_ENONFINITE when a gradient holds NaN or Inf
+    virtual int adjoint(const double* g_psik, double* dpsi0, double* dparams) = 0;
+    // the fields of the full restart state, in the order to set them back
+    virtual int state_fields(int* f, int max) const = 0;
+    // loop counter of schedules that depend on the step (the quasi switch)
+    virtual void set_loop_count(long) {}
 };
+
+int list_fields(std::initializer_list<int> ids, int* f, int max)
+{
+    int c = 0;
+    for (int id : ids) {
+        if (c < max) f[c] = id;
+        ++c;
+    }
+    return c;
+}
 
 template <typename T>
 int download(hipStream_t s, const T* dev, size_t n, double* out)
@@ -467,7 +483,7 @@ struct PlainEngine : EngineBase {
         fresh = false;
     }
 
-    void adjoint(const double* g_psik, double* dpsi0, double* dparams) override
+    int adjoint(const double* g_psik, double* dpsi0, double* dparams) override
     {
         if (tape_K < 1) throw ApiError{SMAD_ESTATE, "adjoint needs a preceding forward_tape"};
         const int K = tape_K, ck = tape_ck, nseg = tape_nseg;
@@ -512,8 +528,11 @@ struct PlainEngine : EngineBase {
         last_adj_ms = ev.ms();
         double h[4];
         HIPCHK(hipMemcpyAsync(h, gsum.p, sizeof(h), hipMemcpyDeviceToHost, s));
-        (void)download(s, psi.p, n, dpsi0);
-        for (int q = 0; q < 4; ++q) dparams[q] = h[q];
+        int rc = download(s, psi.p, n, dpsi0);
+        for (int q = 0; q < 4; ++q) {
+            dparams[q] = h[q];
+            if (!std::isfinite(h[q])) rc = SMAD_ENONFINITE;
+        }
         // leave the context at the end of the taped forward run
         copy_dd(psihat.p, end_ph.p);
         copy_dd(nhat.p, end_nh.p);
@@ -523,6 +542,7 @@ struct PlainEngine : EngineBase {
         HIPCHK(hipStreamSynchronize(s));
         psi_valid = false;
         prev_valid = false;
+        return rc;
     }
 
     // ---------------------------------------------------- surface --
@@ -554,7 +574,8 @@ struct PlainEngine : EngineBase {
         const double pi = M_PI;
         const double dx = 2.0 * pi / (p.q0 * p.nw);  // L/N on every axis
         SurfCoef sc{2.0 * dx, 2.0 * dx, 2.0 * dx, envelope ? 1 : 0};
-        Launch<T>::surf_track(s, psi.p, sf_dxx.p, sf_dyy.p, sf_dzz.p, sf_dt.p, d, sc, sf_out.p, sf_out.p + cols,
+        Launch<T>::surf_track(s, psi.p, sf_dxx.p, sf_dyy.p, sf_dzz.p, sf_dt.p, GradSrc<T>{}, d, sc, sf_out.p,
+                              sf_out.p + cols,
                               sf_out.p + 2 * cols, sf_out.p + 3 * cols);
         LAUNCHCHK();
         double* outs[4] = {surf, vel, ch, ck};
@@ -578,8 +599,14 @@ struct PlainEngine : EngineBase {
         case SMAD_FIELD_PSI: materialize(); return download(s, psi.p, n, out);
         case SMAD_FIELD_PSI_HAT: return download(s, psihat.p, n, out);
         case SMAD_FIELD_N_HAT: return download(s, nhat.p, n, out);
+        case SMAD_FIELD_SPEC_XY: return download(s, w.p, n, out);
         default: throw ApiError{SMAD_EINVAL, "field not available in the plain model"};
         }
+    }
+
+    int state_fields(int* f, int max) const override
+    {
+        return list_fields({SMAD_FIELD_PSI_HAT, SMAD_FIELD_N_HAT, SMAD_FIELD_SPEC_XY}, f, max);
     }
 
     void set_field(int field, const double* in) override
@@ -601,6 +628,12 @@ struct PlainEngine : EngineBase {
             break;
         case SMAD_FIELD_N_HAT:
             upload(s, nhat.p, n, in);
+            fresh = false;
+            break;
+        case SMAD_FIELD_SPEC_XY:
+            upload(s, w.p, n, in);
+            psi_valid = false;
+            prev_valid = false;
             fresh = false;
             break;
         default: throw ApiError{SMAD_EINVAL, "field cannot be set in the plain model"};
@@ -703,7 +736,8 @@ struct AdvEngine : EngineBase {
         preloop();
     }
 
-    void step_one(bool keep_prev, Prof* prof)
+    // psi, grad psi, the force and the velocity of the current psihat
+    void front(bool keep_prev, Prof* prof)
     {
         auto mk = [&](int c) { if (prof) prof->mark(c); };
         // psi, grad psi and Sp2 = REDFT01^3(-aLin psihat) (:847-991) with
@@ -742,6 +776,12 @@ struct AdvEngine : EngineBase {
         Launch<T>::axis(s, uz.p, uz.p, d, 1, REDFT01, T(1), twy.tw);
         Launch<T>::axis(s, uz.p, uz.p, d, 0, REDFT01, T(1), twx.tw);
         mk(0);
+    }
+
+    void step_one(bool keep_prev, Prof* prof)
+    {
+        auto mk = [&](int c) { if (prof) prof->mark(c); };
+        front(keep_prev, prof);
         if (!ac.pc.conserved) {
             Launch<T>::adv_nl_z(s, psi.p, ux.p, uy.p, uz.p, gx.p, gy.p, gz.p, nr.p, d, ac, twz.tw);
             mk(4);
@@ -831,11 +871,53 @@ struct AdvEngine : EngineBase {
         return h[2] > 0 ? SMAD_ENONFINITE : SMAD_OK;
     }
 
+    int state_fields(int* f, int max) const override
+    {
+        return list_fields({SMAD_FIELD_PSI_HAT, SMAD_FIELD_N_HAT}, f, max);
+    }
+
+    // ---------------------------------------------------- surface --
+    // The save block's aux fields and tracker (adv/cosAdvTrackMult.cpp:
+    // 1311-1347, 1412-1500) for the current psi: psi, grad psi and u of the
+    // current psihat, its N_hat with the advection term (kept apart from the
+    // AB2 history), dTpsi = REDFT01^3(aLin psihat + s N_hat), the spectral
+    // second derivatives, and the crossing scan with the spectral gradient.
+    DevBuf<T> sf_dxx, sf_dyy, sf_dzz, sf_dt, sf_nq;
+    DevBuf<double> sf_out;
+
+    void surface(double* surf, double* vel, double* ch, double* ck, int envelope) override
+    {
+        if (ac.pc.conserved)
+            throw ApiError{SMAD_EINVAL, "surface diagnostics are defined for the plain and advective models"};
+        const size_t cols = (size_t)p.nx * p.ny;
+        if (!sf_nq.p) {
+            for (DevBuf<T>* b : {&sf_dxx, &sf_dyy, &sf_dzz, &sf_dt, &sf_nq}) b->alloc(n);
+            sf_out.alloc(4 * cols);
+        }
+        front(false, nullptr);
+        psi_valid = true;  // psi now holds the current psi
+        Launch<T>::adv_nl_z(s, psi.p, ux.p, uy.p, uz.p, gx.p, gy.p, gz.p, sf_nq.p, d, ac, twz.tw);
+        Launch<T>::axis(s, sf_nq.p, sf_nq.p, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(s, sf_nq.p, sf_nq.p, d, 0, REDFT10, T(1), twx.tw);
+        Launch<T>::surf_spec(s, ph.p, sf_nq.p, sf_dxx.p, sf_dyy.p, sf_dzz.p, sf_dt.p, d, ac.pc);
+        for (DevBuf<T>* b : {&sf_dxx, &sf_dyy, &sf_dzz, &sf_dt}) tr3(b->p, b->p, REDFT01, REDFT01, REDFT01);
+        const double dx = 2.0 * M_PI / (p.q0 * p.nw);
+        SurfCoef sc{2.0 * dx, 2.0 * dx, 2.0 * dx, envelope ? 1 : 0};
+        GradSrc<T> gs{{gx.p, gy.p, gz.p}};
+        Launch<T>::surf_track(s, psi.p, sf_dxx.p, sf_dyy.p, sf_dzz.p, sf_dt.p, gs, d, sc, sf_out.p, sf_out.p + cols,
+                              sf_out.p + 2 * cols, sf_out.p + 3 * cols);
+        LAUNCHCHK();
+        double* outs[4] = {surf, vel, ch, ck};
+        for (int q = 0; q < 4; ++q)
+            HIPCHK(hipMemcpyAsync(outs[q], sf_out.p + q * cols, cols * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+
     void forward_tape(int, int) override
     {
         throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
     }
-    void adjoint(const double*, double*, double*) override
+    int adjoint(const double*, double*, double*) override
     {
         throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
     }
@@ -956,7 +1038,9 @@ struct QuasiEngine : EngineBase {
     {
         auto mk = [&](int c) { if (prof) prof->mark(c); };
         ++nloop;
-        qc.pre_switch = nloop <= p.div_switch ? 1 : 0;
+        // load == 1 (quasi/quasi.cpp:1365-1369): divv and grad^2(rho f) from
+        // the first step, with rho_old = rho at that step
+        qc.pre_switch = (nloop <= p.div_switch && !p.load) ? 1 : 0;
         const SpecCoef sc{qc.pc, qc.sig2};
         // psi, f, grad psi (and lap_hat before the switch) from psihat
         // (:1121-1256, :1411-1421) with shared passes: z from psihat, y from
@@ -992,6 +1076,8 @@ struct QuasiEngine : EngineBase {
         Launch<T>::axis(s, gry2.p, gry2.p, d, 0, REDFT01, T(1), twx.tw);
         Launch<T>::axis(s, grz2.p, grz2.p, d, 0, REDFT01, T(1), twx.tw);
         mk(0);
+        if (p.load && nloop == 1)
+            HIPCHK(hipMemcpyAsync(rho_old.p, rho.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
         Launch<T>::q_divv(s, rho.p, rho_old.p, ux.p, uy.p, uz.p, grx.p, gry.p, grz.p, f.p, divv.p,
                           lap.p, d, qc);
         mk(1);
@@ -1065,15 +1151,79 @@ struct QuasiEngine : EngineBase {
         case SMAD_FIELD_RHO: return download(s, rho.p, n, out);
         case SMAD_FIELD_ENERGY: return download(s, e.p, n, out);
         case SMAD_FIELD_DIVV_HAT: return download(s, divv.p, n, out);
-        default: throw ApiError{SMAD_EINVAL, "unknown field"};
+        case SMAD_FIELD_PSI_HAT_OLD: return download(s, pho.p, n, out);
+        case SMAD_FIELD_RHO_OLD: return download(s, rho_old.p, n, out);
+        case SMAD_FIELD_GRAD_RHO_X: return download(s, grx.p, n, out);
+        case SMAD_FIELD_GRAD_RHO_Y: return download(s, gry.p, n, out);
+        case SMAD_FIELD_GRAD_RHO_Z: return download(s, grz.p, n, out);
+        default: break;
         }
+        if (field >= SMAD_FIELD_VSOL_X && field <= SMAD_FIELD_DIVV) return save_output(field, out);
+        throw ApiError{SMAD_EINVAL, "unknown field"};
     }
+
+    // Save-block outputs of the last step (quasi/quasi.cpp:2379-2487), from
+    // the spectral force S_hat and divv_hat the step leaves behind.
+    DevBuf<T> outbuf;
+    int save_output(int field, double* out)
+    {
+        if (!prev_valid) throw ApiError{SMAD_ESTATE, "save-block outputs need at least one step"};
+        if (!outbuf.p) outbuf.alloc(n);
+        static const int ist_kind[3][3] = {{RODFT01, REDFT01, REDFT01}, {REDFT01, RODFT01, REDFT01},
+                                           {REDFT01, REDFT01, RODFT01}};
+        const T* S[3] = {sx.p, sy.p, sz.p};
+        if (field <= SMAD_FIELD_FPROJ_Z) {
+            const int what = field <= SMAD_FIELD_VSOL_Z ? 0 : 1;
+            const int comp = field - (what ? SMAD_FIELD_FPROJ_X : SMAD_FIELD_VSOL_X);
+            Launch<T>::q_proj_out(s, sx.p, sy.p, sz.p, outbuf.p, comp, what, d, qc);
+            tr3(outbuf.p, outbuf.p, ist_kind[comp][0], ist_kind[comp][1], ist_kind[comp][2]);
+        } else if (field <= SMAD_FIELD_FORCE_Z) {
+            // S = s iST(S_hat): the round trip of the forward transforms
+            const int comp = field - SMAD_FIELD_FORCE_X;
+            Launch<T>::scale_copy(s, S[comp], outbuf.p, (long)n, qc.pc.scale);
+            tr3(outbuf.p, outbuf.p, ist_kind[comp][0], ist_kind[comp][1], ist_kind[comp][2]);
+        } else {
+            // divv = iCT(s divv_hat); zero before the switch
+            Launch<T>::scale_copy(s, divv.p, outbuf.p, (long)n, qc.pre_switch ? 0.0 : qc.pc.scale);
+            tr3(outbuf.p, outbuf.p, REDFT01, REDFT01, REDFT01);
+        }
+        LAUNCHCHK();
+        return download(s, outbuf.p, n, out);
+    }
+
+    int state_fields(int* f, int max) const override
+    {
+        return list_fields({SMAD_FIELD_PSI_HAT, SMAD_FIELD_PSI_HAT_OLD, SMAD_FIELD_N_HAT, SMAD_FIELD_RHO_OLD,
+                            SMAD_FIELD_GRAD_RHO_X, SMAD_FIELD_GRAD_RHO_Y, SMAD_FIELD_GRAD_RHO_Z, SMAD_FIELD_VEL_X,
+                            SMAD_FIELD_VEL_Y, SMAD_FIELD_VEL_Z},
+                           f, max);
+    }
+
+    void set_loop_count(long steps) override { nloop = (int)steps; }
 
     void set_field(int field, const double* in) override
     {
-        if (field != SMAD_FIELD_PSI) throw ApiError{SMAD_EINVAL, "only psi can be set in the quasi model"};
-        upload(s, psi.p, n, in);
-        preloop();
+        T* dst = nullptr;
+        switch (field) {
+        case SMAD_FIELD_PSI:
+            upload(s, psi.p, n, in);
+            preloop();
+            return;
+        case SMAD_FIELD_PSI_HAT: dst = ph.p; break;
+        case SMAD_FIELD_PSI_HAT_OLD: dst = pho.p; break;
+        case SMAD_FIELD_N_HAT: dst = nlo.p; break;
+        case SMAD_FIELD_RHO_OLD: dst = rho_old.p; break;
+        case SMAD_FIELD_GRAD_RHO_X: dst = grx.p; break;
+        case SMAD_FIELD_GRAD_RHO_Y: dst = gry.p; break;
+        case SMAD_FIELD_GRAD_RHO_Z: dst = grz.p; break;
+        case SMAD_FIELD_VEL_X: dst = ux.p; break;
+        case SMAD_FIELD_VEL_Y: dst = uy.p; break;
+        case SMAD_FIELD_VEL_Z: dst = uz.p; break;
+        default: throw ApiError{SMAD_EINVAL, "field cannot be set in the quasi model"};
+        }
+        upload(s, dst, n, in);
+        psi_valid = false;
+        prev_valid = false;
     }
 
     double field_sum(const T* a, double* nonfinite)
@@ -1117,7 +1267,7 @@ struct QuasiEngine : EngineBase {
     {
         throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
     }
-    void adjoint(const double*, double*, double*) override
+    int adjoint(const double*, double*, double*) override
     {
         throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
     }
@@ -1220,6 +1370,10 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
         g_init_error = "smad_init: dt, nw and q0 must be positive";
         return SMAD_EINVAL;
     }
+    if (p.load != 0 && p.load != 1) {
+        g_init_error = "smad_init: load must be 0 or 1";
+        return SMAD_EINVAL;
+    }
     auto* c = new smad_ctx;
     c->p = p;
     int rc = SMAD_OK;
@@ -1306,8 +1460,9 @@ int smad_adjoint(smad_ctx* c, const double* dl_dpsik, double* dl_dpsi0, double* 
 {
     return guarded(c, [&] {
         if (!dl_dpsik || !dl_dpsi0 || !dl_dparams) throw ApiError{SMAD_EINVAL, "null argument"};
-        c->eng->adjoint(dl_dpsik, dl_dpsi0, dl_dparams);
-        return SMAD_OK;
+        int rc = c->eng->adjoint(dl_dpsik, dl_dpsi0, dl_dparams);
+        if (rc == SMAD_ENONFINITE) c->err = "a gradient holds NaN or Inf";
+        return rc;
     });
 }
 
@@ -1405,6 +1560,22 @@ int smad_reduce(smad_ctx* c, int what, double* out)
         int rc = c->eng->reduce(what, out);
         if (rc == SMAD_ENONFINITE) c->err = "psi holds NaN or Inf";
         return rc;
+    });
+}
+
+int smad_state_fields(const smad_ctx* c, int* fields, int max)
+{
+    if (!c || !c->eng || (max > 0 && !fields) || max < 0) return SMAD_EINVAL;
+    return c->eng->state_fields(fields, max);
+}
+
+int smad_set_steps_taken(smad_ctx* c, long steps)
+{
+    return guarded(c, [&] {
+        if (steps < 0) throw ApiError{SMAD_EINVAL, "steps < 0"};
+        c->eng->set_loop_count(steps);
+        c->steps = steps;
+        return SMAD_OK;
     });
 }
 

@@ -29,8 +29,8 @@ def test_library_exports_every_declared_symbol(smadft):
 
 
 def test_params_struct_layout(smadft):
-    # model..nz ints (16 B), 13 doubles (104 B), 2 ints (8 B) = 128 B
-    assert ctypes.sizeof(smadft.CParams) == 128
+    # model..nz ints (16 B), 13 doubles (104 B), 3 ints (12 B), padded to 136 B
+    assert ctypes.sizeof(smadft.CParams) == 136
 
 
 def test_init_rejects_bad_sizes(smadft):

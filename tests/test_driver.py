@@ -1,7 +1,8 @@
 """smadft_run, the host driver with the reference programs' loop control
 (no-adv/cosNoAdvTrackMult.cpp:692-850): argument handling on the CPU, and on
-the GPU the L1.dat / psi.dat / psi.bin it writes against the library driven
-directly through the ABI."""
+the GPU the files it writes (L1.dat, psi0.dat, state.bin, the save-block
+slices and surface files) against the library driven directly through the
+ABI, its restarts and the programs' argv meaning."""
 import os
 import struct
 import subprocess
@@ -22,6 +23,8 @@ def run(args, **kw):
 
 
 @pytest.mark.parametrize("args", [[], ["nosuch"], ["adv", "1", "0.7"], ["quasi", "1"],
+                                  ["quasi", "1", "700", "8"], ["adv", "1", "70", "8", "2"],
+                                  ["--config"], ["--config", "7"],
                                   ["plain", "--bogus"], ["plain", "--n"], ["plain", "--steps", "x"],
                                   ["plain", "--steps", "0"], ["plain", "--tracker", "top"]])
 def test_usage_errors(args):
@@ -30,14 +33,26 @@ def test_usage_errors(args):
     assert "usage" in r.stderr
 
 
-def read_bin(path):
+def read_state(path):
+    """state.bin: model, grid, precision, step count and the fields."""
     with open(path, "rb") as f:
-        assert f.read(8) == b"SMADPSI1"
-        nx, ny, nz = struct.unpack("<3i", f.read(12))
+        assert f.read(8) == b"SMADSTA1"
+        model, nx, ny, nz, prec = struct.unpack("<5i", f.read(20))
         (steps,) = struct.unpack("<q", f.read(8))
+        (nf,) = struct.unpack("<i", f.read(4))
+        ids = struct.unpack(f"<{nf}i", f.read(4 * nf))
         data = np.frombuffer(f.read(), dtype=np.float64)
-    assert data.size == nx * ny * nz
-    return (nx, ny, nz), steps, data.reshape(nx, ny, nz)
+    assert data.size == nf * nx * ny * nz
+    fields = dict(zip(ids, data.reshape(nf, nx, ny, nz)))
+    return (model, (nx, ny, nz), prec), steps, fields
+
+
+def info(path):
+    out = {}
+    for line in open(path):
+        k, _, v = line.partition(":")
+        out[k.strip()] = v.strip()
+    return out
 
 
 @pytest.mark.gpu
@@ -48,10 +63,9 @@ def test_plain_run_matches_library(smadft, tmp_path):
     assert r.returncode == 0, r.stderr
     l1 = np.loadtxt(out / "L1.dat")
     assert l1.shape == (3,)
-    dims, steps, psi = read_bin(out / "psi.bin")
-    assert dims == (32, 32, 32) and steps == 300
-    txt = np.loadtxt(out / "psi.dat")
-    assert np.allclose(txt, psi.ravel(), rtol=1e-5, atol=1e-6)  # 6 significant digits
+    (model, dims, prec), steps, fields = read_state(out / "state.bin")
+    assert (model, dims, prec, steps) == (0, (32, 32, 32), 64, 300)
+    txt = np.loadtxt(out / "psi0.dat")
     p = smadft.Params.default(smadft.MODEL_PLAIN, 32)
     psi0 = smadft.ic_slab(p, 1.328, 1e-3, seed=3)
     with smadft.Stepper(p, psi0) as s:
@@ -60,28 +74,46 @@ def test_plain_run_matches_library(smadft, tmp_path):
             s.step(100)
             ref.append(s.l1())
         ref_psi = s.fetch()
+        ref_state = s.save_state()[1]
     assert np.allclose(l1, ref, rtol=1e-5)
-    assert rel_l2(psi, ref_psi) < 1e-13
+    assert np.allclose(txt, ref_psi.ravel(), rtol=1e-5, atol=1e-6)  # 6 significant digits
+    for f, arr in ref_state.items():
+        assert np.array_equal(fields[f], arr)
 
 
 @pytest.mark.gpu
-def test_plain_restart_and_l1_stop(smadft, tmp_path):
+def test_plain_restarts_and_l1_stop(smadft, tmp_path):
     out = tmp_path / "run"
-    r = run(["plain", "--n", "16", "--steps", "200", "--ic", "slab", "--amp", "1.328",
-             "--out", str(out)])
+    r = run(["plain", "--n", "16", "--steps", "200", "--ic", "slab", "--amp", "1.328", "--out", str(out)])
     assert r.returncode == 0, r.stderr
-    _, steps, psi200 = read_bin(out / "psi.bin")
-    assert steps == 200
-    # restart re-runs the pre-loop from psi, as the reference's load path
-    r = run(["plain", "--n", "16", "--steps", "100", "--restart", str(out / "psi.bin"),
+    # the full state continues exactly
+    r = run(["plain", "--n", "16", "--steps", "100", "--restart", str(out / "state.bin"),
              "--out", str(tmp_path / "cont")])
     assert r.returncode == 0, r.stderr
-    _, steps, psi300 = read_bin(tmp_path / "cont" / "psi.bin")
+    _, steps, fields = read_state(tmp_path / "cont" / "state.bin")
     assert steps == 300
     p = smadft.Params.default(smadft.MODEL_PLAIN, 16)
-    with smadft.Stepper(p, psi200.copy()) as s:
+    psi0 = smadft.ic_slab(p, 1.328, 1e-3, seed=1)
+    with smadft.Stepper(p, psi0) as s:
+        s.step(300)
+        want = s.fetch()
+        want_ph = s.fetch(smadft.FIELD_PSI_HAT)
+    assert np.array_equal(fields[smadft.FIELD_PSI_HAT], want_ph)
+    assert np.allclose(np.loadtxt(tmp_path / "cont" / "psi0.dat"), want.ravel(), rtol=1e-5, atol=1e-6)
+    # a psi-only restart (the previous psi.bin format) re-runs the pre-loop, as
+    # the reference's load path
+    _, _, f200 = read_state(out / "state.bin")
+    with smadft.Stepper(p, psi0) as s:
+        s.step(200)
+        psi200 = s.fetch()
+    legacy = tmp_path / "psi.bin"
+    with open(legacy, "wb") as f:
+        f.write(b"SMADPSI1" + struct.pack("<3i", 16, 16, 16) + struct.pack("<q", 200) + psi200.tobytes())
+    r = run(["plain", "--n", "16", "--steps", "100", "--restart", str(legacy), "--out", str(tmp_path / "old")])
+    assert r.returncode == 0, r.stderr
+    with smadft.Stepper(p, psi200) as s:
         s.step(100)
-        assert rel_l2(psi300, s.fetch()) < 1e-13
+        assert np.allclose(np.loadtxt(tmp_path / "old" / "psi0.dat"), s.fetch().ravel(), rtol=1e-5, atol=1e-6)
     # L1-stop: runs until L1 <= limit, last line of L1.dat below it
     r = run(["plain", "--n", "16", "--l1", "1e-3", "--max-steps", "20000", "--ic", "slab",
              "--amp", "1.328", "--out", str(tmp_path / "stop")])
@@ -117,17 +149,94 @@ def test_plain_save_block_surface_files(smadft, tmp_path, tracker):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("argv,model,l1_every,n", [
-    (["adv", "1", "700", "8", "0"], 1, 100, 16),
-    (["quasi", "1", "700", "8"], 2, 50, 16),
-    (["conserved", "70", "8", "16"], 3, 100, 16),
-    (["advcons", "1", "70", "8", "16", "0"], 4, 100, 16),
-])
-def test_model_modes_match_library(smadft, tmp_path, argv, model, l1_every, n):
-    """The positional argv of each reference main and its default cadence:
-    two L1 blocks, L1.dat against the library driven directly."""
+def test_config2_advective_save_block(smadft, tmp_path):
+    """--config 2 (advective, two bumped interfaces) on a 32^3 grid: the
+    save block's surface files (adv/cosAdvTrackMult.cpp:1412-1600) one step
+    before the block's end, against the library."""
     out = tmp_path / "run"
-    extra = [] if "--n" in argv or model in (3, 4) else ["--n", str(n)]
+    r = run(["--config", "2", "--n", "32", "--steps", "40", "--l1-every", "10", "--save-every", "2",
+             "--out", str(out)])
+    assert r.returncode == 0, r.stderr
+    files = {k: np.loadtxt(out / f"{k}.dat").reshape(2, -1) for k in ("surfPsi", "velSurf", "curvH", "psiMid")}
+    p = smadft.Params.config(2, 32)
+    with smadft.Stepper(p, smadft.config_ic(2, p)) as s:
+        for blk, nsteps in enumerate((19, 20)):
+            s.step(nsteps)
+            sf = s.surface()
+            psi = s.fetch()
+            assert (sf["surf"] >= 0).sum() > 100
+            assert np.array_equal(files["surfPsi"][blk], sf["surf"].ravel())
+            for key, name in (("vel", "velSurf"), ("curv_h", "curvH")):
+                assert np.allclose(files[name][blk], sf[key].ravel(), rtol=1e-5, atol=1e-6 * np.abs(sf[key]).max())
+            assert np.allclose(files["psiMid"][blk], psi[:, 16, :].ravel(), rtol=1e-5, atol=1e-6)
+    assert float(info(out / "info")["dt"]) == 0.02
+
+
+@pytest.mark.gpu
+def test_config3_quasi_save_block_slices(smadft, tmp_path):
+    """--config 3 (quasi, tilted slab) on a 32^3 grid: the save block's
+    y = Ny/2 slices (quasi/quasi.cpp:2379-2487) after the block's end."""
+    out = tmp_path / "run"
+    r = run(["--config", "3", "--n", "32", "--steps", "100", "--save-every", "2", "--out", str(out)])
+    assert r.returncode == 0, r.stderr
+    names = ("vx", "vy", "vz", "vsolx", "vsoly", "vsolz", "fx", "fy", "fz", "sx", "sy", "sz", "divv", "p", "rho",
+             "energyMid")
+    fields = (smadft.FIELD_VEL_X, smadft.FIELD_VEL_Y, smadft.FIELD_VEL_Z, smadft.FIELD_VSOL_X, smadft.FIELD_VSOL_Y,
+              smadft.FIELD_VSOL_Z, smadft.FIELD_FPROJ_X, smadft.FIELD_FPROJ_Y, smadft.FIELD_FPROJ_Z,
+              smadft.FIELD_FORCE_X, smadft.FIELD_FORCE_Y, smadft.FIELD_FORCE_Z, smadft.FIELD_DIVV,
+              smadft.FIELD_PRESSURE, smadft.FIELD_RHO, smadft.FIELD_ENERGY)
+    p = smadft.Params.config(3, 32)
+    with smadft.Stepper(p, smadft.config_ic(3, p)) as s:
+        s.step(100)
+        want = {nm: s.fetch(f)[:, 16, :].ravel() for nm, f in zip(names, fields)}
+    for nm in names:
+        got = np.loadtxt(out / f"{nm}.dat")
+        assert got.size == 32 * 32, nm
+        scale = np.abs(want[nm]).max()
+        assert np.allclose(got, want[nm], rtol=1e-5, atol=1e-6 * max(scale, 1e-300)), nm
+    assert np.loadtxt(out / "surfPsi.dat").size == 32 * 32
+    assert np.loadtxt(out / "mass.dat").shape == (2,)
+
+
+@pytest.mark.gpu
+def test_quasi_load_start_from_rank_files(smadft, tmp_path):
+    """load = 1 reads the reference's per-rank psi<rank>.dat (x slabs in rank
+    order) and starts quasi with divv from the first step."""
+    p = smadft.Params.default(smadft.MODEL_QUASI, 32)
+    p.eps = -0.001 * 700
+    psi0 = smadft.ic_curved(p, 1.34164, 1e-3, seed=3)
+    src = tmp_path / "src"
+    src.mkdir()
+    for r, slab in enumerate(np.split(psi0, 2, axis=0)):  # two ranks
+        np.savetxt(src / f"psi{r}.dat", slab.ravel(), fmt="%.17g")
+    out = tmp_path / "run"
+    r = run(["quasi", "1", "700", "8", "1", "--n", "32", "--steps", "50", "--load-dir", str(src),
+             "--ranks", "2", "--out", str(out)])
+    assert r.returncode == 0, r.stderr
+    p.load = 1
+    with smadft.Stepper(p, psi0) as s:
+        s.step(50)
+        want = s.fetch()
+        l1 = s.l1()
+    assert np.allclose(np.loadtxt(out / "psi0.dat"), want.ravel(), rtol=1e-5, atol=1e-6)
+    assert np.loadtxt(out / "L1.dat") == pytest.approx(l1, rel=1e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("argv,model,l1_every,n,factor", [
+    (["adv", "1", "84", "8", "0"], 1, 100, 16, -0.01),
+    (["quasi", "1", "700", "8", "0"], 2, 50, 16, -0.001),
+    (["conserved", "70", "8", "16"], 3, 100, 16, -0.01),
+    (["advcons", "1", "70", "8", "16", "0"], 4, 100, 16, -0.01),
+])
+def test_model_modes_match_library(smadft, tmp_path, argv, model, l1_every, n, factor):
+    """The positional argv of each reference main (eps = -0.01 ep_arg in the
+    advective and conserved programs, adv/cosAdvTrackMult.cpp:171; -0.001
+    ep_arg in quasi, quasi/quasi.cpp:234), the programs' own constants and
+    their default cadence: two L1 blocks, L1.dat against the library driven
+    directly."""
+    out = tmp_path / "run"
+    extra = [] if model in (3, 4) else ["--n", str(n)]
     r = run(argv + extra + ["--steps", str(2 * l1_every), "--ic", "slab", "--amp", "1.328",
                             "--seed", "2", "--out", str(out)])
     assert r.returncode == 0, r.stderr
@@ -137,7 +246,9 @@ def test_model_modes_match_library(smadft, tmp_path, argv, model, l1_every, n):
     if model in (1, 2, 4):
         p.nu = 1.0
         p.lambda_ = -2.0 / 3.0
-    p.eps = (-0.01 if model in (3, 4) else -0.001) * float(argv[2 if model in (1, 2, 4) else 1])
+    ep_arg = float(argv[2 if model in (1, 2, 4) else 1])
+    p.eps = factor * ep_arg
+    assert float(info(out / "info")["epsilon"]) == pytest.approx(p.eps)
     psi0 = smadft.ic_slab(p, 1.328, 1e-3, seed=2)
     with smadft.Stepper(p, psi0) as s:
         ref = []

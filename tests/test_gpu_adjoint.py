@@ -109,3 +109,16 @@ def test_autograd_rollout_matches_oracle_and_fd(smadft, oracle):
         v = getattr(p, name)
         fd = (J(**{name: v + h}) - J(**{name: v - h})) / (2 * h)
         assert theta.grad[i].item() == pytest.approx(fd, rel=1e-6), name
+
+
+def test_adjoint_reports_nonfinite_gradients(smadft):
+    """A NaN in the cotangent gives SMAD_ENONFINITE (as smad_fetch does for a
+    non-finite field), not a silent SMAD_OK."""
+    p, psi0, tgt = setup(smadft, shape=(16, 16, 16))
+    g = tgt.copy()
+    g[3, 4, 5] = np.nan
+    with smadft.Stepper(p, psi0) as s:
+        s.forward_tape(4, 2)
+        with pytest.raises(smadft.SmadError) as e:
+            s.adjoint(g)
+        assert e.value.code == smadft.ENONFINITE

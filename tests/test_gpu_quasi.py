@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 
 
 def qparams(smadft, shape, prec=64, **kw):
-    p = smadft.Params.default(smadft.MODEL_QUASI, shape[0])
+    p = smadft.Params.config(3, shape[0])  # quasi, rho0 = 0.9, kp = 0.1/1.34164
     p.nx, p.ny, p.nz = shape
     p.precision = prec
     for k, v in kw.items():

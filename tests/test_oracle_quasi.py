@@ -14,7 +14,7 @@ import pytest
 
 
 def qparams(smadft, shape, **kw):
-    p = smadft.Params.default(smadft.MODEL_QUASI, shape[0])
+    p = smadft.Params.config(3, shape[0])  # quasi, rho0 = 0.9, kp = 0.1/1.34164
     p.nx, p.ny, p.nz = shape
     for k, v in kw.items():
         setattr(p, k, v)
@@ -316,3 +316,31 @@ def test_divv_coupling_and_force_with_a_manufactured_grad_rho(smadft, oracle):
     rest = nl + q.mu + sum(q.vel[ax] * gpsi[ax] for ax in range(3))
     assert np.abs(coupling).max() > 1e-6 * np.abs(nl).max()
     assert np.abs(rest - coupling).max() < 1e-10 * np.abs(coupling).max()
+
+
+def test_load_start_takes_rho_old_from_the_first_step(smadft, oracle):
+    """The reference's load == 1 start (quasi/quasi.cpp:1365-1369): divv and
+    grad^2(rho f) from the first step with rho_old = rho there, so the first
+    divv has no (rho - rho_old)/dt term.  With kp = 0 (uniform rho) the
+    trajectory equals the one switched at step 0, whose first divv is the
+    uniform -2/dt of rho_old = 0 (it only reaches the mean mode)."""
+    kw = dict(kp=0.0, rho0=0.9, rho_m=0.6)
+    pl = qparams(smadft, (16, 16, 16), div_switch=200, load=1, **kw)
+    ps = qparams(smadft, (16, 16, 16), div_switch=0, **kw)
+    psi0 = smadft.ic_curved(pl, 1.34164, 1e-3, seed=3)
+    a, b = oracle.QuasiOracle(pl, psi0), oracle.QuasiOracle(ps, psi0)
+    a.step(1)
+    b.step(1)
+    n3 = 8 * pl.nx * pl.ny * pl.nz
+    assert abs(a.divv[0, 0, 0]) < 1e-9 * n3
+    assert b.divv[0, 0, 0] == pytest.approx(-2.0 / pl.dt * n3, rel=1e-12)
+    a.step(4)
+    b.step(4)
+    assert np.abs(a.psi - b.psi).max() < 1e-12 * np.abs(b.psi).max()
+    # with a varying density the load start differs from a fresh start
+    pk = qparams(smadft, (16, 16, 16), div_switch=200, load=1, kp=0.2)
+    pf = qparams(smadft, (16, 16, 16), div_switch=200, load=0, kp=0.2)
+    c, d = oracle.QuasiOracle(pk, psi0), oracle.QuasiOracle(pf, psi0)
+    c.step(3)
+    d.step(3)
+    assert np.abs(c.psi - d.psi).max() > 1e-9

@@ -244,3 +244,56 @@ def test_envelope_curvatures_of_spherical_level_sets(oracle):
     # level (the walk's end and the swap to the next point)
     rmin, rmax = R + w * np.arctanh(1 - 2 * 0.6) - 1.0, R + w * np.arctanh(1 - 2 * 0.1) + 2.5
     assert np.all(-h / 2 > 1 / rmax) and np.all(-h / 2 < 1 / rmin)
+
+
+def test_gradient_array_scan_of_spheres(oracle):
+    """The advective program's scan reads gradient arrays instead of
+    differencing psi (adv/cosAdvTrackMult.cpp:1431-1435): fed the analytic
+    gradient of spherical level sets it gives 2H = 2/r and K = 1/r^2."""
+    R, c = 20.0, (31.3, 30.6, 14.2)
+    psi, (dxx, dyy, dzz), (X, Y, Z, r), f1 = radial_case(R=R, c=c)
+    grad = [f1 * A / r for A in (X, Y, Z)]
+    out = oracle.surface_track_grad(psi, grad, dxx, dyy, dzz, np.ones_like(psi), 1.0)
+    s = out["surf"]
+    ii, jj = np.nonzero(s >= 0)
+    assert len(ii) > 1000
+    rp = r[ii, jj, s[ii, jj].astype(int)]
+    assert np.allclose(out["curv_h"][ii, jj], 2 / rp, rtol=1e-2)
+    assert np.allclose(out["curv_k"][ii, jj], 1 / rp ** 2, rtol=2e-2)
+    # vel = dTpsi/|grad psi| with the given gradient
+    assert np.allclose(out["vel"][ii, jj], 1 / np.abs(f1[ii, jj, s[ii, jj].astype(int)]), rtol=1e-12)
+
+
+def test_adv_surface_is_the_scan_of_closed_form_fields(smadft, oracle):
+    """orc_adv_surface (adv/cosAdvTrackMult.cpp:1311-1347, 1412-1500) equals
+    the gradient-array scan of closed forms: for psi a sum of cosine modes,
+    psi_xx = sum -qx^2 a_m m, grad psi analytic, and dTpsi = d psi/dt of the
+    advective model = sum L(q_m) a_m m + beta psi^3 - gamma psi^5 - u . grad
+    psi with u from the (pinned) Stokes solve of the closed-form force."""
+    import forms
+    from test_oracle_adv import first_step_closed_form
+    p = smadft.Params.default(smadft.MODEL_ADV, 16)
+    p.nx, p.ny, p.nz = 16, 16, 32
+    p.nw, p.dt = 8.0, 0.02
+    am = [(1.2, (0, 0, 7)), (0.35, (2, 1, 3)), (0.25, (1, 3, 5))]
+    psi = np.zeros(p.shape)
+    grad = [np.zeros(p.shape) for _ in range(3)]
+    d2 = [np.zeros(p.shape) for _ in range(3)]
+    dtp = np.zeros(p.shape)
+    for amp, m in am:
+        f, g = forms.mode(p, *m)
+        q = [w[i] for w, i in zip(forms.wavenumbers(p), m)]
+        psi += amp * f
+        dtp += forms.alin(p, forms.mq2(p, *m)) * amp * f
+        for ax in range(3):
+            grad[ax] += amp * g[ax]
+            d2[ax] += -q[ax] ** 2 * amp * f
+    _, u, _ = first_step_closed_form(p, am)
+    dtp += p.beta * psi ** 3 - p.gamma * psi ** 5 - sum(u[ax] * grad[ax] for ax in range(3))
+    h = forms.spacing(p)
+    ref = oracle.surface_track_grad(psi, grad, *d2, dtp, h)
+    got = oracle.adv_surface(p, psi)
+    assert (ref["surf"] >= 0).sum() > p.nx * p.ny // 2
+    assert np.array_equal(got["surf"], ref["surf"])
+    for key in ("vel", "curv_h", "curv_k"):
+        assert np.abs(got[key] - ref[key]).max() < 1e-10 * np.abs(ref[key]).max(), key
