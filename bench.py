@@ -64,9 +64,9 @@ def main():
                     help="HBM budget (GiB) for the hbm_tape adjoint leg; < 0: the library default")
     ap.add_argument("--model-steps", type=int, default=10)
     ap.add_argument("--adv-grid", type=int, default=256)
-    ap.add_argument("--adv-acc", type=int, default=200)
+    ap.add_argument("--adv-acc", type=int, default=1000)
     ap.add_argument("--quasi-grid", type=int, default=384)
-    ap.add_argument("--quasi-acc", type=int, default=100)
+    ap.add_argument("--quasi-acc", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--large-grid", type=int, default=0,
                     help="steps of the plain and quasi models on 1024x512x512 and of the plain "
@@ -108,12 +108,12 @@ def main():
 
     smadft = importlib.import_module("sma-dft_amd")
     n = args.grid
-    p64 = smadft.Params.default(smadft.MODEL_PLAIN, n)
-    psi0 = smadft.ic_curved(p64, 0.6, 1e-3, seed=1)
+    p64 = smadft.Params.config(1, n)
+    psi0 = smadft.config_ic(1, p64)
     res = {}
 
     def run(prec):
-        p = smadft.Params.default(smadft.MODEL_PLAIN, n)
+        p = smadft.Params.config(1, n)
         p.precision = prec
         s = smadft.Stepper(p, psi0)
         # untimed device warm-up (clocks settle under load), then the W
@@ -169,7 +169,7 @@ def main():
             ic = smadft.ic_curved(p64, amp, 1e-3, seed=1)
             out = {}
             for prec in (64, 32):
-                p = smadft.Params.default(smadft.MODEL_PLAIN, n)
+                p = smadft.Params.config(1, n)
                 p.precision = prec
                 with smadft.Stepper(p, ic) as s:
                     s.step(args.acc_steps)
@@ -184,31 +184,39 @@ def main():
                        "melts to ~1e-10, so its relative error compares two vanishing fields; "
                        "amplitude 1.328 (the reference's) keeps the smectic")
 
-    # adjoint / forward time ratio: K steps with checkpoints every ckpt,
-    # (taped forward + adjoint) over the plain forward of the same K steps.
-    # "hbm_tape": the library's default tape budget (a quarter of the free
-    # HBM, or --adj-tape-gib when given), which tapes every segment at
-    # 256^3, K = 200; "strict": budget 0, only the last segment is taped and
-    # every other one is recomputed from its checkpoint
+    # adjoint / forward time ratio, configuration 4: K config-1 steps with
+    # checkpoints every ckpt, loss |psi_K - psi_target|^2 with psi_target from
+    # the same initial condition at eps = -0.65, alpha = 1.1.  Ratio =
+    # (taped forward + adjoint) / plain forward of the same K steps.
+    # "strict": checkpointed recomputation (tape budget 0: only the last
+    # segment is taped, every other one is recomputed from its checkpoint) --
+    # the headline adjoint_fwd_ratio; "hbm_tape": the library's default budget
+    # (a quarter of the free HBM, or --adj-tape-gib), which tapes every
+    # segment at 256^3, K = 200
     adj = None
     if args.adj_steps > 0:
-        adj = {"K": args.adj_steps, "ckpt_every": args.adj_ckpt,
+        K = args.adj_steps
+        adj = {"K": K, "ckpt_every": args.adj_ckpt, "config": 4,
                "timing": f"best of {args.adj_reps} runs per leg",
-               "tape_budget": ("library default (a quarter of the free HBM)" if args.adj_tape_gib < 0
-                               else f"{args.adj_tape_gib} GiB")}
+               "hbm_tape_budget": ("library default (a quarter of the free HBM)" if args.adj_tape_gib < 0
+                                   else f"{args.adj_tape_gib} GiB")}
+        ic = smadft.config_ic(4, p64)
+        pt = smadft.Params.config(4, n)
+        pt.eps, pt.alpha = -0.65, 1.1
+        with smadft.Stepper(pt, ic) as s:
+            s.step(K)
+            target = s.fetch()
         grads = {}
         for prec in precs:
-            p = smadft.Params.default(smadft.MODEL_PLAIN, n)
+            p = smadft.Params.config(4, n)
             p.precision = prec
-            ic = smadft.ic_curved(p64, 1.328, 1e-3, seed=1)
-            target = smadft.ic_curved(p64, 1.328, 1e-3, seed=5)
             with smadft.Stepper(p, ic) as s:
                 s.step(5)
                 # best of `adj_reps` runs of each leg (device time, HIP events)
                 fwd_ms = float("inf")
                 for _ in range(args.adj_reps):
                     s.set_field(smadft.FIELD_PSI, ic)
-                    fwd_ms = min(fwd_ms, s.time_steps(args.adj_steps))
+                    fwd_ms = min(fwd_ms, s.time_steps(K))
                 entry = {"forward_ms": fwd_ms}
                 # the default-budget leg first: a context keeps the last budget set
                 for mode, gib in (("hbm_tape", args.adj_tape_gib), ("strict", 0.0)):
@@ -217,7 +225,7 @@ def main():
                     tape_ms = adj_ms = float("inf")
                     for _ in range(args.adj_reps):
                         s.set_field(smadft.FIELD_PSI, ic)
-                        s.forward_tape(args.adj_steps, args.adj_ckpt)
+                        s.forward_tape(K, args.adj_ckpt)
                         psik = s.fetch()
                         d0, dp = s.adjoint(2.0 * (psik - target))
                         t_ms, a_ms = s.adjoint_timing()
@@ -227,14 +235,26 @@ def main():
                                    "ratio": (tape_ms + adj_ms) / fwd_ms}
                     log(f"adjoint fp{prec} {mode}: fwd {fwd_ms:.2f} ms, taped fwd {tape_ms:.2f} ms, "
                         f"adjoint {adj_ms:.2f} ms, ratio {(tape_ms + adj_ms) / fwd_ms:.3f}")
-            grads[prec] = (d0, dp)
+                grads[prec] = (d0, dp, psik)
+                if prec == 32 and 64 in grads:
+                    # the FP64 run's cotangent: the adjoint's own error, apart
+                    # from the FP32 forward state's
+                    s.set_field(smadft.FIELD_PSI, ic)
+                    s.forward_tape(K, args.adj_ckpt)
+                    grads["32_same_cotangent"] = s.adjoint(2.0 * (grads[64][2] - target))
+            entry["tape_gib_hbm_tape"] = 2 * K * n ** 3 * (8 if prec == 64 else 4) / 2 ** 30
             adj[f"fp{prec}"] = entry
         if 64 in grads and 32 in grads:
-            d64, p64g = grads[64]
-            d32, p32g = grads[32]
-            adj["fp32_grad_rel_err"] = {
-                "dpsi0": float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64)),
-                "dtheta": [float(abs(a - b) / abs(b)) for a, b in zip(p32g, p64g)]}
+            d64, p64g, _ = grads[64]
+
+            def rel(d, pg):
+                return {"dpsi0": float(np.linalg.norm(d - d64) / np.linalg.norm(d64)),
+                        "dtheta": [float(abs(a - b) / abs(b)) for a, b in zip(pg, p64g)]}
+
+            adj["fp32_grad_rel_err"] = rel(*grads[32][:2])
+            adj["fp32_grad_rel_err_same_cotangent"] = rel(*grads["32_same_cotangent"])
+            log(f"adjoint fp32 vs fp64: {adj['fp32_grad_rel_err']}, same cotangent "
+                f"{adj['fp32_grad_rel_err_same_cotangent']}")
 
     # advective (256^3) and quasi-incompressible (384^3)
     # models: steps/s in both precisions and per-field FP32-mixed error
@@ -242,17 +262,17 @@ def main():
     if args.model_steps > 0 and args.only_fp is None:
         for label, model, grid, nacc in (("advective", smadft.MODEL_ADV, args.adv_grid, args.adv_acc),
                                          ("quasi", smadft.MODEL_QUASI, args.quasi_grid, args.quasi_acc)):
-            pm = smadft.Params.default(model, grid)
-            ic = smadft.ic_curved(pm, 1.34164 if model == smadft.MODEL_QUASI else 1.328, 1e-3, seed=2)
-            entry = {"grid": [grid] * 3, "dt": pm.dt}
+            cfg = 2 if model == smadft.MODEL_ADV else 3
+            pm = smadft.Params.config(cfg, grid)
+            ic = smadft.config_ic(cfg, pm)
+            entry = {"config": cfg, "grid": [grid] * 3, "dt": pm.dt}
             fields = {"psi": smadft.FIELD_PSI, "ux": smadft.FIELD_VEL_X, "uy": smadft.FIELD_VEL_Y,
                       "uz": smadft.FIELD_VEL_Z}
             acc_switch = None
             if model == smadft.MODEL_QUASI:
-                # timing: the post-switch step (the switch after step 1); accuracy:
-                # a switch halfway through the run, so both phases are exercised
-                acc_switch = min(pm.div_switch, max(1, nacc // 2))
-                pm.div_switch = 1
+                # timing: the post-switch step (switch after step 1); accuracy:
+                # the configuration's own switch (step 200 of 500)
+                acc_switch = pm.div_switch
                 fields.update({"p": smadft.FIELD_PRESSURE, "rho": smadft.FIELD_RHO})
                 entry["timed_after_div_switch"] = True
                 entry["accuracy_div_switch"] = acc_switch
@@ -260,7 +280,7 @@ def main():
             for prec in (64, 32):
                 pm.precision = prec
                 if acc_switch is not None:
-                    pm.div_switch = 1
+                    pm.div_switch = 1  # time the post-switch step
                 with smadft.Stepper(pm, ic) as s:
                     s.step(2)
                     ms = s.time_steps(args.model_steps)
@@ -295,7 +315,7 @@ def main():
     if args.large_grid > 0:
         for label, model in (("plain", smadft.MODEL_PLAIN), ("quasi", smadft.MODEL_QUASI)):
             for prec in (64, 32):
-                pm = smadft.Params.default(model, 512)
+                pm = smadft.Params.config(1 if model == smadft.MODEL_PLAIN else 3, 512)
                 pm.nx, pm.ny, pm.nz = 1024, 512, 512
                 pm.precision = prec
                 if model == smadft.MODEL_QUASI:
@@ -309,7 +329,7 @@ def main():
                 log(f"large grid 1024x512x512 {label} fp{prec}: {sps:.2f} steps/s")
         # the 1024^3 programs: the plain step (5 fields, 43 GB in FP64)
         for prec in (64, 32):
-            pm = smadft.Params.default(smadft.MODEL_PLAIN, 1024)
+            pm = smadft.Params.config(1, 1024)
             pm.precision = prec
             ic = smadft.ic_slab(pm, 1.328, 1e-3, seed=1)
             with smadft.Stepper(pm, ic) as s:
@@ -366,8 +386,9 @@ def main():
             "accuracy_after_steps": args.acc_steps,
             "accuracy": acc,
         },
-        "adjoint_fwd_ratio": adj["fp64"]["hbm_tape"]["ratio"] if adj and "fp64" in adj else None,
-        "adjoint_fwd_ratio_strict": adj["fp64"]["strict"]["ratio"] if adj and "fp64" in adj else None,
+        "adjoint_fwd_ratio": adj["fp64"]["strict"]["ratio"] if adj and "fp64" in adj else None,
+        "adjoint_fwd_ratio_fp32": adj["fp32"]["strict"]["ratio"] if adj and "fp32" in adj else None,
+        "adjoint_fwd_ratio_hbm_tape": adj["fp64"]["hbm_tape"]["ratio"] if adj and "fp64" in adj else None,
         "adjoint": adj,
         "models": models,
         "large_grids": large or None,

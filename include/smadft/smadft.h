@@ -329,7 +329,7 @@ void smad_ic_two_bumps(const smad_params *p, double amp, double noise, unsigned 
 void smad_default_params(int model, int n, smad_params *out);
 
 /* The benchmark configurations: 0 plain 128^3, 1 plain 256^3, 2 advective
- * 256^3 (nw = 8, nu = 1, dt = 0.02), 3 quasi 384^3 (rho0 = 0.9,
+ * 256^3 (nw = 8, nu = 1, dt = 0.01), 3 quasi 384^3 (rho0 = 0.9,
  * kp = 0.1/1.34164, rho_m = 0.5, dt = 1e-3), 4 plain 256^3 (the adjoint
  * case); eps = -0.7, alpha = 1, beta = 2, gamma = 1, q0 = 1.  Configuration
  * 2 specifies dt = 0.05 and 3 dt = 0.02; both diverge in these schemes

@@ -405,13 +405,16 @@ __device__ __forceinline__ double cn_ab2(const PlainCoef& cf, double mq2, double
 }
 
 // x pass: Nq = REDFT10_x(w); CN/AB2 update of psihat with the history nhat
-// (in place: nhat holds Nq_past on entry, Nq on exit); w = REDFT01_x(psihat').
+// (in place: nhat holds Nq_past on entry, Nq on exit; with HOOP the history
+// is read from nh_in instead and nhat only written); w = REDFT01_x(psihat').
 // psihat is the pre-scaled spectral field.
-template <typename T, int N, bool CONS>
+template <typename T, int N, bool CONS, bool HOOP = false>
 __device__ __forceinline__ void plain_x_body(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
                                              T* __restrict__ nhat, const T* __restrict__ sub, const Dims& d,
-                                             const PlainCoef& cf, int inverse, const Tw<T>& tw)
+                                             const PlainCoef& cf, int inverse, const Tw<T>& tw,
+                                             const T* __restrict__ nh_in = nullptr)
 {
+    const T* __restrict__ nh_src = HOOP ? nh_in : nhat;
     constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
     constexpr int W = V16<T>::W;
     __shared__ T lds[NL * LS];
@@ -437,7 +440,7 @@ __device__ __forceinline__ void plain_x_body(const T* a_in, T* w_out, const T* p
     if (g.ncol == NL) {
         rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
             V16<T> ph = *reinterpret_cast<const V16<T>*>(ph_in + gi);
-            V16<T> np = *reinterpret_cast<const V16<T>*>(nhat + gi);
+            V16<T> np = *reinterpret_cast<const V16<T>*>(nh_src + gi);
             V16<T> sb{};
             if constexpr (CONS)
                 if (sub) sb = *reinterpret_cast<const V16<T>*>(sub + gi);
@@ -464,7 +467,7 @@ __device__ __forceinline__ void plain_x_body(const T* a_in, T* w_out, const T* p
                 double mq2 = qx * qx + qy * qy + qz * qz;
                 T nqt = lds[c * LS + PR(p)];
                 if constexpr (CONS) nqt = T(mq2 * (double)nqt - (sub ? (double)sub[gi] : 0.0));
-                double r = cn_ab2<CONS>(cf, mq2, (double)ph_in[gi], (double)nqt, (double)nhat[gi]);
+                double r = cn_ab2<CONS>(cf, mq2, (double)ph_in[gi], (double)nqt, (double)nh_src[gi]);
                 ph_out[gi] = T(r);
                 nhat[gi] = nqt;
                 lds[c * LS + PR(p)] = T(r);
@@ -489,6 +492,15 @@ __global__ __launch_bounds__(WG) void k_plain_x(const T* a_in, T* w_out, const T
 }
 // conserved variants: aLin times |q|^2 and Nq = |q|^2 REDFT10_x(a_in) - sub
 // (sub = the advection term's spectrum, or null)
+// the first step of a checkpointed segment: the history comes from the
+// checkpoint nh_in, which stays untouched
+template <typename T, int N>
+__global__ __launch_bounds__(WG) void k_plain_x_hoop(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
+                                                     T* __restrict__ nhat, const T* __restrict__ nh_in, Dims d,
+                                                     PlainCoef cf, int inverse, Tw<T> tw)
+{
+    plain_x_body<T, N, false, true>(a_in, w_out, ph_in, ph_out, nhat, nullptr, d, cf, inverse, tw, nh_in);
+}
 template <typename T, int N>
 __global__ __launch_bounds__(WG) void k_plain_x_cons(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
                                                      T* __restrict__ nhat, const T* __restrict__ sub, Dims d,
@@ -1835,7 +1847,7 @@ struct Launch {
                         const PlainCoef& cf, int from_real, const Tw<T>& tw);
     static void plain_x(hipStream_t s, const T* a_in, T* w_out, const T* ph_in, T* ph_out,
                         T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse = 1,
-                        const T* sub = nullptr);
+                        const T* sub = nullptr, const T* nh_in = nullptr);
     // z pass with one output per op of the set `zs` (ZS_*); strided pass
     // with the outputs (OP_SHIFT, OP_ID)
     static void zpass(hipStream_t s, int zs, const T* in, const PassSet<T>& ps, Dims d, const SpecCoef& sc,

@@ -333,14 +333,22 @@ struct PlainEngine : EngineBase {
     // psi^n, the real field the step starts from.
     void step_one(const T* ph_in, T* ph_out, T* psi_out, Prof* prof, bool z_only = false)
     {
-        Launch<T>::plain_z(s, w.p, w.p, psi_out, d, cf, 0, twz.tw);
+        step_on(s, ph_in, ph_out, psi_out, prof, z_only);
+    }
+    // The step on stream st (the adjoint recomputes segments on a second
+    // stream).  w_in and nh_in, when set, hold the state's w and N_hat (a
+    // checkpoint, left untouched); the step writes the live w and nhat.
+    void step_on(hipStream_t st, const T* ph_in, T* ph_out, T* psi_out, Prof* prof, bool z_only,
+                 const T* w_in = nullptr, const T* nh_in = nullptr)
+    {
+        Launch<T>::plain_z(st, w_in ? w_in : w.p, w.p, psi_out, d, cf, 0, twz.tw);
         if (prof) prof->mark(0);
         if (z_only) return;
-        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(st, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
         if (prof) prof->mark(1);
-        Launch<T>::plain_x(s, w.p, w.p, ph_in, ph_out, nhat.p, d, cf, twx.tw, 1);
+        Launch<T>::plain_x(st, w.p, w.p, ph_in, ph_out, nhat.p, d, cf, twx.tw, 1, nullptr, nh_in);
         if (prof) prof->mark(2);
-        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
+        Launch<T>::axis(st, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
         if (prof) prof->mark(3);
     }
 
@@ -375,8 +383,20 @@ struct PlainEngine : EngineBase {
     double tape_budget = -1.0;
     std::vector<std::unique_ptr<DevBuf<T>>> ck_ph, ck_nh, ck_w;
     std::vector<std::unique_ptr<Tape>> tapes;  // per segment (taped ones), plus scratch
-    std::unique_ptr<Tape> scratch;
-    DevBuf<T> end_ph, end_nh, lbuf, gbuf;
+    std::unique_ptr<Tape> scratch[2];  // recomputed segments, double-buffered
+    hipStream_t s2 = nullptr;           // the recomputation's stream
+    hipEvent_t ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+
+    ~PlainEngine()
+    {
+        if (s2) {
+            (void)hipStreamSynchronize(s2);
+            (void)hipStreamDestroy(s2);
+        }
+        for (hipEvent_t e : {ev_ready[0], ev_ready[1], ev_free[0], ev_free[1]})
+            if (e) (void)hipEventDestroy(e);
+    }
+    DevBuf<T> end_nh, lbuf, gbuf;
     DevBuf<double> acc_x, acc_z, gsum;
     bool fresh = true;
 
@@ -414,15 +434,25 @@ struct PlainEngine : EngineBase {
         ensure(t->ph, ck, n);
     }
 
-    // slot m of segment g on tape t: slot 0 is the checkpoint
-    const T* slot(const Tape& t, int g, int m) const { return m == 0 ? ck_ph[g]->p : t.ph[m - 1]->p; }
+    // slot m of segment g on tape t: slot 0 is the checkpoint, or for a taped
+    // segment after the first taped one the previous segment's last slot
+    const T* slot(const Tape& t, int g, int m) const
+    {
+        if (m > 0) return t.ph[m - 1]->p;
+        return g > tape_first ? tapes[g - 1]->ph[tape_ck - 1]->p : ck_ph[g]->p;
+    }
 
     // recorded run of ns steps of segment g from (w, ck_ph[g], nhat) into t
-    void record(Tape& t, int g, int ns, bool last_z_only)
+    // recorded run of ns steps of segment g into t, from the live w and nhat,
+    // or from the checkpoint's (from_ck)
+    void record(Tape& t, int g, int ns, bool last_z_only, hipStream_t st = nullptr, bool from_ck = false)
     {
+        if (!st) st = s;
         for (int m = 0; m < ns; ++m) {
             bool zo = last_z_only && m == ns - 1;
-            step_one(slot(t, g, m), zo ? nullptr : t.ph[m]->p, t.psi[m]->p, nullptr, zo);
+            const bool ckin = from_ck && m == 0;
+            step_on(st, slot(t, g, m), zo ? nullptr : t.ph[m]->p, t.psi[m]->p, nullptr, zo,
+                    ckin ? ck_w[g]->p : nullptr, ckin ? ck_nh[g]->p : nullptr);
         }
         LAUNCHCHK();
     }
@@ -448,36 +478,44 @@ struct PlainEngine : EngineBase {
         ensure(ck_w, nseg, n);
         if ((int)tapes.size() < nseg) tapes.resize(nseg);
         for (int g = first; g < nseg; ++g) ensure_tape(tapes[g], ck);
-        if (first > 0) ensure_tape(scratch, ck);
-        if (!end_ph.p) {
-            for (DevBuf<T>* b : {&end_ph, &end_nh, &lbuf, &gbuf}) b->alloc(n);
+        if (first > 0) ensure_tape(scratch[0], ck);
+        if (first > 1) ensure_tape(scratch[1], ck);
+        if (!end_nh.p) {
+            for (DevBuf<T>* b : {&end_nh, &lbuf, &gbuf}) b->alloc(n);
             acc_x.alloc(2 * (size_t)num_tiles(d, 0, Lines<T>::NL));
             acc_z.alloc(2 * (size_t)num_tiles(d, 2, linesz_nl<T>(d.nz)));
             gsum.alloc(4);
         }
+        tape_ck = ck;
+        tape_first = first;
         EvPair ev;
         HIPCHK(hipEventRecord(ev.a, s));
+        // A checkpoint takes the live state's buffers (pointer swaps, no
+        // copies) and the segment's first step reads them and writes the
+        // live buffers; a taped segment after the first taped one starts
+        // from the previous segment's last tape slot.
         for (int g = 0; g < nseg; ++g) {
-            copy_dd(ck_ph[g]->p, psihat.p);
-            copy_dd(ck_nh[g]->p, nhat.p);
-            int ns = std::min(ck, K - g * ck);
+            const int ns = std::min(ck, K - g * ck);
             if (g < first) {
-                copy_dd(ck_w[g]->p, w.p);
-                for (int m = 0; m < ns; ++m) step_one(psihat.p, psihat.p, nullptr, nullptr, false);
+                std::swap(ck_ph[g]->p, psihat.p);
+                std::swap(ck_nh[g]->p, nhat.p);
+                std::swap(ck_w[g]->p, w.p);
+                step_on(s, ck_ph[g]->p, psihat.p, nullptr, nullptr, false, ck_w[g]->p, ck_nh[g]->p);
+                for (int m = 1; m < ns; ++m) step_one(psihat.p, psihat.p, nullptr, nullptr, false);
                 LAUNCHCHK();
             } else {
+                if (g == first) std::swap(ck_ph[g]->p, psihat.p);
                 record(*tapes[g], g, ns, false);
-                copy_dd(psihat.p, tapes[g]->ph[ns - 1]->p);
             }
         }
-        copy_dd(end_ph.p, psihat.p);
+        // the context ends at step K: psihat from the last tape slot; the
+        // history is kept apart because recomputation reuses nhat
+        copy_dd(psihat.p, tapes[nseg - 1]->ph[std::min(ck, K - (nseg - 1) * ck) - 1]->p);
         copy_dd(end_nh.p, nhat.p);
         HIPCHK(hipEventRecord(ev.b, s));
         last_fwd_ms = ev.ms();
         tape_K = K;
-        tape_ck = ck;
         tape_nseg = nseg;
-        tape_first = first;
         psi_valid = false;
         prev_valid = false;
         fresh = false;
@@ -496,17 +534,34 @@ struct PlainEngine : EngineBase {
         HIPCHK(hipMemsetAsync(lbuf.p, 0, n * sizeof(T), s));
         HIPCHK(hipMemsetAsync(acc_x.p, 0, acc_x.n * sizeof(double), s));
         HIPCHK(hipMemsetAsync(acc_z.p, 0, acc_z.n * sizeof(double), s));
+        // Untaped segments are recomputed from their checkpoints on a second
+        // stream into two alternating scratch tapes, segment g-1 while the
+        // backward sweep of segment g runs: the recomputation does not depend
+        // on the sweep, and the two overlap on the GPU.  ev_ready[b]: scratch
+        // tape b holds its segment; ev_free[b]: the sweep is done with it.
+        if (tape_first > 0) {
+            if (!s2) {
+                HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+                for (hipEvent_t* e : {&ev_ready[0], &ev_ready[1], &ev_free[0], &ev_free[1]})
+                    HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+            }
+            HIPCHK(hipEventRecord(ev_free[0], s));
+            HIPCHK(hipEventRecord(ev_free[1], s));
+        }
+        auto recompute = [&](int g) {
+            const int b = g & 1, ns = std::min(ck, K - g * ck);
+            HIPCHK(hipStreamWaitEvent(s2, ev_free[b], 0));
+            // from the checkpoint's w and N_hat; the final step needs only its
+            // z pass: psihat at the segment end is checkpoint g+1
+            record(*scratch[b], g, ns, true, s2, true);
+            HIPCHK(hipEventRecord(ev_ready[b], s2));
+        };
+        if (nseg >= 2 && nseg - 2 < tape_first) recompute(nseg - 2);
         for (int g = nseg - 1; g >= 0; --g) {
             const int n0 = g * ck, ns = std::min(ck, K - n0);
             const bool taped = g >= tape_first;
-            Tape& t = taped ? *tapes[g] : *scratch;
-            if (!taped) {
-                // recompute from the checkpoint; the final step needs only its
-                // z pass because psihat at the segment end is checkpoint g+1
-                copy_dd(nhat.p, ck_nh[g]->p);
-                copy_dd(w.p, ck_w[g]->p);
-                record(t, g, ns, true);
-            }
+            Tape& t = taped ? *tapes[g] : *scratch[g & 1];
+            if (!taped) HIPCHK(hipStreamWaitEvent(s, ev_ready[g & 1], 0));
             for (int m = ns - 1; m >= 0; --m) {
                 int nn = n0 + m;
                 const T* ph_next = (!taped && m == ns - 1) ? ck_ph[g + 1]->p : t.ph[m]->p;
@@ -516,6 +571,10 @@ struct PlainEngine : EngineBase {
                 Launch<T>::axis(s, gbuf.p, gbuf.p, d, 1, REDFT10, T(1), twy.tw);
             }
             LAUNCHCHK();
+            if (!taped) HIPCHK(hipEventRecord(ev_free[g & 1], s));
+            // segment g-2 goes into the tape this sweep has just released; it
+            // is recomputed while the sweep of g-1 (recomputed before) runs
+            if (g >= 2 && g - 2 < tape_first) recompute(g - 2);
         }
         // dJ/dpsi0 = s REDFT01^3(a lt^1 + REDFT10_x(G)), psihat^0 = checkpoint 0
         Launch<T>::adj_x(s, gbuf.p, lbuf.p, ck_ph[0]->p, d, cf, 2, acc_x.p, twx.tw);
@@ -534,7 +593,7 @@ struct PlainEngine : EngineBase {
             if (!std::isfinite(h[q])) rc = SMAD_ENONFINITE;
         }
         // leave the context at the end of the taped forward run
-        copy_dd(psihat.p, end_ph.p);
+        copy_dd(psihat.p, tapes[nseg - 1]->ph[std::min(ck, K - (nseg - 1) * ck) - 1]->p);
         copy_dd(nhat.p, end_nh.p);
         Launch<T>::axis(s, psihat.p, w.p, d, 0, REDFT01, T(1), twx.tw);
         Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);

@@ -169,7 +169,7 @@ def test_config2_advective_save_block(smadft, tmp_path):
             for key, name in (("vel", "velSurf"), ("curv_h", "curvH")):
                 assert np.allclose(files[name][blk], sf[key].ravel(), rtol=1e-5, atol=1e-6 * np.abs(sf[key]).max())
             assert np.allclose(files["psiMid"][blk], psi[:, 16, :].ravel(), rtol=1e-5, atol=1e-6)
-    assert float(info(out / "info")["dt"]) == 0.02
+    assert float(info(out / "info")["dt"]) == 0.01
 
 
 @pytest.mark.gpu

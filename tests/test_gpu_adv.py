@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 
 
 def adv_params(smadft, shape, prec=64, **kw):
-    p = smadft.Params.config(2, shape[0])  # advective, nw = 8, dt = 0.02
+    p = smadft.Params.config(2, shape[0])  # advective, nw = 8, dt = 0.01
     p.nx, p.ny, p.nz = shape
     p.precision = prec
     for k, v in kw.items():

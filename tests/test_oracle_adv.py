@@ -16,7 +16,7 @@ from conftest import rel_l2
 
 
 def adv_params(smadft, shape, **kw):
-    p = smadft.Params.config(2, shape[0])  # advective, nw = 8, dt = 0.02
+    p = smadft.Params.config(2, shape[0])  # advective, nw = 8, dt = 0.01
     p.nx, p.ny, p.nz = shape
     for k, v in kw.items():
         setattr(p, k, v)
