@@ -105,7 +105,9 @@ enum {
     SMAD_FIELD_FORCE_X = 25,     /* the force S = mu grad psi + e grad rho */
     SMAD_FIELD_FORCE_Y = 26,
     SMAD_FIELD_FORCE_Z = 27,
-    SMAD_FIELD_DIVV = 28         /* the imposed divergence in real space */
+    SMAD_FIELD_DIVV = 28,        /* the imposed divergence in real space */
+    SMAD_FIELD_DRHO_OLD = 29     /* quasi: rho_old - rho0 as stored (the restart state's
+                                    form of SMAD_FIELD_RHO_OLD, exact in both precisions) */
 };
 
 /* Reductions (FP64 on the device, deterministic order). */
@@ -148,7 +150,7 @@ int smad_set_field(smad_ctx *ctx, int field, const double *host_in);
  * re-runs the pre-loop (no-adv/cosNoAdvTrackMult.cpp:517-537); that is
  * smad_set_field(SMAD_FIELD_PSI).  Plain: psi_hat, N_hat and the carried
  * xy-transformed psi_hat; advective: psi_hat, N_hat; quasi: psi_hat, its
- * previous value, N_hat, rho_old, grad rho and u. */
+ * previous value, N_hat, rho_old - rho0, grad rho and u. */
 int smad_state_fields(const smad_ctx *ctx, int *fields, int max);
 
 /* Sets the step count (smad_steps_taken) and, for the quasi model, the loop

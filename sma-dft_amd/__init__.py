@@ -34,6 +34,7 @@ FIELD_VSOL_X, FIELD_VSOL_Y, FIELD_VSOL_Z = 19, 20, 21
 FIELD_FPROJ_X, FIELD_FPROJ_Y, FIELD_FPROJ_Z = 22, 23, 24
 FIELD_FORCE_X, FIELD_FORCE_Y, FIELD_FORCE_Z = 25, 26, 27
 FIELD_DIVV = 28
+FIELD_DRHO_OLD = 29
 
 RED_L1, RED_MASS, RED_ENERGY = 0, 1, 2
 

@@ -1006,7 +1006,8 @@ __device__ __forceinline__ double mq2_at(const PlainCoef& cf, int i, int j, int 
 }
 
 // after the switch (:1369-1384): divv = -((rho - rho_old)/dt + u . grad rho) / (rho + rho_old)/2,
-// rf = rho f.  Before it: lap = rho * lap.  Then rho_old = rho.
+// rf = rho f.  Before it: lap = rho * lap.  Then rho_old = rho.  rho and
+// rho_old hold rho - rho0: the difference is taken of the stored values.
 template <typename T>
 __global__ __launch_bounds__(WG) void k_q_divv(const T* __restrict__ rho, T* __restrict__ rho_old,
                                                const T* __restrict__ ux, const T* __restrict__ uy,
@@ -1016,13 +1017,13 @@ __global__ __launch_bounds__(WG) void k_q_divv(const T* __restrict__ rho, T* __r
                                                T* __restrict__ lap, long n, QuasiCoef qc)
 {
     for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        double r = (double)rho[id], ro = (double)rho_old[id];
+        const double dr = (double)rho[id], dro = (double)rho_old[id], r = qc.rho0 + dr;
         if (qc.pre_switch) {
             lap[id] = T(r * (double)lap[id]);
         } else {
             double adv = (double)ux[id] * (double)grx[id] + (double)uy[id] * (double)gry[id] +
                          (double)uz[id] * (double)grz[id];
-            divv[id] = T(-((r - ro) / qc.dt + adv) / (0.5 * (r + ro)));
+            divv[id] = T(-((dr - dro) / qc.dt + adv) / (qc.rho0 + 0.5 * (dr + dro)));
             lap[id] = T(r * (double)f[id]);
         }
         rho_old[id] = rho[id];
@@ -1044,7 +1045,7 @@ __global__ __launch_bounds__(WG) void k_q_force(const T* __restrict__ psi, const
     for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
         double x = (double)psi[id], x2 = x * x, x3 = x2 * x, fv = (double)f[id];
         double ev = -0.5 * cf.eps * x2 + 0.5 * fv * fv - 0.25 * cf.beta * x2 * x2 + (cf.gamma / 6) * x3 * x3;
-        double mv = (double)lap[id] + (double)rho[id] * (fv - cf.eps * x - cf.beta * x3 + cf.gamma * x3 * x2);
+        double mv = (double)lap[id] + (qc.rho0 + (double)rho[id]) * (fv - cf.eps * x - cf.beta * x3 + cf.gamma * x3 * x2);
         e[id] = T(ev);
         mu[id] = T(mv);
         sx[id] = T(mv * (double)gpx[id] + ev * (double)grx[id]);
@@ -1154,7 +1155,7 @@ __global__ __launch_bounds__(WG) void k_q_proj_out(const T* __restrict__ Sx, con
     }
 }
 
-// REDFT10_z of rho = kp sqrt(q0^2 psi^2 + |grad psi|^2) + rho0 (:1322-1344),
+// REDFT10_z of rho - rho0 = kp sqrt(q0^2 psi^2 + |grad psi|^2) (:1322-1344),
 // the density law formed while the z lines are loaded.
 template <typename T, int N>
 __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_rho_z(const T* __restrict__ psi, const T* __restrict__ gx,
@@ -1177,8 +1178,10 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_rho_z(const T* __restrict_
 #pragma unroll
         for (int t = 0; t < W; ++t) {
             double x = (double)a.v[t], g1 = (double)hx.v[t], g2 = (double)hy.v[t], g3 = (double)hz.v[t];
-            lds[c * LS + PR(mk_pos<N>(p + t))] =
-                T(qc.kp * sqrt(qc.pc.q0sq * x * x + g1 * g1 + g2 * g2 + g3 * g3) + qc.rho0);
+            // the density is stored as rho - rho0 (its FP32 rounding then
+            // scales with kp |psi| instead of rho0); the filter keeps the
+            // mean mode, so rho0 is added back exactly where rho is used
+            lds[c * LS + PR(mk_pos<N>(p + t))] = T(qc.kp * sqrt(qc.pc.q0sq * x * x + g1 * g1 + g2 * g2 + g3 * g3));
         }
     });
     lsync<true>();
@@ -1233,7 +1236,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_nl_z(const T* __restrict__
             const V16<T> az = *reinterpret_cast<const V16<T>*>(grz + gi);
 #pragma unroll
             for (int t = 0; t < W; ++t) {
-                double rr = (double)r.v[t], a = (double)ax.v[t], b = (double)ay.v[t], cc = (double)az.v[t];
+                double rr = qc.rho0 + (double)r.v[t], a = (double)ax.v[t], b = (double)ay.v[t], cc = (double)az.v[t];
                 v[t] += (pr[gi + t] + rr * (double)ev.v[t]) * qc.kp * sqrt(a * a + b * b + cc * cc) / (rr * rr);
             }
         }
@@ -1252,6 +1255,13 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_nl_z(const T* __restrict__
 
 // a <- |q|^2 a on the spectral grid (the conserved variant's N_hat^0,
 // no-adv/cosNoAdvConserv.cpp:690-697)
+// b = v
+template <typename T>
+__global__ __launch_bounds__(WG) void k_fill(T* __restrict__ b, long n, double v)
+{
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) b[id] = T(v);
+}
+
 // b = f a (FP64 arithmetic)
 template <typename T>
 __global__ __launch_bounds__(WG) void k_scale_copy(const T* __restrict__ a, T* __restrict__ b, long n, double f)
@@ -1890,6 +1900,7 @@ struct Launch {
     static void q_proj_out(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, T* out, int comp, int what, Dims d,
                            const QuasiCoef& qc);
     static void scale_copy(hipStream_t s, const T* a, T* b, long n, double f);
+    static void fill(hipStream_t s, T* b, long n, double v);
     static void q_x(hipStream_t s, const T* nl_zy, T* ph, T* pho, T* nlo, Dims d, const QuasiCoef& qc, int first,
                     const Tw<T>& tw);
     static void l1(hipStream_t s, const T* a, const T* b, long n, double* part, int nparts,

@@ -153,6 +153,12 @@ void Launch<T>::q_proj_out(hipStream_t s, const T* Sx, const T* Sy, const T* Sz,
 }
 
 template <typename T>
+void Launch<T>::fill(hipStream_t s, T* b, long n, double v)
+{
+    hipLaunchKernelGGL(k_fill<T>, flat_grid(n), dim3(WG), 0, s, b, n, v);
+}
+
+template <typename T>
 void Launch<T>::scale_copy(hipStream_t s, const T* a, T* b, long n, double f)
 {
     hipLaunchKernelGGL(k_scale_copy<T>, flat_grid(n), dim3(WG), 0, s, a, b, n, f);

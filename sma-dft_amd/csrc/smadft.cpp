@@ -1076,7 +1076,8 @@ struct QuasiEngine : EngineBase {
         zero(grx);  // grad of the initial filtered rho, which is zero
         zero(gry);
         zero(grz);
-        zero(rho_old);
+        // rho_old = 0 before the first step; rho and rho_old hold rho - rho0
+        Launch<T>::fill(s, rho_old.p, (long)n, -qc.rho0);
         zero(ux);
         zero(uy);
         zero(uz);
@@ -1207,11 +1208,12 @@ struct QuasiEngine : EngineBase {
         case SMAD_FIELD_VEL_Y: return download(s, uy.p, n, out);
         case SMAD_FIELD_VEL_Z: return download(s, uz.p, n, out);
         case SMAD_FIELD_PRESSURE: return download(s, pr.p, n, out);
-        case SMAD_FIELD_RHO: return download(s, rho.p, n, out);
+        case SMAD_FIELD_RHO: return download_rho(rho.p, out);
         case SMAD_FIELD_ENERGY: return download(s, e.p, n, out);
         case SMAD_FIELD_DIVV_HAT: return download(s, divv.p, n, out);
         case SMAD_FIELD_PSI_HAT_OLD: return download(s, pho.p, n, out);
-        case SMAD_FIELD_RHO_OLD: return download(s, rho_old.p, n, out);
+        case SMAD_FIELD_RHO_OLD: return download_rho(rho_old.p, out);
+        case SMAD_FIELD_DRHO_OLD: return download(s, rho_old.p, n, out);
         case SMAD_FIELD_GRAD_RHO_X: return download(s, grx.p, n, out);
         case SMAD_FIELD_GRAD_RHO_Y: return download(s, gry.p, n, out);
         case SMAD_FIELD_GRAD_RHO_Z: return download(s, grz.p, n, out);
@@ -1219,6 +1221,14 @@ struct QuasiEngine : EngineBase {
         }
         if (field >= SMAD_FIELD_VSOL_X && field <= SMAD_FIELD_DIVV) return save_output(field, out);
         throw ApiError{SMAD_EINVAL, "unknown field"};
+    }
+
+    // rho and rho_old are stored as rho - rho0
+    int download_rho(const T* dev, double* out)
+    {
+        int rc = download(s, dev, n, out);
+        for (size_t i = 0; i < n; ++i) out[i] += qc.rho0;
+        return rc;
     }
 
     // Save-block outputs of the last step (quasi/quasi.cpp:2379-2487), from
@@ -1252,7 +1262,7 @@ struct QuasiEngine : EngineBase {
 
     int state_fields(int* f, int max) const override
     {
-        return list_fields({SMAD_FIELD_PSI_HAT, SMAD_FIELD_PSI_HAT_OLD, SMAD_FIELD_N_HAT, SMAD_FIELD_RHO_OLD,
+        return list_fields({SMAD_FIELD_PSI_HAT, SMAD_FIELD_PSI_HAT_OLD, SMAD_FIELD_N_HAT, SMAD_FIELD_DRHO_OLD,
                             SMAD_FIELD_GRAD_RHO_X, SMAD_FIELD_GRAD_RHO_Y, SMAD_FIELD_GRAD_RHO_Z, SMAD_FIELD_VEL_X,
                             SMAD_FIELD_VEL_Y, SMAD_FIELD_VEL_Z},
                            f, max);
@@ -1270,8 +1280,16 @@ struct QuasiEngine : EngineBase {
             return;
         case SMAD_FIELD_PSI_HAT: dst = ph.p; break;
         case SMAD_FIELD_PSI_HAT_OLD: dst = pho.p; break;
+        case SMAD_FIELD_DRHO_OLD: dst = rho_old.p; break;
         case SMAD_FIELD_N_HAT: dst = nlo.p; break;
-        case SMAD_FIELD_RHO_OLD: dst = rho_old.p; break;
+        case SMAD_FIELD_RHO_OLD: {
+            std::vector<double> h(in, in + n);
+            for (double& v : h) v -= qc.rho0;
+            upload(s, rho_old.p, n, h.data());
+            psi_valid = false;
+            prev_valid = false;
+            return;
+        }
         case SMAD_FIELD_GRAD_RHO_X: dst = grx.p; break;
         case SMAD_FIELD_GRAD_RHO_Y: dst = gry.p; break;
         case SMAD_FIELD_GRAD_RHO_Z: dst = grz.p; break;
@@ -1313,7 +1331,7 @@ struct QuasiEngine : EngineBase {
             out[2] = h[0] / (p.dt * h[1]);
             bad = h[2];
         } else if (what == SMAD_RED_MASS) {
-            out[0] = field_sum(rho.p, &bad) * dx * dx * dx;
+            out[0] = (field_sum(rho.p, &bad) + (double)n * qc.rho0) * dx * dx * dx;
         } else if (what == SMAD_RED_ENERGY) {
             out[0] = field_sum(e.p, &bad) * dx * dx * dx;
         } else {
