@@ -25,6 +25,11 @@ import time
 
 import numpy as np
 
+try:  # torch carries its own HIP runtime, which must load before the library's
+    import torch  # noqa: F401
+except ImportError:
+    torch = None
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 

@@ -223,6 +223,28 @@ int smad_forward_tape(smad_ctx *ctx, int nsteps, int ckpt_every);
 int smad_set_tape_budget(smad_ctx *ctx, double gib);
 int smad_adjoint(smad_ctx *ctx, const double *dl_dpsik, double *dl_dpsi0, double *dl_dparams);
 
+/* Device-buffer forms of smad_forward_tape and smad_adjoint, for callers
+ * whose fields live on the GPU (the torch.autograd.Function of the plain
+ * model): psi0_dev, psik_dev, dl_dpsik_dev and dl_dpsi0_dev are FP64 device
+ * buffers of nx*ny*nz values; dl_dparams stays a host array of 4.
+ * smad_forward_tape_dev takes psi0_dev as the initial condition (running
+ * the pre-loop on it, as smad_set_field(SMAD_FIELD_PSI)), runs the taped
+ * forward and writes psi_K to psik_dev.  Both read and write the buffers on
+ * the context's stream and return when the work is complete; the caller
+ * makes sure its own writes to the inputs are complete before the call.
+ * Plain model only, as smad_forward_tape. */
+int smad_forward_tape_dev(smad_ctx *ctx, const double *psi0_dev, int nsteps, int ckpt_every,
+                          double *psik_dev);
+int smad_adjoint_dev(smad_ctx *ctx, const double *dl_dpsik_dev, double *dl_dpsi0_dev,
+                     double *dl_dparams);
+
+/* New coefficients (eps, alpha, beta, gamma, dt, ...) for a context of the
+ * same model, grid and precision (plain model), without rebuilding it; the
+ * state is kept, so a new run starts with smad_set_field(SMAD_FIELD_PSI) or
+ * smad_forward_tape_dev.  The reference fixes its constants at compile time
+ * (no-adv/cosNoAdvTrackMult.cpp:130-172). */
+int smad_set_params(smad_ctx *ctx, const smad_params *params);
+
 /* Device time (HIP events, ms) of the last smad_forward_tape and of the
  * device part of the last smad_adjoint (host copies excluded). */
 int smad_adjoint_timing(const smad_ctx *ctx, double *fwd_tape_ms, double *adjoint_ms);

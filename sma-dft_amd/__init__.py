@@ -8,6 +8,10 @@ no-adv/cosNoAdvTrackMult.cpp:724-1147): build an initial condition, create a
 
 There is no CPU fallback: if the shared library is missing, or no GPU is
 visible when a context is created, the calls raise.
+
+With torch in the same process (the autograd wrapper, device buffers),
+import torch before the first context is created: torch's wheel carries its
+own HIP runtime, which must load before the one the library links.
 """
 from __future__ import annotations
 
@@ -49,7 +53,8 @@ EXPORTS = (
     "smad_profile_steps", "smad_kernel_class_name", "smad_forward_tape", "smad_adjoint", "smad_adjoint_timing", "smad_set_tape_budget", "smad_surface",
     "smad_surface_envelope", "smad_r2r_3d_noalloc", "smad_noalloc_last_error",
     "smad_state_fields", "smad_set_steps_taken", "smad_ic_tilted_slab", "smad_ic_two_bumps",
-    "smad_config_params", "smad_config_ic",
+    "smad_config_params", "smad_config_ic", "smad_forward_tape_dev", "smad_adjoint_dev",
+    "smad_set_params",
 )
 
 REDFT10, REDFT01, RODFT10, RODFT01 = 0, 1, 2, 3
@@ -173,6 +178,9 @@ def lib() -> ctypes.CDLL:
         L.smad_kernel_class_name.restype = ctypes.c_char_p
         L.smad_r2r_3d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int),
                                   ctypes.c_int, dp, dp]
+        L.smad_forward_tape_dev.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp]
+        L.smad_adjoint_dev.argtypes = [vp, vp, vp, dp]
+        L.smad_set_params.argtypes = [vp, P(CParams)]
         L.smad_state_fields.argtypes = [vp, P(ctypes.c_int), ctypes.c_int]
         L.smad_set_steps_taken.argtypes = [vp, ctypes.c_long]
         L.smad_r2r_3d_noalloc.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int),
@@ -355,6 +363,27 @@ class Stepper:
         dp = np.zeros(4)
         self._check(lib().smad_adjoint(self._h, _dptr(g), _dptr(d0), _dptr(dp)))
         return d0, dp
+
+    def set_params(self, params: Params):
+        """New coefficients for the same model, grid and precision."""
+        cp = params.c()
+        self._check(lib().smad_set_params(self._h, ctypes.byref(cp)))
+        self.params = params
+        self._cp = cp
+
+    def forward_tape_dev(self, psi0_ptr: int, nsteps: int, ckpt_every: int, psik_ptr: int):
+        """forward_tape on device FP64 buffers given by address: psi0 is the
+        initial condition (the pre-loop runs on it), psi_K is written to
+        psik.  Returns when done."""
+        self._check(lib().smad_forward_tape_dev(self._h, ctypes.c_void_p(psi0_ptr), int(nsteps), int(ckpt_every),
+                                                ctypes.c_void_p(psik_ptr)))
+
+    def adjoint_dev(self, g_ptr: int, d0_ptr: int) -> np.ndarray:
+        """adjoint on device FP64 buffers: cotangent at g, dL/dpsi0 written
+        to d0; returns dL/d(eps, alpha, beta, gamma) (host)."""
+        dp = np.zeros(4)
+        self._check(lib().smad_adjoint_dev(self._h, ctypes.c_void_p(g_ptr), ctypes.c_void_p(d0_ptr), _dptr(dp)))
+        return dp
 
     def adjoint_timing(self):
         """(device ms of the last forward_tape, device ms of the last adjoint)."""

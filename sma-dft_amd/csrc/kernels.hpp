@@ -1255,6 +1255,13 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_nl_z(const T* __restrict__
 
 // a <- |q|^2 a on the spectral grid (the conserved variant's N_hat^0,
 // no-adv/cosNoAdvConserv.cpp:690-697)
+// b = a converted (device FP64 buffers to and from the storage type)
+template <typename S, typename D>
+__global__ __launch_bounds__(WG) void k_convert(const S* __restrict__ a, D* __restrict__ b, long n)
+{
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) b[id] = D(a[id]);
+}
+
 // b = v
 template <typename T>
 __global__ __launch_bounds__(WG) void k_fill(T* __restrict__ b, long n, double v)
@@ -1901,6 +1908,8 @@ struct Launch {
                            const QuasiCoef& qc);
     static void scale_copy(hipStream_t s, const T* a, T* b, long n, double f);
     static void fill(hipStream_t s, T* b, long n, double v);
+    static void from_f64(hipStream_t s, const double* a, T* b, long n);
+    static void to_f64(hipStream_t s, const T* a, double* b, long n);
     static void q_x(hipStream_t s, const T* nl_zy, T* ph, T* pho, T* nlo, Dims d, const QuasiCoef& qc, int first,
                     const Tw<T>& tw);
     static void l1(hipStream_t s, const T* a, const T* b, long n, double* part, int nparts,

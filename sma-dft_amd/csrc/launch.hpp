@@ -153,6 +153,18 @@ void Launch<T>::q_proj_out(hipStream_t s, const T* Sx, const T* Sy, const T* Sz,
 }
 
 template <typename T>
+void Launch<T>::from_f64(hipStream_t s, const double* a, T* b, long n)
+{
+    hipLaunchKernelGGL((k_convert<double, T>), flat_grid(n), dim3(WG), 0, s, a, b, n);
+}
+
+template <typename T>
+void Launch<T>::to_f64(hipStream_t s, const T* a, double* b, long n)
+{
+    hipLaunchKernelGGL((k_convert<T, double>), flat_grid(n), dim3(WG), 0, s, a, b, n);
+}
+
+template <typename T>
 void Launch<T>::fill(hipStream_t s, T* b, long n, double v)
 {
     hipLaunchKernelGGL(k_fill<T>, flat_grid(n), dim3(WG), 0, s, b, n, v);

@@ -166,6 +166,20 @@ struct EngineBase {
     double last_fwd_ms = 0.0, last_adj_ms = 0.0;
     // returns SMAD_ENONFINITE when a gradient holds NaN or Inf
     virtual int adjoint(const double* g_psik, double* dpsi0, double* dparams) = 0;
+    // the same on device FP64 buffers
+    virtual void forward_tape_dev(const double*, int, int, double*)
+    {
+        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
+    }
+    virtual int adjoint_dev(const double*, double*, double*)
+    {
+        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
+    }
+    // new coefficients for the same model, grid and precision
+    virtual void set_params(const smad_params&)
+    {
+        throw ApiError{SMAD_EINVAL, "smad_set_params is implemented for the plain model"};
+    }
     // the fields of the full restart state, in the order to set them back
     virtual int state_fields(int* f, int max) const = 0;
     // loop counter of schedules that depend on the step (the quasi switch)
@@ -258,10 +272,8 @@ struct PlainEngine : EngineBase {
     int nparts = 1024;
     bool psi_valid = false, prev_valid = false;
 
-    PlainEngine(const smad_params& pp, hipStream_t st) : p(pp), s(st)
+    void coefs()
     {
-        d = Dims{p.nx, p.ny, p.nz};
-        n = (size_t)d.total();
         const double pi = M_PI;
         double lx = p.nx * 2.0 * pi / (p.q0 * p.nw), ly = p.ny * 2.0 * pi / (p.q0 * p.nw),
                lz = p.nz * 2.0 * pi / (p.q0 * p.nw);
@@ -277,6 +289,24 @@ struct PlainEngine : EngineBase {
         cf.dtd2 = p.dt / 2;
         cf.scale = 0.125 / ((double)p.nx * p.ny * p.nz);
         cf.conserved = p.model == SMAD_MODEL_PLAIN_CONSERVED ? 1 : 0;
+    }
+
+    void set_params(const smad_params& np) override
+    {
+        if (np.model != p.model || np.nx != p.nx || np.ny != p.ny || np.nz != p.nz || np.precision != p.precision)
+            throw ApiError{SMAD_EINVAL, "smad_set_params: model, grid and precision must not change"};
+        if (!(np.dt > 0) || !(np.nw > 0) || !(np.q0 > 0))
+            throw ApiError{SMAD_EINVAL, "smad_set_params: dt, nw and q0 must be positive"};
+        HIPCHK(hipStreamSynchronize(s));
+        p = np;
+        coefs();
+    }
+
+    PlainEngine(const smad_params& pp, hipStream_t st) : p(pp), s(st)
+    {
+        d = Dims{p.nx, p.ny, p.nz};
+        n = (size_t)d.total();
+        coefs();
         twx.make(p.nx, s);
         twy.make(p.ny, s);
         twz.make(p.nz, s);
@@ -524,8 +554,60 @@ struct PlainEngine : EngineBase {
     int adjoint(const double* g_psik, double* dpsi0, double* dparams) override
     {
         if (tape_K < 1) throw ApiError{SMAD_ESTATE, "adjoint needs a preceding forward_tape"};
-        const int K = tape_K, ck = tape_ck, nseg = tape_nseg;
         upload(s, psi.p, n, g_psik);
+        adjoint_core();
+        double h[4];
+        HIPCHK(hipMemcpyAsync(h, gsum.p, sizeof(h), hipMemcpyDeviceToHost, s));
+        int rc = download(s, psi.p, n, dpsi0);
+        for (int q = 0; q < 4; ++q) {
+            dparams[q] = h[q];
+            if (!std::isfinite(h[q])) rc = SMAD_ENONFINITE;
+        }
+        return rc;
+    }
+
+    // device buffers: psi0_dev is taken as the initial condition (the
+    // pre-loop runs on it), psik_dev receives psi_K
+    void forward_tape_dev(const double* psi0_dev, int K, int ck, double* psik_dev) override
+    {
+        Launch<T>::from_f64(s, psi0_dev, psi.p, (long)n);
+        LAUNCHCHK();
+        preloop();
+        fresh = true;
+        forward_tape(K, ck);
+        materialize();
+        Launch<T>::to_f64(s, psi.p, psik_dev, (long)n);
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(s));
+    }
+
+    int adjoint_dev(const double* g_dev, double* d0_dev, double* dparams) override
+    {
+        if (tape_K < 1) throw ApiError{SMAD_ESTATE, "adjoint needs a preceding forward_tape"};
+        Launch<T>::from_f64(s, g_dev, psi.p, (long)n);
+        LAUNCHCHK();
+        adjoint_core();
+        Launch<T>::to_f64(s, psi.p, d0_dev, (long)n);
+        Launch<T>::sum(s, psi.p, (long)n, part.p, nparts, red.p);  // red[2]: non-finite count
+        LAUNCHCHK();
+        double h[4], r[3];
+        HIPCHK(hipMemcpyAsync(h, gsum.p, sizeof(h), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(r, red.p, sizeof(r), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        int rc = r[2] > 0 ? SMAD_ENONFINITE : SMAD_OK;
+        for (int q = 0; q < 4; ++q) {
+            dparams[q] = h[q];
+            if (!std::isfinite(h[q])) rc = SMAD_ENONFINITE;
+        }
+        return rc;
+    }
+
+    // The adjoint sweep for the cotangent in psi (storage type); leaves
+    // dJ/dpsi0 in psi, the parameter gradients in gsum, and the context at
+    // the end of the taped forward run.
+    void adjoint_core()
+    {
+        const int K = tape_K, ck = tape_ck, nseg = tape_nseg;
         EvPair ev;
         HIPCHK(hipEventRecord(ev.a, s));
         // lt^K = REDFT10^3(g): z and y here, x inside the first adjoint x pass
@@ -585,13 +667,6 @@ struct PlainEngine : EngineBase {
         LAUNCHCHK();
         HIPCHK(hipEventRecord(ev.b, s));
         last_adj_ms = ev.ms();
-        double h[4];
-        HIPCHK(hipMemcpyAsync(h, gsum.p, sizeof(h), hipMemcpyDeviceToHost, s));
-        int rc = download(s, psi.p, n, dpsi0);
-        for (int q = 0; q < 4; ++q) {
-            dparams[q] = h[q];
-            if (!std::isfinite(h[q])) rc = SMAD_ENONFINITE;
-        }
         // leave the context at the end of the taped forward run
         copy_dd(psihat.p, tapes[nseg - 1]->ph[std::min(ck, K - (nseg - 1) * ck) - 1]->p);
         copy_dd(nhat.p, end_nh.p);
@@ -601,7 +676,6 @@ struct PlainEngine : EngineBase {
         HIPCHK(hipStreamSynchronize(s));
         psi_valid = false;
         prev_valid = false;
-        return rc;
     }
 
     // ---------------------------------------------------- surface --
@@ -1538,6 +1612,36 @@ int smad_adjoint(smad_ctx* c, const double* dl_dpsik, double* dl_dpsi0, double* 
     return guarded(c, [&] {
         if (!dl_dpsik || !dl_dpsi0 || !dl_dparams) throw ApiError{SMAD_EINVAL, "null argument"};
         int rc = c->eng->adjoint(dl_dpsik, dl_dpsi0, dl_dparams);
+        if (rc == SMAD_ENONFINITE) c->err = "a gradient holds NaN or Inf";
+        return rc;
+    });
+}
+
+int smad_set_params(smad_ctx* c, const smad_params* params)
+{
+    return guarded(c, [&] {
+        if (!params) throw ApiError{SMAD_EINVAL, "null params"};
+        c->eng->set_params(*params);
+        c->p = *params;
+        return SMAD_OK;
+    });
+}
+
+int smad_forward_tape_dev(smad_ctx* c, const double* psi0_dev, int nsteps, int ckpt_every, double* psik_dev)
+{
+    return guarded(c, [&] {
+        if (!psi0_dev || !psik_dev) throw ApiError{SMAD_EINVAL, "null device buffer"};
+        c->eng->forward_tape_dev(psi0_dev, nsteps, ckpt_every, psik_dev);
+        c->steps = nsteps;
+        return SMAD_OK;
+    });
+}
+
+int smad_adjoint_dev(smad_ctx* c, const double* dl_dpsik_dev, double* dl_dpsi0_dev, double* dl_dparams)
+{
+    return guarded(c, [&] {
+        if (!dl_dpsik_dev || !dl_dpsi0_dev || !dl_dparams) throw ApiError{SMAD_EINVAL, "null argument"};
+        int rc = c->eng->adjoint_dev(dl_dpsik_dev, dl_dpsi0_dev, dl_dparams);
         if (rc == SMAD_ENONFINITE) c->err = "a gradient holds NaN or Inf";
         return rc;
     });

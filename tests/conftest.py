@@ -5,6 +5,11 @@ import sys
 import numpy as np
 import pytest
 
+try:  # torch first: it carries its own HIP runtime, which must load before the library's
+    import torch  # noqa: F401
+except ImportError:
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
