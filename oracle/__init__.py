@@ -52,6 +52,12 @@ def lib():
         L.orc_quasi_step.argtypes = [pp, ctypes.POINTER(QState), ctypes.c_int, dp]
         L.orc_quasi_preloop.restype = None
         L.orc_quasi_step.restype = None
+        L.orc_r2r3d_T.argtypes = [dp, dp] + [ctypes.c_int] * 6
+        L.orc_r2r3d_T.restype = None
+        L.orc_adv_stokes_T.argtypes = [pp, dp, dp, dp, dp, dp, dp]
+        L.orc_adv_stokes_T.restype = None
+        L.orc_adv_adjoint.argtypes = [pp, dp, ctypes.c_int, dp, dp, dp, dp]
+        L.orc_adv_adjoint.restype = None
         L.orc_quasi_project.argtypes = [pp, dp, dp, dp, dp, dp, dp, dp, dp]
         L.orc_quasi_project.restype = None
         L.orc_adv_stokes.argtypes = [pp, dp, dp, dp, dp, dp, dp]
@@ -183,6 +189,38 @@ def plain_adjoint(p, psi0, nsteps, g_psik):
     op = oparams(p)
     lib().orc_plain_adjoint(ctypes.byref(op), _p(psi0), int(nsteps), _p(g), _p(psik), _p(d0), _p(dp))
     return psik, d0, dp
+
+
+def r2r3d_T(x, kinds):
+    """Transpose of the separable transform r2r3d(., kinds)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    lib().orc_r2r3d_T(_p(x), _p(y), *x.shape, *kinds)
+    return y
+
+
+def adv_stokes_T(p, ux, uy, uz):
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (ux, uy, uz)]
+    out = [np.empty_like(a[0]) for _ in range(3)]
+    lib().orc_adv_stokes_T(ctypes.byref(oparams(p)), *[_p(v) for v in a], *[_p(v) for v in out])
+    return tuple(out)
+
+
+def adv_adjoint(p, psi0, nsteps, g_psik):
+    """Advective model: (psi_K, dJ/dpsi0, dJ/d(eps, alpha, beta, gamma))."""
+    psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
+    g = np.ascontiguousarray(g_psik, dtype=np.float64)
+    psik = np.empty_like(psi0)
+    d0 = np.empty_like(psi0)
+    dp = np.zeros(4)
+    lib().orc_adv_adjoint(ctypes.byref(oparams(p)), _p(psi0), int(nsteps), _p(g), _p(psik), _p(d0), _p(dp))
+    return psik, d0, dp
+
+
+def adv_forward(p, psi0, nsteps):
+    o = AdvOracle(p, psi0)
+    o.step(nsteps)
+    return o.psi
 
 
 def plain_forward(p, psi0, nsteps):

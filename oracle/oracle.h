@@ -21,6 +21,10 @@ void orc_r2r1d_naive(const double *x, double *y, int n, int kind);
 void orc_r2r1d(const double *x, double *y, int n, int kind);
 /* separable 3D transform, row-major (i,j,k) with k fastest; in may equal out */
 void orc_r2r3d(const double *in, double *out, int nx, int ny, int nz, int kx, int ky, int kz);
+/* its transpose (per axis: REDFT10^T = REDFT01 diag(2,1,..), REDFT01^T =
+ * diag(1/2,1,..) REDFT10, RODFT10^T = RODFT01 diag(..,1,2), RODFT01^T =
+ * diag(..,1,1/2) RODFT10) */
+void orc_r2r3d_T(const double *in, double *out, int nx, int ny, int nz, int kx, int ky, int kz);
 
 /* Model parameters.  Field order and meaning mirror smad_params of the product
  * header so that one ctypes structure serves both. */
@@ -88,6 +92,13 @@ void orc_adv_surface(const orc_params *p, const double *psi, double *surf, doubl
 /* the velocity projection alone (adv/cosAdvTrackMult.cpp:1103-1274) */
 void orc_adv_stokes(const orc_params *p, const double *sx, const double *sy, const double *sz,
                     double *vx, double *vy, double *vz);
+
+/* transpose of orc_adv_stokes; discrete adjoint of K advective steps
+ * (outputs as orc_plain_adjoint) */
+void orc_adv_stokes_T(const orc_params *p, const double *ux, const double *uy, const double *uz, double *sx,
+                      double *sy, double *sz);
+void orc_adv_adjoint(const orc_params *p, const double *psi0, int nsteps, const double *g_psik,
+                     double *psik_out, double *dpsi0, double *dparams);
 
 /* ------------------------------------------- quasi-incompressible model -- */
 /* quasi/quasi.cpp:990-2251.  All arrays nx*ny*nz (vel, grad_psi: 3x). */

@@ -753,3 +753,202 @@ void orc_quasi_step(const orc_params *p, orc_quasi_state *st, int nsteps, double
     free(lap); free(tmp); free(mu); free(Sx); free(Sy); free(Sz); free(psi_old); free(psiq_prev);
     grid_free(&g);
 }
+
+/* ------------------------------------------------- advective adjoint -- */
+/*
+ * Transpose of the velocity projection orc_adv_stokes (same index shifts
+ * and truncations, scattered back): s* = P^T u*.
+ */
+void orc_adv_stokes_T(const orc_params *p, const double *ux, const double *uy, const double *uz, double *sx,
+                      double *sy, double *sz)
+{
+    grid_t g;
+    grid_make(p, &g);
+    const double nu = p->nu, sc = g.scale;
+    memset(sx, 0, sizeof(double) * g.n);
+    memset(sy, 0, sizeof(double) * g.n);
+    memset(sz, 0, sizeof(double) * g.n);
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                long id = IDX(i, j, k);
+                if (i < g.nx - 1) {
+                    double mq2 = pow(g.sx[i], 2) + pow(g.qy[j], 2) + pow(g.qz[k], 2);
+                    double cm1 = sc / (nu * mq2), cm2 = g.sx[i] / mq2, u = ux[id];
+                    sx[id] += cm1 * (1.0 - cm2 * g.sx[i]) * u;
+                    if (j != 0) sy[IDX(i + 1, j - 1, k)] -= cm1 * cm2 * g.qy[j] * u;
+                    if (k != 0) sz[IDX(i + 1, j, k - 1)] -= cm1 * cm2 * g.qz[k] * u;
+                }
+                if (j < g.ny - 1 && k < g.nz - 1) {
+                    double mq2 = pow(g.qx[i], 2) + pow(g.sy[j], 2) + pow(g.qz[k], 2);
+                    double cm1 = sc / (nu * mq2), cm2 = g.sy[j] / mq2, u = uy[id];
+                    if (i != 0) sx[IDX(i - 1, j + 1, k)] -= cm1 * cm2 * g.qx[i] * u;
+                    sy[id] += cm1 * (1.0 - cm2 * g.sy[j]) * u;
+                    if (k != 0) sz[IDX(i, j + 1, k - 1)] -= cm1 * cm2 * g.qz[k] * u;
+                    mq2 = pow(g.qx[i], 2) + pow(g.qy[j], 2) + pow(g.sz[k], 2);
+                    cm1 = sc / (nu * mq2);
+                    cm2 = g.sz[k] / mq2;
+                    u = uz[id];
+                    if (i != 0) sx[IDX(i - 1, j, k + 1)] -= cm1 * cm2 * g.qx[i] * u;
+                    if (j != 0) sy[IDX(i, j - 1, k + 1)] -= cm1 * cm2 * g.qy[j] * u;
+                    sz[id] += cm1 * (1.0 - cm2 * g.sz[k]) * u;
+                }
+            }
+    grid_free(&g);
+}
+
+static void tr_T(const grid_t *g, double *a, int kx, int ky, int kz)
+{
+    orc_r2r3d_T(a, a, g->nx, g->ny, g->nz, kx, ky, kz);
+}
+
+/*
+ * Discrete adjoint of K advective steps (no reference counterpart; the
+ * forward is orc_adv_step, adv/cosAdvTrackMult.cpp:760-1369).  With the
+ * state psiq^n = s CT(psi^n), N^n = F(psiq^n) (the advective nonlinear
+ * spectrum, steps 1-10 of the loop body) and F_pre (the pre-loop's, without
+ * advection):
+ *   psiq^{n+1} = a psiq^n + c (3 N^n - N^{n-1}),  N^{-1} = F_pre(psi^0),
+ *   a = C1/C2, c = s dt/2/C2 per mode.
+ * With lam^n = dJ/dpsiq^n:  lam^K = iCT^T g,
+ *   lam^n = a lam^{n+1} + J_F(psiq^n)^T w^n,  w^n = 3c lam^{n+1} - c lam^{n+2},
+ *   dJ/dpsi^0 = s CT^T lam^0 + f'(psi^0) CT^T(-c lam^1)   (the pre-loop term)
+ * and J_F^T w, backward through the front half: r = CT^T w;
+ *   u* = -g r, g* = -u r;  S^* = P^T iST^T u*;  S* = ST^T S^*;
+ *   mu* = sum S* g, g* += mu S*;  psi* = f'(psi)(r - mu*);
+ *   J_F^T w = iCT^T psi* - L iCT^T mu* + sum_a D_a^T iST_a^T g*_a
+ * (D_a the DST-shift gradient).  Parameters: dJ/dbeta = sum (r - mu*) psi^3,
+ * dJ/dgamma = -sum (r - mu*) psi^5 (plus the pre-loop term), and
+ * dJ/dL(q) = sum_n [lam^{n+1} dt/2/C2 (psiq^n + psiq^{n+1}) - (iCT^T mu*) psiq^n].
+ * Pinned by central differences and the dot products of the transposes
+ * (tests/test_oracle_adv.py).
+ */
+void orc_adv_adjoint(const orc_params *p, const double *psi0, int nsteps, const double *g_psik,
+                     double *psik_out, double *dpsi0, double *dparams)
+{
+    grid_t g;
+    grid_make(p, &g);
+    const long n = g.n;
+    const int K = nsteps;
+    const double dtd2 = p->dt / 2, sc = g.scale;
+    double *psis = malloc(sizeof(double) * n * (K + 1));
+    double *nq = malloc(sizeof(double) * n), *nqp = malloc(sizeof(double) * n);
+    memcpy(psis, psi0, sizeof(double) * n);
+    orc_adv_preloop(p, psi0, nq);
+    for (int s = 0; s < K; ++s) {
+        memcpy(psis + (long)(s + 1) * n, psis + (long)s * n, sizeof(double) * n);
+        orc_adv_step(p, psis + (long)(s + 1) * n, nq, nqp, NULL, NULL, 1);
+    }
+    if (psik_out) memcpy(psik_out, psis + (long)K * n, sizeof(double) * n);
+
+    double *psi = malloc(sizeof(double) * n), *psiq = malloc(sizeof(double) * n), *psiq1 = malloc(sizeof(double) * n);
+    double *gx = malloc(sizeof(double) * n), *gy = malloc(sizeof(double) * n), *gz = malloc(sizeof(double) * n);
+    double *vx = malloc(sizeof(double) * n), *vy = malloc(sizeof(double) * n), *vz = malloc(sizeof(double) * n);
+    double *sp2 = malloc(sizeof(double) * n), *sx = malloc(sizeof(double) * n), *sy = malloc(sizeof(double) * n);
+    double *sz = malloc(sizeof(double) * n);
+    double *lam1 = calloc(n, sizeof(double)), *lam2 = calloc(n, sizeof(double)), *lam0 = malloc(sizeof(double) * n);
+    double *r = malloc(sizeof(double) * n), *mub = malloc(sizeof(double) * n), *psb = malloc(sizeof(double) * n);
+    double *ub[3], *gb[3], *Sb[3];
+    for (int a = 0; a < 3; ++a) {
+        ub[a] = malloc(sizeof(double) * n);
+        gb[a] = malloc(sizeof(double) * n);
+        Sb[a] = malloc(sizeof(double) * n);
+    }
+    static const int ist[3][3] = {{ORC_RODFT01, ORC_REDFT01, ORC_REDFT01},
+                                  {ORC_REDFT01, ORC_RODFT01, ORC_REDFT01},
+                                  {ORC_REDFT01, ORC_REDFT01, ORC_RODFT01}};
+    static const int st[3][3] = {{ORC_RODFT10, ORC_REDFT10, ORC_REDFT10},
+                                 {ORC_REDFT10, ORC_RODFT10, ORC_REDFT10},
+                                 {ORC_REDFT10, ORC_REDFT10, ORC_RODFT10}};
+    double deps = 0, dalpha = 0, dbeta = 0, dgamma = 0;
+
+    /* lam^K = iCT^T g */
+    memcpy(lam1, g_psik, sizeof(double) * n);
+    tr_T(&g, lam1, ORC_REDFT01, ORC_REDFT01, ORC_REDFT01);
+    for (int s = K - 1; s >= 0; --s) {
+        /* the front half of step s from psi^s, as the forward computes it */
+        memcpy(psi, psis + (long)s * n, sizeof(double) * n);
+        adv_front(p, &g, psi, psiq, gx, gy, gz, vx, vy, vz, sp2, sx, sy, sz);
+        ct(&g, psis + (long)(s + 1) * n, psiq1);
+        for (long i = 0; i < n; ++i) psiq1[i] *= sc;
+        /* w = 3c lam^{s+1} - c lam^{s+2}; the update's own L terms */
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    double al = alin_at(p, &g, i, j, k), c2 = 1.0 - dtd2 * al;
+                    double c = sc * dtd2 / c2, t = mq2c_at(&g, i, j, k) - p->q0 * p->q0;
+                    double dl = lam1[id] * dtd2 / c2 * (psiq[id] + psiq1[id]);
+                    deps += dl;
+                    dalpha -= t * t * dl;
+                    r[id] = 3.0 * c * lam1[id] - c * lam2[id];
+                }
+        tr_T(&g, r, ORC_REDFT10, ORC_REDFT10, ORC_REDFT10); /* r = CT^T w */
+        for (long id = 0; id < n; ++id) {
+            ub[0][id] = -gx[id] * r[id]; ub[1][id] = -gy[id] * r[id]; ub[2][id] = -gz[id] * r[id];
+            gb[0][id] = -vx[id] * r[id]; gb[1][id] = -vy[id] * r[id]; gb[2][id] = -vz[id] * r[id];
+        }
+        for (int a = 0; a < 3; ++a) tr_T(&g, ub[a], ist[a][0], ist[a][1], ist[a][2]);
+        orc_adv_stokes_T(p, ub[0], ub[1], ub[2], Sb[0], Sb[1], Sb[2]);
+        for (int a = 0; a < 3; ++a) tr_T(&g, Sb[a], st[a][0], st[a][1], st[a][2]);
+        for (long id = 0; id < n; ++id) {
+            double x = psi[id], x2 = x * x;
+            double mu = sp2[id] - p->beta * x2 * x + p->gamma * x2 * x2 * x;
+            double fp = 3.0 * p->beta * x2 - 5.0 * p->gamma * x2 * x2;
+            double m = Sb[0][id] * gx[id] + Sb[1][id] * gy[id] + Sb[2][id] * gz[id];
+            gb[0][id] += mu * Sb[0][id]; gb[1][id] += mu * Sb[1][id]; gb[2][id] += mu * Sb[2][id];
+            mub[id] = m;
+            psb[id] = fp * (r[id] - m);
+            dbeta += (r[id] - m) * x2 * x;
+            dgamma -= (r[id] - m) * x2 * x2 * x;
+        }
+        tr_T(&g, psb, ORC_REDFT01, ORC_REDFT01, ORC_REDFT01);
+        tr_T(&g, mub, ORC_REDFT01, ORC_REDFT01, ORC_REDFT01);
+        for (int a = 0; a < 3; ++a) tr_T(&g, gb[a], ist[a][0], ist[a][1], ist[a][2]);
+        /* lam^s = a lam^{s+1} + iCT^T psi* - L iCT^T mu* + sum_a D_a^T g*_a */
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    double al = alin_at(p, &g, i, j, k);
+                    double aa = (1.0 + dtd2 * al) / (1.0 - dtd2 * al);
+                    double t = mq2c_at(&g, i, j, k) - p->q0 * p->q0;
+                    double dl = -mub[id] * psiq[id];
+                    deps += dl;
+                    dalpha -= t * t * dl;
+                    double v = aa * lam1[id] + psb[id] - al * mub[id];
+                    if (i > 0) v -= g.sx[i - 1] * gb[0][IDX(i - 1, j, k)];
+                    if (j > 0) v -= g.sy[j - 1] * gb[1][IDX(i, j - 1, k)];
+                    if (k > 0) v -= g.sz[k - 1] * gb[2][IDX(i, j, k - 1)];
+                    lam0[id] = v;
+                }
+        if (s == 0) {
+            /* the pre-loop history N^{-1} = CT(beta psi0^3 - gamma psi0^5) enters psiq^1 with -c */
+            for (int i = 0; i < g.nx; ++i)
+                for (int j = 0; j < g.ny; ++j)
+                    for (int k = 0; k < g.nz; ++k) {
+                        long id = IDX(i, j, k);
+                        double c2 = 1.0 - dtd2 * alin_at(p, &g, i, j, k);
+                        r[id] = -sc * dtd2 / c2 * lam1[id];
+                    }
+            tr_T(&g, r, ORC_REDFT10, ORC_REDFT10, ORC_REDFT10);
+        }
+        double *sw = lam2; lam2 = lam1; lam1 = lam0; lam0 = sw;
+    }
+    /* dJ/dpsi0 = s CT^T lam^0 + f'(psi0) r_pre */
+    memcpy(dpsi0, lam1, sizeof(double) * n);
+    tr_T(&g, dpsi0, ORC_REDFT10, ORC_REDFT10, ORC_REDFT10);
+    for (long id = 0; id < n; ++id) {
+        double x = psi0[id], x2 = x * x;
+        double fp = 3.0 * p->beta * x2 - 5.0 * p->gamma * x2 * x2;
+        dpsi0[id] = sc * dpsi0[id] + fp * r[id];
+        dbeta += r[id] * x2 * x;
+        dgamma -= r[id] * x2 * x2 * x;
+    }
+    dparams[0] = deps; dparams[1] = dalpha; dparams[2] = dbeta; dparams[3] = dgamma;
+    for (int a = 0; a < 3; ++a) { free(ub[a]); free(gb[a]); free(Sb[a]); }
+    free(psis); free(nq); free(nqp); free(psi); free(psiq); free(psiq1); free(gx); free(gy); free(gz);
+    free(vx); free(vy); free(vz); free(sp2); free(sx); free(sy); free(sz); free(lam1); free(lam2); free(lam0);
+    free(r); free(mub); free(psb);
+    grid_free(&g);
+}

@@ -289,3 +289,45 @@ void orc_r2r3d(const double *in, double *out, int nx, int ny, int nz, int kx, in
     axis_pass(out, nx, ny, nz, 1, ky);
     axis_pass(out, nx, ny, nz, 0, kx);
 }
+
+/* Multiplies the plane `at` (0 or n-1) of one axis by w. */
+static void weight_plane(double *a, int nx, int ny, int nz, int axis, int at, double w)
+{
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                int c = axis == 0 ? i : axis == 1 ? j : k;
+                if (c == at) a[((long)i * ny + j) * nz + k] *= w;
+            }
+}
+
+/* Transpose of the separable transform with kinds (kx, ky, kz), axis by
+ * axis: REDFT10^T = REDFT01 diag(2,1,..,1), REDFT01^T = diag(1/2,1,..,1)
+ * REDFT10, RODFT10^T = RODFT01 diag(1,..,1,2), RODFT01^T = diag(1,..,1,1/2)
+ * RODFT10 (the unnormalised FFTW kinds). */
+void orc_r2r3d_T(const double *in, double *out, int nx, int ny, int nz, int kx, int ky, int kz)
+{
+    if (out != in) memcpy(out, in, sizeof(double) * (size_t)nx * ny * nz);
+    const int kinds[3] = {kx, ky, kz}, len[3] = {nx, ny, nz};
+    for (int axis = 0; axis < 3; ++axis) {
+        const int last = len[axis] - 1;
+        switch (kinds[axis]) {
+        case ORC_REDFT10:
+            weight_plane(out, nx, ny, nz, axis, 0, 2.0);
+            axis_pass(out, nx, ny, nz, axis, ORC_REDFT01);
+            break;
+        case ORC_REDFT01:
+            axis_pass(out, nx, ny, nz, axis, ORC_REDFT10);
+            weight_plane(out, nx, ny, nz, axis, 0, 0.5);
+            break;
+        case ORC_RODFT10:
+            weight_plane(out, nx, ny, nz, axis, last, 2.0);
+            axis_pass(out, nx, ny, nz, axis, ORC_RODFT01);
+            break;
+        default:
+            axis_pass(out, nx, ny, nz, axis, ORC_RODFT10);
+            weight_plane(out, nx, ny, nz, axis, last, 0.5);
+            break;
+        }
+    }
+}

@@ -206,3 +206,66 @@ def test_stokes_flow_dissipates(smadft, oracle):
     work = sum(float(np.sum(o.vel[ax] * S[ax])) for ax in range(3))
     scale_ = sum(float(np.sum(np.abs(o.vel[ax] * S[ax]))) for ax in range(3))
     assert work > 0.2 * scale_
+
+
+# ---------------------------------------------------------------------------
+# The advective adjoint (orc_adv_adjoint): transposes by dot products, the
+# gradients by central differences.
+
+@pytest.mark.parametrize("kinds", [(0, 0, 0), (1, 1, 1), (2, 0, 0), (1, 3, 1), (3, 2, 1)])
+def test_transform_transposes(oracle, kinds):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((8, 16, 12))
+    y = rng.standard_normal((8, 16, 12))
+    lhs = np.sum(oracle.r2r3d(x, kinds) * y)
+    rhs = np.sum(x * oracle.r2r3d_T(y, kinds))
+    assert lhs == pytest.approx(rhs, rel=1e-13)
+
+
+def test_projection_transpose(smadft, oracle):
+    """<P S, U> = <S, P^T U> for the projection with its shifted reads and
+    truncations (adv/cosAdvTrackMult.cpp:1103-1274)."""
+    p = adv_params(smadft, (8, 12, 16))
+    rng = np.random.default_rng(2)
+    S = [rng.standard_normal(p.shape) for _ in range(3)]
+    U = [rng.standard_normal(p.shape) for _ in range(3)]
+    PS = oracle.adv_stokes(p, *S)
+    PtU = oracle.adv_stokes_T(p, *U)
+    lhs = sum(np.sum(a * b) for a, b in zip(PS, U))
+    rhs = sum(np.sum(a * b) for a, b in zip(S, PtU))
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+
+
+def test_adv_adjoint_central_differences(smadft, oracle):
+    """dJ/d(eps, alpha, beta, gamma) and dJ/dpsi0 along a random direction
+    of J = |psi_K - target|^2 after K advective steps, against central
+    differences (h = 1e-6) of the forward oracle."""
+    p = adv_params(smadft, (16, 16, 16), dt=0.01)
+    psi0 = smadft.ic_curved(p, 1.328, 1e-2, seed=2)
+    K = 6
+    tgt = np.random.default_rng(3).standard_normal(p.shape) * 0.1
+    psik = oracle.adv_forward(p, psi0, K)
+    _, d0, dp = oracle.adv_adjoint(p, psi0, K, 2 * (psik - tgt))
+    import dataclasses
+    h = 1e-6
+
+    def J(q, x0):
+        return float(np.sum((oracle.adv_forward(q, x0, K) - tgt) ** 2))
+
+    for i, name in enumerate(("eps", "alpha", "beta", "gamma")):
+        v = getattr(p, name)
+        fd = (J(dataclasses.replace(p, **{name: v + h}), psi0) - J(dataclasses.replace(p, **{name: v - h}), psi0)) / (2 * h)
+        assert dp[i] == pytest.approx(fd, rel=1e-6), name
+    # dJ/dpsi0: along a random unit direction the derivative is ~1e-5 of J,
+    # so h = 1e-6 would be roundoff-limited; h = 1e-4 (central differences
+    # converge there to 1e-8), and h = 1e-6 along the gradient itself
+    v = np.random.default_rng(4).standard_normal(p.shape)
+    v /= np.linalg.norm(v)
+    fd = (J(p, psi0 + 1e-4 * v) - J(p, psi0 - 1e-4 * v)) / 2e-4
+    assert float(np.sum(d0 * v)) == pytest.approx(fd, rel=1e-6)
+    u = d0 / np.linalg.norm(d0)
+    fd = (J(p, psi0 + h * u) - J(p, psi0 - h * u)) / (2 * h)
+    assert float(np.sum(d0 * u)) == pytest.approx(fd, rel=1e-6)
+    # the flow matters: the advective gradient differs from the plain model's
+    _, d0p, _ = oracle.plain_adjoint(p, psi0, K, 2 * (oracle.plain_forward(p, psi0, K) - tgt))
+    assert np.linalg.norm(d0 - d0p) > 1e-6 * np.linalg.norm(d0)
