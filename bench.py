@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--adj-tape-gib", type=float, default=-1.0,
                     help="HBM budget (GiB) for the hbm_tape adjoint leg; < 0: the library default")
     ap.add_argument("--model-steps", type=int, default=10)
+    ap.add_argument("--adv-adj-steps", type=int, default=20,
+                    help="K of the advective adjoint timing leg (0 = skip)")
     ap.add_argument("--adv-grid", type=int, default=256)
     ap.add_argument("--adv-acc", type=int, default=1000)
     ap.add_argument("--quasi-grid", type=int, default=384)
@@ -313,6 +315,25 @@ def main():
                     for k in fields}
                 entry["accuracy_after_steps"] = nacc
                 log(f"{label} fp32 errors: {entry['fp32_rel_l2_vs_fp64']}")
+            if model == smadft.MODEL_ADV and args.adv_adj_steps > 0:
+                # advective adjoint: (taped forward + adjoint) / forward over K steps
+                Ka = args.adv_adj_steps
+                aentry = {"K": Ka, "tape": "psi_hat of every step"}
+                for prec in (64, 32):
+                    pm.precision = prec
+                    with smadft.Stepper(pm, ic) as s:
+                        s.step(2)
+                        s.set_field(smadft.FIELD_PSI, ic)
+                        fwd = s.time_steps(Ka)
+                        s.set_field(smadft.FIELD_PSI, ic)
+                        s.forward_tape(Ka, Ka)
+                        s.adjoint(np.zeros(pm.shape))
+                        t_ms, a_ms = s.adjoint_timing()
+                    aentry[f"fp{prec}"] = {"forward_ms": fwd, "forward_tape_ms": t_ms, "adjoint_ms": a_ms,
+                                           "ratio": (t_ms + a_ms) / fwd}
+                    log(f"advective adjoint fp{prec}: fwd {fwd:.1f} ms, taped {t_ms:.1f} ms, adjoint {a_ms:.1f} ms, "
+                        f"ratio {(t_ms + a_ms) / fwd:.3f}")
+                entry["adjoint"] = aentry
             models[label] = entry
 
     # the reference programs' 1024x512x512 grid, resident on one GPU

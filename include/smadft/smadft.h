@@ -201,14 +201,21 @@ int smad_surface_envelope(smad_ctx *ctx, double *surf, double *vel, double *curv
 int smad_sync(smad_ctx *ctx);
 
 /* ----------------------------------------------------------------------
- * Adjoint (plain model; no counterpart in the reference, which has no
- * gradients).  smad_forward_tape runs nsteps steps from the pre-loop state
- * (right after smad_init or smad_set_field(PSI)), saving (psi_hat, N_hat)
- * every ckpt_every steps.  smad_adjoint then takes the cotangent
- * dL/dpsi_K (host, FP64) and returns dL/dpsi_0 (host) and
- * dL/d(eps, alpha, beta, gamma) by the discrete adjoint of every step,
- * recomputing each segment from its checkpoint.  Afterwards the context is
- * left at step K, as after the forward run. */
+ * Adjoint (plain and advective models; no counterpart in the reference,
+ * which has no gradients).  smad_forward_tape runs nsteps steps from the
+ * pre-loop state (right after smad_init or smad_set_field(PSI)).  The plain
+ * model saves (psi_hat, N_hat, its xy-transformed psi_hat) every ckpt_every
+ * steps and tapes segments as the budget allows (smad_set_tape_budget); the
+ * advective model tapes psi_hat of every step (nsteps + 1 fields;
+ * ckpt_every is not used) and recomputes each step's front half (psi,
+ * grad psi, the force, the Stokes velocity) in the backward sweep.
+ * smad_adjoint then takes the cotangent dL/dpsi_K (host, FP64) and returns
+ * dL/dpsi_0 (host) and dL/d(eps, alpha, beta, gamma) by the discrete adjoint
+ * of every step (for the advective model the linearised psi^3/psi^5 terms,
+ * the force mu grad psi, the projection with its shifted reads and u . grad
+ * psi, transposed).  Afterwards the context is left at step K, as after the
+ * forward run (the advective work fields GRAD and VEL then hold the sweep's
+ * last recomputation). */
 int smad_forward_tape(smad_ctx *ctx, int nsteps, int ckpt_every);
 
 /* Device memory (GiB) the forward run may spend on tape beyond the one

@@ -1856,6 +1856,258 @@ __global__ __launch_bounds__(WG) void k_fold(const double* __restrict__ part, in
 
 // ------------------------------------------------------------------------
 // Host-side launchers (explicitly instantiated for float and double).
+// ------------------------------------------------------------------------
+// Adjoint of the advective step (adv/cosAdvTrackMult.cpp:847-1369).  The
+// backward sweep of step n takes the front half of the step recomputed from
+// the taped psi_hat^n (psi, grad psi, Sp2, S_hat, u) and propagates
+// lam^n = a lam^{n+1} + J_F^T w^n,  w^n = 3c lam^{n+1} - c lam^{n+2}
+// through the transposed stages (transform transposes as axis passes with
+// edge weights: REDFT10^T = REDFT01 diag(2,1,..), REDFT01^T = diag(1/2,1,..)
+// REDFT10, RODFT10^T = RODFT01 diag(..,1,2), RODFT01^T = diag(..,1,1/2)
+// RODFT10).  FP64 arithmetic; parameter gradients as FP64 per-block sums.
+
+// DCT weight w0 on index 0 of every axis (2 for REDFT10^T's input, 1/2 for
+// REDFT01^T's output)
+__device__ __forceinline__ double dct_edges(int i, int j, int k, double w0)
+{
+    return (i == 0 ? w0 : 1.0) * (j == 0 ? w0 : 1.0) * (k == 0 ? w0 : 1.0);
+}
+// mixed ST_a (RODFT on axis a, REDFT on the others): weight wl on the DST
+// axis's last index, w0 on the DCT axes' index 0
+__device__ __forceinline__ double st_edges(int a, int i, int j, int k, const Dims& d, double wl, double w0)
+{
+    const int c[3] = {i, j, k}, nn[3] = {d.nx, d.ny, d.nz};
+    double w = 1.0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) w *= (b == a) ? (c[b] == nn[b] - 1 ? wl : 1.0) : (c[b] == 0 ? w0 : 1.0);
+    return w;
+}
+
+// mode 0: r = D (3c lam1 - c lam2) (CT^T's input weights), and the update's
+//         own dJ/dL(q) += lam1 dt/2/C2 (ph_n + ph_n1) into acc (eps, alpha);
+// mode 1: r = D (-c lam1) (the pre-loop history term of the first step);
+// mode 2: r = D s lam1 (s CT^T lam^0, the initial condition's cotangent);
+// mode 3: r = Wc lam1 with Wc = diag(1/2,1,..) per axis (iCT^T's output)
+template <typename T>
+__global__ __launch_bounds__(WG) void k_aa_w(const T* __restrict__ lam1, const T* __restrict__ lam2,
+                                             const T* __restrict__ phn, const T* __restrict__ phn1,
+                                             T* __restrict__ r, Dims d, PlainCoef cf, int mode,
+                                             double* __restrict__ acc)
+{
+    const long n = d.total();
+    double sums[2] = {0.0, 0.0};
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        const double mq2 = mq2_at(cf, i, j, k), t = mq2 - cf.q0sq;
+        const double al = cf.eps - cf.alpha * t * t, c2 = 1.0 - cf.dtd2 * al, c = cf.scale * cf.dtd2 / c2;
+        const double D = dct_edges(i, j, k, 2.0), l1 = (double)lam1[id];
+        double v;
+        if (mode == 0) {
+            v = 3.0 * c * l1 - c * (double)lam2[id];
+            const double dl = l1 * cf.dtd2 / c2 * ((double)phn[id] + (double)phn1[id]);
+            sums[0] += dl;
+            sums[1] -= t * t * dl;
+        } else if (mode == 3) {
+            r[id] = T(dct_edges(i, j, k, 0.5) * l1);  // REDFT01^T's output weights (iCT^T)
+            continue;
+        } else {
+            v = mode == 1 ? -c * l1 : cf.scale * l1;
+        }
+        r[id] = T(D * v);
+    }
+    if (mode == 0) block_accumulate<2>(sums, acc);
+}
+
+// ub_a = -g_a r, gb_a = -u_a r (the cotangents of u and of grad psi from
+// N = beta psi^3 - gamma psi^5 - u . grad psi)
+template <typename T>
+__global__ __launch_bounds__(WG) void k_aa_p1(const T* __restrict__ r, const T* __restrict__ gx,
+                                              const T* __restrict__ gy, const T* __restrict__ gz,
+                                              const T* __restrict__ ux, const T* __restrict__ uy,
+                                              const T* __restrict__ uz, T* __restrict__ ubx, T* __restrict__ uby,
+                                              T* __restrict__ ubz, T* __restrict__ gbx, T* __restrict__ gby,
+                                              T* __restrict__ gbz, long n)
+{
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        const double rv = (double)r[id];
+        ubx[id] = T(-(double)gx[id] * rv);
+        uby[id] = T(-(double)gy[id] * rv);
+        ubz[id] = T(-(double)gz[id] * rv);
+        gbx[id] = T(-(double)ux[id] * rv);
+        gby[id] = T(-(double)uy[id] * rv);
+        gbz[id] = T(-(double)uz[id] * rv);
+    }
+}
+
+// the projection's coefficients at a component's own index (as
+// k_adv_stokes_z): comp 0 x, 1 y, 2 z; returns cm1, cm2 for that component
+__device__ __forceinline__ void proj_coefs(int comp, int i, int j, int k, const PlainCoef& cf, double nu, double& cm1,
+                                           double& cm2)
+{
+    const double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
+    double m, qs;
+    if (comp == 0) {
+        qs = (i + 1) * cf.qfx;
+        m = qs * qs + qy * qy + qz * qz;
+    } else if (comp == 1) {
+        qs = (j + 1) * cf.qfy;
+        m = qx * qx + qs * qs + qz * qz;
+    } else {
+        qs = (k + 1) * cf.qfz;
+        m = qx * qx + qy * qy + qs * qs;
+    }
+    cm1 = cf.scale / (nu * m);
+    cm2 = qs / m;
+}
+
+// S_bar_hat = V (P^T (W u_bar_hat)): W the post-weights of iST_a^T applied
+// to the raw forward transforms ub of the velocity cotangent, P^T the
+// transposed projection in gather form (each S_hat component collects the
+// velocity outputs that read it), V the pre-weights of ST_a^T
+template <typename T>
+__global__ __launch_bounds__(WG) void k_aa_projT(const T* __restrict__ ubx, const T* __restrict__ uby,
+                                                 const T* __restrict__ ubz, T* __restrict__ sbx,
+                                                 T* __restrict__ sby, T* __restrict__ sbz, Dims d, AdvCoef ac)
+{
+    const PlainCoef& cf = ac.pc;
+    const long n = d.total(), SX = (long)d.ny * d.nz, SY = d.nz;
+    const double nu = ac.nu;
+    auto U = [&](int comp, long id, int i, int j, int k) {
+        const T* u = comp == 0 ? ubx : comp == 1 ? uby : ubz;
+        return (double)u[id] * st_edges(comp, i, j, k, d, 0.5, 0.5);
+    };
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int I, J, K;
+        unflat(id, d, I, J, K);
+        double cm1, cm2, vx = 0.0, vy = 0.0, vz = 0.0;
+        // S_x[I,J,K]
+        if (I <= d.nx - 2) {
+            proj_coefs(0, I, J, K, cf, nu, cm1, cm2);
+            vx += cm1 * (1.0 - cm2 * (I + 1) * cf.qfx) * U(0, id, I, J, K);
+            if (J >= 1 && K <= d.nz - 2) {  // u_y at (I+1, J-1, K)
+                proj_coefs(1, I + 1, J - 1, K, cf, nu, cm1, cm2);
+                vx -= cm1 * cm2 * (I + 1) * cf.qfx * U(1, id + SX - SY, I + 1, J - 1, K);
+            }
+            if (K >= 1 && J <= d.ny - 2) {  // u_z at (I+1, J, K-1)
+                proj_coefs(2, I + 1, J, K - 1, cf, nu, cm1, cm2);
+                vx -= cm1 * cm2 * (I + 1) * cf.qfx * U(2, id + SX - 1, I + 1, J, K - 1);
+            }
+        }
+        // S_y[I,J,K]
+        if (I >= 1 && J <= d.ny - 2) {  // u_x at (I-1, J+1, K)
+            proj_coefs(0, I - 1, J + 1, K, cf, nu, cm1, cm2);
+            vy -= cm1 * cm2 * (J + 1) * cf.qfy * U(0, id - SX + SY, I - 1, J + 1, K);
+        }
+        if (J <= d.ny - 2 && K <= d.nz - 2) {
+            proj_coefs(1, I, J, K, cf, nu, cm1, cm2);
+            vy += cm1 * (1.0 - cm2 * (J + 1) * cf.qfy) * U(1, id, I, J, K);
+        }
+        if (J <= d.ny - 3 && K >= 1) {  // u_z at (I, J+1, K-1)
+            proj_coefs(2, I, J + 1, K - 1, cf, nu, cm1, cm2);
+            vy -= cm1 * cm2 * (J + 1) * cf.qfy * U(2, id + SY - 1, I, J + 1, K - 1);
+        }
+        // S_z[I,J,K]
+        if (I >= 1 && K <= d.nz - 2) {  // u_x at (I-1, J, K+1)
+            proj_coefs(0, I - 1, J, K + 1, cf, nu, cm1, cm2);
+            vz -= cm1 * cm2 * (K + 1) * cf.qfz * U(0, id - SX + 1, I - 1, J, K + 1);
+        }
+        if (J >= 1 && K <= d.nz - 3) {  // u_y at (I, J-1, K+1)
+            proj_coefs(1, I, J - 1, K + 1, cf, nu, cm1, cm2);
+            vz -= cm1 * cm2 * (K + 1) * cf.qfz * U(1, id - SY + 1, I, J - 1, K + 1);
+        }
+        if (J <= d.ny - 2 && K <= d.nz - 2) {
+            proj_coefs(2, I, J, K, cf, nu, cm1, cm2);
+            vz += cm1 * (1.0 - cm2 * (K + 1) * cf.qfz) * U(2, id, I, J, K);
+        }
+        sbx[id] = T(vx * st_edges(0, I, J, K, d, 2.0, 2.0));
+        sby[id] = T(vy * st_edges(1, I, J, K, d, 2.0, 2.0));
+        sbz[id] = T(vz * st_edges(2, I, J, K, d, 2.0, 2.0));
+    }
+}
+
+// From the force's cotangent S_bar (real): mu_bar = S_bar . grad psi,
+// g_bar += mu S_bar, psi_bar = f'(psi) (r - mu_bar) with mu = Sp2 - beta
+// psi^3 + gamma psi^5 and f' = 3 beta psi^2 - 5 gamma psi^4;
+// dJ/dbeta += (r - mu_bar) psi^3, dJ/dgamma -= (r - mu_bar) psi^5.
+// psi_bar overwrites r, mu_bar goes to mub.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_aa_p2(const T* __restrict__ sbx, const T* __restrict__ sby,
+                                              const T* __restrict__ sbz, const T* __restrict__ gx,
+                                              const T* __restrict__ gy, const T* __restrict__ gz,
+                                              const T* __restrict__ psi, const T* __restrict__ sp2,
+                                              T* __restrict__ r, T* __restrict__ mub, T* __restrict__ gbx,
+                                              T* __restrict__ gby, T* __restrict__ gbz, long n, PlainCoef cf,
+                                              double* __restrict__ acc)
+{
+    double sums[2] = {0.0, 0.0};
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        const double x = (double)psi[id], x2 = x * x, x3 = x2 * x;
+        const double mu = (double)sp2[id] - cf.beta * x3 + cf.gamma * x3 * x2;
+        const double fp = 3.0 * cf.beta * x2 - 5.0 * cf.gamma * x2 * x2;
+        const double s1 = (double)sbx[id], s2 = (double)sby[id], s3 = (double)sbz[id];
+        const double m = s1 * (double)gx[id] + s2 * (double)gy[id] + s3 * (double)gz[id];
+        gbx[id] = T((double)gbx[id] + mu * s1);
+        gby[id] = T((double)gby[id] + mu * s2);
+        gbz[id] = T((double)gbz[id] + mu * s3);
+        const double e = (double)r[id] - m;
+        r[id] = T(fp * e);
+        mub[id] = T(m);
+        sums[0] += e * x3;
+        sums[1] -= e * x3 * x2;
+    }
+    block_accumulate<2>(sums, acc);
+}
+
+// lam0 = a lam1 + Wc psb - L Wc mub + sum_a D_a^T (W_a gb_a): the raw
+// forward transforms psb, mub (REDFT10^3) and gb_a (ST_a) of the real
+// cotangents, Wc / W_a the post-weights of iCT^T / iST_a^T, D_a^T the
+// transposed DST-shift gradient (D_x^T y)[i] = -qs(i-1) y[i-1];
+// dJ/dL(q) -= (Wc mub) ph_n into acc (eps, alpha)
+template <typename T>
+__global__ __launch_bounds__(WG) void k_aa_combine(const T* __restrict__ lam1, const T* __restrict__ psb,
+                                                   const T* __restrict__ mub, const T* __restrict__ gbx,
+                                                   const T* __restrict__ gby, const T* __restrict__ gbz,
+                                                   const T* __restrict__ phn, T* __restrict__ lam0, Dims d,
+                                                   PlainCoef cf, double* __restrict__ acc)
+{
+    const long n = d.total(), SX = (long)d.ny * d.nz, SY = d.nz;
+    double sums[2] = {0.0, 0.0};
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        const double t = mq2_at(cf, i, j, k) - cf.q0sq, al = cf.eps - cf.alpha * t * t;
+        const double a = (1.0 + cf.dtd2 * al) / (1.0 - cf.dtd2 * al);
+        const double wc = dct_edges(i, j, k, 0.5);
+        const double mb = wc * (double)mub[id];
+        double v = a * (double)lam1[id] + wc * (double)psb[id] - al * mb;
+        if (i > 0) v -= i * cf.qfx * (double)gbx[id - SX] * st_edges(0, i - 1, j, k, d, 0.5, 0.5);
+        if (j > 0) v -= j * cf.qfy * (double)gby[id - SY] * st_edges(1, i, j - 1, k, d, 0.5, 0.5);
+        if (k > 0) v -= k * cf.qfz * (double)gbz[id - 1] * st_edges(2, i, j, k - 1, d, 0.5, 0.5);
+        lam0[id] = T(v);
+        const double dl = -mb * (double)phn[id];
+        sums[0] += dl;
+        sums[1] -= t * t * dl;
+    }
+    block_accumulate<2>(sums, acc);
+}
+
+// dpsi0 += f'(psi0) r_pre (the pre-loop history's term), dJ/dbeta +=
+// r_pre psi0^3, dJ/dgamma -= r_pre psi0^5
+template <typename T>
+__global__ __launch_bounds__(WG) void k_aa_pre(const T* __restrict__ psi0, const T* __restrict__ rpre,
+                                               T* __restrict__ d0, long n, PlainCoef cf, double* __restrict__ acc)
+{
+    double sums[2] = {0.0, 0.0};
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        const double x = (double)psi0[id], x2 = x * x, x3 = x2 * x, rv = (double)rpre[id];
+        d0[id] = T((double)d0[id] + (3.0 * cf.beta * x2 - 5.0 * cf.gamma * x2 * x2) * rv);
+        sums[0] += rv * x3;
+        sums[1] -= rv * x3 * x2;
+    }
+    block_accumulate<2>(sums, acc);
+}
+
 template <typename T>
 struct Launch {
     static void axis(hipStream_t s, const T* in, T* out, Dims d, int axis, int kind, T scale,
@@ -1908,6 +2160,20 @@ struct Launch {
                            const QuasiCoef& qc);
     static void scale_copy(hipStream_t s, const T* a, T* b, long n, double f);
     static void fill(hipStream_t s, T* b, long n, double v);
+    // advective adjoint stages (accumulators sized by flat_blocks(n))
+    static int flat_blocks(long n);
+    static void aa_w(hipStream_t s, const T* lam1, const T* lam2, const T* phn, const T* phn1, T* r, Dims d,
+                     const PlainCoef& cf, int mode, double* acc);
+    static void aa_p1(hipStream_t s, const T* r, const T* gx, const T* gy, const T* gz, const T* ux, const T* uy,
+                      const T* uz, T* ubx, T* uby, T* ubz, T* gbx, T* gby, T* gbz, long n);
+    static void aa_projT(hipStream_t s, const T* ubx, const T* uby, const T* ubz, T* sbx, T* sby, T* sbz, Dims d,
+                         const AdvCoef& ac);
+    static void aa_p2(hipStream_t s, const T* sbx, const T* sby, const T* sbz, const T* gx, const T* gy, const T* gz,
+                      const T* psi, const T* sp2, T* r, T* mub, T* gbx, T* gby, T* gbz, long n, const PlainCoef& cf,
+                      double* acc);
+    static void aa_combine(hipStream_t s, const T* lam1, const T* psb, const T* mub, const T* gbx, const T* gby,
+                           const T* gbz, const T* phn, T* lam0, Dims d, const PlainCoef& cf, double* acc);
+    static void aa_pre(hipStream_t s, const T* psi0, const T* rpre, T* d0, long n, const PlainCoef& cf, double* acc);
     static void from_f64(hipStream_t s, const double* a, T* b, long n);
     static void to_f64(hipStream_t s, const T* a, double* b, long n);
     static void q_x(hipStream_t s, const T* nl_zy, T* ph, T* pho, T* nlo, Dims d, const QuasiCoef& qc, int first,

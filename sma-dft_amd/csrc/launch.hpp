@@ -153,6 +153,57 @@ void Launch<T>::q_proj_out(hipStream_t s, const T* Sx, const T* Sy, const T* Sz,
 }
 
 template <typename T>
+int Launch<T>::flat_blocks(long n)
+{
+    return (int)flat_grid(n).x;
+}
+
+template <typename T>
+void Launch<T>::aa_w(hipStream_t s, const T* lam1, const T* lam2, const T* phn, const T* phn1, T* r, Dims d,
+                     const PlainCoef& cf, int mode, double* acc)
+{
+    hipLaunchKernelGGL(k_aa_w<T>, flat_grid(d.total()), dim3(WG), 0, s, lam1, lam2, phn, phn1, r, d, cf, mode, acc);
+}
+
+template <typename T>
+void Launch<T>::aa_p1(hipStream_t s, const T* r, const T* gx, const T* gy, const T* gz, const T* ux, const T* uy,
+                      const T* uz, T* ubx, T* uby, T* ubz, T* gbx, T* gby, T* gbz, long n)
+{
+    hipLaunchKernelGGL(k_aa_p1<T>, flat_grid(n), dim3(WG), 0, s, r, gx, gy, gz, ux, uy, uz, ubx, uby, ubz, gbx, gby,
+                       gbz, n);
+}
+
+template <typename T>
+void Launch<T>::aa_projT(hipStream_t s, const T* ubx, const T* uby, const T* ubz, T* sbx, T* sby, T* sbz, Dims d,
+                         const AdvCoef& ac)
+{
+    hipLaunchKernelGGL(k_aa_projT<T>, flat_grid(d.total()), dim3(WG), 0, s, ubx, uby, ubz, sbx, sby, sbz, d, ac);
+}
+
+template <typename T>
+void Launch<T>::aa_p2(hipStream_t s, const T* sbx, const T* sby, const T* sbz, const T* gx, const T* gy, const T* gz,
+                      const T* psi, const T* sp2, T* r, T* mub, T* gbx, T* gby, T* gbz, long n, const PlainCoef& cf,
+                      double* acc)
+{
+    hipLaunchKernelGGL(k_aa_p2<T>, flat_grid(n), dim3(WG), 0, s, sbx, sby, sbz, gx, gy, gz, psi, sp2, r, mub, gbx, gby,
+                       gbz, n, cf, acc);
+}
+
+template <typename T>
+void Launch<T>::aa_combine(hipStream_t s, const T* lam1, const T* psb, const T* mub, const T* gbx, const T* gby,
+                           const T* gbz, const T* phn, T* lam0, Dims d, const PlainCoef& cf, double* acc)
+{
+    hipLaunchKernelGGL(k_aa_combine<T>, flat_grid(d.total()), dim3(WG), 0, s, lam1, psb, mub, gbx, gby, gbz, phn, lam0,
+                       d, cf, acc);
+}
+
+template <typename T>
+void Launch<T>::aa_pre(hipStream_t s, const T* psi0, const T* rpre, T* d0, long n, const PlainCoef& cf, double* acc)
+{
+    hipLaunchKernelGGL(k_aa_pre<T>, flat_grid(n), dim3(WG), 0, s, psi0, rpre, d0, n, cf, acc);
+}
+
+template <typename T>
 void Launch<T>::from_f64(hipStream_t s, const double* a, T* b, long n)
 {
     hipLaunchKernelGGL((k_convert<double, T>), flat_grid(n), dim3(WG), 0, s, a, b, n);

@@ -863,21 +863,26 @@ struct AdvEngine : EngineBase {
         prev_valid = false;
     }
 
+    bool fresh = true;
+
     void init(const double* psi0) override
     {
         upload(s, psi.p, n, psi0);
         preloop();
+        fresh = true;
     }
 
-    // psi, grad psi, the force and the velocity of the current psihat
-    void front(bool keep_prev, Prof* prof)
+    // psi, grad psi, the force and the velocity of the current psihat (or
+    // of phin)
+    void front(bool keep_prev, Prof* prof, const T* phin = nullptr)
     {
+        if (!phin) phin = ph.p;
         auto mk = [&](int c) { if (prof) prof->mark(c); };
         // psi, grad psi and Sp2 = REDFT01^3(-aLin psihat) (:847-991) with
         // shared passes: z from psihat, y from B = REDFT01_z psihat, x from
         // A = REDFT01_y B
         const SpecCoef cf{ac.pc, 0.0};
-        Launch<T>::zpass(s, ZS_ADV, ph.p,
+        Launch<T>::zpass(s, ZS_ADV, phin,
                          pass3(psi.p, REDFT01, OP_ID, gz.p, RODFT01, OP_SHIFT, sp.p, REDFT01, OP_ALIN), d, cf,
                          twz.tw);
         Launch<T>::spass(s, psi.p, pass2(gy.p, RODFT01, OP_SHIFT, psi.p, REDFT01, OP_ID), d, 1, cf, twy.tw);
@@ -943,6 +948,7 @@ struct AdvEngine : EngineBase {
         if (nsteps > 0) {
             psi_valid = false;
             prev_valid = true;
+            fresh = false;
         }
     }
 
@@ -977,13 +983,18 @@ struct AdvEngine : EngineBase {
         case SMAD_FIELD_PSI:
             upload(s, psi.p, n, in);
             preloop();
+            fresh = true;
             break;
         case SMAD_FIELD_PSI_HAT:
             upload(s, ph.p, n, in);
             psi_valid = false;
             prev_valid = false;
+            fresh = false;
             break;
-        case SMAD_FIELD_N_HAT: upload(s, nhat.p, n, in); break;
+        case SMAD_FIELD_N_HAT:
+            upload(s, nhat.p, n, in);
+            fresh = false;
+            break;
         default: throw ApiError{SMAD_EINVAL, "field cannot be set in the advective model"};
         }
     }
@@ -1046,13 +1057,135 @@ struct AdvEngine : EngineBase {
         HIPCHK(hipStreamSynchronize(s));
     }
 
-    void forward_tape(int, int) override
+    // ------------------------------------------------------------ adjoint --
+    // The forward run tapes psi_hat of every step (K+1 fields) and the
+    // initial psi; the backward sweep of step n recomputes the step's front
+    // half from its psi_hat and transposes the step (the kernels k_aa_*).
+    std::vector<std::unique_ptr<DevBuf<T>>> tape;
+    DevBuf<T> ic, end_nh, lam0, lam1, lam2, mub, ubx, uby, ubz, gbx, gby, gbz;
+    DevBuf<double> acc_l, acc_b, gsum;
+    int tape_K = 0;
+
+    void copy_dd(T* dst, const T* src)
     {
-        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
+        HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, s));
     }
-    int adjoint(const double*, double*, double*) override
+
+    void forward_tape(int K, int) override
     {
-        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
+        if (ac.pc.conserved) throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain and advective models"};
+        if (!fresh) throw ApiError{SMAD_ESTATE, "forward_tape must start from the pre-loop state (after init or set_field(PSI))"};
+        if (K < 1) throw ApiError{SMAD_EINVAL, "K must be >= 1"};
+        if ((int)tape.size() < K + 1) tape.resize(K + 1);
+        for (int m = 0; m <= K; ++m)
+            if (!tape[m]) {
+                tape[m].reset(new DevBuf<T>);
+                tape[m]->alloc(n);
+            }
+        if (!ic.p) {
+            for (DevBuf<T>* b : {&ic, &end_nh, &lam0, &lam1, &lam2, &mub, &ubx, &uby, &ubz, &gbx, &gby, &gbz})
+                b->alloc(n);
+            const int nb = Launch<T>::flat_blocks((long)n);
+            acc_l.alloc(2 * (size_t)nb);
+            acc_b.alloc(2 * (size_t)nb);
+            gsum.alloc(4);
+        }
+        copy_dd(ic.p, psi.p);  // the initial psi (the pre-loop history's argument)
+        EvPair ev;
+        HIPCHK(hipEventRecord(ev.a, s));
+        for (int m = 0; m < K; ++m) {
+            copy_dd(tape[m]->p, ph.p);
+            step_one(m == K - 1, nullptr);
+        }
+        copy_dd(tape[K]->p, ph.p);
+        copy_dd(end_nh.p, nhat.p);
+        LAUNCHCHK();
+        HIPCHK(hipEventRecord(ev.b, s));
+        last_fwd_ms = ev.ms();
+        tape_K = K;
+        psi_valid = false;
+        prev_valid = true;
+        fresh = false;
+    }
+
+    void ist3T_fwd(T* b, int a)  // the transform part of iST_a^T: ST_a's kinds
+    {
+        tr3(b, b, a == 0 ? RODFT10 : REDFT10, a == 1 ? RODFT10 : REDFT10, a == 2 ? RODFT10 : REDFT10);
+    }
+
+    int adjoint(const double* g_psik, double* dpsi0, double* dparams) override
+    {
+        if (tape_K < 1) throw ApiError{SMAD_ESTATE, "adjoint needs a preceding forward_tape"};
+        const int K = tape_K;
+        const PlainCoef& cf = ac.pc;
+        upload(s, nr.p, n, g_psik);
+        EvPair ev;
+        HIPCHK(hipEventRecord(ev.a, s));
+        // lam^K = iCT^T g
+        tr3(nr.p, nr.p, REDFT10, REDFT10, REDFT10);
+        Launch<T>::aa_w(s, nr.p, nullptr, nullptr, nullptr, lam1.p, d, cf, 3, nullptr);
+        Launch<T>::fill(s, lam2.p, (long)n, 0.0);
+        Launch<double>::fill(s, acc_l.p, (long)acc_l.n, 0.0);
+        Launch<double>::fill(s, acc_b.p, (long)acc_b.n, 0.0);
+        T* L0 = lam0.p;
+        T* L1 = lam1.p;
+        T* L2 = lam2.p;
+        for (int m = K - 1; m >= 0; --m) {
+            front(false, nullptr, tape[m]->p);  // psi, grad psi, Sp2, S_hat, u of step m
+            Launch<T>::aa_w(s, L1, L2, tape[m]->p, tape[m + 1]->p, nr.p, d, cf, 0, acc_l.p);
+            tr3(nr.p, nr.p, REDFT01, REDFT01, REDFT01);  // r = CT^T w
+            Launch<T>::aa_p1(s, nr.p, gx.p, gy.p, gz.p, ux.p, uy.p, uz.p, ubx.p, uby.p, ubz.p, gbx.p, gby.p, gbz.p,
+                             (long)n);
+            ist3T_fwd(ubx.p, 0);
+            ist3T_fwd(uby.p, 1);
+            ist3T_fwd(ubz.p, 2);
+            Launch<T>::aa_projT(s, ubx.p, uby.p, ubz.p, sx.p, sy.p, sz.p, d, ac);
+            tr3(sx.p, sx.p, RODFT01, REDFT01, REDFT01);  // ST_a^T's transform part
+            tr3(sy.p, sy.p, REDFT01, RODFT01, REDFT01);
+            tr3(sz.p, sz.p, REDFT01, REDFT01, RODFT01);
+            Launch<T>::aa_p2(s, sx.p, sy.p, sz.p, gx.p, gy.p, gz.p, psi.p, sp.p, nr.p, mub.p, gbx.p, gby.p, gbz.p,
+                             (long)n, cf, acc_b.p);
+            tr3(nr.p, nr.p, REDFT10, REDFT10, REDFT10);
+            tr3(mub.p, mub.p, REDFT10, REDFT10, REDFT10);
+            ist3T_fwd(gbx.p, 0);
+            ist3T_fwd(gby.p, 1);
+            ist3T_fwd(gbz.p, 2);
+            Launch<T>::aa_combine(s, L1, nr.p, mub.p, gbx.p, gby.p, gbz.p, tape[m]->p, L0, d, cf, acc_l.p);
+            if (m == 0) {
+                // the pre-loop history enters psi_hat^1 with -c: r_pre = CT^T(-c lam^1)
+                Launch<T>::aa_w(s, L1, nullptr, nullptr, nullptr, ubx.p, d, cf, 1, nullptr);
+                tr3(ubx.p, ubx.p, REDFT01, REDFT01, REDFT01);
+            }
+            T* t = L2;
+            L2 = L1;
+            L1 = L0;
+            L0 = t;
+            LAUNCHCHK();
+        }
+        // dJ/dpsi0 = s CT^T lam^0 + f'(psi0) r_pre
+        Launch<T>::aa_w(s, L1, nullptr, nullptr, nullptr, nr.p, d, cf, 2, nullptr);
+        tr3(nr.p, nr.p, REDFT01, REDFT01, REDFT01);
+        Launch<T>::aa_pre(s, ic.p, ubx.p, nr.p, (long)n, cf, acc_b.p);
+        Launch<T>::fold(s, acc_l.p, (int)(acc_l.n / 2), 2, gsum.p);
+        Launch<T>::fold(s, acc_b.p, (int)(acc_b.n / 2), 2, gsum.p + 2);
+        LAUNCHCHK();
+        HIPCHK(hipEventRecord(ev.b, s));
+        last_adj_ms = ev.ms();
+        double h[4];
+        HIPCHK(hipMemcpyAsync(h, gsum.p, sizeof(h), hipMemcpyDeviceToHost, s));
+        int rc = download(s, nr.p, n, dpsi0);
+        for (int q = 0; q < 4; ++q) {
+            dparams[q] = h[q];
+            if (!std::isfinite(h[q])) rc = SMAD_ENONFINITE;
+        }
+        // the context stays at step K (the work fields hold the sweep's last
+        // recomputation)
+        copy_dd(ph.p, tape[K]->p);
+        copy_dd(nhat.p, end_nh.p);
+        HIPCHK(hipStreamSynchronize(s));
+        psi_valid = false;
+        prev_valid = false;
+        return rc;
     }
 };
 
