@@ -38,9 +38,12 @@ enum {
     SMAD_MODEL_QUASI = 2, /* quasi-incompressible, varying density, BDF2/AB2 */
     SMAD_MODEL_PLAIN_CONSERVED = 3, /* conserved dynamics: the plain step with aLin and
                                        N_hat times |q|^2 (no-adv/cosNoAdvConserv.cpp:647-697, :809) */
-    SMAD_MODEL_ADV_CONSERVED = 4    /* conserved advective step: C1, C2 with |q|^2 aLin and
+    SMAD_MODEL_ADV_CONSERVED = 4,   /* conserved advective step: C1, C2 with |q|^2 aLin and
                                        N_hat = |q|^2 CT(beta psi^3 - gamma psi^5) - CT(u . grad psi)
                                        (adv/cosAdvConSingle.cpp:741-742, :1290-1369) */
+    SMAD_MODEL_COMPRESSIBLE = 5     /* independent density on an AB2 continuity equation, penalty
+                                       zeta, solenoidal + irrotational velocity
+                                       (compressible/compressible.cpp:1271-2642) */
 };
 
 enum { SMAD_FP64 = 64, SMAD_FP32_MIXED = 32 };
@@ -61,7 +64,15 @@ typedef struct {
     int precision;          /* SMAD_FP64 or SMAD_FP32_MIXED */
     int load;               /* quasi: 1 = the reference's restart start (load == 1,
                                quasi/quasi.cpp:1365-1369): divv and grad^2(rho f)
-                               from the first step, with rho_old = rho at that step */
+                               from the first step, with rho_old = rho at that step;
+                               compressible: 1 = no relaxation phase (the density
+                               is the loaded one from the first step,
+                               compressible/compressible.cpp:1542) */
+    double zeta;            /* compressible: penalty on rho - rho0 - kp amp
+                               (compressible/compressible.cpp:272), reference 0.001 */
+    int rho_up;             /* compressible: planes by which the relaxation density
+                               is lifted along z (compressible/compressible.cpp:228,
+                               :1606-1628), reference 16 */
 } smad_params;
 
 enum {
@@ -323,6 +334,13 @@ const char *smad_noalloc_last_error(void);
 /* Focal-conic / hyperboloid defect of no-adv/cosNoAdvTrackSing.cpp:389-449
  * (amp 1.328 there; a_e = b_e set the conic size in grid units). */
 void smad_ic_focal_conic(const smad_params *p, double amp, double a_e, double b_e, double *psi);
+
+/* Focal-conic torus of compressible/compressible.cpp:703-727 (the FC Torus
+ * block): a half-ellipse (a_e, b_e) of layers about the ring of radius ny/4
+ * centred at (ny/2, ny/2) in (x_scale i, j) on the k = 0 wall.  With
+ * 578 x 484 x 130, x_scale = 5.2/6.2 and amp = Amp it is the reference's
+ * initial psi (centre 242, radius 121). */
+void smad_ic_fc_torus(const smad_params *p, double amp, double a_e, double b_e, double x_scale, double *psi);
 
 /* Planar smectic slab: amp cos(q0 k dz) for |i dx - Lx/2| < Lx/4, else 0,
  * plus noise. */

@@ -25,6 +25,7 @@ class OParams(ctypes.Structure):
         ("rho0", ctypes.c_double), ("kp", ctypes.c_double), ("rho_m", ctypes.c_double),
         ("filter_sigma", ctypes.c_double), ("div_switch", ctypes.c_int),
         ("precision", ctypes.c_int), ("load", ctypes.c_int),
+        ("zeta", ctypes.c_double), ("rho_up", ctypes.c_int),
     ]
 
 
@@ -58,6 +59,11 @@ def lib():
         L.orc_adv_stokes_T.restype = None
         L.orc_adv_adjoint.argtypes = [pp, dp, ctypes.c_int, dp, dp, dp, dp]
         L.orc_adv_adjoint.restype = None
+        L.orc_comp_preloop.argtypes = [pp, ctypes.POINTER(CState)]
+        L.orc_comp_step.argtypes = [pp, ctypes.POINTER(CState), ctypes.c_int, dp]
+        L.orc_comp_velocity.argtypes = [pp] + [dp] * 10
+        for f in ("orc_comp_preloop", "orc_comp_step", "orc_comp_velocity"):
+            getattr(L, f).restype = None
         L.orc_quasi_project.argtypes = [pp, dp, dp, dp, dp, dp, dp, dp, dp]
         L.orc_quasi_project.restype = None
         L.orc_adv_stokes.argtypes = [pp, dp, dp, dp, dp, dp, dp]
@@ -134,6 +140,57 @@ class QuasiOracle:
 
     def step(self, n=1):
         lib().orc_quasi_step(ctypes.byref(self.op), ctypes.byref(self.st), int(n), _p(self.sums))
+
+
+class CState(ctypes.Structure):
+    _fields_ = [(nm, ctypes.POINTER(ctypes.c_double)) for nm in
+                ("psi", "psiq", "psiq_old", "nl", "rho", "frho", "amp", "f", "divv", "adg", "vel", "vsol",
+                 "virr", "grad_psi", "grad_rho")] + [("nloop", ctypes.c_int),
+                                                     ("mu", ctypes.POINTER(ctypes.c_double))]
+
+
+_CARRAYS = ("psi", "psiq", "psiq_old", "nl", "rho", "frho", "amp", "f", "divv", "adg", "vel", "vsol", "virr",
+            "grad_psi", "grad_rho", "mu")
+
+
+class CompOracle:
+    """Reference-order compressible stepper (compressible/compressible.cpp:
+    1114-2700, one slab).  rho0_field: the starting density (zeros for a new
+    start, as the reference, or the loaded density for load == 1)."""
+
+    def __init__(self, p, psi0, rho0_field=None):
+        self.op = oparams(p)
+        sh = tuple(np.shape(psi0))
+        self.arrays = {}
+        for nm in _CARRAYS:
+            shape = (3,) + sh if nm in ("vel", "vsol", "virr", "grad_psi", "grad_rho") else sh
+            self.arrays[nm] = np.zeros(shape)
+        self.arrays["psi"][...] = psi0
+        if rho0_field is not None:
+            self.arrays["rho"][...] = rho0_field
+        self.st = CState(**{nm: _p(a) for nm, a in self.arrays.items()}, nloop=0)
+        lib().orc_comp_preloop(ctypes.byref(self.op), ctypes.byref(self.st))
+        self.sums = np.zeros(4)
+
+    def __getattr__(self, nm):
+        if nm in ("arrays", "op", "st"):
+            raise AttributeError(nm)
+        return self.arrays[nm]
+
+    @property
+    def nloop(self):
+        return self.st.nloop
+
+    def step(self, n=1):
+        lib().orc_comp_step(ctypes.byref(self.op), ctypes.byref(self.st), int(n), _p(self.sums))
+
+
+def comp_velocity(p, sx, sy, sz, g_hat):
+    """(vsol_hat, virr_hat) of compressible/compressible.cpp:1826-2410, each (3,)+shape."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (sx, sy, sz, g_hat)]
+    out = [np.empty_like(arrs[0]) for _ in range(6)]
+    lib().orc_comp_velocity(ctypes.byref(oparams(p)), *[_p(a) for a in arrs], *[_p(a) for a in out])
+    return np.stack(out[:3]), np.stack(out[3:])
 
 
 class PlainOracle:

@@ -35,7 +35,9 @@ typedef struct {
     double nu, lambda, rho0, kp, rho_m, filter_sigma;
     int div_switch;
     int precision;
-    int load; /* quasi: the reference's load == 1 start (quasi/quasi.cpp:1365-1369) */
+    int load; /* quasi, compressible: the reference's load == 1 start (quasi/quasi.cpp:1365-1369) */
+    double zeta; /* compressible: density penalty (compressible/compressible.cpp:272) */
+    int rho_up;  /* compressible: z lift of the relaxation density (compressible/compressible.cpp:228) */
 } orc_params;
 
 /* ------------------------------------------------------------ plain model -- */
@@ -115,6 +117,26 @@ void orc_quasi_project(const orc_params *p, const double *Sx, const double *Sy, 
 /* sums (may be NULL): sum|psi-psi_old|, sum|psi|, mass sum(rho) dx^3, energy sum(e) dx^3
  * for the last step taken */
 void orc_quasi_step(const orc_params *p, orc_quasi_state *st, int nsteps, double *sums);
+
+/* ------------------------------------------------- compressible model -- */
+/* compressible/compressible.cpp:1114-2700 on one slab.  All arrays nx*ny*nz
+ * (vel, vsol, virr, grad_psi, grad_rho: 3x).  rho is the state density
+ * (set it before orc_comp_preloop: 0 for a new start, the loaded density for
+ * load == 1); frho is the continuity's AB2 history. */
+typedef struct {
+    double *psi, *psiq, *psiq_old, *nl, *rho, *frho, *amp, *f, *divv, *adg, *vel, *vsol, *virr, *grad_psi,
+        *grad_rho;
+    int nloop;
+    double *mu; /* optional (may be NULL) */
+} orc_comp_state;
+void orc_comp_preloop(const orc_params *p, orc_comp_state *st);
+/* sums (may be NULL): sum|psi-psi_old|, sum|psi|, mass sum(rho) dx^3,
+ * energy sum(rho e + zeta/2 (rho - rho0 - kp amp)^2) dx^3 of the last step */
+void orc_comp_step(const orc_params *p, orc_comp_state *st, int nsteps, double *sums);
+/* the velocity alone, spectral in and out (see oracle_models.c) */
+void orc_comp_velocity(const orc_params *p, const double *Sx, const double *Sy, const double *Sz,
+                       const double *G, double *vsx, double *vsy, double *vsz, double *vrx, double *vry,
+                       double *vrz);
 
 /* --------------------------------------------------- plain adjoint -- */
 /* K plain steps from psi0, then the discrete adjoint for the cotangent

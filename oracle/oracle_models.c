@@ -26,6 +26,19 @@
  *     single force and divergence mode equals the compressible Stokes
  *     solution; divv, S = mu grad psi + e grad rho and the Nl coupling term
  *     match their forms for a manufactured previous density.
+ *   - compressible: uniform psi follows a scalar recurrence (relaxation
+ *     density, freeze at the switch, penalty, C1/C2, sums); planar layers
+ *     have amp = q0 A exactly, no solenoidal flow and stay 1D, with equal
+ *     profiles along x, y and z; the solenoidal velocity is divergence-free
+ *     and the irrotational one the gradient of one potential (with the
+ *     k = 0 filter of :2350); with no flow Nl = -dF/dpsi for the free energy
+ *     and the zeta terms = -dF_pen/dpsi (central differences); R equals
+ *     -div(rho u) and rho advances by dt sum R; a density mode decays at
+ *     zeta rho^2/(lambda + 2 nu); amp is the Gaussian-filtered amplitude; the
+ *     relaxation density is lifted by rho_up.  Sixteen planted mistakes
+ *     (signs of the penalty, divergence, history, pressure and eps terms,
+ *     dropped filters, the k = 0 filter, the lift direction, shifted reads,
+ *     scales) each fail one of these.
  *   - the truncated top modes of the projections (j = ny-1, k = nz-1,
  *     adv/cosAdvTrackMult.cpp:1202-1203) follow the reference by reading.
  */
@@ -950,5 +963,288 @@ void orc_adv_adjoint(const orc_params *p, const double *psi0, int nsteps, const 
     free(psis); free(nq); free(nqp); free(psi); free(psiq); free(psiq1); free(gx); free(gy); free(gz);
     free(vx); free(vy); free(vz); free(sp2); free(sx); free(sy); free(sz); free(lam1); free(lam2); free(lam0);
     free(r); free(mub); free(psb);
+    grid_free(&g);
+}
+
+/* ------------------------------------------------------- compressible -- */
+/*
+ * compressible/compressible.cpp:1114-1194 (pre-loop) and :1271-2642 (loop
+ * body) on one slab (global indexing: the reference's i_local == i).  The
+ * density is an independent field carried by an AB2 continuity equation
+ * (:2555-2573); during the relaxation phase (nLoop <= divvSwitch, load != 1)
+ * it is reset every step to rho0 + kp amp and lifted by rho_up planes along z
+ * (:1580-1628).
+ */
+
+/* Velocity of compressible/compressible.cpp:1826-2410 for one slab, spectral
+ * in and out: S_hat in the bases (STx, STy, STz), G_hat = CT(fgrad) +
+ * CT(rho e0) exp(-2 sig^2 |q|^2) (unscaled).  The solenoidal part
+ *   vsol = s/(nu |q|^2) (S_hat - q (q . S_hat)/|q|^2)
+ * and the irrotational part
+ *   virr = s/((lambda + 2 nu) |q|^2) (q G_hat + q (q . S_hat) exp(-2 sig^2 |q|^2)/|q|^2)
+ * at the staggered points of each component, with the reference's shifted
+ * reads and truncations (x at i = nx-1, y and z at j = ny-1 or k = nz-1 are
+ * 0).  The z component's filter at k = 0, i >= 1, j >= 1 takes the y
+ * component's |q|^2 (CM1y in the k = 0 loop, :2350), as there. */
+void orc_comp_velocity(const orc_params *p, const double *Sx, const double *Sy, const double *Sz,
+                       const double *G, double *vsx, double *vsy, double *vsz, double *vrx, double *vry,
+                       double *vrz)
+{
+    grid_t g;
+    grid_make(p, &g);
+    const long n = g.n;
+    const double sc = g.scale, nu = p->nu, l2n = p->lambda + 2 * p->nu;
+    const double c = -p->filter_sigma * p->filter_sigma * 2;
+    memset(vsx, 0, sizeof(double) * n); memset(vsy, 0, sizeof(double) * n); memset(vsz, 0, sizeof(double) * n);
+    memset(vrx, 0, sizeof(double) * n); memset(vry, 0, sizeof(double) * n); memset(vrz, 0, sizeof(double) * n);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.ny; ++j)
+            for (int k = 0; k < g.nz; ++k) {
+                long id = IDX(i, j, k);
+                if (i < g.nx - 1) {                                              /* :1940-2082 */
+                    double syv = (j != 0) ? Sy[IDX(i + 1, j - 1, k)] : 0.0;
+                    double szv = (k != 0) ? Sz[IDX(i + 1, j, k - 1)] : 0.0;
+                    double mq2 = pow(g.sx[i], 2) + pow(g.qy[j], 2) + pow(g.qz[k], 2);
+                    double cm1 = sc / mq2, cm2 = g.sx[i] / mq2;
+                    double dot = g.sx[i] * Sx[id] + g.qy[j] * syv + g.qz[k] * szv;
+                    vsx[id] = (cm1 / nu) * (Sx[id] - cm2 * dot);
+                    vrx[id] = (cm1 / l2n) * (g.sx[i] * G[IDX(i + 1, j, k)] + cm2 * dot * exp(c * sc / cm1));
+                }
+                if (j < g.ny - 1 && k < g.nz - 1) {                               /* :2131-2410 */
+                    double sxv = (i != 0) ? Sx[IDX(i - 1, j + 1, k)] : 0.0;
+                    double szv = (k != 0) ? Sz[IDX(i, j + 1, k - 1)] : 0.0;
+                    double mq2 = pow(g.qx[i], 2) + pow(g.sy[j], 2) + pow(g.qz[k], 2);
+                    double cm1y = sc / mq2, cm2 = g.sy[j] / mq2;
+                    double dot = g.qx[i] * sxv + g.sy[j] * Sy[id] + g.qz[k] * szv;
+                    vsy[id] = (cm1y / nu) * (Sy[id] - cm2 * dot);
+                    vry[id] = (cm1y / l2n) * (g.sy[j] * G[IDX(i, j + 1, k)] + cm2 * dot * exp(c * sc / cm1y));
+                    sxv = (i != 0) ? Sx[IDX(i - 1, j, k + 1)] : 0.0;
+                    double syv = (j != 0) ? Sy[IDX(i, j - 1, k + 1)] : 0.0;
+                    mq2 = pow(g.qx[i], 2) + pow(g.qy[j], 2) + pow(g.sz[k], 2);
+                    double cm1 = sc / mq2;
+                    cm2 = g.sz[k] / mq2;
+                    dot = g.qx[i] * sxv + g.qy[j] * syv + g.sz[k] * Sz[id];
+                    vsz[id] = (cm1 / nu) * (Sz[id] - cm2 * dot);
+                    double cf = (i >= 1 && j >= 1 && k == 0) ? cm1y : cm1;
+                    vrz[id] = (cm1 / l2n) * (g.sz[k] * G[IDX(i, j, k + 1)] + cm2 * dot * exp(c * sc / cf));
+                }
+            }
+    grid_free(&g);
+}
+
+static void comp_relax_rho(const orc_params *p, const grid_t *gp, const double *amp, double *rho, double *lap)
+{
+    grid_t g = *gp;
+    /* :1594-1603 rho_1 + ((1 - rho_1)/Amp) amp with rho_1 = rho_0 and
+     * kp = (1 - rho_0)/Amp */
+    for (long id = 0; id < g.n; ++id) {
+        rho[id] = p->rho0 + p->kp * amp[id];
+        lap[id] = rho[id] * lap[id];
+    }
+    if (p->rho_up <= 0) return;
+    const int up = p->rho_up;
+    for (int i = 0; i < g.nx; ++i)                                            /* :1606-1628 */
+        for (int j = 0; j < g.ny; ++j) {
+            for (int k = g.nz - 1; k > up - 1; --k) {
+                long id = IDX(i, j, k);
+                rho[id] = rho[IDX(i, j, k - up)];
+                lap[id] = rho[id] * lap[id];
+            }
+            for (int k = 0; k < up && k < g.nz; ++k) {
+                long id = IDX(i, j, k);
+                rho[id] = 1;
+                lap[id] = rho[id] * lap[id];
+            }
+        }
+}
+
+void orc_comp_preloop(const orc_params *p, orc_comp_state *st)
+{
+    grid_t g;
+    grid_make(p, &g);
+    const long n = g.n;
+    double *tmp = malloc(sizeof(double) * n);
+    ct(&g, st->psi, st->psiq);                                   /* :1151-1155 */
+    for (long i = 0; i < n; ++i) st->psiq[i] *= g.scale;         /* :1159-1168 */
+    ict(&g, st->psiq, st->psi);                                  /* :1169-1173 */
+    nonlin(p, st->psi, tmp, n);                                  /* :1174-1182 */
+    memcpy(st->psiq_old, st->psiq, sizeof(double) * n);          /* :1185 */
+    ct(&g, tmp, st->nl);                                         /* :1189-1193 */
+    memset(st->frho, 0, sizeof(double) * n);                     /* f_rho starts at zero (:426) */
+    memset(st->vel, 0, sizeof(double) * 3 * n);
+    st->nloop = 0;
+    free(tmp);
+    grid_free(&g);
+}
+
+void orc_comp_step(const orc_params *p, orc_comp_state *st, int nsteps, double *sums)
+{
+    grid_t g;
+    grid_make(p, &g);
+    const long n = g.n;
+    const double dt = p->dt, dtd2 = dt / 2, sc = g.scale, q02 = p->q0 * p->q0;
+    const double sig2 = p->filter_sigma * p->filter_sigma, zeta = p->zeta, kp = p->kp, r0 = p->rho0;
+    double *nl_old = malloc(sizeof(double) * n), *rhoq = malloc(sizeof(double) * n);
+    double *gpx = st->grad_psi, *gpy = st->grad_psi + n, *gpz = st->grad_psi + 2 * n;
+    double *grx = st->grad_rho, *gry = st->grad_rho + n, *grz = st->grad_rho + 2 * n;
+    double *rho = st->rho, *amp = st->amp, *f = st->f, *lap = malloc(sizeof(double) * n);
+    double *mu = malloc(sizeof(double) * n), *fg = malloc(sizeof(double) * n), *E = malloc(sizeof(double) * n);
+    double *Sx = malloc(sizeof(double) * n), *Sy = malloc(sizeof(double) * n), *Sz = malloc(sizeof(double) * n);
+    double *fx = malloc(sizeof(double) * n), *fy = malloc(sizeof(double) * n), *fz = malloc(sizeof(double) * n);
+    double *vsx = st->vsol, *vsy = st->vsol + n, *vsz = st->vsol + 2 * n;
+    double *vrx = st->virr, *vry = st->virr + n, *vrz = st->virr + 2 * n;
+    double *ux = st->vel, *uy = st->vel + n, *uz = st->vel + 2 * n;
+    double *adg = st->adg, *divv = st->divv;
+    double *psi_old = malloc(sizeof(double) * n), *psiq_prev = malloc(sizeof(double) * n);
+
+    for (int s = 0; s < nsteps; ++s) {
+        st->nloop++;
+        const int nloop = st->nloop;
+        memcpy(nl_old, st->nl, sizeof(double) * n);                         /* :1287 */
+        /* gradients of psi and rho (:1288-1472): grad rho = iST(-s q rhoq) */
+        ct(&g, rho, rhoq);
+        dst_shift(&g, st->psiq, gpx, gpy, gpz);
+        dst_shift(&g, rhoq, grx, gry, grz);
+        for (long id = 0; id < n; ++id) { grx[id] *= sc; gry[id] *= sc; grz[id] *= sc; }
+        ist3(&g, gpx, gpy, gpz);
+        ist3(&g, grx, gry, grz);
+        /* amplitude of psi, filtered (:1473-1515) */
+        for (long id = 0; id < n; ++id)
+            amp[id] = sqrt(q02 * pow(st->psi[id], 2) + pow(gpx[id], 2) + pow(gpy[id], 2) + pow(gpz[id], 2));
+        ct(&g, amp, amp);
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    amp[id] = sc * amp[id] * exp(-sig2 * mq2c_at(&g, i, j, k) * 2);
+                }
+        ict(&g, amp, amp);
+        /* f = iCT(alpha (q0^2 - |q|^2) psiq) (:1517-1534) */
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    f[id] = p->alpha * (q02 - mq2c_at(&g, i, j, k)) * st->psiq[id];
+                }
+        ict(&g, f, f);
+        /* lap(rho f) (:1536-1630) */
+        if (nloop > p->div_switch || p->load == 1) {
+            for (long id = 0; id < n; ++id) lap[id] = rho[id] * f[id];
+            ct(&g, lap, lap);
+            for (int i = 0; i < g.nx; ++i)
+                for (int j = 0; j < g.ny; ++j)
+                    for (int k = 0; k < g.nz; ++k) {
+                        long id = IDX(i, j, k);
+                        lap[id] = -sc * mq2c_at(&g, i, j, k) * lap[id];
+                    }
+            ict(&g, lap, lap);
+        } else {
+            for (int i = 0; i < g.nx; ++i)
+                for (int j = 0; j < g.ny; ++j)
+                    for (int k = 0; k < g.nz; ++k) {
+                        long id = IDX(i, j, k);
+                        double m2 = mq2c_at(&g, i, j, k);
+                        lap[id] = -m2 * p->alpha * (q02 - m2) * st->psiq[id];
+                    }
+            ict(&g, lap, lap);
+            comp_relax_rho(p, &g, amp, rho, lap);
+        }
+        /* stresses, mu, penalty force (:1632-1671) */
+        for (long id = 0; id < n; ++id) {
+            double x = st->psi[id], e0 = 0.5 * pow(f[id], 2), r = rho[id];
+            fg[id] = 0.5 * zeta * (pow(r, 2) - pow(r0 + kp * amp[id], 2));
+            double mt = lap[id] + r * f[id];
+            Sx[id] = mt * gpx[id] + e0 * grx[id];
+            Sy[id] = mt * gpy[id] + e0 * gry[id];
+            Sz[id] = mt * gpz[id] + e0 * grz[id];
+            mu[id] = lap[id] + r * (f[id] - p->eps * x - p->beta * pow(x, 3) + p->gamma * pow(x, 5));
+            E[id] = r * e0;
+            double pen = kp * zeta * (r - r0 - kp * amp[id]) / (amp[id] + 0.001);
+            fx[id] = pen * gpx[id];
+            fy[id] = pen * gpy[id];
+            fz[id] = pen * gpz[id];
+        }
+        if (st->mu) memcpy(st->mu, mu, sizeof(double) * n);
+        ct(&g, fg, fg);                                                       /* :1680-1718 */
+        ct(&g, E, E);
+        orc_r2r3d(Sx, Sx, g.nx, g.ny, g.nz, ORC_RODFT10, ORC_REDFT10, ORC_REDFT10);
+        orc_r2r3d(Sy, Sy, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_RODFT10, ORC_REDFT10);
+        orc_r2r3d(Sz, Sz, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_REDFT10, ORC_RODFT10);
+        orc_r2r3d(fx, fx, g.nx, g.ny, g.nz, ORC_RODFT10, ORC_REDFT10, ORC_REDFT10);
+        orc_r2r3d(fy, fy, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_RODFT10, ORC_REDFT10);
+        orc_r2r3d(fz, fz, g.nx, g.ny, g.nz, ORC_REDFT10, ORC_REDFT10, ORC_RODFT10);
+        /* G = CT(fgrad) + CT(rho e0) exp(-2 sig^2 |q|^2) (:1826-1834) */
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    fg[id] = fg[id] + E[id] * exp(-sig2 * mq2c_at(&g, i, j, k) * 2);
+                }
+        orc_comp_velocity(p, Sx, Sy, Sz, fg, vsx, vsy, vsz, vrx, vry, vrz);
+        /* divergence of virr and of the penalty force (:2413-2520) */
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    double a = i > 0 ? vrx[IDX(i - 1, j, k)] : 0.0, af = i > 0 ? fx[IDX(i - 1, j, k)] : 0.0;
+                    double b = j > 0 ? vry[IDX(i, j - 1, k)] : 0.0, bf = j > 0 ? fy[IDX(i, j - 1, k)] : 0.0;
+                    double cz = k > 0 ? vrz[IDX(i, j, k - 1)] : 0.0, cf = k > 0 ? fz[IDX(i, j, k - 1)] : 0.0;
+                    divv[id] = g.qx[i] * a + g.qy[j] * b + g.qz[k] * cz;
+                    adg[id] = sc * (g.qx[i] * af + g.qy[j] * bf + g.qz[k] * cf);
+                }
+        ict(&g, divv, divv);                                                  /* :2521-2553 */
+        ict(&g, adg, adg);
+        ist3(&g, vsx, vsy, vsz);
+        ist3(&g, vrx, vry, vrz);
+        /* continuity, AB2 (:2555-2573) */
+        for (long id = 0; id < n; ++id) {
+            ux[id] = vsx[id] + vrx[id];
+            uy[id] = vsy[id] + vry[id];
+            uz[id] = vsz[id] + vrz[id];
+            double r = -(vsx[id] + vrx[id]) * grx[id] - (vsy[id] + vry[id]) * gry[id] -
+                       (vsz[id] + vrz[id]) * grz[id] - rho[id] * divv[id];
+            rho[id] = rho[id] + dt * (1.5 * r - 0.5 * st->frho[id]);
+            st->frho[id] = r;
+        }
+        /* Nl (:2578-2608) */
+        for (long id = 0; id < n; ++id)
+            st->nl[id] = -mu[id] + (kp * zeta / (amp[id] + 0.001)) * (rho[id] - r0 - kp * amp[id]) * st->psi[id] -
+                         adg[id] - (vsx[id] + vrx[id]) * gpx[id] - (vsy[id] + vry[id]) * gpy[id] -
+                         (vsz[id] + vrz[id]) * gpz[id];
+        ct(&g, st->nl, st->nl);                                               /* :2610-2613 */
+        if (nloop == 1) memcpy(nl_old, st->nl, sizeof(double) * n);          /* :2615-2617 */
+        /* BDF2/AB2 update (:2620-2635) */
+        memcpy(psiq_prev, st->psiq, sizeof(double) * n);
+        for (int i = 0; i < g.nx; ++i)
+            for (int j = 0; j < g.ny; ++j)
+                for (int k = 0; k < g.nz; ++k) {
+                    long id = IDX(i, j, k);
+                    double m2 = mq2c_at(&g, i, j, k);
+                    double al = p->eps - p->alpha * pow(q02 - m2, 2);
+                    double c1 = 2.0 - dt * p->rho_m * al, c2 = 1.5 - dt * p->rho_m * al;
+                    st->psiq[id] = (c1 * st->psiq[id] - 0.5 * st->psiq_old[id] +
+                                    dtd2 * sc * (3.0 * st->nl[id] - nl_old[id])) / c2;
+                }
+        memcpy(st->psiq_old, psiq_prev, sizeof(double) * n);
+        memcpy(psi_old, st->psi, sizeof(double) * n);                         /* :2639 */
+        ict(&g, st->psiq, st->psi);                                           /* :2640-2643 */
+    }
+    if (sums && nsteps > 0) { /* :2645-2700 for the last step */
+        double sa = 0, sb = 0, sr = 0, se = 0;
+        for (long id = 0; id < n; ++id) {
+            double x = st->psi[id];
+            sa += fabs(x - psi_old[id]);
+            sb += fabs(x);
+            sr += rho[id];
+            double e = -0.5 * p->eps * pow(x, 2) + 0.5 * pow(f[id], 2) - 0.25 * p->beta * pow(x, 4) +
+                       (p->gamma / 6) * pow(x, 6);
+            se += rho[id] * e + 0.5 * zeta * pow(rho[id] - r0 - kp * amp[id], 2);
+        }
+        double dv = g.dx * g.dx * g.dx;
+        sums[0] = sa; sums[1] = sb; sums[2] = sr * dv; sums[3] = se * dv;
+    }
+    free(nl_old); free(rhoq); free(lap); free(mu); free(fg); free(E); free(Sx); free(Sy); free(Sz);
+    free(fx); free(fy); free(fz); free(psi_old); free(psiq_prev);
     grid_free(&g);
 }

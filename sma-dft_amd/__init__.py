@@ -25,6 +25,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsmadft.so")
 
 MODEL_PLAIN, MODEL_ADV, MODEL_QUASI, MODEL_PLAIN_CONSERVED, MODEL_ADV_CONSERVED = 0, 1, 2, 3, 4
+MODEL_COMPRESSIBLE = 5
 FP64, FP32_MIXED = 64, 32
 
 FIELD_PSI, FIELD_PSI_HAT, FIELD_N_HAT = 0, 1, 2
@@ -54,7 +55,7 @@ EXPORTS = (
     "smad_surface_envelope", "smad_r2r_3d_noalloc", "smad_noalloc_last_error",
     "smad_state_fields", "smad_set_steps_taken", "smad_ic_tilted_slab", "smad_ic_two_bumps",
     "smad_config_params", "smad_config_ic", "smad_forward_tape_dev", "smad_adjoint_dev",
-    "smad_set_params",
+    "smad_set_params", "smad_ic_fc_torus",
 )
 
 REDFT10, REDFT01, RODFT10, RODFT01 = 0, 1, 2, 3
@@ -71,6 +72,7 @@ class CParams(ctypes.Structure):
         ("rho0", ctypes.c_double), ("kp", ctypes.c_double), ("rho_m", ctypes.c_double),
         ("filter_sigma", ctypes.c_double), ("div_switch", ctypes.c_int),
         ("precision", ctypes.c_int), ("load", ctypes.c_int),
+        ("zeta", ctypes.c_double), ("rho_up", ctypes.c_int),
     ]
 
 
@@ -98,6 +100,8 @@ class Params:
     div_switch: int = 200
     precision: int = FP64
     load: int = 0
+    zeta: float = 0.001
+    rho_up: int = 16
 
     def c(self) -> CParams:
         return CParams(**{f.name: getattr(self, f.name) for f in dc_fields(self)})
@@ -156,6 +160,7 @@ def lib() -> ctypes.CDLL:
         L.smad_last_error.argtypes = [vp]
         L.smad_last_error.restype = ctypes.c_char_p
         L.smad_ic_focal_conic.argtypes = [P(CParams), ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+        L.smad_ic_fc_torus.argtypes = [P(CParams)] + [ctypes.c_double] * 4 + [dp]
         L.smad_ic_slab.argtypes = [P(CParams), ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, dp]
         L.smad_ic_curved.argtypes = [P(CParams), ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, dp]
         L.smad_ic_add_noise.argtypes = [P(CParams), ctypes.c_double, ctypes.c_ulonglong, dp]
@@ -188,7 +193,7 @@ def lib() -> ctypes.CDLL:
         L.smad_noalloc_last_error.argtypes = []
         L.smad_noalloc_last_error.restype = ctypes.c_char_p
         for fn in ("smad_ic_focal_conic", "smad_ic_slab", "smad_ic_curved", "smad_ic_add_noise",
-                   "smad_default_params", "smad_ic_two_bumps", "smad_ic_tilted_slab"):
+                   "smad_default_params", "smad_ic_two_bumps", "smad_ic_tilted_slab", "smad_ic_fc_torus"):
             getattr(L, fn).restype = None
         _LIB = L
     return _LIB
@@ -246,6 +251,19 @@ def ic_focal_conic(p: Params, amp=1.328, a_e=None, b_e=None) -> np.ndarray:
     b_e = a_e if b_e is None else b_e
     out = np.empty(p.shape, dtype=np.float64)
     lib().smad_ic_focal_conic(ctypes.byref(p.c()), amp, a_e, b_e, _dptr(out))
+    return out
+
+
+def ic_fc_torus(p: Params, amp=None, a_e=None, b_e=None, x_scale=1.0) -> np.ndarray:
+    """Focal-conic torus of the compressible program (compressible/compressible.cpp:
+    703-727); amp defaults to the smectic amplitude of p's eps, beta, gamma,
+    a_e = b_e to 124/130 of nz as in the shipped run."""
+    if amp is None:
+        amp = np.sqrt((3 * p.beta + np.sqrt(9 * p.beta ** 2 + 40 * p.eps * p.gamma)) / 5)
+    a_e = p.nz * 124.0 / 130.0 if a_e is None else a_e
+    b_e = a_e if b_e is None else b_e
+    out = np.empty(p.shape, dtype=np.float64)
+    lib().smad_ic_fc_torus(ctypes.byref(p.c()), amp, a_e, b_e, x_scale, _dptr(out))
     return out
 
 

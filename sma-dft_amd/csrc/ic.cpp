@@ -83,6 +83,30 @@ void smad_ic_focal_conic(const smad_params* p, double amp, double a_e, double b_
             }
 }
 
+// Focal-conic torus of the compressible program: on the k = 0 wall, about
+// the ring of radius ny/4 centred at (ny/2, ny/2) in (x_scale i, j), the
+// layers follow amp cos(q0 dz sqrt((b_e/a_e)^2 (ds - ny/4)^2 + k^2)) inside
+// the half-ellipse ((ds - ny/4)/a_e)^2 + (k/b_e)^2 <= 1; 0 elsewhere.  The
+// reference grid 578 x 484 x 130 with x_scale = 5.2/6.2 gives its centre 242
+// and radius 121.
+void smad_ic_fc_torus(const smad_params* p, double amp, double a_e, double b_e, double x_scale, double* psi)
+{
+    const Box b = box_of(p);
+    const double c = p->ny / 2, r = p->ny / 4;
+    for (int i = 0; i < p->nx; ++i)
+        for (int j = 0; j < p->ny; ++j)
+            for (int k = 0; k < p->nz; ++k) {
+                double v = 0.0;
+                if (k < b_e + 1) {
+                    double zs = k, xs = x_scale * i - c, ys = j - c;
+                    double ds = std::sqrt(xs * xs + ys * ys);
+                    if (!(std::sqrt(std::pow((ds - r) / a_e, 2) + std::pow(zs / b_e, 2)) > 1))
+                        v = amp * std::cos(p->q0 * b.dz * std::sqrt(std::pow((b_e / a_e) * (ds - r), 2) + zs * zs));
+                }
+                psi[lin(p, i, j, k)] = v;
+            }
+}
+
 void smad_ic_slab(const smad_params* p, double amp, double noise, unsigned long long seed, double* psi)
 {
     const Box b = box_of(p);
@@ -142,11 +166,25 @@ void smad_default_params(int model, int n, smad_params* o)
     p.div_switch = 200;
     p.precision = SMAD_FP64;
     p.load = 0;
+    p.zeta = 0.001;
+    p.rho_up = 16;
     if (model == SMAD_MODEL_ADV || model == SMAD_MODEL_PLAIN_CONSERVED || model == SMAD_MODEL_ADV_CONSERVED) {
         p.nw = 16;
         p.dt = 0.0005;
     }
     if (model == SMAD_MODEL_QUASI) p.dt = 0.001;
+    if (model == SMAD_MODEL_COMPRESSIBLE) {
+        // compressible/compressible.cpp:255-281 with the arguments of its
+        // new-start run (nu = 1, ep_arg = 500: eps = -0.5; runCompComet:30)
+        p.dt = 0.001;
+        p.eps = -0.5;
+        p.nu = 1.0;
+        p.lambda = p.nu;
+        const double amp = std::sqrt((3 * p.beta + std::sqrt(9 * p.beta * p.beta + 40 * p.eps * p.gamma)) / 5);
+        p.rho0 = 0.01;
+        p.kp = (1 - p.rho0) / amp;
+        p.rho_m = (p.kp * amp + p.rho0) / 2;
+    }
     *o = p;
 }
 

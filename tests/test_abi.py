@@ -4,6 +4,7 @@ deterministic and match their defining formulas."""
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -28,9 +29,21 @@ def test_library_exports_every_declared_symbol(smadft):
         assert hasattr(L, name), name
 
 
-def test_params_struct_layout(smadft):
-    # model..nz ints (16 B), 13 doubles (104 B), 3 ints (12 B), padded to 136 B
-    assert ctypes.sizeof(smadft.CParams) == 136
+def test_params_struct_layout(smadft, tmp_path):
+    # model..nz ints (16 B), 13 doubles (104 B), 3 ints (12 B), zeta at 136, rho_up at 144,
+    # padded to 152 B; the offsets the C compiler gives the header's struct
+    assert ctypes.sizeof(smadft.CParams) == 152
+    names = [n for n, _ in smadft.CParams._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"smadft/smadft.h\"\n"
+                   "int main(void){printf(\"%zu\\n\", sizeof(smad_params));\n" +
+                   "".join(f'printf("%zu\\n", offsetof(smad_params, {"lambda" if n == "lambda_" else n}));\n'
+                           for n in names) + "return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got[0] == ctypes.sizeof(smadft.CParams)
+    assert got[1:] == [getattr(smadft.CParams, n).offset for n in names]
 
 
 def test_init_rejects_bad_sizes(smadft):
