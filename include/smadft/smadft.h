@@ -117,8 +117,15 @@ enum {
     SMAD_FIELD_FORCE_Y = 26,
     SMAD_FIELD_FORCE_Z = 27,
     SMAD_FIELD_DIVV = 28,        /* the imposed divergence in real space */
-    SMAD_FIELD_DRHO_OLD = 29     /* quasi: rho_old - rho0 as stored (the restart state's
+    SMAD_FIELD_DRHO_OLD = 29,    /* quasi: rho_old - rho0 as stored (the restart state's
                                     form of SMAD_FIELD_RHO_OLD, exact in both precisions) */
+    /* compressible (SMAD_FIELD_RHO there is the carried density: fetch and
+       set, e.g. the loaded density of a load == 1 start) */
+    SMAD_FIELD_F_RHO = 30,       /* continuity right-hand side of the last step (AB2 history) */
+    SMAD_FIELD_AMP = 31,         /* filtered amplitude sqrt(q0^2 psi^2 + |grad psi|^2) of the last step */
+    SMAD_FIELD_VIRR_X = 32,      /* irrotational part of the velocity of the last step */
+    SMAD_FIELD_VIRR_Y = 33,
+    SMAD_FIELD_VIRR_Z = 34
 };
 
 /* Reductions (FP64 on the device, deterministic order). */
@@ -126,8 +133,10 @@ enum {
     SMAD_RED_L1 = 0,    /* out[0] = sum|psi_n+1 - psi_n| over the last step taken,
                            out[1] = sum|psi_n+1|, out[2] = out[0]/(dt out[1]);
                            reference L1 check, no-adv/cosNoAdvTrackMult.cpp:814-842 */
-    SMAD_RED_MASS = 1,  /* quasi: out[0] = sum(rho) dx^3 (quasi/quasi.cpp:2238-2243) */
-    SMAD_RED_ENERGY = 2 /* quasi: out[0] = sum(e) dx^3 (quasi/quasi.cpp:2244-2248) */
+    SMAD_RED_MASS = 1,  /* quasi, compressible: out[0] = sum(rho) dx^3 (quasi/quasi.cpp:2238-2243,
+                           compressible/compressible.cpp:2697-2701) */
+    SMAD_RED_ENERGY = 2 /* quasi: out[0] = sum(e) dx^3 (quasi/quasi.cpp:2244-2248); compressible:
+                           sum(rho e + zeta (rho - rho0 - kp amp)^2/2) dx^3 (compressible.cpp:2669-2702) */
 };
 
 /* Create a context and run the reference pre-loop on psi0 (host, FP64,

@@ -40,6 +40,8 @@ FIELD_FPROJ_X, FIELD_FPROJ_Y, FIELD_FPROJ_Z = 22, 23, 24
 FIELD_FORCE_X, FIELD_FORCE_Y, FIELD_FORCE_Z = 25, 26, 27
 FIELD_DIVV = 28
 FIELD_DRHO_OLD = 29
+FIELD_F_RHO, FIELD_AMP = 30, 31
+FIELD_VIRR_X, FIELD_VIRR_Y, FIELD_VIRR_Z = 32, 33, 34
 
 RED_L1, RED_MASS, RED_ENERGY = 0, 1, 2
 

@@ -565,6 +565,8 @@ enum ZSet : int {
     ZS_QPSI_LAP = 2,  // ZS_QPSI + OP_LAPF (before the divv switch)
     ZS_QRHO = 3,      // (OP_FILT, OP_FILT_SHIFT): filtered rho and grad_z rho
     ZS_NMQ2 = 4,      // (OP_NMQ2): laplacian of rho f
+    ZS_GRAD = 5,      // (OP_ID, OP_SHIFT): a field and its z derivative (compressible grad rho)
+    ZS_FILT = 6,      // (OP_FILT): Gaussian filter (compressible amplitude)
 };
 
 // Coefficients of the spectral multipliers.
@@ -1539,6 +1541,208 @@ __global__ __launch_bounds__(WG) void k_q_x(const T* __restrict__ nl_zy, T* __re
     }
 }
 
+// ------------------------------------------------------------------------
+// Compressible model (compressible/compressible.cpp:1271-2642), pointwise
+// and spectral stages on the global grid, FP64 arithmetic.  The density is
+// an independent field with its own AB2 history.
+struct CompCoef {
+    PlainCoef pc;
+    double nu, lambda, rho0, kp, sig2, dt, zeta;
+    int rho_up;
+};
+
+// Relaxation density (:1594-1628): rho = rho0 + kp amp and lap <- rho lap;
+// with rho_up > 0 the density is lifted by rho_up planes along z (1 below)
+// and lap is multiplied once more by the lifted density, as there.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_c_relax(const T* __restrict__ amp, T* __restrict__ lap, T* __restrict__ rho,
+                                                Dims d, CompCoef cc)
+{
+    const long n = d.total();
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        const int k = (int)(id % d.nz);
+        const double r = cc.rho0 + cc.kp * (double)amp[id];
+        double l = r * (double)lap[id], rn = r;
+        if (cc.rho_up > 0) {
+            rn = k >= cc.rho_up ? cc.rho0 + cc.kp * (double)amp[id - cc.rho_up] : 1.0;
+            l = rn * l;
+        }
+        lap[id] = T(l);
+        rho[id] = T(rn);
+    }
+}
+
+// rho f, the argument of the spectral Laplacian after the switch (:1546-1551)
+template <typename T>
+__global__ __launch_bounds__(WG) void k_c_rhof(const T* __restrict__ rho, const T* __restrict__ f,
+                                               T* __restrict__ out, long n)
+{
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG)
+        out[id] = T((double)rho[id] * (double)f[id]);
+}
+
+// Stresses and forces (:1636-1671), e0 = f^2/2, w = rho - rho0 - kp amp:
+//   fgrad = zeta (rho^2 - (rho0 + kp amp)^2)/2,  E = rho e0,
+//   S = (lap + rho f) grad psi + e0 grad rho,
+//   mu = lap + rho (f - eps psi - beta psi^3 + gamma psi^5),
+//   fv = kp zeta w/(amp + 0.001) grad psi.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_c_force(const T* __restrict__ psi, const T* __restrict__ f,
+                                                const T* __restrict__ lap, const T* __restrict__ rho,
+                                                const T* __restrict__ amp, const T* __restrict__ gpx,
+                                                const T* __restrict__ gpy, const T* __restrict__ gpz,
+                                                const T* __restrict__ grx, const T* __restrict__ gry,
+                                                const T* __restrict__ grz, T* __restrict__ fg, T* __restrict__ E,
+                                                T* __restrict__ mu, T* __restrict__ sx, T* __restrict__ sy,
+                                                T* __restrict__ sz, T* __restrict__ fvx, T* __restrict__ fvy,
+                                                T* __restrict__ fvz, long n, CompCoef cc)
+{
+    const PlainCoef& cf = cc.pc;
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        const double x = (double)psi[id], x2 = x * x, x3 = x2 * x, fv = (double)f[id], r = (double)rho[id];
+        const double a = (double)amp[id], lp = (double)lap[id], e0 = 0.5 * fv * fv;
+        const double req = cc.rho0 + cc.kp * a;
+        fg[id] = T(0.5 * cc.zeta * (r * r - req * req));
+        const double mt = lp + r * fv;
+        const double hx = (double)gpx[id], hy = (double)gpy[id], hz = (double)gpz[id];
+        sx[id] = T(mt * hx + e0 * (double)grx[id]);
+        sy[id] = T(mt * hy + e0 * (double)gry[id]);
+        sz[id] = T(mt * hz + e0 * (double)grz[id]);
+        mu[id] = T(lp + r * (fv - cf.eps * x - cf.beta * x3 + cf.gamma * x3 * x2));
+        E[id] = T(r * e0);
+        const double pen = cc.kp * cc.zeta * (r - cc.rho0 - cc.kp * a) / (a + 0.001);
+        fvx[id] = T(pen * hx);
+        fvy[id] = T(pen * hy);
+        fvz[id] = T(pen * hz);
+    }
+}
+
+// Velocity (:1826-2410), spectral.  S_hat in the bases (STx, STy, STz),
+// fg_hat and E_hat unscaled REDFT10^3; with G = fg_hat + E_hat g(|q|^2),
+// g = exp(-2 sig^2 |q|^2), at the staggered point P of each component
+// (P = (i+1, j, k) for x):
+//   vsol = s/(nu |q|^2) (S - q (q . S)/|q|^2)
+//   virr = s/((lambda + 2 nu)|q|^2) (q G(P) + q (q . S) g/|q|^2)
+// with the shifted reads and truncations of the quasi projection.  The z
+// component's filter at k = 0, i >= 1, j >= 1 takes the y component's |q|^2
+// (:2350).  Writes u_hat = vsol + virr and virr_hat.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_c_vel(const T* __restrict__ Sx, const T* __restrict__ Sy,
+                                              const T* __restrict__ Sz, const T* __restrict__ fg,
+                                              const T* __restrict__ E, T* __restrict__ ux, T* __restrict__ uy,
+                                              T* __restrict__ uz, T* __restrict__ vrx, T* __restrict__ vry,
+                                              T* __restrict__ vrz, Dims d, CompCoef cc)
+{
+    const PlainCoef& cf = cc.pc;
+    const long n = d.total(), sx = (long)d.ny * d.nz, sy = d.nz;
+    const double sc = cf.scale, c2 = -cc.sig2 * 2.0, l2n = cc.lambda + 2 * cc.nu;
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        const double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
+        const double qsx = (i + 1) * cf.qfx, qsy = (j + 1) * cf.qfy, qsz = (k + 1) * cf.qfz;
+        double vs[3] = {0.0, 0.0, 0.0}, vr[3] = {0.0, 0.0, 0.0};
+        if (i < d.nx - 1) {
+            const double s2 = j != 0 ? (double)Sy[id + sx - sy] : 0.0;
+            const double s3 = k != 0 ? (double)Sz[id + sx - 1] : 0.0;
+            const double mq2 = qsx * qsx + qy * qy + qz * qz, cm1 = sc / mq2, cm2 = qsx / mq2;
+            const double dot = qsx * (double)Sx[id] + qy * s2 + qz * s3, g = exp(c2 * mq2);
+            const double G = (double)fg[id + sx] + (double)E[id + sx] * g;
+            vs[0] = (cm1 / cc.nu) * ((double)Sx[id] - cm2 * dot);
+            vr[0] = (cm1 / l2n) * (qsx * G + cm2 * dot * g);
+        }
+        if (j < d.ny - 1 && k < d.nz - 1) {
+            const double s1 = i != 0 ? (double)Sx[id - sx + sy] : 0.0;
+            const double s3 = k != 0 ? (double)Sz[id + sy - 1] : 0.0;
+            const double mq2y = qx * qx + qsy * qsy + qz * qz, cm1 = sc / mq2y, cm2 = qsy / mq2y;
+            const double dot = qx * s1 + qsy * (double)Sy[id] + qz * s3, g = exp(c2 * mq2y);
+            const double G = (double)fg[id + sy] + (double)E[id + sy] * g;
+            vs[1] = (cm1 / cc.nu) * ((double)Sy[id] - cm2 * dot);
+            vr[1] = (cm1 / l2n) * (qsy * G + cm2 * dot * g);
+            const double t1 = i != 0 ? (double)Sx[id - sx + 1] : 0.0;
+            const double t2 = j != 0 ? (double)Sy[id - sy + 1] : 0.0;
+            const double mq2 = qx * qx + qy * qy + qsz * qsz, cz1 = sc / mq2, cz2 = qsz / mq2;
+            const double dz = qx * t1 + qy * t2 + qsz * (double)Sz[id], gz = exp(c2 * mq2);
+            const double gd = (i >= 1 && j >= 1 && k == 0) ? g : gz;
+            const double Gz = (double)fg[id + 1] + (double)E[id + 1] * gz;
+            vs[2] = (cz1 / cc.nu) * ((double)Sz[id] - cz2 * dz);
+            vr[2] = (cz1 / l2n) * (qsz * Gz + cz2 * dz * gd);
+        }
+        ux[id] = T(vs[0] + vr[0]);
+        uy[id] = T(vs[1] + vr[1]);
+        uz[id] = T(vs[2] + vr[2]);
+        vrx[id] = T(vr[0]);
+        vry[id] = T(vr[1]);
+        vrz[id] = T(vr[2]);
+    }
+}
+
+// Divergences (:2413-2465), spectral: divv_hat = q . virr_hat and
+// adg_hat = s q . fv_hat at the DCT point, reading each component one
+// point back along its own axis (0 at the first plane).
+template <typename T>
+__global__ __launch_bounds__(WG) void k_c_div(const T* __restrict__ vrx, const T* __restrict__ vry,
+                                              const T* __restrict__ vrz, const T* __restrict__ fvx,
+                                              const T* __restrict__ fvy, const T* __restrict__ fvz,
+                                              T* __restrict__ divv, T* __restrict__ adg, Dims d, CompCoef cc)
+{
+    const PlainCoef& cf = cc.pc;
+    const long n = d.total(), sx = (long)d.ny * d.nz, sy = d.nz;
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        const double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
+        double a = 0.0, b = 0.0, c = 0.0, af = 0.0, bf = 0.0, cfz = 0.0;
+        if (i > 0) { a = (double)vrx[id - sx]; af = (double)fvx[id - sx]; }
+        if (j > 0) { b = (double)vry[id - sy]; bf = (double)fvy[id - sy]; }
+        if (k > 0) { c = (double)vrz[id - 1]; cfz = (double)fvz[id - 1]; }
+        divv[id] = T(qx * a + qy * b + qz * c);
+        adg[id] = T(cf.scale * (qx * af + qy * bf + qz * cfz));
+    }
+}
+
+// Continuity, AB2 (:2555-2573), and Nl (:2578-2595):
+//   R = -u . grad rho - rho divv,  rho' = rho + dt (3 R - R_old)/2,
+//   Nl = -mu + kp zeta (rho' - rho0 - kp amp)/(amp + 0.001) psi - adg - u . grad psi.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_c_update(const T* __restrict__ ux, const T* __restrict__ uy,
+                                                 const T* __restrict__ uz, const T* __restrict__ grx,
+                                                 const T* __restrict__ gry, const T* __restrict__ grz,
+                                                 const T* __restrict__ divv, const T* __restrict__ mu,
+                                                 const T* __restrict__ amp, const T* __restrict__ psi,
+                                                 const T* __restrict__ adg, const T* __restrict__ gpx,
+                                                 const T* __restrict__ gpy, const T* __restrict__ gpz,
+                                                 T* __restrict__ rho, T* __restrict__ frho, T* __restrict__ nl,
+                                                 long n, CompCoef cc)
+{
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        const double vx = (double)ux[id], vy = (double)uy[id], vz = (double)uz[id], r = (double)rho[id];
+        const double R = -vx * (double)grx[id] - vy * (double)gry[id] - vz * (double)grz[id] - r * (double)divv[id];
+        const double rn = r + cc.dt * (1.5 * R - 0.5 * (double)frho[id]);
+        rho[id] = T(rn);
+        frho[id] = T(R);
+        const double a = (double)amp[id];
+        nl[id] = T(-(double)mu[id] + (cc.kp * cc.zeta / (a + 0.001)) * (rn - cc.rho0 - cc.kp * a) * (double)psi[id] -
+                   (double)adg[id] - vx * (double)gpx[id] - vy * (double)gpy[id] - vz * (double)gpz[id]);
+    }
+}
+
+// energy density of the L1 block (:2669-2678): rho e + zeta (rho - rho0 - kp amp)^2/2,
+// e = -eps psi^2/2 + f^2/2 - beta psi^4/4 + gamma psi^6/6
+template <typename T>
+__global__ __launch_bounds__(WG) void k_c_edens(const T* __restrict__ psi, const T* __restrict__ f,
+                                                const T* __restrict__ rho, const T* __restrict__ amp,
+                                                T* __restrict__ out, long n, CompCoef cc)
+{
+    const PlainCoef& cf = cc.pc;
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        const double x = (double)psi[id], x2 = x * x, fv = (double)f[id], r = (double)rho[id];
+        const double e = -0.5 * cf.eps * x2 + 0.5 * fv * fv - 0.25 * cf.beta * x2 * x2 + (cf.gamma / 6) * x2 * x2 * x2;
+        const double w = r - cc.rho0 - cc.kp * (double)amp[id];
+        out[id] = T(r * e + 0.5 * cc.zeta * w * w);
+    }
+}
+
 // sums {sum a, #non-finite(a)} in FP64
 template <typename T>
 __global__ __launch_bounds__(WG) void k_sum_partial(const T* __restrict__ a, long n, double* __restrict__ part)
@@ -2180,6 +2384,22 @@ struct Launch {
                     const Tw<T>& tw);
     static void l1(hipStream_t s, const T* a, const T* b, long n, double* part, int nparts,
                    double* out);
+    // compressible stages
+    static void c_relax(hipStream_t s, const T* amp, T* lap, T* rho, Dims d, const CompCoef& cc);
+    static void c_rhof(hipStream_t s, const T* rho, const T* f, T* out, long n);
+    static void c_force(hipStream_t s, const T* psi, const T* f, const T* lap, const T* rho, const T* amp,
+                        const T* gpx, const T* gpy, const T* gpz, const T* grx, const T* gry, const T* grz, T* fg,
+                        T* E, T* mu, T* sx, T* sy, T* sz, T* fvx, T* fvy, T* fvz, long n, const CompCoef& cc);
+    static void c_vel(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, const T* fg, const T* E, T* ux, T* uy,
+                      T* uz, T* vrx, T* vry, T* vrz, Dims d, const CompCoef& cc);
+    static void c_div(hipStream_t s, const T* vrx, const T* vry, const T* vrz, const T* fvx, const T* fvy,
+                      const T* fvz, T* divv, T* adg, Dims d, const CompCoef& cc);
+    static void c_update(hipStream_t s, const T* ux, const T* uy, const T* uz, const T* grx, const T* gry,
+                         const T* grz, const T* divv, const T* mu, const T* amp, const T* psi, const T* adg,
+                         const T* gpx, const T* gpy, const T* gpz, T* rho, T* frho, T* nl, long n,
+                         const CompCoef& cc);
+    static void c_edens(hipStream_t s, const T* psi, const T* f, const T* rho, const T* amp, T* out, long n,
+                        const CompCoef& cc);
 };
 
 bool supported_length(int n);

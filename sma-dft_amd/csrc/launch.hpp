@@ -105,12 +105,14 @@ template <typename T>
 void Launch<T>::zpass(hipStream_t s, int zs, const T* in, const PassSet<T>& ps, Dims d, const SpecCoef& sc,
                       const Tw<T>& tw)
 {
-    static const int ops[5][4] = {{OP_ID, OP_SHIFT, OP_ALIN, -1},
+    static const int ops[7][4] = {{OP_ID, OP_SHIFT, OP_ALIN, -1},
                                   {OP_ID, OP_SHIFT, OP_FQ, -1},
                                   {OP_ID, OP_SHIFT, OP_FQ, OP_LAPF},
                                   {OP_FILT, OP_FILT_SHIFT, -1, -1},
-                                  {OP_NMQ2, -1, -1, -1}};
-    if (zs < 0 || zs > ZS_NMQ2) throw std::logic_error("zpass: unknown op set");
+                                  {OP_NMQ2, -1, -1, -1},
+                                  {OP_ID, OP_SHIFT, -1, -1},
+                                  {OP_FILT, -1, -1, -1}};
+    if (zs < 0 || zs > ZS_FILT) throw std::logic_error("zpass: unknown op set");
     for (int o = 0; o < 4; ++o)
         if ((o < ps.n ? ps.op[o] : -1) != ops[zs][o]) throw std::logic_error("zpass: outputs do not match the op set");
     for (int o = 0; o < ps.n; ++o)
@@ -123,6 +125,8 @@ void Launch<T>::zpass(hipStream_t s, int zs, const T* in, const PassSet<T>& ps, 
         SMAD_DISPATCH_OPS(d.nz, k_zmulti, (OP_ID, OP_SHIFT, OP_FQ, OP_LAPF), grid, block, in, ps, d, sc, tw);
         break;
     case ZS_QRHO: SMAD_DISPATCH_OPS(d.nz, k_zmulti, (OP_FILT, OP_FILT_SHIFT), grid, block, in, ps, d, sc, tw); break;
+    case ZS_GRAD: SMAD_DISPATCH_OPS(d.nz, k_zmulti, (OP_ID, OP_SHIFT), grid, block, in, ps, d, sc, tw); break;
+    case ZS_FILT: SMAD_DISPATCH_OPS(d.nz, k_zmulti, (OP_FILT), grid, block, in, ps, d, sc, tw); break;
     default: SMAD_DISPATCH_OPS(d.nz, k_zmulti, (OP_NMQ2), grid, block, in, ps, d, sc, tw); break;
     }
 }
@@ -321,6 +325,58 @@ void Launch<T>::l1(hipStream_t s, const T* a, const T* b, long n, double* part, 
 
 #define SMAD_FLAT(KERNEL, ...) \
     hipLaunchKernelGGL(KERNEL<T>, flat_grid(d.total()), dim3(WG), 0, s, __VA_ARGS__)
+
+template <typename T>
+void Launch<T>::c_relax(hipStream_t s, const T* amp, T* lap, T* rho, Dims d, const CompCoef& cc)
+{
+    SMAD_FLAT(k_c_relax, amp, lap, rho, d, cc);
+}
+
+template <typename T>
+void Launch<T>::c_rhof(hipStream_t s, const T* rho, const T* f, T* out, long n)
+{
+    hipLaunchKernelGGL(k_c_rhof<T>, flat_grid(n), dim3(WG), 0, s, rho, f, out, n);
+}
+
+template <typename T>
+void Launch<T>::c_force(hipStream_t s, const T* psi, const T* f, const T* lap, const T* rho, const T* amp,
+                        const T* gpx, const T* gpy, const T* gpz, const T* grx, const T* gry, const T* grz, T* fg,
+                        T* E, T* mu, T* sx, T* sy, T* sz, T* fvx, T* fvy, T* fvz, long n, const CompCoef& cc)
+{
+    hipLaunchKernelGGL(k_c_force<T>, flat_grid(n), dim3(WG), 0, s, psi, f, lap, rho, amp, gpx, gpy, gpz, grx, gry,
+                       grz, fg, E, mu, sx, sy, sz, fvx, fvy, fvz, n, cc);
+}
+
+template <typename T>
+void Launch<T>::c_vel(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, const T* fg, const T* E, T* ux, T* uy,
+                      T* uz, T* vrx, T* vry, T* vrz, Dims d, const CompCoef& cc)
+{
+    SMAD_FLAT(k_c_vel, Sx, Sy, Sz, fg, E, ux, uy, uz, vrx, vry, vrz, d, cc);
+}
+
+template <typename T>
+void Launch<T>::c_div(hipStream_t s, const T* vrx, const T* vry, const T* vrz, const T* fvx, const T* fvy,
+                      const T* fvz, T* divv, T* adg, Dims d, const CompCoef& cc)
+{
+    SMAD_FLAT(k_c_div, vrx, vry, vrz, fvx, fvy, fvz, divv, adg, d, cc);
+}
+
+template <typename T>
+void Launch<T>::c_update(hipStream_t s, const T* ux, const T* uy, const T* uz, const T* grx, const T* gry,
+                         const T* grz, const T* divv, const T* mu, const T* amp, const T* psi, const T* adg,
+                         const T* gpx, const T* gpy, const T* gpz, T* rho, T* frho, T* nl, long n,
+                         const CompCoef& cc)
+{
+    hipLaunchKernelGGL(k_c_update<T>, flat_grid(n), dim3(WG), 0, s, ux, uy, uz, grx, gry, grz, divv, mu, amp, psi, adg,
+                       gpx, gpy, gpz, rho, frho, nl, n, cc);
+}
+
+template <typename T>
+void Launch<T>::c_edens(hipStream_t s, const T* psi, const T* f, const T* rho, const T* amp, T* out, long n,
+                        const CompCoef& cc)
+{
+    hipLaunchKernelGGL(k_c_edens<T>, flat_grid(n), dim3(WG), 0, s, psi, f, rho, amp, out, n, cc);
+}
 
 template <typename T>
 void Launch<T>::sum(hipStream_t s, const T* a, long n, double* part, int nparts, double* out)

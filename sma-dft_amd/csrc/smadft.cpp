@@ -1557,6 +1557,344 @@ struct QuasiEngine : EngineBase {
     }
 };
 
+// Compressible model (compressible/compressible.cpp:1114-2700): the quasi
+// model's psi chain and BDF2/AB2 update with an independent density carried
+// by an AB2 continuity equation, the penalty zeta and the solenoidal +
+// irrotational velocity.  The density is stored as rho (it spans rho0 = 0.01
+// to 1, so an offset form gains nothing).
+template <typename T>
+struct CompEngine : EngineBase {
+    smad_params p;
+    hipStream_t s;
+    Dims d;
+    size_t n;
+    CompCoef cc{};
+    QuasiCoef qc{};   // the update's C1/C2 (rho_m, dt)
+    QuasiCoef qamp{}; // the amplitude law: kp = 1 in the density-law z pass
+    TwTable<T> twx, twy, twz;
+    DevBuf<T> ph, pho, nl, nlo, rho, frho, psi, psi_prev, f, gpx, gpy, gpz, grx, gry, grz, amp, lap, mu, fg, E, sx,
+        sy, sz, fvx, fvy, fvz, ux, uy, uz, vrx, vry, vrz, divv, adg, tmp;
+    DevBuf<double> part, red;
+    int nparts = 1024;
+    int nloop = 0;
+    bool psi_valid = false, prev_valid = false;
+
+    CompEngine(const smad_params& pp, hipStream_t st) : p(pp), s(st)
+    {
+        d = Dims{p.nx, p.ny, p.nz};
+        n = (size_t)d.total();
+        PlainCoef& cf = cc.pc;
+        const double pi = M_PI;
+        double dx = 2.0 * pi / (p.q0 * p.nw);
+        cf.qfx = pi / (dx * p.nx);
+        cf.qfy = pi / (dx * p.ny);
+        cf.qfz = pi / (dx * p.nz);
+        cf.eps = p.eps;
+        cf.alpha = p.alpha;
+        cf.q0sq = p.q0 * p.q0;
+        cf.beta = p.beta;
+        cf.gamma = p.gamma;
+        cf.dtd2 = p.dt / 2;
+        cf.scale = 0.125 / ((double)p.nx * p.ny * p.nz);
+        cc.nu = p.nu;
+        cc.lambda = p.lambda;
+        cc.rho0 = p.rho0;
+        cc.kp = p.kp;
+        cc.sig2 = p.filter_sigma * p.filter_sigma;
+        cc.dt = p.dt;
+        cc.zeta = p.zeta;
+        cc.rho_up = p.rho_up;
+        qc.pc = cf;
+        qc.nu = p.nu;
+        qc.lambda = p.lambda;
+        qc.rho0 = p.rho0;
+        qc.kp = p.kp;
+        qc.rho_m = p.rho_m;
+        qc.sig2 = cc.sig2;
+        qc.dt = p.dt;
+        qamp = qc;
+        qamp.kp = 1.0;
+        twx.make(p.nx, s);
+        twy.make(p.ny, s);
+        twz.make(p.nz, s);
+        for (DevBuf<T>* b : {&ph, &pho, &nl, &nlo, &rho, &frho, &psi, &psi_prev, &f, &gpx, &gpy, &gpz, &grx, &gry,
+                             &grz, &amp, &lap, &mu, &fg, &E, &sx, &sy, &sz, &fvx, &fvy, &fvz, &ux, &uy, &uz, &vrx,
+                             &vry, &vrz, &divv, &adg, &tmp})
+            b->alloc(n);
+        part.alloc(3 * nparts);
+        red.alloc(8);
+    }
+
+    const char* kernel_class(int c) const override
+    {
+        static const char* names[] = {"axis passes (r2r)", "pointwise / spectral stages", "velocity (FP64)"};
+        return (c >= 0 && c < 3) ? names[c] : nullptr;
+    }
+
+    void tr3(const T* in, T* out, int kx, int ky, int kz, T scale = T(1))
+    {
+        Launch<T>::axis(s, in, out, d, 2, kz, T(1), twz.tw);
+        Launch<T>::axis(s, out, out, d, 1, ky, T(1), twy.tw);
+        Launch<T>::axis(s, out, out, d, 0, kx, scale, twx.tw);
+    }
+    void zero(DevBuf<T>& b) { HIPCHK(hipMemsetAsync(b.p, 0, n * sizeof(T), s)); }
+
+    void preloop()  // compressible.cpp:1149-1194; f_rho starts at zero (:426)
+    {
+        tr3(psi.p, ph.p, REDFT10, REDFT10, REDFT10, T(cc.pc.scale));
+        tr3(ph.p, psi.p, REDFT01, REDFT01, REDFT01);
+        Launch<T>::plain_z(s, psi.p, nl.p, nullptr, d, cc.pc, 1, twz.tw);
+        Launch<T>::axis(s, nl.p, nl.p, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(s, nl.p, nl.p, d, 0, REDFT10, T(1), twx.tw);
+        HIPCHK(hipMemcpyAsync(pho.p, ph.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(nlo.p, nl.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        zero(frho);
+        for (DevBuf<T>* b : {&ux, &uy, &uz, &vrx, &vry, &vrz, &divv, &amp}) zero(*b);
+        LAUNCHCHK();
+        nloop = 0;
+        psi_valid = true;
+        prev_valid = false;
+    }
+
+    void init(const double* psi0) override
+    {
+        upload(s, psi.p, n, psi0);
+        zero(rho);  // a new start: rho = 0 (compressible.cpp:612)
+        preloop();
+    }
+
+    void step_one(bool keep_prev, Prof* prof)
+    {
+        auto mk = [&](int c) { if (prof) prof->mark(c); };
+        ++nloop;
+        const bool relax = nloop <= p.div_switch && !p.load;  // :1542
+        const SpecCoef sc{cc.pc, cc.sig2};
+        // grad rho of the carried density (:1290-1472): s REDFT10^3(rho),
+        // then the shifted inverse passes; E is free until the stresses
+        tr3(rho.p, tmp.p, REDFT10, REDFT10, REDFT10, T(cc.pc.scale));
+        Launch<T>::zpass(s, ZS_GRAD, tmp.p, pass2(E.p, REDFT01, OP_ID, grz.p, RODFT01, OP_SHIFT), d, sc, twz.tw);
+        Launch<T>::spass(s, E.p, pass2(gry.p, RODFT01, OP_SHIFT, E.p, REDFT01, OP_ID), d, 1, sc, twy.tw);
+        Launch<T>::axis(s, grz.p, grz.p, d, 1, REDFT01, T(1), twy.tw);
+        Launch<T>::spass(s, E.p, pass2(grx.p, RODFT01, OP_SHIFT, E.p, REDFT01, OP_ID), d, 0, sc, twx.tw);
+        Launch<T>::axis(s, gry.p, gry.p, d, 0, REDFT01, T(1), twx.tw);
+        Launch<T>::axis(s, grz.p, grz.p, d, 0, REDFT01, T(1), twx.tw);
+        // psi, grad psi, f (and lap_hat of the relaxation branch) from psihat
+        // (:1289-1472, :1517-1534, :1582-1591)
+        if (relax)
+            Launch<T>::zpass(s, ZS_QPSI_LAP, ph.p,
+                             pass4(psi.p, REDFT01, OP_ID, gpz.p, RODFT01, OP_SHIFT, f.p, REDFT01, OP_FQ, lap.p,
+                                   REDFT01, OP_LAPF),
+                             d, sc, twz.tw);
+        else
+            Launch<T>::zpass(s, ZS_QPSI, ph.p,
+                             pass3(psi.p, REDFT01, OP_ID, gpz.p, RODFT01, OP_SHIFT, f.p, REDFT01, OP_FQ), d, sc,
+                             twz.tw);
+        Launch<T>::spass(s, psi.p, pass2(gpy.p, RODFT01, OP_SHIFT, psi.p, REDFT01, OP_ID), d, 1, sc, twy.tw);
+        for (T* b : {gpz.p, f.p, relax ? lap.p : nullptr})
+            if (b) Launch<T>::axis(s, b, b, d, 1, REDFT01, T(1), twy.tw);
+        Launch<T>::spass(s, psi.p, pass2(gpx.p, RODFT01, OP_SHIFT, psi.p, REDFT01, OP_ID), d, 0, sc, twx.tw);
+        for (T* b : {gpy.p, gpz.p, f.p, relax ? lap.p : nullptr})
+            if (b) Launch<T>::axis(s, b, b, d, 0, REDFT01, T(1), twx.tw);
+        mk(0);
+        if (keep_prev) HIPCHK(hipMemcpyAsync(psi_prev.p, psi.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        // filtered amplitude (:1473-1515): the law in the forward z pass,
+        // the filter s exp(-2 sig^2 |q|^2) in the first inverse pass
+        Launch<T>::q_rho_z(s, psi.p, gpx.p, gpy.p, gpz.p, amp.p, d, qamp, twz.tw);
+        Launch<T>::axis(s, amp.p, amp.p, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(s, amp.p, amp.p, d, 0, REDFT10, T(1), twx.tw);
+        Launch<T>::zpass(s, ZS_FILT, amp.p, pass1(amp.p, REDFT01, OP_FILT), d, sc, twz.tw);
+        Launch<T>::axis(s, amp.p, amp.p, d, 1, REDFT01, T(1), twy.tw);
+        Launch<T>::axis(s, amp.p, amp.p, d, 0, REDFT01, T(1), twx.tw);
+        mk(0);
+        // grad^2(rho f) (:1542-1628)
+        if (relax) {
+            Launch<T>::c_relax(s, amp.p, lap.p, rho.p, d, cc);
+            mk(1);
+        } else {
+            Launch<T>::c_rhof(s, rho.p, f.p, lap.p, (long)n);
+            mk(1);
+            tr3(lap.p, lap.p, REDFT10, REDFT10, REDFT10);
+            Launch<T>::zpass(s, ZS_NMQ2, lap.p, pass1(lap.p, REDFT01, OP_NMQ2), d, sc, twz.tw);
+            Launch<T>::axis(s, lap.p, lap.p, d, 1, REDFT01, T(1), twy.tw);
+            Launch<T>::axis(s, lap.p, lap.p, d, 0, REDFT01, T(1), twx.tw);
+            mk(0);
+        }
+        Launch<T>::c_force(s, psi.p, f.p, lap.p, rho.p, amp.p, gpx.p, gpy.p, gpz.p, grx.p, gry.p, grz.p, fg.p, E.p,
+                           mu.p, sx.p, sy.p, sz.p, fvx.p, fvy.p, fvz.p, (long)n, cc);
+        mk(1);
+        // to spectral space (:1675-1718)
+        tr3(fg.p, fg.p, REDFT10, REDFT10, REDFT10);
+        tr3(E.p, E.p, REDFT10, REDFT10, REDFT10);
+        tr3(sx.p, sx.p, RODFT10, REDFT10, REDFT10);
+        tr3(sy.p, sy.p, REDFT10, RODFT10, REDFT10);
+        tr3(sz.p, sz.p, REDFT10, REDFT10, RODFT10);
+        tr3(fvx.p, fvx.p, RODFT10, REDFT10, REDFT10);
+        tr3(fvy.p, fvy.p, REDFT10, RODFT10, REDFT10);
+        tr3(fvz.p, fvz.p, REDFT10, REDFT10, RODFT10);
+        mk(0);
+        // velocity and divergences (:1826-2465)
+        Launch<T>::c_vel(s, sx.p, sy.p, sz.p, fg.p, E.p, ux.p, uy.p, uz.p, vrx.p, vry.p, vrz.p, d, cc);
+        mk(2);
+        Launch<T>::c_div(s, vrx.p, vry.p, vrz.p, fvx.p, fvy.p, fvz.p, divv.p, adg.p, d, cc);
+        mk(1);
+        // back to real space (:2521-2553); virr is kept spectral (fetched on demand)
+        tr3(divv.p, divv.p, REDFT01, REDFT01, REDFT01);
+        tr3(adg.p, adg.p, REDFT01, REDFT01, REDFT01);
+        tr3(ux.p, ux.p, RODFT01, REDFT01, REDFT01);
+        tr3(uy.p, uy.p, REDFT01, RODFT01, REDFT01);
+        tr3(uz.p, uz.p, REDFT01, REDFT01, RODFT01);
+        mk(0);
+        // continuity and Nl (:2555-2595)
+        Launch<T>::c_update(s, ux.p, uy.p, uz.p, grx.p, gry.p, grz.p, divv.p, mu.p, amp.p, psi.p, adg.p, gpx.p, gpy.p,
+                            gpz.p, rho.p, frho.p, nl.p, (long)n, cc);
+        mk(1);
+        Launch<T>::axis(s, nl.p, nl.p, d, 2, REDFT10, T(1), twz.tw);
+        Launch<T>::axis(s, nl.p, nl.p, d, 1, REDFT10, T(1), twy.tw);
+        mk(0);
+        // REDFT10_x of Nl and the BDF2/AB2 update (:2599-2635)
+        Launch<T>::q_x(s, nl.p, ph.p, pho.p, nlo.p, d, qc, nloop == 1 ? 1 : 0, twx.tw);
+        mk(1);
+        psi_valid = false;
+    }
+
+    void step(int nsteps, Prof* prof) override
+    {
+        for (int it = 0; it < nsteps; ++it) step_one(it == nsteps - 1, prof);
+        LAUNCHCHK();
+        if (nsteps > 0) prev_valid = true;
+    }
+
+    void materialize()
+    {
+        if (!psi_valid) {
+            tr3(ph.p, psi.p, REDFT01, REDFT01, REDFT01);
+            LAUNCHCHK();
+            psi_valid = true;
+        }
+    }
+
+    int fetch_virr(int comp, double* out)
+    {
+        static const int ist_kind[3][3] = {{RODFT01, REDFT01, REDFT01}, {REDFT01, RODFT01, REDFT01},
+                                           {REDFT01, REDFT01, RODFT01}};
+        const T* v[3] = {vrx.p, vry.p, vrz.p};
+        HIPCHK(hipMemcpyAsync(tmp.p, v[comp], n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        tr3(tmp.p, tmp.p, ist_kind[comp][0], ist_kind[comp][1], ist_kind[comp][2]);
+        LAUNCHCHK();
+        return download(s, tmp.p, n, out);
+    }
+
+    int fetch(int field, double* out) override
+    {
+        switch (field) {
+        case SMAD_FIELD_PSI: materialize(); return download(s, psi.p, n, out);
+        case SMAD_FIELD_PSI_HAT: return download(s, ph.p, n, out);
+        case SMAD_FIELD_PSI_HAT_OLD: return download(s, pho.p, n, out);
+        case SMAD_FIELD_N_HAT: return download(s, nlo.p, n, out);
+        case SMAD_FIELD_GRAD_X: return download(s, gpx.p, n, out);
+        case SMAD_FIELD_GRAD_Y: return download(s, gpy.p, n, out);
+        case SMAD_FIELD_GRAD_Z: return download(s, gpz.p, n, out);
+        case SMAD_FIELD_GRAD_RHO_X: return download(s, grx.p, n, out);
+        case SMAD_FIELD_GRAD_RHO_Y: return download(s, gry.p, n, out);
+        case SMAD_FIELD_GRAD_RHO_Z: return download(s, grz.p, n, out);
+        case SMAD_FIELD_VEL_X: return download(s, ux.p, n, out);
+        case SMAD_FIELD_VEL_Y: return download(s, uy.p, n, out);
+        case SMAD_FIELD_VEL_Z: return download(s, uz.p, n, out);
+        case SMAD_FIELD_RHO: return download(s, rho.p, n, out);
+        case SMAD_FIELD_F_RHO: return download(s, frho.p, n, out);
+        case SMAD_FIELD_AMP: return download(s, amp.p, n, out);
+        case SMAD_FIELD_DIVV:
+            if (!prev_valid) throw ApiError{SMAD_ESTATE, "the divergence needs at least one step"};
+            return download(s, divv.p, n, out);
+        case SMAD_FIELD_VIRR_X:
+        case SMAD_FIELD_VIRR_Y:
+        case SMAD_FIELD_VIRR_Z:
+            if (!prev_valid) throw ApiError{SMAD_ESTATE, "the velocity parts need at least one step"};
+            return fetch_virr(field - SMAD_FIELD_VIRR_X, out);
+        default: break;
+        }
+        throw ApiError{SMAD_EINVAL, "unknown field for the compressible model"};
+    }
+
+    int state_fields(int* f, int max) const override
+    {
+        return list_fields({SMAD_FIELD_PSI_HAT, SMAD_FIELD_PSI_HAT_OLD, SMAD_FIELD_N_HAT, SMAD_FIELD_RHO,
+                            SMAD_FIELD_F_RHO},
+                           f, max);
+    }
+
+    void set_loop_count(long steps) override { nloop = (int)steps; }
+
+    void set_field(int field, const double* in) override
+    {
+        T* dst = nullptr;
+        switch (field) {
+        case SMAD_FIELD_PSI:
+            upload(s, psi.p, n, in);
+            preloop();
+            return;
+        case SMAD_FIELD_PSI_HAT: dst = ph.p; break;
+        case SMAD_FIELD_PSI_HAT_OLD: dst = pho.p; break;
+        case SMAD_FIELD_N_HAT: dst = nlo.p; break;
+        case SMAD_FIELD_RHO: dst = rho.p; break;
+        case SMAD_FIELD_F_RHO: dst = frho.p; break;
+        default: throw ApiError{SMAD_EINVAL, "field cannot be set in the compressible model"};
+        }
+        upload(s, dst, n, in);
+        if (field != SMAD_FIELD_RHO && field != SMAD_FIELD_F_RHO) psi_valid = false;
+        prev_valid = false;
+    }
+
+    double field_sum(const T* a, double* nonfinite)
+    {
+        Launch<T>::sum(s, a, (long)n, part.p, nparts, red.p);
+        LAUNCHCHK();
+        double h[3];
+        HIPCHK(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        *nonfinite += h[2];
+        return h[0];
+    }
+
+    int reduce(int what, double* out) override
+    {
+        if (!prev_valid) throw ApiError{SMAD_ESTATE, "reductions need at least one step since init"};
+        const double dx = 2.0 * M_PI / (p.q0 * p.nw);
+        double bad = 0.0;
+        if (what == SMAD_RED_L1) {
+            materialize();
+            Launch<T>::l1(s, psi.p, psi_prev.p, (long)n, part.p, nparts, red.p);
+            LAUNCHCHK();
+            double h[3];
+            HIPCHK(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            out[0] = h[0];
+            out[1] = h[1];
+            out[2] = h[0] / (p.dt * h[1]);
+            bad = h[2];
+        } else if (what == SMAD_RED_MASS) {
+            out[0] = field_sum(rho.p, &bad) * dx * dx * dx;
+        } else if (what == SMAD_RED_ENERGY) {
+            // psi of the new step with f and amp of the last one (:2669-2678)
+            materialize();
+            Launch<T>::c_edens(s, psi.p, f.p, rho.p, amp.p, tmp.p, (long)n, cc);
+            out[0] = field_sum(tmp.p, &bad) * dx * dx * dx;
+        } else {
+            throw ApiError{SMAD_EINVAL, "unknown reduction"};
+        }
+        return bad > 0 ? SMAD_ENONFINITE : SMAD_OK;
+    }
+
+    void forward_tape(int, int) override
+    {
+        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain and advective models"};
+    }
+    int adjoint(const double*, double*, double*) override
+    {
+        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain and advective models"};
+    }
+};
+
 bool length_ok(int n)
 {
     return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 384 || n == 512 || n == 1024;
@@ -1641,11 +1979,13 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
         return SMAD_EINVAL;
     }
     if (p.model != SMAD_MODEL_PLAIN && p.model != SMAD_MODEL_ADV && p.model != SMAD_MODEL_QUASI &&
-        p.model != SMAD_MODEL_PLAIN_CONSERVED && p.model != SMAD_MODEL_ADV_CONSERVED) {
+        p.model != SMAD_MODEL_PLAIN_CONSERVED && p.model != SMAD_MODEL_ADV_CONSERVED &&
+        p.model != SMAD_MODEL_COMPRESSIBLE) {
         g_init_error = "smad_init: unknown model";
         return SMAD_EINVAL;
     }
-    if ((p.model == SMAD_MODEL_ADV || p.model == SMAD_MODEL_QUASI || p.model == SMAD_MODEL_ADV_CONSERVED) &&
+    if ((p.model == SMAD_MODEL_ADV || p.model == SMAD_MODEL_QUASI || p.model == SMAD_MODEL_ADV_CONSERVED ||
+         p.model == SMAD_MODEL_COMPRESSIBLE) &&
         !(p.nu > 0)) {
         g_init_error = "smad_init: nu must be positive";
         return SMAD_EINVAL;
@@ -1656,6 +1996,10 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
     }
     if (p.load != 0 && p.load != 1) {
         g_init_error = "smad_init: load must be 0 or 1";
+        return SMAD_EINVAL;
+    }
+    if (p.model == SMAD_MODEL_COMPRESSIBLE && (!(p.lambda + 2 * p.nu > 0) || p.rho_up < 0)) {
+        g_init_error = "smad_init: the compressible model needs lambda + 2 nu > 0 and rho_up >= 0";
         return SMAD_EINVAL;
     }
     auto* c = new smad_ctx;
@@ -1670,9 +2014,12 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
         } else if (p.model == SMAD_MODEL_ADV || p.model == SMAD_MODEL_ADV_CONSERVED) {
             if (f64) c->eng.reset(new AdvEngine<double>(p, c->stream));
             else c->eng.reset(new AdvEngine<float>(p, c->stream));
-        } else {
+        } else if (p.model == SMAD_MODEL_QUASI) {
             if (f64) c->eng.reset(new QuasiEngine<double>(p, c->stream));
             else c->eng.reset(new QuasiEngine<float>(p, c->stream));
+        } else {
+            if (f64) c->eng.reset(new CompEngine<double>(p, c->stream));
+            else c->eng.reset(new CompEngine<float>(p, c->stream));
         }
         c->eng->init(psi0);
         HIPCHK(hipStreamSynchronize(c->stream));
