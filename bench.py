@@ -72,6 +72,9 @@ def main():
                     help="K of the advective adjoint timing leg (0 = skip)")
     ap.add_argument("--adv-grid", type=int, default=256)
     ap.add_argument("--adv-acc", type=int, default=1000)
+    ap.add_argument("--comp-grid", type=int, default=256, help="compressible model grid (0 = skip)")
+    ap.add_argument("--comp-acc", type=int, default=300,
+                    help="steps of the compressible FP32-vs-FP64 accuracy leg (across its switch at 200)")
     ap.add_argument("--quasi-grid", type=int, default=384)
     ap.add_argument("--quasi-acc", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
@@ -335,6 +338,46 @@ def main():
                         f"ratio {(t_ms + a_ms) / fwd:.3f}")
                 entry["adjoint"] = aentry
             models[label] = entry
+
+    # compressible (256^3): the post-relaxation step (div switch after step
+    # 1) in both precisions; accuracy across the program's own switch (200)
+    if args.model_steps > 0 and args.only_fp is None and args.comp_grid > 0:
+        grid = args.comp_grid
+        pm = smadft.Params.default(smadft.MODEL_COMPRESSIBLE, grid)
+        ic = smadft.ic_fc_torus(pm)
+        entry = {"grid": [grid] * 3, "dt": pm.dt, "ic": "fc torus", "timed_after_relaxation": True}
+        acc_switch = pm.div_switch
+        out = {}
+        for prec in (64, 32):
+            pm.precision = prec
+            pm.div_switch = 1
+            with smadft.Stepper(pm, ic) as s:
+                s.step(2)
+                ms = s.time_steps(args.model_steps)
+            sps = 1e3 * args.model_steps / ms
+            # algorithmic bytes per step: 218 field transfers (axis passes,
+            # multi-output passes, stresses, velocity, continuity and update)
+            fb = grid ** 3 * (8 if prec == 64 else 4)
+            entry[f"fp{prec}_steps_per_s"] = sps
+            entry[f"fp{prec}_alg_GBps"] = 218 * fb * sps / 1e9
+            log(f"compressible {grid}^3 fp{prec}: {sps:.1f} steps/s ({218 * fb * sps / 1e12:.2f} TB/s algorithmic)")
+            if args.comp_acc > 0:
+                pm.div_switch = acc_switch
+                with smadft.Stepper(pm, ic) as s:
+                    s.step(args.comp_acc)
+                    try:
+                        out[prec] = {k: s.fetch(f) for k, f in (("psi", smadft.FIELD_PSI), ("rho", smadft.FIELD_RHO),
+                                                                ("ux", smadft.FIELD_VEL_X),
+                                                                ("uz", smadft.FIELD_VEL_Z))}
+                    except smadft.SmadError as e:
+                        entry[f"fp{prec}_error"] = str(e)
+        if len(out) == 2:
+            entry["fp32_rel_l2_vs_fp64"] = {
+                k: float(np.linalg.norm(out[32][k] - out[64][k]) / max(np.linalg.norm(out[64][k]), 1e-300))
+                for k in out[64]}
+            entry["accuracy_after_steps"] = args.comp_acc
+            log(f"compressible fp32 errors: {entry['fp32_rel_l2_vs_fp64']}")
+        models["compressible"] = entry
 
     # the reference programs' 1024x512x512 grid, resident on one GPU
     large = {}

@@ -6,7 +6,10 @@
 // one (adv/cosAdvTrackMult.cpp:125-183) reads nu, ep_arg (eps = -0.01
 // ep_arg), the focal-conic size aE and a restart flag from argv; quasi
 // (quasi/quasi.cpp:173-252) reads nu, ep_arg (eps = -0.001 ep_arg), aE and
-// the restart flag.  This driver keeps their meaning as positional
+// the restart flag; compressible (compressible/compressible.cpp:190-281)
+// reads nu (lambda = nu), ep_arg (eps = -0.001 ep_arg, which also sets the
+// smectic amplitude Amp, kp = (1 - rho0)/Amp and rho_m), aE and the restart
+// flag.  This driver keeps their meaning as positional
 // arguments after the model name, starts from the programs' own constants
 // (smad_default_params) and adds options for the grid, the precision and
 // the run length:
@@ -14,6 +17,7 @@
 //   smadft_run plain [opts]
 //   smadft_run adv   <nu> <ep_arg> <aE> <load> [opts]
 //   smadft_run quasi <nu> <ep_arg> <aE> <load> [opts]
+//   smadft_run comp  <nu> <ep_arg> <aE> <load> [opts]
 //   smadft_run conserved <ep_arg> <aE> <N> [opts]   (no-adv/cosNoAdvConserv.cpp:137-163)
 //   smadft_run advcons <nu> <ep_arg> <aE> <N> <load> [opts]   (adv/cosAdvConSingle.cpp:126-180)
 //   smadft_run --config <0-4> [opts]   (the benchmark configurations: parameters
@@ -23,8 +27,8 @@
 // Loop control follows the reference: step `--l1-every` steps (100 for
 // plain/adv, 50 for quasi), append L1 = sum|psi'-psi| / (dt sum|psi'|) to
 // L1.dat, and stop once L1 <= --l1 (1e-6, cosNoAdvTrackMult.cpp:696) or
-// after --steps steps.  Quasi also appends mass.dat and energy.dat
-// (quasi.cpp:2223-2248).  Every --save-every L1 blocks (10) the save block
+// after --steps steps.  Quasi and compressible also append mass.dat and
+// energy.dat (quasi.cpp:2223-2248, compressible.cpp:2689-2707).  Every --save-every L1 blocks (10) the save block
 // runs: for plain and advective one step before the block's end, for quasi
 // at its end, as in the reference programs.  It rewrites psi0.dat (the
 // single-rank name of the reference's per-rank psi<rank>.dat, one value per
@@ -36,7 +40,10 @@
 // (cosNoAdvTrackMult.cpp:1049-1138, cosAdvTrackMult.cpp:1412-1600); quasi
 // appends surfPsi.dat (the first rho > rho0 + 0.2 from the top of each
 // column) and the y = Ny/2 slices vx, vy, vz, vsolx-z, fx-z, sx-z, divv, p,
-// rho and energyMid (quasi.cpp:2259-2487).  --track-every B writes the plain
+// rho and energyMid (quasi.cpp:2259-2487); compressible rewrites rho0.dat
+// (the full density) and appends surfPsi.dat (the first rho > rho_s - 0.15
+// from the top of each column; a column without one keeps its previous
+// value, 0 at first, compressible.cpp:2716-2790).  --track-every B writes the plain
 // or advective surface files every B blocks instead; --tracker envelope
 // takes the |psi|-envelope column scan of cosNoAdvTrackBump.cpp:929-1031.
 //
@@ -45,7 +52,9 @@
 // load = 1 re-run the pre-loop from psi as the reference does; load = 1
 // reads the reference's per-rank ASCII psi0.dat, psi1.dat, ... (concatenated
 // in rank order, as its x slabs) from --load-dir (default --out), and the
-// quasi model then uses divv from its first step (quasi.cpp:1365-1369).
+// quasi model then uses divv from its first step (quasi.cpp:1365-1369);
+// the compressible model also reads rho<rank>.dat (compressible.cpp:904-924)
+// and skips its relaxation phase (:1542).
 // At the end psi0.dat and state.bin hold the final state, and the wall time
 // and steps/s of the loop are printed.  A non-finite field ends the run with
 // exit code 3 instead of the reference's silent loop exit.
@@ -78,7 +87,7 @@ struct Opts {
     int track_every = -1;    // L1 blocks between surface outputs; -1 = with the saves
     int envelope = 0;        // --tracker envelope: the |psi|-envelope column scan
     long max_steps = 1000000;
-    std::string ic = "";     // focal | slab | curved | bumps | tilted
+    std::string ic = "";     // focal | slab | curved | bumps | tilted | torus
     double amp = 0, noise = 1e-3, ae = -1;
     unsigned long long seed = 1;
     double nu = NAN, ep_arg = NAN, eps = NAN, dt = NAN;
@@ -95,12 +104,13 @@ struct Opts {
     std::cerr << "usage: smadft_run plain [opts]\n"
                  "       smadft_run adv <nu> <ep_arg> <aE> <load> [opts]\n"
                  "       smadft_run quasi <nu> <ep_arg> <aE> <load> [opts]\n"
+                 "       smadft_run comp <nu> <ep_arg> <aE> <load> [opts]\n"
                  "       smadft_run conserved <ep_arg> <aE> <N> [opts]\n"
                  "       smadft_run advcons <nu> <ep_arg> <aE> <N> <load> [opts]\n"
                  "       smadft_run --config <0-4> [opts]\n"
                  "opts: --n N | --nx/--ny/--nz N, --fp32, --steps K, --l1 LIM, --l1-every S,\n"
                  "      --save-every B, --track-every B, --tracker crossing|envelope, --max-steps K,\n"
-                 "      --ic focal|slab|curved|bumps|tilted, --amp A,\n"
+                 "      --ic focal|slab|curved|bumps|tilted|torus, --amp A,\n"
                  "      --noise X, --seed S, --ae A, --eps E, --dt DT, --out DIR,\n"
                  "      --restart state.bin|psi.bin, --load-dir DIR, --ranks P\n";
     std::exit(2);
@@ -141,6 +151,14 @@ Opts parse(int argc, char** argv)
     } else if (m == "quasi") {
         o.model = SMAD_MODEL_QUASI;
         if (argc < 6) usage("quasi needs <nu> <ep_arg> <aE> <load>");
+        o.nu = num(argv[2], "nu");
+        o.ep_arg = num(argv[3], "ep_arg");
+        o.ae = num(argv[4], "aE");
+        o.load = load_flag(argv[5]);
+        pos = 6;
+    } else if (m == "comp") {
+        o.model = SMAD_MODEL_COMPRESSIBLE;
+        if (argc < 6) usage("comp needs <nu> <ep_arg> <aE> <load>");
         o.nu = num(argv[2], "nu");
         o.ep_arg = num(argv[3], "ep_arg");
         o.ae = num(argv[4], "aE");
@@ -287,14 +305,14 @@ bool read_restart(const std::string& path, const smad_params& p, long& steps, st
     return ok;
 }
 
-// The reference's per-rank restart files psi<r>.dat (whitespace-separated
+// The reference's per-rank restart files <name><r>.dat (whitespace-separated
 // values of each rank's x slab, no-adv/cosNoAdvTrackMult.cpp:517-537),
 // concatenated in rank order.
-bool read_rank_files(const std::string& dir, int ranks, std::vector<double>& psi)
+bool read_rank_files(const std::string& dir, const char* name, int ranks, std::vector<double>& psi)
 {
     size_t at = 0;
     for (int r = 0; ranks == 0 || r < ranks; ++r) {
-        std::ifstream f(dir + "psi" + std::to_string(r) + ".dat");
+        std::ifstream f(dir + name + std::to_string(r) + ".dat");
         if (!f) {
             if (ranks == 0 && r > 0) break;
             return false;
@@ -346,19 +364,31 @@ int main(int argc, char** argv)
     if (o.ny) p.ny = o.ny;
     if (o.nz) p.nz = o.nz;
     p.precision = o.precision;
-    p.load = o.model == SMAD_MODEL_QUASI ? o.load : 0;
+    const bool comp = o.model == SMAD_MODEL_COMPRESSIBLE;
+    p.load = (o.model == SMAD_MODEL_QUASI || comp) ? o.load : 0;
     if (!std::isnan(o.nu)) {
         p.nu = o.nu;
-        p.lambda = -2.0 * o.nu / 3.0;  // quasi.cpp:248
+        // quasi.cpp:248 (-2 nu/3); compressible.cpp:271 (lambda = nu)
+        p.lambda = comp ? o.nu : -2.0 * o.nu / 3.0;
     }
-    // eps from argv: -0.001 ep_arg in quasi (quasi.cpp:234), -0.01 ep_arg in
-    // the advective and conserved programs (cosAdvTrackMult.cpp:171)
-    if (!std::isnan(o.ep_arg)) p.eps = (o.model == SMAD_MODEL_QUASI ? -0.001 : -0.01) * o.ep_arg;
+    // eps from argv: -0.001 ep_arg in quasi and compressible (quasi.cpp:234,
+    // compressible.cpp:258), -0.01 ep_arg in the advective and conserved
+    // programs (cosAdvTrackMult.cpp:171)
+    if (!std::isnan(o.ep_arg))
+        p.eps = (o.model == SMAD_MODEL_QUASI || comp ? -0.001 : -0.01) * o.ep_arg;
     if (!std::isnan(o.eps)) p.eps = o.eps;
     if (!std::isnan(o.dt)) p.dt = o.dt;
+    // the compressible program derives its density law from eps
+    // (compressible.cpp:265-270): Amp, kp = (1 - rho0)/Amp, rho_s, rho_m = rho_s/2
+    const double amp_s = std::sqrt((3 * p.beta + std::sqrt(9 * p.beta * p.beta + 40 * p.eps * p.gamma)) / 5);
+    if (comp) {
+        p.kp = (1 - p.rho0) / amp_s;
+        p.rho_m = (p.kp * amp_s + p.rho0) / 2;
+    }
     const bool quasi = o.model == SMAD_MODEL_QUASI;
+    const bool dens = quasi || comp;  // mass and energy every L1 block
     const bool tracked = o.model == SMAD_MODEL_PLAIN || o.model == SMAD_MODEL_ADV;
-    const int l1_every = o.l1_every ? o.l1_every : (quasi ? 50 : 100);
+    const int l1_every = o.l1_every ? o.l1_every : (dens ? 50 : 100);
     const size_t n = (size_t)p.nx * p.ny * p.nz;
     mkdir(o.out.c_str(), 0755);
     const std::string dir = o.out + "/";
@@ -366,16 +396,24 @@ int main(int argc, char** argv)
     // initial condition: load == 1 (the reference's per-rank files), else the
     // configuration's or the model's default morphology; a restart file is
     // read into the context after init
-    std::vector<double> psi(n, 0.0);
+    std::vector<double> psi(n, 0.0), rho_in;
     long taken = 0;
     std::string ic = o.ic;
-    if (ic.empty()) ic = o.config >= 0 ? "config" : (o.model == SMAD_MODEL_PLAIN ? "slab" : "focal");
-    const double amp = o.amp ? o.amp : (quasi ? 1.34164 : 1.328);
+    if (ic.empty())
+        ic = o.config >= 0 ? "config" : (o.model == SMAD_MODEL_PLAIN ? "slab" : comp ? "torus" : "focal");
+    const double amp = o.amp ? o.amp : (quasi ? 1.34164 : comp ? amp_s : 1.328);
     if (o.load == 1) {
         const std::string ld = (o.load_dir.empty() ? o.out : o.load_dir) + "/";
-        if (!read_rank_files(ld, o.ranks, psi)) {
+        if (!read_rank_files(ld, "psi", o.ranks, psi)) {
             std::cerr << "smadft_run: cannot read " << n << " values from " << ld << "psi<rank>.dat\n";
             return 1;
+        }
+        if (comp) {
+            rho_in.assign(n, 0.0);
+            if (!read_rank_files(ld, "rho", o.ranks, rho_in)) {
+                std::cerr << "smadft_run: cannot read " << n << " values from " << ld << "rho<rank>.dat\n";
+                return 1;
+            }
         }
     } else if (!o.restart.empty()) {
         // the context is initialised on zeros and the restart read into it
@@ -392,12 +430,19 @@ int main(int argc, char** argv)
         smad_ic_two_bumps(&p, amp, o.noise, o.seed, psi.data());
     } else if (ic == "tilted") {
         smad_ic_tilted_slab(&p, amp, 15.0, o.noise, o.seed, psi.data(), nullptr);
+    } else if (ic == "torus") {
+        // compressible.cpp:703-727 on 578 x 484 (x scaled by 5.2/6.2); other
+        // grids keep the torus's shape relative to the box
+        const double xs = (p.nx == 578 && p.ny == 484) ? 5.2 / 6.2 : (5.2 / 6.2) * (578.0 / 484.0) * p.ny / p.nx;
+        const double ae = o.ae > 0 ? o.ae : p.nz * 124.0 / 130.0;
+        smad_ic_fc_torus(&p, amp, ae, ae, xs, psi.data());
     } else {
         usage(("unknown --ic " + ic).c_str());
     }
 
     smad_ctx* ctx = nullptr;
     if (smad_init(&p, psi.data(), &ctx) != SMAD_OK) return fail(nullptr, "init");
+    if (!rho_in.empty() && smad_set_field(ctx, SMAD_FIELD_RHO, rho_in.data()) != SMAD_OK) return fail(ctx, "load rho");
     if (!o.restart.empty()) {
         bool full = false;
         if (!read_restart(o.restart, p, taken, psi, ctx, full)) {
@@ -437,7 +482,7 @@ int main(int argc, char** argv)
             for (const char* f : kQuasiSlices) std::ofstream(dir + f).close();
     }
     const size_t cols = (size_t)p.nx * p.ny;
-    std::vector<double> sf(4 * cols), mid((size_t)p.nx * p.nz), fld(n);
+    std::vector<double> sf(4 * cols), mid((size_t)p.nx * p.nz), fld(n), surf_z(cols, 0.0);
     auto slice = [&](const std::vector<double>& v) {
         for (int i = 0; i < p.nx; ++i)
             for (int k = 0; k < p.nz; ++k) mid[(size_t)i * p.nz + k] = v[((size_t)i * p.ny + p.ny / 2) * p.nz + k];
@@ -452,6 +497,19 @@ int main(int argc, char** argv)
             !append_all(dir + "psiMid.dat", mid.data(), mid.size(), "\n")) {
             std::cerr << "smadft_run: cannot write fields in " << dir << "\n";
             return 1;
+        }
+        if (comp) {
+            // compressible.cpp:2716-2790: the full density and the density surface
+            if (smad_fetch(ctx, SMAD_FIELD_RHO, fld.data()) != SMAD_OK) return fail(ctx, "fetch");
+            const double rho_s = p.kp * amp_s + p.rho0;
+            for (size_t c = 0; c < cols; ++c)
+                for (int k = p.nz - 1; k >= 0; --k)
+                    if (fld[c * p.nz + k] > rho_s - 0.15) {
+                        surf_z[c] = k;
+                        break;
+                    }
+            if (!write_ascii(dir + "rho0.dat", fld) || !append_all(dir + "surfPsi.dat", surf_z.data(), cols, "\n"))
+                return 1;
         }
         if (quasi) {
             for (size_t q = 0; q < sizeof(kQuasiFields) / sizeof(int); ++q) {
@@ -502,7 +560,7 @@ int main(int argc, char** argv)
         // the plain and advective save blocks see the state one step before
         // the block's end (they run before psi <- psi_new); quasi's runs
         // after its update
-        const bool early = (save || trk) && !quasi;
+        const bool early = (save || trk) && !dens;
         if (smad_step(ctx, early ? k - 1 : k) != SMAD_OK) { rc = fail(ctx, "step"); break; }
         if (early) {
             // the advective surface call recomputes u and grad psi of this
@@ -524,7 +582,7 @@ int main(int argc, char** argv)
         if (r != SMAD_OK) { rc = fail(ctx, "reduce"); break; }
         l1 = red[2];
         append(dir + "L1.dat", l1);
-        if (quasi) {
+        if (dens) {
             double m[1], e[1];
             if (smad_reduce(ctx, SMAD_RED_MASS, m) != SMAD_OK || smad_reduce(ctx, SMAD_RED_ENERGY, e) != SMAD_OK) {
                 rc = fail(ctx, "reduce");

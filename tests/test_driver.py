@@ -26,7 +26,8 @@ def run(args, **kw):
                                   ["quasi", "1", "700", "8"], ["adv", "1", "70", "8", "2"],
                                   ["--config"], ["--config", "7"],
                                   ["plain", "--bogus"], ["plain", "--n"], ["plain", "--steps", "x"],
-                                  ["plain", "--steps", "0"], ["plain", "--tracker", "top"]])
+                                  ["plain", "--steps", "0"], ["plain", "--tracker", "top"],
+                                  ["comp", "1", "500"], ["comp", "1", "500", "8", "3"]])
 def test_usage_errors(args):
     r = run(args)
     assert r.returncode == 2
@@ -258,3 +259,75 @@ def test_model_modes_match_library(smadft, tmp_path, argv, model, l1_every, n, f
     assert np.allclose(l1, ref, rtol=1e-5)
     if model == 2:
         assert np.loadtxt(out / "mass.dat").shape == (2,)
+
+
+def comp_params(smadft, n, ep_arg):
+    """compressible/compressible.cpp:255-281 with nu = 1 (lambda = nu) and eps
+    = -0.001 ep_arg, from which Amp, kp and rho_m follow."""
+    p = smadft.Params.default(smadft.MODEL_COMPRESSIBLE, n)
+    p.nu, p.lambda_ = 1.0, 1.0
+    p.eps = -0.001 * ep_arg
+    amp = np.sqrt((3 * p.beta + np.sqrt(9 * p.beta ** 2 + 40 * p.eps * p.gamma)) / 5)
+    p.kp = (1 - p.rho0) / amp
+    p.rho_m = (p.kp * amp + p.rho0) / 2
+    return p, amp
+
+
+@pytest.mark.gpu
+def test_comp_mode_matches_library(smadft, tmp_path):
+    """comp <nu> <ep_arg> <aE> <load>: the FC torus start, L1/mass/energy every
+    50 steps and the save block's full density rho0.dat and density surface."""
+    out = tmp_path / "run"
+    r = run(["comp", "1", "850", "12", "0", "--n", "16", "--steps", "100", "--save-every", "1",
+             "--out", str(out)])
+    assert r.returncode == 0, r.stderr
+    p, amp = comp_params(smadft, 16, 850)
+    assert float(info(out / "info")["epsilon"]) == pytest.approx(p.eps)
+    psi0 = smadft.ic_fc_torus(p, amp, 12.0, 12.0, (5.2 / 6.2) * (578.0 / 484.0))
+    with smadft.Stepper(p, psi0) as s:
+        l1, mass, en = [], [], []
+        for _ in range(2):
+            s.step(50)
+            l1.append(s.l1())
+            mass.append(s.reduce(smadft.RED_MASS)[0])
+            en.append(s.reduce(smadft.RED_ENERGY)[0])
+        rho = s.fetch(smadft.FIELD_RHO)
+        psi = s.fetch()
+    assert np.allclose(np.loadtxt(out / "L1.dat"), l1, rtol=1e-5)
+    assert np.allclose(np.loadtxt(out / "mass.dat"), mass, rtol=1e-5)
+    assert np.allclose(np.loadtxt(out / "energy.dat"), en, rtol=1e-5)
+    assert np.allclose(np.loadtxt(out / "rho0.dat"), rho.ravel(), rtol=1e-5, atol=1e-6)
+    assert np.allclose(np.loadtxt(out / "psi0.dat"), psi.ravel(), rtol=1e-5, atol=1e-6)
+    surf = np.loadtxt(out / "surfPsi.dat").reshape(2, 16, 16)
+    rho_s = p.kp * amp + p.rho0
+    hit = rho > rho_s - 0.15
+    assert hit.any()
+    top = np.where(hit.any(axis=2), 15 - np.argmax(hit[:, :, ::-1], axis=2), surf[0])
+    assert np.array_equal(surf[1], top)
+
+
+@pytest.mark.gpu
+def test_comp_load_start_from_rank_files(smadft, tmp_path):
+    """load = 1 reads psi<rank>.dat and rho<rank>.dat (compressible.cpp:874-924)
+    and runs without the relaxation phase."""
+    p, amp = comp_params(smadft, 16, 500)
+    psi0 = smadft.ic_curved(p, amp, 1e-3, seed=3)
+    x = (np.indices(p.shape)[2] + 0.5) / 16
+    rho0 = 0.5 + 0.3 * np.cos(np.pi * x)
+    src = tmp_path / "src"
+    src.mkdir()
+    for r, (a, b) in enumerate(zip(np.split(psi0, 2, axis=0), np.split(rho0, 2, axis=0))):
+        np.savetxt(src / f"psi{r}.dat", a.ravel(), fmt="%.17g")
+        np.savetxt(src / f"rho{r}.dat", b.ravel(), fmt="%.17g")
+    out = tmp_path / "run"
+    r = run(["comp", "1", "500", "8", "1", "--n", "16", "--steps", "50", "--load-dir", str(src),
+             "--ranks", "2", "--out", str(out)])
+    assert r.returncode == 0, r.stderr
+    p.load = 1
+    with smadft.Stepper(p, psi0) as s:
+        s.set_field(smadft.FIELD_RHO, rho0)
+        s.step(50)
+        want = s.fetch()
+        l1 = s.l1()
+    assert np.allclose(np.loadtxt(out / "psi0.dat"), want.ravel(), rtol=1e-5, atol=1e-6)
+    assert np.loadtxt(out / "L1.dat") == pytest.approx(l1, rel=1e-5)
