@@ -26,6 +26,8 @@
 #ifndef SMADFT_SMADFT_H
 #define SMADFT_SMADFT_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -172,6 +174,13 @@ int smad_set_field(smad_ctx *ctx, int field, const double *host_in);
  * xy-transformed psi_hat; advective: psi_hat, N_hat; quasi: psi_hat, its
  * previous value, N_hat, rho_old - rho0, grad rho and u. */
 int smad_state_fields(const smad_ctx *ctx, int *fields, int max);
+
+/* Device-to-device streaming copy of `bytes` (a multiple of 16, both
+ * pointers 16-byte aligned) on `stream` (a hipStream_t; NULL = the null
+ * stream), asynchronous: 16-byte nontemporal loads and stores, one per
+ * thread.  No counterpart in the reference; it measures the device's copy
+ * ceiling that the passes' bandwidth is compared with. */
+int smad_stream_copy(void *dst, const void *src, size_t bytes, void *stream);
 
 /* Sets the step count (smad_steps_taken) and, for the quasi model, the loop
  * counter its divv switch follows; used to restore a full restart state. */

@@ -57,7 +57,7 @@ EXPORTS = (
     "smad_surface_envelope", "smad_r2r_3d_noalloc", "smad_noalloc_last_error",
     "smad_state_fields", "smad_set_steps_taken", "smad_ic_tilted_slab", "smad_ic_two_bumps",
     "smad_config_params", "smad_config_ic", "smad_forward_tape_dev", "smad_adjoint_dev",
-    "smad_set_params", "smad_ic_fc_torus",
+    "smad_set_params", "smad_ic_fc_torus", "smad_stream_copy",
 )
 
 REDFT10, REDFT01, RODFT10, RODFT01 = 0, 1, 2, 3
@@ -163,6 +163,7 @@ def lib() -> ctypes.CDLL:
         L.smad_last_error.restype = ctypes.c_char_p
         L.smad_ic_focal_conic.argtypes = [P(CParams), ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
         L.smad_ic_fc_torus.argtypes = [P(CParams)] + [ctypes.c_double] * 4 + [dp]
+        L.smad_stream_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
         L.smad_ic_slab.argtypes = [P(CParams), ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, dp]
         L.smad_ic_curved.argtypes = [P(CParams), ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, dp]
         L.smad_ic_add_noise.argtypes = [P(CParams), ctypes.c_double, ctypes.c_ulonglong, dp]
@@ -267,6 +268,13 @@ def ic_fc_torus(p: Params, amp=None, a_e=None, b_e=None, x_scale=1.0) -> np.ndar
     out = np.empty(p.shape, dtype=np.float64)
     lib().smad_ic_fc_torus(ctypes.byref(p.c()), amp, a_e, b_e, x_scale, _dptr(out))
     return out
+
+
+def stream_copy(dst_ptr: int, src_ptr: int, nbytes: int, stream: int = 0) -> None:
+    """Streaming device copy (smad_stream_copy): the copy-ceiling kernel."""
+    rc = lib().smad_stream_copy(dst_ptr, src_ptr, nbytes, stream or None)
+    if rc != OK:
+        raise SmadError(rc, lib().smad_last_error(None).decode())
 
 
 def add_noise(p: Params, psi: np.ndarray, amp=1e-3, seed=0) -> np.ndarray:

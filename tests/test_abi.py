@@ -119,3 +119,9 @@ def test_noalloc_rejects_bad_arguments_before_any_device_call(smadft):
     with pytest.raises(smadft.SmadError) as e:
         smadft.r2r_3d_noalloc(fake, fake, (16, 16, 16), (0, 4, 0))
     assert e.value.code == smadft.EINVAL
+
+
+def test_stream_copy_rejects_unaligned_arguments(smadft):
+    L = smadft.lib()
+    for dst, src, nb in ((0, 0, 16), (16, 32, 15), (17, 32, 16), (16, 33, 16)):
+        assert L.smad_stream_copy(dst or None, src or None, nb, None) == smadft.EINVAL

@@ -38,3 +38,17 @@ def test_round_trip_scale(smadft):
     assert rel_l2(y, 8 * 64 ** 3 * x) < 1e-14
     y = smadft.r2r_3d(smadft.r2r_3d(x, (2, 2, 2)), (3, 3, 3))
     assert rel_l2(y, 8 * 64 ** 3 * x) < 1e-14
+
+
+@pytest.mark.parametrize("nbytes", [16, 4096 + 16, 3 * (1 << 20) + 48])
+def test_stream_copy_is_exact(smadft, nbytes):
+    """smad_stream_copy (the copy-ceiling kernel) copies every 16-byte group,
+    including a partial last block, and nothing beyond."""
+    import torch
+    src = torch.arange(nbytes // 8 + 4, dtype=torch.float64, device="cuda:0")
+    dst = torch.full_like(src, -1.0)
+    smadft.stream_copy(dst.data_ptr(), src.data_ptr(), nbytes, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    k = nbytes // 8
+    assert torch.equal(dst[:k], src[:k])
+    assert torch.all(dst[k:] == -1.0)
