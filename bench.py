@@ -478,7 +478,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "alg_bytes_per_launch": alg_bytes[key],
                      "avg_launch_us": 1e3 * avg_ms,
-                     "measured_copy_GBps": copy_gbps,
+                     "measured_copy_GBps": copy_gbps, "runtime_copy_GBps": rt_copy_gbps,
                      "frac_of_measured_copy": (achieved / copy_gbps) if copy_gbps else None},
         "step_alg_GBps": (18 * F) / (ms64 / args.steps * 1e-3) / 1e9 if ms64 else None,
         "device_warmup_s": args.device_warmup_s,
