@@ -1825,31 +1825,32 @@ __global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__
     const double qy = j * cf.qfy;
     const double cm = mode == 1 ? 2.0 : 3.0;
     double sums[2] = {0.0, 0.0};
-    // FP32 storage: the point arithmetic in FP32, each gradient term
-    // converted to FP64 before it is summed (FP64 temporaries of several
-    // groups in flight cost occupancy)
-    const float qyf = float(qy), qfxf = float(cf.qfx), qfzf = float(cf.qfz), q0f = float(cf.q0sq);
-    const float epf = float(cf.eps), alf = float(cf.alpha), dtf = float(cf.dtd2), scf = float(cf.scale);
-    const float cmf = float(cm);
+    // FP32 storage: the CN coefficients C1/C2 and dt/2 /C2 and the lambda
+    // recurrence in FP64, as the forward update computes them (an FP32
+    // C1/C2 is off by a fixed relative 1e-8..1e-7 per mode, which the
+    // recurrence compounds over K steps); 1/C2 from the FP32 reciprocal and
+    // one Newton step (relative error ~1e-15) instead of an FP64 division
     auto point_f = [&](int p, int c, float& lb, float phv) {
-        const float qx = p * qfxf, qz = (k0 + c) * qfzf;
-        const float t = qx * qx + qyf * qyf + qz * qz - q0f;
-        const float al = epf - alf * t * t;
-        const float inv = 1.0f / (1.0f - dtf * al);
-        const float h = dtf * inv;
-        const float a = (1.0f + dtf * al) * inv;
-        const float lold = lb;
-        const float lnew = a * lold + lds[c * LS + PR(p)];
+        const double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
+        const double t = qx * qx + qy * qy + qz * qz - cf.q0sq;
+        const double al = cf.eps - cf.alpha * t * t;
+        const double c2 = 1.0 - cf.dtd2 * al;
+        double inv = (double)(1.0f / (float)c2);
+        inv = inv * (2.0 - c2 * inv);
+        const double h = cf.dtd2 * inv;
+        const double a = (1.0 + cf.dtd2 * al) * inv;
+        const double lold = (double)lb;
+        const double lnew = a * lold + (double)lds[c * LS + PR(p)];
         const float wgt = (p == 0 ? 0.5f : 1.0f) * (j == 0 ? 0.5f : 1.0f) * (k0 + c == 0 ? 0.5f : 1.0f);
-        const float term = wgt * h * phv * (mode == 2 ? lold : lnew + lold);
+        const float term = float(wgt * h) * phv * float(mode == 2 ? lold : lnew + lold);
         sums[0] += (double)term;
-        sums[1] -= (double)(t * t * term);
+        sums[1] -= (double)(float(t * t) * term);
         if (mode == 2) {
-            lds[c * LS + PR(p)] = scf * lnew;
+            lds[c * LS + PR(p)] = float(cf.scale * lnew);
             return;
         }
-        lb = lnew;
-        lds[c * LS + PR(p)] = scf * h * (cmf * lnew - lold);
+        lb = float(lnew);
+        lds[c * LS + PR(p)] = float(cf.scale * h * (cm * lnew - lold));
     };
     auto point = [&](int p, int c, T& lb, T phv) {
         if constexpr (sizeof(T) == 4) {

@@ -61,8 +61,10 @@ def test_adjoint_fp32_mixed_close_to_oracle(smadft, oracle):
     with smadft.Stepper(p, psi0) as s:
         s.forward_tape(K, 5)
         d0, dp = s.adjoint(g)
-    assert rel_l2(d0, d0_o) < 1e-4
-    assert np.max(np.abs(dp - dp_o) / np.abs(dp_o)) < 1e-3
+    # FP32 storage, FP64 CN coefficients and lambda recurrence: measured
+    # 1.3e-6 (dpsi0) and <= 9.2e-7 (theta) on this case
+    assert rel_l2(d0, d0_o) < 3e-6
+    assert np.max(np.abs(dp - dp_o) / np.abs(dp_o)) < 3e-6
 
 
 def test_adjoint_state_errors(smadft):
