@@ -7,6 +7,7 @@
  *   plain      no-adv/cosNoAdvTrackMult.cpp:729-1147 (cosNoAdvTrackSing.cpp:681-1072)
  *   advective  adv/cosAdvTrackMult.cpp:827-1742
  *   quasi      quasi/quasi.cpp:1099-2498
+ *   compressible  compressible/compressible.cpp:1271-2716
  * and below it the FFTW-MPI calls fftw_mpi_plan_r2r_3d / fftw_execute.  This
  * header cuts at that seam: everything above the loop (parameters, initial
  * conditions, L1 cadence, output writers) stays host code, and the loop body
@@ -59,10 +60,14 @@ typedef struct {
     double nw;              /* points per wavelength: L = N 2 pi/(q0 nw) */
     double q0, eps, alpha, beta, gamma, dt;
     double nu;              /* viscosity (advective, quasi) */
-    double lambda;          /* bulk viscosity (quasi), reference -2 nu/3 */
-    double rho0, kp, rho_m; /* density law rho = kp sqrt(q0^2 psi^2 + |grad psi|^2) + rho0 (quasi) */
-    double filter_sigma;    /* Gaussian filter width of rho (quasi), reference 1.57 */
-    int div_switch;         /* quasi: step after which divv and grad^2(rho f) are used */
+    double lambda;          /* bulk viscosity (quasi, reference -2 nu/3; compressible, reference nu) */
+    double rho0, kp, rho_m; /* density law rho = kp sqrt(q0^2 psi^2 + |grad psi|^2) + rho0 (quasi;
+                               compressible: the relaxation density and the penalty's target,
+                               kp = (1 - rho0)/Amp there) */
+    double filter_sigma;    /* Gaussian filter width of rho (quasi) and of the amplitude and the
+                               pressure's rho e term (compressible), reference 1.57 */
+    int div_switch;         /* quasi: step after which divv and grad^2(rho f) are used;
+                               compressible: the end of the relaxation phase (divvSwitch) */
     int precision;          /* SMAD_FP64 or SMAD_FP32_MIXED */
     int load;               /* quasi: 1 = the reference's restart start (load == 1,
                                quasi/quasi.cpp:1365-1369): divv and grad^2(rho f)
@@ -98,7 +103,7 @@ enum {
     SMAD_FIELD_VEL_Y = 7,
     SMAD_FIELD_VEL_Z = 8,
     SMAD_FIELD_PRESSURE = 9, /* real pressure (quasi), FP64 storage */
-    SMAD_FIELD_RHO = 10,     /* filtered density of the last step (quasi) */
+    SMAD_FIELD_RHO = 10,     /* filtered density of the last step (quasi); the carried density (compressible) */
     SMAD_FIELD_ENERGY = 11,  /* free-energy density e of the last step (quasi) */
     SMAD_FIELD_DIVV_HAT = 12, /* REDFT10^3 of the imposed divergence (quasi, unscaled) */
     /* restart state (fetch and set; see smad_state_fields) */
@@ -172,7 +177,8 @@ int smad_set_field(smad_ctx *ctx, int field, const double *host_in);
  * re-runs the pre-loop (no-adv/cosNoAdvTrackMult.cpp:517-537); that is
  * smad_set_field(SMAD_FIELD_PSI).  Plain: psi_hat, N_hat and the carried
  * xy-transformed psi_hat; advective: psi_hat, N_hat; quasi: psi_hat, its
- * previous value, N_hat, rho_old - rho0, grad rho and u. */
+ * previous value, N_hat, rho_old - rho0, grad rho and u; compressible:
+ * psi_hat, its previous value, N_hat, rho and the continuity history. */
 int smad_state_fields(const smad_ctx *ctx, int *fields, int max);
 
 /* Device-to-device streaming copy of `bytes` (a multiple of 16, both
@@ -182,8 +188,9 @@ int smad_state_fields(const smad_ctx *ctx, int *fields, int max);
  * ceiling that the passes' bandwidth is compared with. */
 int smad_stream_copy(void *dst, const void *src, size_t bytes, void *stream);
 
-/* Sets the step count (smad_steps_taken) and, for the quasi model, the loop
- * counter its divv switch follows; used to restore a full restart state. */
+/* Sets the step count (smad_steps_taken) and, for the quasi and
+ * compressible models, the loop counter their switch follows; used to
+ * restore a full restart state. */
 int smad_set_steps_taken(smad_ctx *ctx, long steps);
 
 /* Global FP64 reduction (synchronises).  Returns SMAD_ENONFINITE when psi
