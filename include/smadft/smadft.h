@@ -275,7 +275,7 @@ int smad_adjoint(smad_ctx *ctx, const double *dl_dpsik, double *dl_dpsi0, double
  * forward and writes psi_K to psik_dev.  Both read and write the buffers on
  * the context's stream and return when the work is complete; the caller
  * makes sure its own writes to the inputs are complete before the call.
- * Plain model only, as smad_forward_tape. */
+ * Plain model only (the advective adjoint takes host buffers). */
 int smad_forward_tape_dev(smad_ctx *ctx, const double *psi0_dev, int nsteps, int ckpt_every,
                           double *psik_dev);
 int smad_adjoint_dev(smad_ctx *ctx, const double *dl_dpsik_dev, double *dl_dpsi0_dev,

@@ -169,11 +169,11 @@ struct EngineBase {
     // the same on device FP64 buffers
     virtual void forward_tape_dev(const double*, int, int, double*)
     {
-        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
+        throw ApiError{SMAD_EINVAL, "the device-buffer adjoint is implemented for the plain model"};
     }
     virtual int adjoint_dev(const double*, double*, double*)
     {
-        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
+        throw ApiError{SMAD_EINVAL, "the device-buffer adjoint is implemented for the plain model"};
     }
     // new coefficients for the same model, grid and precision
     virtual void set_params(const smad_params&)
@@ -489,7 +489,7 @@ struct PlainEngine : EngineBase {
 
     void forward_tape(int K, int ck) override
     {
-        if (cf.conserved) throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
+        if (cf.conserved) throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain and advective models"};
         if (!fresh) throw ApiError{SMAD_ESTATE, "forward_tape must start from the pre-loop state (after init or set_field(PSI))"};
         if (K < 1 || ck < 1) throw ApiError{SMAD_EINVAL, "K and ckpt_every must be >= 1"};
         ck = std::min(ck, K);
@@ -1549,11 +1549,11 @@ struct QuasiEngine : EngineBase {
 
     void forward_tape(int, int) override
     {
-        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
+        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain and advective models"};
     }
     int adjoint(const double*, double*, double*) override
     {
-        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain model only"};
+        throw ApiError{SMAD_EINVAL, "the adjoint is implemented for the plain and advective models"};
     }
 };
 
