@@ -70,12 +70,12 @@ def main():
     ap.add_argument("--model-steps", type=int, default=10)
     ap.add_argument("--adv-adj-steps", type=int, default=20,
                     help="K of the advective adjoint timing leg (0 = skip)")
-    ap.add_argument("--adv-grid", type=int, default=256)
+    ap.add_argument("--adv-grid", type=int, default=256, help="advective model grid (0 = skip)")
     ap.add_argument("--adv-acc", type=int, default=1000)
     ap.add_argument("--comp-grid", type=int, default=256, help="compressible model grid (0 = skip)")
     ap.add_argument("--comp-acc", type=int, default=300,
                     help="steps of the compressible FP32-vs-FP64 accuracy leg (across its switch at 200)")
-    ap.add_argument("--quasi-grid", type=int, default=384)
+    ap.add_argument("--quasi-grid", type=int, default=384, help="quasi model grid (0 = skip)")
     ap.add_argument("--quasi-acc", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--large-grid", type=int, default=0,
@@ -283,6 +283,8 @@ def main():
     if args.model_steps > 0 and args.only_fp is None:
         for label, model, grid, nacc in (("advective", smadft.MODEL_ADV, args.adv_grid, args.adv_acc),
                                          ("quasi", smadft.MODEL_QUASI, args.quasi_grid, args.quasi_acc)):
+            if grid <= 0:
+                continue
             cfg = 2 if model == smadft.MODEL_ADV else 3
             pm = smadft.Params.config(cfg, grid)
             ic = smadft.config_ic(cfg, pm)
