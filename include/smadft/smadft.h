@@ -56,7 +56,7 @@ enum { SMAD_FP64 = 64, SMAD_FP32_MIXED = 32 };
  * quasi/quasi.cpp:204-267). */
 typedef struct {
     int model;              /* SMAD_MODEL_* */
-    int nx, ny, nz;         /* grid (each 16, 32, 64, 128, 256, 384, 512 or 1024) */
+    int nx, ny, nz;         /* grid (each 16, 32, 64, 128, 256, 384, 512 or 1024; or 130, 484, 578 in FP64) */
     double nw;              /* points per wavelength: L = N 2 pi/(q0 nw) */
     double q0, eps, alpha, beta, gamma, dt;
     double nu;              /* viscosity (advective, quasi) */

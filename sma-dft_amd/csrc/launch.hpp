@@ -9,7 +9,9 @@
 
 namespace smad {
 
-#define SMAD_LENGTHS(X) X(16) X(32) X(64) X(128) X(256) X(384) X(512)
+// Line lengths of the kernels: 16-1024 and 384, and 130, 484 and 578 (the
+// compressible program's grid, odd prime radices) in FP64 only, where every
+// line is a multiple of 16 bytes.
 
 #define SMAD_DISPATCH_B(n, KERNEL, GRID, BLOCK, ...)                                    \
     switch (n) {                                                                       \
@@ -21,6 +23,9 @@ namespace smad {
     case 384: hipLaunchKernelGGL((KERNEL<T, 384>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
     case 512: hipLaunchKernelGGL((KERNEL<T, 512>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
     case 1024: hipLaunchKernelGGL((KERNEL<T, 1024>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 130: if constexpr (sizeof(T) == 8) hipLaunchKernelGGL((KERNEL<T, 130>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 484: if constexpr (sizeof(T) == 8) hipLaunchKernelGGL((KERNEL<T, 484>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 578: if constexpr (sizeof(T) == 8) hipLaunchKernelGGL((KERNEL<T, 578>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
     default: break;                                                                    \
     }
 #define SMAD_DISPATCH(n, KERNEL, GRID, ...) SMAD_DISPATCH_B(n, KERNEL, GRID, dim3(WG), __VA_ARGS__)
@@ -38,6 +43,9 @@ namespace smad {
     case 384: hipLaunchKernelGGL((KERNEL<T, 384, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
     case 512: hipLaunchKernelGGL((KERNEL<T, 512, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
     case 1024: hipLaunchKernelGGL((KERNEL<T, 1024, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 130: if constexpr (sizeof(T) == 8) hipLaunchKernelGGL((KERNEL<T, 130, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 484: if constexpr (sizeof(T) == 8) hipLaunchKernelGGL((KERNEL<T, 484, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
+    case 578: if constexpr (sizeof(T) == 8) hipLaunchKernelGGL((KERNEL<T, 578, SMAD_UNPAREN OPS>), GRID, BLOCK, 0, s, __VA_ARGS__); break; \
     default: break;                                                                                     \
     }
 

@@ -244,6 +244,65 @@ __device__ __forceinline__ void dft8(cpx<T>* a)
     a[7] = sub(e[3], o3);
 }
 
+// cos(2 pi m/R) and sin(2 pi m/R), m = 0..(R-1)/2, correctly rounded (from
+// 50-digit decimal arithmetic): the constants of the odd prime radices that
+// lines of 130, 484 and 578 points need (65 = 5 13, 242 = 2 11 11, 289 = 17 17).
+constexpr double kCos5[3] = {1.0, 0.30901699437494745, -0.8090169943749475};
+constexpr double kSin5[3] = {0.0, 0.9510565162951535, 0.5877852522924731};
+constexpr double kCos11[6] = {1.0, 0.8412535328311812, 0.41541501300188644, -0.14231483827328514, -0.6548607339452851, -0.9594929736144974};
+constexpr double kSin11[6] = {0.0, 0.5406408174555976, 0.9096319953545183, 0.9898214418809327, 0.7557495743542583, 0.28173255684142967};
+constexpr double kCos13[7] = {1.0, 0.8854560256532099, 0.5680647467311558, 0.12053668025532305, -0.3546048870425356, -0.7485107481711011, -0.970941817426052};
+constexpr double kSin13[7] = {0.0, 0.46472317204376856, 0.8229838658936564, 0.992708874098054, 0.9350162426854148, 0.6631226582407952, 0.23931566428755777};
+constexpr double kCos17[9] = {1.0, 0.9324722294043558, 0.7390089172206591, 0.44573835577653825, 0.092268359463302, -0.2736629900720829, -0.6026346363792564, -0.8502171357296141, -0.9829730996839018};
+constexpr double kSin17[9] = {0.0, 0.3612416661871529, 0.6736956436465572, 0.8951632913550623, 0.9957341762950345, 0.961825643172819, 0.7980172272802395, 0.5264321628773558, 0.18374951781657034};
+
+template <int R>
+__host__ __device__ constexpr double odd_cos(int m)
+{
+    const int a = m <= R / 2 ? m : R - m;
+    return R == 5 ? kCos5[a] : R == 11 ? kCos11[a] : R == 13 ? kCos13[a] : kCos17[a];
+}
+template <int R>
+__host__ __device__ constexpr double odd_sin(int m)
+{
+    const int a = m <= R / 2 ? m : R - m;
+    const double v = R == 5 ? kSin5[a] : R == 11 ? kSin11[a] : R == 13 ? kSin13[a] : kSin17[a];
+    return m <= R / 2 ? v : -v;
+}
+
+// Odd prime radix R by the symmetric form: with s_j = a_j + a_{R-j},
+// d_j = a_j - a_{R-j} (j = 1..(R-1)/2), C_k = a_0 + sum_j s_j cos(2 pi jk/R)
+// and S_k = sum_j d_j sin(2 pi jk/R), the forward DFT is X_k = C_k - i S_k,
+// X_{R-k} = C_k + i S_k (the inverse swaps the signs of i S_k).
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft_odd(cpx<T>* a)
+{
+    constexpr int H = (R - 1) / 2;
+    cpx<T> sj[H + 1], dj[H + 1];
+    const cpx<T> x0 = a[0];
+    cpx<T> sum = x0;
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+        sj[j] = add(a[j], a[R - j]);
+        dj[j] = sub(a[j], a[R - j]);
+        sum = add(sum, sj[j]);
+    }
+    a[0] = sum;
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        cpx<T> C = x0, S = mk(T(0), T(0));
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            const int m = (j * k) % R;
+            C = add(C, scl(sj[j], T(odd_cos<R>(m))));
+            S = add(S, scl(dj[j], T(odd_sin<R>(m))));
+        }
+        const cpx<T> miS = mk(S.im, -S.re);  // -i S
+        a[k] = INV ? sub(C, miS) : add(C, miS);
+        a[R - k] = INV ? add(C, miS) : sub(C, miS);
+    }
+}
+
 template <typename T, int R, bool INV>
 __device__ __forceinline__ void dftR(cpx<T>* a)
 {
@@ -251,6 +310,7 @@ __device__ __forceinline__ void dftR(cpx<T>* a)
     else if constexpr (R == 3) dft3<T, INV>(a);
     else if constexpr (R == 4) dft4<T, INV>(a);
     else if constexpr (R == 8) dft8<T, INV>(a);
+    else if constexpr (R == 5 || R == 11 || R == 13 || R == 17) dft_odd<T, R, INV>(a);
     else static_assert(R == 2, "unsupported radix");
 }
 
@@ -406,6 +466,10 @@ template <> struct Plan<128> { using inv = RList<4, 4, 8>; using fwd = RList<8, 
 template <> struct Plan<192> { using inv = RList<3, 8, 8>; using fwd = RList<8, 8, 3>; };
 template <> struct Plan<256> { using inv = RList<4, 8, 8>; using fwd = RList<8, 8, 4>; };
 template <> struct Plan<512> { using inv = RList<8, 8, 8>; using fwd = RList<8, 8, 8>; };
+// the compressible program's 578 x 484 x 130 grid (FP64)
+template <> struct Plan<65> { using inv = RList<5, 13>; using fwd = RList<13, 5>; };
+template <> struct Plan<242> { using inv = RList<2, 11, 11>; using fwd = RList<11, 11, 2>; };
+template <> struct Plan<289> { using inv = RList<17, 17>; using fwd = RList<17, 17>; };
 
 // Stages of a radix list, starting at span Ns.
 template <typename T, int N, int TPL, bool INV, bool WAVE, class L>
@@ -498,12 +562,17 @@ __device__ __forceinline__ void dct3_first_stage(T* lr, int u, const T* __restri
         int b = u + p * TPL;
         if (B::live(b)) {
             // element r of butterfly b is k = b + r M/R: h_k = h_b e^{-i pi r/4R},
-            // s_k = s_b e^{-i pi r/R}
+            // s_k = s_b e^{-i pi r/R} (constants for R | 24, table reads else)
             const cpx<T> hb = omega<T, N>(ct, b), sb = omega<T, N>(ct, 4 * b);
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                v[p][r] = dct3_pre<T, N>(lr, b + r * B::NB, mul(hb, cis96<T>(r * (24 / R))),
-                                         mul(sb, cis96<T>(r * (96 / R))));
+            for (int r = 0; r < R; ++r) {
+                const int k = b + r * B::NB;
+                if constexpr (24 % R == 0)
+                    v[p][r] = dct3_pre<T, N>(lr, k, mul(hb, cis96<T>(r * (24 / R))),
+                                             mul(sb, cis96<T>(r * (96 / R))));
+                else
+                    v[p][r] = dct3_pre<T, N>(lr, k, omega<T, N>(ct, k), omega<T, N>(ct, 4 * k));
+            }
         }
     }
     lsync<WAVE>();

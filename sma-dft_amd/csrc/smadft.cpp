@@ -1916,8 +1916,12 @@ void stream_copy(void* dst, const void* src, size_t bytes, hipStream_t s)
 
 bool length_ok(int n)
 {
-    return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 384 || n == 512 || n == 1024;
+    return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 384 || n == 512 || n == 1024 ||
+           n == 130 || n == 484 || n == 578;
 }
+
+// 130, 484 and 578 points (radices 5, 11, 13, 17): FP64 only
+bool fp64_only_length(int n) { return n == 130 || n == 484 || n == 578; }
 
 }  // namespace
 
@@ -1990,7 +1994,13 @@ int smad_init(const smad_params* params, const double* psi0, smad_ctx** out)
     *out = nullptr;
     const smad_params& p = *params;
     if (!length_ok(p.nx) || !length_ok(p.ny) || !length_ok(p.nz)) {
-        g_init_error = "smad_init: each grid size must be one of 16, 32, 64, 128, 256, 384, 512, 1024";
+        g_init_error = "smad_init: each grid size must be one of 16, 32, 64, 128, 256, 384, 512, 1024 "
+                       "(or 130, 484, 578 in FP64)";
+        return SMAD_EINVAL;
+    }
+    if (p.precision == SMAD_FP32_MIXED &&
+        (fp64_only_length(p.nx) || fp64_only_length(p.ny) || fp64_only_length(p.nz))) {
+        g_init_error = "smad_init: grid sizes 130, 484 and 578 run in FP64 only";
         return SMAD_EINVAL;
     }
     if (p.precision != SMAD_FP64 && p.precision != SMAD_FP32_MIXED) {
@@ -2272,7 +2282,8 @@ int smad_r2r_3d(int nx, int ny, int nz, const int kinds[3], int precision, const
                 double* out)
 {
     g_init_error.clear();
-    if (!kinds || !in || !out || !length_ok(nx) || !length_ok(ny) || !length_ok(nz)) {
+    if (!kinds || !in || !out || !length_ok(nx) || !length_ok(ny) || !length_ok(nz) ||
+        (precision == SMAD_FP32_MIXED && (fp64_only_length(nx) || fp64_only_length(ny) || fp64_only_length(nz)))) {
         g_init_error = "smad_r2r_3d: bad argument or unsupported size";
         return SMAD_EINVAL;
     }

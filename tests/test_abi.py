@@ -125,3 +125,18 @@ def test_stream_copy_rejects_unaligned_arguments(smadft):
     L = smadft.lib()
     for dst, src, nb in ((0, 0, 16), (16, 32, 15), (17, 32, 16), (16, 33, 16)):
         assert L.smad_stream_copy(dst or None, src or None, nb, None) == smadft.EINVAL
+
+
+def test_odd_radix_lengths_are_fp64_only(smadft):
+    """578, 484 and 130 points (the compressible program's grid) run in FP64;
+    FP32 is rejected before any device work."""
+    for shape in ((578, 16, 16), (16, 484, 16), (16, 16, 130)):
+        with pytest.raises(smadft.SmadError) as e:
+            smadft.r2r_3d(np.zeros(shape), (0, 0, 0), smadft.FP32_MIXED)
+        assert e.value.code == smadft.EINVAL
+        p = smadft.Params.default(smadft.MODEL_COMPRESSIBLE, 16)
+        p.nx, p.ny, p.nz = shape
+        p.precision = 32
+        with pytest.raises(smadft.SmadError) as e:
+            smadft.Stepper(p, np.zeros(p.shape))
+        assert e.value.code == smadft.EINVAL

@@ -18,7 +18,8 @@ def adv_params(smadft, shape, prec=64, **kw):
 
 
 @pytest.mark.parametrize("shape,nsteps", [((32, 32, 32), 40), ((16, 32, 64), 25), ((384, 16, 32), 20),
-                                          ((16, 384, 32), 20), ((32, 16, 384), 20)])
+                                          ((16, 384, 32), 20), ((32, 16, 384), 20), ((130, 16, 32), 15),
+                                          ((16, 484, 16), 10), ((16, 16, 578), 10)])
 def test_adv_fp64_matches_oracle(smadft, oracle, shape, nsteps):
     p = adv_params(smadft, shape)
     psi0 = smadft.ic_curved(p, 1.328, 1e-3, seed=2)

@@ -46,7 +46,9 @@ def check_fp64(g, o, tol=1e-9):
 
 
 @pytest.mark.parametrize("shape,nsteps,switch,up", [((32, 32, 32), 10, 4, 3), ((16, 32, 64), 8, 3, 16),
-                                                    ((384, 16, 32), 5, 2, 2), ((32, 16, 384), 5, 2, 16)])
+                                                    ((384, 16, 32), 5, 2, 2), ((32, 16, 384), 5, 2, 16),
+                                                    ((130, 16, 32), 5, 2, 3), ((16, 484, 16), 4, 2, 2),
+                                                    ((32, 16, 578), 4, 2, 16), ((578, 16, 130), 3, 1, 16)])
 def test_comp_fp64_matches_oracle(smadft, oracle, shape, nsteps, switch, up):
     p = cparams(smadft, shape, div_switch=switch, rho_up=up)
     psi0 = smadft.ic_curved(p, 1.4142, 1e-3, seed=3)
@@ -128,3 +130,22 @@ def test_comp_full_state_restart_is_bit_identical(smadft, prec):
     assert np.array_equal(got, want)
     assert np.array_equal(got_rho, want_rho)
     assert np.array_equal(got_l1, want_l1)
+
+
+def test_comp_reference_grid_runs(smadft):
+    """The program's own grid, 578 x 484 x 130, FP64, from its FC torus
+    (x scaled by 5.2/6.2, centre 242, radius 121): a few steps across the
+    switch stay finite, and the density stays within [rho0, 1] up to the
+    continuity's small excursions."""
+    p = cparams(smadft, (578, 484, 130), div_switch=2)
+    amp = np.sqrt((3 * p.beta + np.sqrt(9 * p.beta ** 2 + 40 * p.eps * p.gamma)) / 5)
+    psi0 = smadft.ic_fc_torus(p, amp, 124.0, 124.0, 5.2 / 6.2)
+    print("comp reference grid 578x484x130", flush=True)
+    with smadft.Stepper(p, psi0) as s:
+        s.step(4)
+        l1 = s.reduce(smadft.RED_L1)
+        rho = s.fetch(smadft.FIELD_RHO)
+        psi = s.fetch()
+    assert np.isfinite(l1).all() and l1[0] > 0
+    assert np.isfinite(psi).all() and np.abs(psi).max() < 2
+    assert rho.min() > 0.0 and rho.max() < 1.1

@@ -20,7 +20,9 @@ def params(smadft, shape, prec=64, **kw):
 
 @pytest.mark.parametrize("shape,nsteps", [((32, 32, 32), 100), ((16, 32, 64), 60), ((64, 64, 64), 400),
                                           ((1024, 16, 32), 30), ((32, 16, 1024), 30), ((16, 16, 512), 30),
-                                          ((384, 16, 32), 30), ((32, 384, 16), 30), ((16, 32, 384), 30)])
+                                          ((384, 16, 32), 30), ((32, 384, 16), 30), ((16, 32, 384), 30),
+                                          ((578, 16, 16), 20), ((16, 484, 16), 20), ((16, 16, 130), 30),
+                                          ((130, 32, 578), 10)])
 def test_plain_fp64_matches_oracle(smadft, oracle, shape, nsteps):
     p = params(smadft, shape)
     psi0 = smadft.ic_slab(p, 0.6, 1e-3, seed=0)

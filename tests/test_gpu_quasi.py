@@ -18,7 +18,8 @@ def qparams(smadft, shape, prec=64, **kw):
 
 
 @pytest.mark.parametrize("shape,nsteps,switch", [((32, 32, 32), 14, 5), ((16, 32, 64), 9, 3),
-                                                 ((384, 16, 32), 6, 2), ((32, 16, 384), 6, 2)])
+                                                 ((384, 16, 32), 6, 2), ((32, 16, 384), 6, 2),
+                                                 ((578, 16, 16), 4, 2), ((16, 484, 16), 4, 2), ((16, 16, 130), 5, 2)])
 def test_quasi_fp64_matches_oracle(smadft, oracle, shape, nsteps, switch):
     p = qparams(smadft, shape, div_switch=switch)
     psi0 = smadft.ic_curved(p, 1.34164, 1e-3, seed=3)

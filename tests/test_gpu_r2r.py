@@ -31,6 +31,21 @@ def test_r2r3d_fp32(smadft, oracle, shape, kinds):
     assert rel_l2(y, ref) < 3e-7
 
 
+# the compressible program's grid lengths (578 = 2 17^2, 484 = 4 11^2,
+# 130 = 2 5 13: radices 17, 11, 13, 5), FP64 only
+ODD_SHAPES = [(130, 16, 16), (16, 484, 16), (16, 16, 578), (16, 16, 130), (578, 16, 16), (16, 130, 484)]
+
+
+@pytest.mark.parametrize("shape", ODD_SHAPES)
+@pytest.mark.parametrize("kinds", COMBOS)
+def test_r2r3d_fp64_odd_radices(smadft, oracle, shape, kinds):
+    rng = np.random.default_rng(hash((shape, kinds)) % 2**32)
+    x = rng.standard_normal(shape)
+    ref = oracle.r2r3d(x, kinds)
+    y = smadft.r2r_3d(x, kinds, smadft.FP64)
+    assert rel_l2(y, ref) < 1e-14
+
+
 def test_round_trip_scale(smadft):
     rng = np.random.default_rng(1)
     x = rng.standard_normal((64, 64, 64))
