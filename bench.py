@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--comp-grid", type=int, default=256, help="compressible model grid (0 = skip)")
     ap.add_argument("--comp-acc", type=int, default=300,
                     help="steps of the compressible FP32-vs-FP64 accuracy leg (across its switch at 200)")
+    ap.add_argument("--comp-ref-steps", type=int, default=5,
+                    help="timed compressible steps on the program's own 578x484x130 grid (0 = skip)")
     ap.add_argument("--quasi-grid", type=int, default=384, help="quasi model grid (0 = skip)")
     ap.add_argument("--quasi-acc", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
@@ -390,6 +392,20 @@ def main():
                 for k in out[64]}
             entry["accuracy_after_steps"] = args.comp_acc
             log(f"compressible fp32 errors: {entry['fp32_rel_l2_vs_fp64']}")
+        if args.comp_ref_steps > 0:
+            # the program's own grid, 578 x 484 x 130 (FP64: radix 17, 11, 13, 5
+            # lines), from its FC torus, after the relaxation phase
+            pr = smadft.Params.default(smadft.MODEL_COMPRESSIBLE, 578)
+            pr.nx, pr.ny, pr.nz = 578, 484, 130
+            pr.div_switch = 1
+            amp_s = np.sqrt((3 * pr.beta + np.sqrt(9 * pr.beta ** 2 + 40 * pr.eps * pr.gamma)) / 5)
+            with smadft.Stepper(pr, smadft.ic_fc_torus(pr, amp_s, 124.0, 124.0, 5.2 / 6.2)) as s:
+                s.step(2)
+                ms = s.time_steps(args.comp_ref_steps)
+            sps = 1e3 * args.comp_ref_steps / ms
+            fb = 578 * 484 * 130 * 8
+            entry["reference_grid_578x484x130_fp64"] = {"steps_per_s": sps, "alg_GBps": 218 * fb * sps / 1e9}
+            log(f"compressible 578x484x130 fp64: {sps:.1f} steps/s ({218 * fb * sps / 1e12:.2f} TB/s algorithmic)")
         models["compressible"] = entry
 
     # the reference programs' 1024x512x512 grid, resident on one GPU
