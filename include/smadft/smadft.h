@@ -132,7 +132,11 @@ enum {
     SMAD_FIELD_AMP = 31,         /* filtered amplitude sqrt(q0^2 psi^2 + |grad psi|^2) of the last step */
     SMAD_FIELD_VIRR_X = 32,      /* irrotational part of the velocity of the last step */
     SMAD_FIELD_VIRR_Y = 33,
-    SMAD_FIELD_VIRR_Z = 34
+    SMAD_FIELD_VIRR_Z = 34,
+    /* plain (and conserved plain): the carried CN/AB2 history
+       H = C1 psi_hat - (dt/2) s N_hat_past per mode, C1 = 1 + (dt/2) aLin(q),
+       s = 1/(8 nx ny nz); with SMAD_FIELD_SPEC_XY the whole plain state */
+    SMAD_FIELD_CN_HIST = 35
 };
 
 /* Reductions (FP64 on the device, deterministic order). */
@@ -175,8 +179,9 @@ int smad_set_field(smad_ctx *ctx, int field, const double *host_in);
  * continues the run exactly as if it had not been interrupted (FP32 storage
  * is converted exactly).  The reference's own restart keeps psi only and
  * re-runs the pre-loop (no-adv/cosNoAdvTrackMult.cpp:517-537); that is
- * smad_set_field(SMAD_FIELD_PSI).  Plain: psi_hat, N_hat and the carried
- * xy-transformed psi_hat; advective: psi_hat, N_hat; quasi: psi_hat, its
+ * smad_set_field(SMAD_FIELD_PSI).  Plain: the carried xy-transformed
+ * psi_hat and the combined CN/AB2 history (SMAD_FIELD_SPEC_XY,
+ * SMAD_FIELD_CN_HIST); advective: psi_hat, N_hat; quasi: psi_hat, its
  * previous value, N_hat, rho_old - rho0, grad rho and u; compressible:
  * psi_hat, its previous value, N_hat, rho and the continuity history. */
 int smad_state_fields(const smad_ctx *ctx, int *fields, int max);

@@ -42,6 +42,7 @@ FIELD_DIVV = 28
 FIELD_DRHO_OLD = 29
 FIELD_F_RHO, FIELD_AMP = 30, 31
 FIELD_VIRR_X, FIELD_VIRR_Y, FIELD_VIRR_Z = 32, 33, 34
+FIELD_CN_HIST = 35  # plain: the combined CN/AB2 history H = C1 psi_hat - (dt/2) s N_hat_past
 
 RED_L1, RED_MASS, RED_ENERGY = 0, 1, 2
 

@@ -510,14 +510,18 @@ __global__ __launch_bounds__(WG) void k_plain_x_cons(const T* a_in, T* w_out, co
 }
 
 // ------------------------------------------------------------------------
-// Advective model (adv/cosAdvTrackMult.cpp:827-1369).  Pointwise and
-// spectral stages with the reference's index shifts and truncations on the
-// global (single-slab) grid; the transforms are separate axis passes.
-struct AdvCoef {
-    PlainCoef pc;
-    double nu;
-};
+// The plain step in combined-history form.  The CN/AB2 update
+//   psihat' = (C1 psihat + c (3 Nq - Nq_past)) / C2,   c = dtd2 s,
+// reads psihat and Nq_past only in the combination H = C1 psihat - c Nq_past,
+// so the step carries H and w = REDFT01_y REDFT01_x(psihat) and never stores
+// psihat or Nq:
+//   psihat' = (H + 3 c Nq) / C2,   H' = C1 psihat' - c Nq.
+// The x pass reads 2 fields (the transformed nonlinearity and H) and writes 2
+// (REDFT01_x(psihat') and H') instead of 3 and 3.  The pre-loop history
+// (Nq_past = Nq of psi^0, the reference's first-step Euler) enters as
+// H^0 = C1 psihat^0 - c Nq^0.
 
+// (i, j, k) of a flat field index
 __device__ __forceinline__ void unflat(long id, const Dims& d, int& i, int& j, int& k)
 {
     k = (int)(id % d.nz);
@@ -525,6 +529,135 @@ __device__ __forceinline__ void unflat(long id, const Dims& d, int& i, int& j, i
     j = (int)(r % d.ny);
     i = (int)(r / d.ny);
 }
+
+// Per-mode coefficients of the update (FP64): C1, 1/C2 and c.
+struct CnMode {
+    double c1, ic2, c;
+};
+template <bool CONS>
+__device__ __forceinline__ CnMode cn_mode(const PlainCoef& cf, double mq2)
+{
+    const double t = mq2 - cf.q0sq;
+    double al = cf.eps - cf.alpha * t * t;
+    if constexpr (CONS) al = mq2 * al;
+    return CnMode{1.0 + cf.dtd2 * al, 1.0 / (1.0 - cf.dtd2 * al), cf.dtd2 * cf.scale};
+}
+
+// x pass: Nq = REDFT10_x(a_in) (times |q|^2 in the conserved variant);
+// psihat' = (H + 3 c Nq)/C2, H' = C1 psihat' - c Nq; w_out = REDFT01_x(psihat').
+// a_in may equal w_out and h_in may equal h_out.  TAPE: psihat' is also
+// written to ph_tape (the adjoint's psihat tape).
+template <typename T, int N, bool CONS, bool TAPE>
+__device__ __forceinline__ void plain_xh_body(const T* a_in, T* w_out, const T* h_in, T* h_out,
+                                              T* __restrict__ ph_tape, const Dims& d, const PlainCoef& cf,
+                                              const Tw<T>& tw)
+{
+    constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
+    constexpr int W = V16<T>::W;
+    __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    TileGeo g = tile_geo(d, 0, blockIdx.x, NL);
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    struct Mk {
+        __device__ T operator()(int p, int& q) const { q = mk_pos<N>(p); return T(1); }
+    };
+    load_rows<T, N, NL, LS>(g, a_in, lds, Mk{});
+    __syncthreads();
+    dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    __syncthreads();
+    const int nct = (d.nz + NL - 1) / NL;
+    const int j = blockIdx.x / nct;
+    const int k0 = (blockIdx.x % nct) * NL;
+    const double qy = j * cf.qfy;
+    auto point = [&](int p, int c, T hv, T& hn, T& ph) {
+        const double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
+        const double mq2 = qx * qx + qy * qy + qz * qz;
+        const CnMode m = cn_mode<CONS>(cf, mq2);
+        double nq = (double)lds[c * LS + PR(p)];
+        if constexpr (CONS) nq *= mq2;
+        const double r = ((double)hv + 3.0 * m.c * nq) * m.ic2;
+        hn = T(m.c1 * r - m.c * nq);
+        ph = T(r);
+        lds[c * LS + PR(p)] = ph;
+    };
+    if (g.ncol == NL) {
+        rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
+            const V16<T> hv = *reinterpret_cast<const V16<T>*>(h_in + gi);
+            V16<T> hn, ph;
+#pragma unroll
+            for (int t = 0; t < W; ++t) point(p, c0 + t, hv.v[t], hn.v[t], ph.v[t]);
+            *reinterpret_cast<V16<T>*>(h_out + gi) = hn;
+            if constexpr (TAPE) *reinterpret_cast<V16<T>*>(ph_tape + gi) = ph;
+        });
+    } else {
+        tile_for<N, NL>(g, 0, [&](int p, int c, long gi, bool ok) {
+            if (ok) {
+                T hn, ph;
+                point(p, c, h_in[gi], hn, ph);
+                h_out[gi] = hn;
+                if constexpr (TAPE) ph_tape[gi] = ph;
+            } else {
+                lds[c * LS + PR(p)] = T(0);
+            }
+        });
+    }
+    __syncthreads();
+    const int e0 = opaque_zero_if<N>();
+    dct3_line<T, N, TPL, true>(lds + l * LS, u + e0, ct);
+    __syncthreads();
+    store_rows<T, N, NL, LS>(g, w_out, lds, Mk{}, T(1), e0);
+}
+
+template <typename T, int N, bool CONS, bool TAPE>
+__global__ __launch_bounds__(WG) void k_plain_xh(const T* a_in, T* w_out, const T* h_in, T* h_out,
+                                                 T* __restrict__ ph_tape, Dims d, PlainCoef cf, Tw<T> tw)
+{
+    plain_xh_body<T, N, CONS, TAPE>(a_in, w_out, h_in, h_out, ph_tape, d, cf, tw);
+}
+
+// Pointwise conversions between (psihat, Nq_past) and the combined history
+// H = C1 psihat - c Nq_past (spectral fields, FP64 arithmetic):
+//   mode 0: h = C1 ph - c nq           (nq: the history)
+//   mode 1: nq = (C1 ph - h) / c       (recovers the history from H)
+//   mode 2: h = C1' ph - c' (C1 ph - h)/c  (new coefficients cf2 for the same
+//           psihat and history: smad_set_params on a running context)
+template <typename T>
+__global__ __launch_bounds__(WG) void k_plain_hist(const T* __restrict__ ph, T* __restrict__ h, T* __restrict__ nq,
+                                                   Dims d, PlainCoef cf, PlainCoef cf2, int mode)
+{
+    const long n = d.total();
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
+        int i, j, k;
+        unflat(id, d, i, j, k);
+        const double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
+        const double mq2 = qx * qx + qy * qy + qz * qz;
+        const CnMode m = cf.conserved ? cn_mode<true>(cf, mq2) : cn_mode<false>(cf, mq2);
+        const double p = (double)ph[id];
+        if (mode == 0) {
+            h[id] = T(m.c1 * p - m.c * (double)nq[id]);
+        } else {
+            const double past = (m.c1 * p - (double)h[id]) / m.c;
+            if (mode == 1) {
+                nq[id] = T(past);
+            } else {
+                const double rx = i * cf2.qfx, ry = j * cf2.qfy, rz = k * cf2.qfz;
+                const double mq22 = rx * rx + ry * ry + rz * rz;
+                const CnMode m2 = cf2.conserved ? cn_mode<true>(cf2, mq22) : cn_mode<false>(cf2, mq22);
+                h[id] = T(m2.c1 * p - m2.c * past);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Advective model (adv/cosAdvTrackMult.cpp:827-1369).  Pointwise and
+// spectral stages with the reference's index shifts and truncations on the
+// global (single-slab) grid; the transforms are separate axis passes.
+struct AdvCoef {
+    PlainCoef pc;
+    double nu;
+};
 
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* out);
@@ -2322,6 +2455,12 @@ struct Launch {
     static void plain_x(hipStream_t s, const T* a_in, T* w_out, const T* ph_in, T* ph_out,
                         T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse = 1,
                         const T* sub = nullptr, const T* nh_in = nullptr);
+    // the plain step's x pass in combined-history form (k_plain_xh); ph_tape
+    // non-null: psihat' to the adjoint's tape as well
+    static void plain_xh(hipStream_t s, const T* a_in, T* w_out, const T* h_in, T* h_out, T* ph_tape, Dims d,
+                         const PlainCoef& cf, const Tw<T>& tw);
+    static void plain_hist(hipStream_t s, const T* ph, T* h, T* nq, Dims d, const PlainCoef& cf,
+                           const PlainCoef& cf2, int mode);
     // z pass with one output per op of the set `zs` (ZS_*); strided pass
     // with the outputs (OP_SHIFT, OP_ID)
     static void zpass(hipStream_t s, int zs, const T* in, const PassSet<T>& ps, Dims d, const SpecCoef& sc,

@@ -110,6 +110,30 @@ inline dim3 flat_grid(long n)
 }
 
 template <typename T>
+void Launch<T>::plain_xh(hipStream_t s, const T* a_in, T* w_out, const T* h_in, T* h_out, T* ph_tape, Dims d,
+                         const PlainCoef& cf, const Tw<T>& tw)
+{
+    dim3 grid(num_tiles(d, 0, Lines<T>::NL));
+#define SMAD_XH(C, TP) SMAD_DISPATCH_OPS(d.nx, k_plain_xh, (C, TP), grid, dim3(WG), a_in, w_out, h_in, h_out, ph_tape, d, cf, tw)
+    if (cf.conserved) {
+        if (ph_tape) throw std::logic_error("plain_xh: no tape in the conserved variant");
+        SMAD_XH(true, false);
+    } else if (ph_tape) {
+        SMAD_XH(false, true);
+    } else {
+        SMAD_XH(false, false);
+    }
+#undef SMAD_XH
+}
+
+template <typename T>
+void Launch<T>::plain_hist(hipStream_t s, const T* ph, T* h, T* nq, Dims d, const PlainCoef& cf,
+                           const PlainCoef& cf2, int mode)
+{
+    hipLaunchKernelGGL(k_plain_hist<T>, flat_grid(d.total()), dim3(WG), 0, s, ph, h, nq, d, cf, cf2, mode);
+}
+
+template <typename T>
 void Launch<T>::zpass(hipStream_t s, int zs, const T* in, const PassSet<T>& ps, Dims d, const SpecCoef& sc,
                       const Tw<T>& tw)
 {

@@ -267,10 +267,15 @@ struct PlainEngine : EngineBase {
     size_t n;
     PlainCoef cf{};
     TwTable<T> twx, twy, twz;
-    DevBuf<T> w, psihat, nhat, psi, psi_prev;
+    // Carried state: w = REDFT01_y REDFT01_x(psihat) and the combined history
+    // hh = C1 psihat - c Nq_past (k_plain_xh).  psihat and nhat hold the
+    // pre-loop's psihat^0 and Nq^0 while the context is fresh, and serve as
+    // scratch for on-demand fetches afterwards.
+    DevBuf<T> w, hh, psihat, nhat, psi, psi_prev;
     DevBuf<double> part, red;
     int nparts = 1024;
     bool psi_valid = false, prev_valid = false;
+    bool fresh = true;  // the state is the pre-loop's (psihat, nhat exact)
 
     void coefs()
     {
@@ -298,8 +303,22 @@ struct PlainEngine : EngineBase {
         if (!(np.dt > 0) || !(np.nw > 0) || !(np.q0 > 0))
             throw ApiError{SMAD_EINVAL, "smad_set_params: dt, nw and q0 must be positive"};
         HIPCHK(hipStreamSynchronize(s));
+        const PlainCoef old = cf;
         p = np;
         coefs();
+        // the carried history follows the new coefficients (same psihat and
+        // Nq_past, the latter formed with the old beta and gamma)
+        if (fresh || prev_valid) {
+            const PlainCoef now = cf;
+            cf = old;
+            cur_history();
+            cf = now;
+            Launch<T>::plain_hist(s, cur_psihat(), hh.p, nhat.p, d, cf, cf, 0);
+        } else {
+            from_w(psihat.p);
+            Launch<T>::plain_hist(s, psihat.p, hh.p, nullptr, d, old, cf, 2);
+        }
+        LAUNCHCHK();
     }
 
     PlainEngine(const smad_params& pp, hipStream_t st) : p(pp), s(st)
@@ -310,11 +329,7 @@ struct PlainEngine : EngineBase {
         twx.make(p.nx, s);
         twy.make(p.ny, s);
         twz.make(p.nz, s);
-        w.alloc(n);
-        psihat.alloc(n);
-        nhat.alloc(n);
-        psi.alloc(n);
-        psi_prev.alloc(n);
+        for (DevBuf<T>* b : {&w, &hh, &psihat, &nhat, &psi, &psi_prev}) b->alloc(n);
         part.alloc(3 * nparts);
         red.alloc(8);
     }
@@ -327,7 +342,41 @@ struct PlainEngine : EngineBase {
         LAUNCHCHK();
     }
 
-    // pre-loop: N_hat^0, psi_hat^0 = s CT(psi0), w = iCT_y iCT_x psi_hat^0
+    // psihat of the carried state: REDFT10_x REDFT10_y(w) / (2Nx 2Ny)
+    void from_w(T* dst)
+    {
+        Launch<T>::axis(s, w.p, dst, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(s, dst, dst, d, 0, REDFT10, T(0.25 / ((double)p.nx * p.ny)), twx.tw);
+        LAUNCHCHK();
+    }
+    // psihat of the current state in psihat (exact while fresh)
+    const T* cur_psihat()
+    {
+        if (!fresh) from_w(psihat.p);
+        return psihat.p;
+    }
+    // the AB2 history Nq_past of the current state in nhat: the pre-loop's
+    // while fresh; after steps, REDFT10^3 of the nonlinearity of psi_prev (the
+    // psi the last step started from), as the step formed it; otherwise
+    // (after a taped forward run or a set field) from H as (C1 psihat - H)/c,
+    // which loses digits where the nonlinearity is small against psihat/c
+    void cur_history()
+    {
+        if (fresh) return;
+        if (prev_valid) {
+            Launch<T>::plain_z(s, psi_prev.p, nhat.p, nullptr, d, cf, 1, twz.tw);
+            Launch<T>::axis(s, nhat.p, nhat.p, d, 1, REDFT10, T(1), twy.tw);
+            Launch<T>::axis(s, nhat.p, nhat.p, d, 0, REDFT10, T(1), twx.tw);
+            if (cf.conserved) Launch<T>::mul_mq2(s, nhat.p, d, cf);
+        } else {
+            from_w(psihat.p);
+            Launch<T>::plain_hist(s, psihat.p, hh.p, nhat.p, d, cf, cf, 1);
+        }
+        LAUNCHCHK();
+    }
+
+    // pre-loop: Nq^0 = REDFT10^3(N(psi0)), psihat^0 = s CT(psi0),
+    // w = iCT_y iCT_x psihat^0, H^0 = C1 psihat^0 - c Nq^0 (first step Euler)
     void preloop()
     {
         Launch<T>::plain_z(s, psi.p, w.p, nullptr, d, cf, 1, twz.tw);
@@ -337,16 +386,17 @@ struct PlainEngine : EngineBase {
         ct3(psi.p, psihat.p, T(cf.scale));
         Launch<T>::axis(s, psihat.p, w.p, d, 0, REDFT01, T(1), twx.tw);
         Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
+        Launch<T>::plain_hist(s, psihat.p, hh.p, nhat.p, d, cf, cf, 0);
         LAUNCHCHK();
         psi_valid = true;
         prev_valid = false;
+        fresh = true;
     }
 
     void init(const double* psi0) override
     {
         upload(s, psi.p, n, psi0);
         preloop();
-        fresh = true;
     }
 
     const char* kernel_class(int c) const override
@@ -358,34 +408,29 @@ struct PlainEngine : EngineBase {
         return (c >= 0 && c < 4) ? names[c] : nullptr;
     }
 
-    // One loop-body iteration: psihat is read from ph_in and written to
-    // ph_out (they alias in ordinary stepping); psi_out, if set, receives
-    // psi^n, the real field the step starts from.
-    void step_one(const T* ph_in, T* ph_out, T* psi_out, Prof* prof, bool z_only = false)
+    // One loop-body iteration on stream st: from (w_in, h_in) into the
+    // buffers (w_buf, h_buf) (w_in may equal w_buf, h_in h_buf).  psi_out,
+    // if set, receives psi^n, the real field the step starts from;
+    // ph_tape, if set, psihat^{n+1}; no_yinv leaves w_buf at REDFT01_x(psihat)
+    // (a recomputed segment's last step, whose w the adjoint does not read).
+    void step_on(hipStream_t st, const T* w_in, T* w_buf, const T* h_in, T* h_buf, T* psi_out, T* ph_tape,
+                 Prof* prof, bool no_yinv = false)
     {
-        step_on(s, ph_in, ph_out, psi_out, prof, z_only);
-    }
-    // The step on stream st (the adjoint recomputes segments on a second
-    // stream).  w_in and nh_in, when set, hold the state's w and N_hat (a
-    // checkpoint, left untouched); the step writes the live w and nhat.
-    void step_on(hipStream_t st, const T* ph_in, T* ph_out, T* psi_out, Prof* prof, bool z_only,
-                 const T* w_in = nullptr, const T* nh_in = nullptr)
-    {
-        Launch<T>::plain_z(st, w_in ? w_in : w.p, w.p, psi_out, d, cf, 0, twz.tw);
+        Launch<T>::plain_z(st, w_in, w_buf, psi_out, d, cf, 0, twz.tw);
         if (prof) prof->mark(0);
-        if (z_only) return;
-        Launch<T>::axis(st, w.p, w.p, d, 1, REDFT10, T(1), twy.tw);
+        Launch<T>::axis(st, w_buf, w_buf, d, 1, REDFT10, T(1), twy.tw);
         if (prof) prof->mark(1);
-        Launch<T>::plain_x(st, w.p, w.p, ph_in, ph_out, nhat.p, d, cf, twx.tw, 1, nullptr, nh_in);
+        Launch<T>::plain_xh(st, w_buf, w_buf, h_in, h_buf, ph_tape, d, cf, twx.tw);
         if (prof) prof->mark(2);
-        Launch<T>::axis(st, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
+        if (no_yinv) return;
+        Launch<T>::axis(st, w_buf, w_buf, d, 1, REDFT01, T(1), twy.tw);
         if (prof) prof->mark(3);
     }
 
     void step(int nsteps, Prof* prof) override
     {
         for (int it = 0; it < nsteps; ++it)
-            step_one(psihat.p, psihat.p, it == nsteps - 1 ? psi_prev.p : nullptr, prof);
+            step_on(s, w.p, w.p, hh.p, hh.p, it == nsteps - 1 ? psi_prev.p : nullptr, nullptr, prof);
         LAUNCHCHK();
         if (nsteps > 0) {
             psi_valid = false;
@@ -395,24 +440,24 @@ struct PlainEngine : EngineBase {
     }
 
     // ------------------------------------------------------------ adjoint --
-    // Checkpointed forward: (psihat, nhat, w) saved at the start of every
-    // segment of `ck` steps.  A recorded segment writes psi^m (from its z
-    // pass) to tape slot m and steps psihat through rotating slots (slot 0 is
-    // the segment's checkpoint, slot m+1 receives psihat^{m+1}), so
-    // recording psihat costs no extra traffic.  The forward run records the
-    // last segments directly, as many as the tape budget allows (at least
-    // one); every other segment is recomputed from its checkpoint into a
-    // scratch tape when the adjoint reaches it.
+    // Checkpointed forward.  The state (w, H) is saved at the start of every
+    // segment of `ck` steps that is not taped (pointer swaps, no copies).  A
+    // recorded segment writes psi^m (its z pass) and psihat^{m+1} (its x pass)
+    // to tape slot m.  The forward run records the last segments directly, as
+    // many as the tape budget allows (at least one); every other segment is
+    // recomputed from its checkpoint into a scratch tape, on a second stream
+    // with buffers of its own, when the adjoint reaches it, so the live state
+    // stays at step K throughout.
     struct Tape {
-        std::vector<std::unique_ptr<DevBuf<T>>> psi, ph;  // ph[m-1] holds slot m
+        std::vector<std::unique_ptr<DevBuf<T>>> psi, ph;  // slot m: psi^{n0+m}, psihat^{n0+m+1}
     };
     int tape_K = 0, tape_ck = 0, tape_nseg = 0, tape_first = 0;  // segments >= tape_first are taped
     // bytes of tape beyond one segment; < 0 (the default): a quarter of the
     // device memory that is free or already held by tapes when forward_tape
     // runs, i.e. on a 288 GB GPU every segment of a 256^3 K = 200 run
     double tape_budget = -1.0;
-    std::vector<std::unique_ptr<DevBuf<T>>> ck_ph, ck_nh, ck_w;
-    std::vector<std::unique_ptr<Tape>> tapes;  // per segment (taped ones), plus scratch
+    std::vector<std::unique_ptr<DevBuf<T>>> ck_w, ck_h;
+    std::vector<std::unique_ptr<Tape>> tapes;  // per segment (taped ones)
     std::unique_ptr<Tape> scratch[2];  // recomputed segments, double-buffered
     hipStream_t s2 = nullptr;           // the recomputation's stream
     hipEvent_t ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
@@ -426,9 +471,8 @@ struct PlainEngine : EngineBase {
         for (hipEvent_t e : {ev_ready[0], ev_ready[1], ev_free[0], ev_free[1]})
             if (e) (void)hipEventDestroy(e);
     }
-    DevBuf<T> end_nh, lbuf, gbuf;
+    DevBuf<T> ph0, rw, rh, lbuf, gbuf;  // psihat^0; the recomputation's state
     DevBuf<double> acc_x, acc_z, gsum;
-    bool fresh = true;
 
     void set_tape_budget(double bytes) override { tape_budget = bytes > 0 ? bytes : 0.0; }
 
@@ -464,26 +508,15 @@ struct PlainEngine : EngineBase {
         ensure(t->ph, ck, n);
     }
 
-    // slot m of segment g on tape t: slot 0 is the checkpoint, or for a taped
-    // segment after the first taped one the previous segment's last slot
-    const T* slot(const Tape& t, int g, int m) const
+    // recorded run of ns steps into t from (w_in, h_in), stepping in the
+    // buffers (w_buf, h_buf); last_no_yinv: the final step stops after its x
+    // pass
+    void record(Tape& t, int ns, hipStream_t st, const T* w_in, const T* h_in, T* w_buf, T* h_buf,
+                bool last_no_yinv)
     {
-        if (m > 0) return t.ph[m - 1]->p;
-        return g > tape_first ? tapes[g - 1]->ph[tape_ck - 1]->p : ck_ph[g]->p;
-    }
-
-    // recorded run of ns steps of segment g from (w, ck_ph[g], nhat) into t
-    // recorded run of ns steps of segment g into t, from the live w and nhat,
-    // or from the checkpoint's (from_ck)
-    void record(Tape& t, int g, int ns, bool last_z_only, hipStream_t st = nullptr, bool from_ck = false)
-    {
-        if (!st) st = s;
-        for (int m = 0; m < ns; ++m) {
-            bool zo = last_z_only && m == ns - 1;
-            const bool ckin = from_ck && m == 0;
-            step_on(st, slot(t, g, m), zo ? nullptr : t.ph[m]->p, t.psi[m]->p, nullptr, zo,
-                    ckin ? ck_w[g]->p : nullptr, ckin ? ck_nh[g]->p : nullptr);
-        }
+        for (int m = 0; m < ns; ++m)
+            step_on(st, m == 0 ? w_in : w_buf, w_buf, m == 0 ? h_in : h_buf, h_buf, t.psi[m]->p, t.ph[m]->p,
+                    nullptr, last_no_yinv && m == ns - 1);
         LAUNCHCHK();
     }
 
@@ -503,45 +536,43 @@ struct PlainEngine : EngineBase {
         }
         const int ntaped = std::min(nseg, 1 + (int)std::min(budget / seg_bytes, 1e9));
         const int first = nseg - ntaped;
-        ensure(ck_ph, nseg, n);
-        ensure(ck_nh, nseg, n);
-        ensure(ck_w, nseg, n);
+        ensure(ck_w, first, n);
+        ensure(ck_h, first, n);
         if ((int)tapes.size() < nseg) tapes.resize(nseg);
         for (int g = first; g < nseg; ++g) ensure_tape(tapes[g], ck);
-        if (first > 0) ensure_tape(scratch[0], ck);
+        if (first > 0) {
+            ensure_tape(scratch[0], ck);
+            if (!rw.p) {
+                rw.alloc(n);
+                rh.alloc(n);
+            }
+        }
         if (first > 1) ensure_tape(scratch[1], ck);
-        if (!end_nh.p) {
-            for (DevBuf<T>* b : {&end_nh, &lbuf, &gbuf}) b->alloc(n);
+        if (!ph0.p) {
+            for (DevBuf<T>* b : {&ph0, &lbuf, &gbuf}) b->alloc(n);
             acc_x.alloc(2 * (size_t)num_tiles(d, 0, Lines<T>::NL));
             acc_z.alloc(2 * (size_t)num_tiles(d, 2, linesz_nl<T>(d.nz)));
             gsum.alloc(4);
         }
         tape_ck = ck;
         tape_first = first;
+        copy_dd(ph0.p, psihat.p);  // psihat^0 for the adjoint's last pass
         EvPair ev;
         HIPCHK(hipEventRecord(ev.a, s));
-        // A checkpoint takes the live state's buffers (pointer swaps, no
-        // copies) and the segment's first step reads them and writes the
-        // live buffers; a taped segment after the first taped one starts
-        // from the previous segment's last tape slot.
         for (int g = 0; g < nseg; ++g) {
             const int ns = std::min(ck, K - g * ck);
             if (g < first) {
-                std::swap(ck_ph[g]->p, psihat.p);
-                std::swap(ck_nh[g]->p, nhat.p);
+                // the checkpoint takes the live buffers; the segment's first
+                // step reads it and writes the live buffers
                 std::swap(ck_w[g]->p, w.p);
-                step_on(s, ck_ph[g]->p, psihat.p, nullptr, nullptr, false, ck_w[g]->p, ck_nh[g]->p);
-                for (int m = 1; m < ns; ++m) step_one(psihat.p, psihat.p, nullptr, nullptr, false);
+                std::swap(ck_h[g]->p, hh.p);
+                step_on(s, ck_w[g]->p, w.p, ck_h[g]->p, hh.p, nullptr, nullptr, nullptr);
+                for (int m = 1; m < ns; ++m) step_on(s, w.p, w.p, hh.p, hh.p, nullptr, nullptr, nullptr);
                 LAUNCHCHK();
             } else {
-                if (g == first) std::swap(ck_ph[g]->p, psihat.p);
-                record(*tapes[g], g, ns, false);
+                record(*tapes[g], ns, s, w.p, hh.p, w.p, hh.p, false);
             }
         }
-        // the context ends at step K: psihat from the last tape slot; the
-        // history is kept apart because recomputation reuses nhat
-        copy_dd(psihat.p, tapes[nseg - 1]->ph[std::min(ck, K - (nseg - 1) * ck) - 1]->p);
-        copy_dd(end_nh.p, nhat.p);
         HIPCHK(hipEventRecord(ev.b, s));
         last_fwd_ms = ev.ms();
         tape_K = K;
@@ -573,7 +604,6 @@ struct PlainEngine : EngineBase {
         Launch<T>::from_f64(s, psi0_dev, psi.p, (long)n);
         LAUNCHCHK();
         preloop();
-        fresh = true;
         forward_tape(K, ck);
         materialize();
         Launch<T>::to_f64(s, psi.p, psik_dev, (long)n);
@@ -603,8 +633,8 @@ struct PlainEngine : EngineBase {
     }
 
     // The adjoint sweep for the cotangent in psi (storage type); leaves
-    // dJ/dpsi0 in psi, the parameter gradients in gsum, and the context at
-    // the end of the taped forward run.
+    // dJ/dpsi0 in psi and the parameter gradients in gsum.  The carried state
+    // stays at the end of the taped forward run.
     void adjoint_core()
     {
         const int K = tape_K, ck = tape_ck, nseg = tape_nseg;
@@ -633,9 +663,9 @@ struct PlainEngine : EngineBase {
         auto recompute = [&](int g) {
             const int b = g & 1, ns = std::min(ck, K - g * ck);
             HIPCHK(hipStreamWaitEvent(s2, ev_free[b], 0));
-            // from the checkpoint's w and N_hat; the final step needs only its
-            // z pass: psihat at the segment end is checkpoint g+1
-            record(*scratch[b], g, ns, true, s2, true);
+            // from the checkpoint's w and H; the final step stops after its x
+            // pass (psihat at the segment end is its last tape slot)
+            record(*scratch[b], ns, s2, ck_w[g]->p, ck_h[g]->p, rw.p, rh.p, true);
             HIPCHK(hipEventRecord(ev_ready[b], s2));
         };
         if (nseg >= 2 && nseg - 2 < tape_first) recompute(nseg - 2);
@@ -646,8 +676,7 @@ struct PlainEngine : EngineBase {
             if (!taped) HIPCHK(hipStreamWaitEvent(s, ev_ready[g & 1], 0));
             for (int m = ns - 1; m >= 0; --m) {
                 int nn = n0 + m;
-                const T* ph_next = (!taped && m == ns - 1) ? ck_ph[g + 1]->p : t.ph[m]->p;
-                Launch<T>::adj_x(s, gbuf.p, lbuf.p, ph_next, d, cf, nn >= 1 ? 0 : 1, acc_x.p, twx.tw);
+                Launch<T>::adj_x(s, gbuf.p, lbuf.p, t.ph[m]->p, d, cf, nn >= 1 ? 0 : 1, acc_x.p, twx.tw);
                 Launch<T>::axis(s, gbuf.p, gbuf.p, d, 1, REDFT01, T(1), twy.tw);
                 Launch<T>::adj_z(s, gbuf.p, t.psi[m]->p, d, cf, acc_z.p, twz.tw);
                 Launch<T>::axis(s, gbuf.p, gbuf.p, d, 1, REDFT10, T(1), twy.tw);
@@ -658,8 +687,8 @@ struct PlainEngine : EngineBase {
             // is recomputed while the sweep of g-1 (recomputed before) runs
             if (g >= 2 && g - 2 < tape_first) recompute(g - 2);
         }
-        // dJ/dpsi0 = s REDFT01^3(a lt^1 + REDFT10_x(G)), psihat^0 = checkpoint 0
-        Launch<T>::adj_x(s, gbuf.p, lbuf.p, ck_ph[0]->p, d, cf, 2, acc_x.p, twx.tw);
+        // dJ/dpsi0 = s REDFT01^3(a lt^1 + REDFT10_x(G)) with psihat^0
+        Launch<T>::adj_x(s, gbuf.p, lbuf.p, ph0.p, d, cf, 2, acc_x.p, twx.tw);
         Launch<T>::axis(s, gbuf.p, gbuf.p, d, 1, REDFT01, T(1), twy.tw);
         Launch<T>::axis(s, gbuf.p, psi.p, d, 2, REDFT01, T(1), twz.tw);
         Launch<T>::fold(s, acc_x.p, (int)(acc_x.n / 2), 2, gsum.p);
@@ -667,12 +696,6 @@ struct PlainEngine : EngineBase {
         LAUNCHCHK();
         HIPCHK(hipEventRecord(ev.b, s));
         last_adj_ms = ev.ms();
-        // leave the context at the end of the taped forward run
-        copy_dd(psihat.p, tapes[nseg - 1]->ph[std::min(ck, K - (nseg - 1) * ck) - 1]->p);
-        copy_dd(nhat.p, end_nh.p);
-        Launch<T>::axis(s, psihat.p, w.p, d, 0, REDFT01, T(1), twx.tw);
-        Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
-        LAUNCHCHK();
         HIPCHK(hipStreamSynchronize(s));
         psi_valid = false;
         prev_valid = false;
@@ -698,7 +721,7 @@ struct PlainEngine : EngineBase {
         Launch<T>::plain_z(s, psi.p, sf_nq.p, nullptr, d, cf, 1, twz.tw);
         Launch<T>::axis(s, sf_nq.p, sf_nq.p, d, 1, REDFT10, T(1), twy.tw);
         Launch<T>::axis(s, sf_nq.p, sf_nq.p, d, 0, REDFT10, T(1), twx.tw);
-        Launch<T>::surf_spec(s, psihat.p, sf_nq.p, sf_dxx.p, sf_dyy.p, sf_dzz.p, sf_dt.p, d, cf);
+        Launch<T>::surf_spec(s, cur_psihat(), sf_nq.p, sf_dxx.p, sf_dyy.p, sf_dzz.p, sf_dt.p, d, cf);
         for (DevBuf<T>* b : {&sf_dxx, &sf_dyy, &sf_dzz, &sf_dt}) {
             Launch<T>::axis(s, b->p, b->p, d, 2, REDFT01, T(1), twz.tw);
             Launch<T>::axis(s, b->p, b->p, d, 1, REDFT01, T(1), twy.tw);
@@ -730,16 +753,17 @@ struct PlainEngine : EngineBase {
     {
         switch (field) {
         case SMAD_FIELD_PSI: materialize(); return download(s, psi.p, n, out);
-        case SMAD_FIELD_PSI_HAT: return download(s, psihat.p, n, out);
-        case SMAD_FIELD_N_HAT: return download(s, nhat.p, n, out);
+        case SMAD_FIELD_PSI_HAT: return download(s, cur_psihat(), n, out);
+        case SMAD_FIELD_N_HAT: cur_history(); return download(s, nhat.p, n, out);
         case SMAD_FIELD_SPEC_XY: return download(s, w.p, n, out);
+        case SMAD_FIELD_CN_HIST: return download(s, hh.p, n, out);
         default: throw ApiError{SMAD_EINVAL, "field not available in the plain model"};
         }
     }
 
     int state_fields(int* f, int max) const override
     {
-        return list_fields({SMAD_FIELD_PSI_HAT, SMAD_FIELD_N_HAT, SMAD_FIELD_SPEC_XY}, f, max);
+        return list_fields({SMAD_FIELD_SPEC_XY, SMAD_FIELD_CN_HIST}, f, max);
     }
 
     void set_field(int field, const double* in) override
@@ -748,29 +772,30 @@ struct PlainEngine : EngineBase {
         case SMAD_FIELD_PSI:
             upload(s, psi.p, n, in);
             preloop();
-            fresh = true;
-            break;
+            return;
         case SMAD_FIELD_PSI_HAT:
+            // a new psihat with the current history Nq_past
+            cur_history();
             upload(s, psihat.p, n, in);
             Launch<T>::axis(s, psihat.p, w.p, d, 0, REDFT01, T(1), twx.tw);
             Launch<T>::axis(s, w.p, w.p, d, 1, REDFT01, T(1), twy.tw);
+            Launch<T>::plain_hist(s, psihat.p, hh.p, nhat.p, d, cf, cf, 0);
             LAUNCHCHK();
-            psi_valid = false;
-            prev_valid = false;
-            fresh = false;
             break;
         case SMAD_FIELD_N_HAT:
+            // a new history with the current psihat
+            cur_psihat();
             upload(s, nhat.p, n, in);
-            fresh = false;
+            Launch<T>::plain_hist(s, psihat.p, hh.p, nhat.p, d, cf, cf, 0);
+            LAUNCHCHK();
             break;
-        case SMAD_FIELD_SPEC_XY:
-            upload(s, w.p, n, in);
-            psi_valid = false;
-            prev_valid = false;
-            fresh = false;
-            break;
+        case SMAD_FIELD_SPEC_XY: upload(s, w.p, n, in); break;
+        case SMAD_FIELD_CN_HIST: upload(s, hh.p, n, in); break;
         default: throw ApiError{SMAD_EINVAL, "field cannot be set in the plain model"};
         }
+        psi_valid = false;
+        prev_valid = false;
+        fresh = false;
     }
 
     int reduce(int what, double* out) override

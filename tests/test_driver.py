@@ -98,8 +98,10 @@ def test_plain_restarts_and_l1_stop(smadft, tmp_path):
     with smadft.Stepper(p, psi0) as s:
         s.step(300)
         want = s.fetch()
-        want_ph = s.fetch(smadft.FIELD_PSI_HAT)
-    assert np.array_equal(fields[smadft.FIELD_PSI_HAT], want_ph)
+        want_state = {f: s.fetch(f) for f in (smadft.FIELD_SPEC_XY, smadft.FIELD_CN_HIST)}
+    assert sorted(fields) == sorted(want_state)
+    for f, v in want_state.items():
+        assert np.array_equal(fields[f], v)
     assert np.allclose(np.loadtxt(tmp_path / "cont" / "psi0.dat"), want.ravel(), rtol=1e-5, atol=1e-6)
     # a psi-only restart (the previous psi.bin format) re-runs the pre-loop, as
     # the reference's load path

@@ -132,3 +132,64 @@ def test_restart_equals_continuous(smadft):
     o = orc.PlainOracle(p, mid)
     o.step(10)
     assert rel_l2(gpu, o.psi) < 1e-10
+
+
+def test_history_fields_match_oracle(smadft, oracle):
+    """The step carries w = REDFT01_y REDFT01_x(psi_hat) and the combined
+    history H = C1 psi_hat - (dt/2) s N_hat_past; psi_hat and N_hat are
+    recovered from them on fetch."""
+    p = params(smadft, (32, 16, 64))
+    psi0 = smadft.ic_curved(p, 1.328, 1e-3, seed=3)
+    o = oracle.PlainOracle(p, psi0)
+    o.step(40)
+    with smadft.Stepper(p, psi0) as s:
+        ph0 = s.fetch(smadft.FIELD_PSI_HAT)  # pre-loop: exact
+        nq0 = s.fetch(smadft.FIELD_N_HAT)
+        s.step(40)
+        ph = s.fetch(smadft.FIELD_PSI_HAT)
+        nq = s.fetch(smadft.FIELD_N_HAT)
+    scale = 0.125 / psi0.size
+    kinds = (oracle.REDFT10,) * 3
+    assert rel_l2(ph0, scale * oracle.r2r3d(psi0, kinds)) < 1e-13
+    assert rel_l2(ph, scale * oracle.r2r3d(o.psi, kinds)) < 1e-10
+    assert rel_l2(nq, o.nq) < 1e-10
+    o0 = oracle.PlainOracle(p, psi0)
+    assert rel_l2(nq0, o0.nq) < 1e-13
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_set_params_mid_run_keeps_psi_and_history(smadft, oracle, prec):
+    """smad_set_params on a running context re-expresses the carried history
+    for the new coefficients: the run continues from the same psi and AB2
+    history, as the oracle does when its parameters change between steps."""
+    pa = params(smadft, (32, 32, 16), prec=prec)
+    pb = params(smadft, (32, 32, 16), prec=prec, eps=-0.65, alpha=1.1, dt=0.05)
+    psi0 = smadft.ic_curved(pa, 1.328, 1e-3, seed=5)
+    o = oracle.PlainOracle(pa, psi0)
+    o.step(30)
+    o.op = oracle.oparams(pb)
+    o.step(30)
+    with smadft.Stepper(pa, psi0) as s:
+        s.step(30)
+        s.set_params(pb)
+        s.step(30)
+        psi = s.fetch()
+    assert rel_l2(psi, o.psi) < (1e-10 if prec == 64 else 1e-5)
+
+
+def test_set_psi_hat_and_n_hat_restart(smadft):
+    """The classic state (psi_hat, N_hat) set on a fresh context continues
+    the run: setting psi_hat keeps the history, setting N_hat keeps psi_hat."""
+    p = params(smadft, (16, 32, 32))
+    psi0 = smadft.ic_curved(p, 1.328, 1e-3, seed=6)
+    with smadft.Stepper(p, psi0) as a:
+        a.step(25)
+        ph, nq = a.fetch(smadft.FIELD_PSI_HAT), a.fetch(smadft.FIELD_N_HAT)
+        a.step(25)
+        want = a.fetch()
+    for order in ((smadft.FIELD_PSI_HAT, smadft.FIELD_N_HAT), (smadft.FIELD_N_HAT, smadft.FIELD_PSI_HAT)):
+        with smadft.Stepper(p, np.zeros(p.shape)) as b:
+            for f in order:
+                b.set_field(f, ph if f == smadft.FIELD_PSI_HAT else nq)
+            b.step(25)
+            assert rel_l2(b.fetch(), want) < 1e-12
