@@ -530,17 +530,44 @@ __device__ __forceinline__ void unflat(long id, const Dims& d, int& i, int& j, i
     i = (int)(r / d.ny);
 }
 
-// Per-mode coefficients of the update (FP64): C1, 1/C2 and c.
+// Per-mode coefficients of the update (FP64): C1, 1/C2 and c.  FAST: 1/C2
+// from the FP32 reciprocal and one FP64 Newton step (relative error ~1e-14,
+// far below FP32 storage rounding) instead of the FP64 division; the FP32
+// storage form uses it, the FP64 form divides.
 struct CnMode {
     double c1, ic2, c;
 };
-template <bool CONS>
+// t = |q|^2 - q0^2 given
+template <bool CONS, bool FAST = false>
+__device__ __forceinline__ CnMode cn_mode_t(const PlainCoef& cf, double t)
+{
+    double al = cf.eps - cf.alpha * t * t;
+    if constexpr (CONS) al = (t + cf.q0sq) * al;
+    const double c2 = 1.0 - cf.dtd2 * al;
+    double ic2;
+    if constexpr (FAST) {
+        ic2 = (double)__builtin_amdgcn_rcpf((float)c2);
+        ic2 = ic2 * (2.0 - c2 * ic2);
+    } else {
+        ic2 = 1.0 / c2;
+    }
+    return CnMode{2.0 - c2, ic2, cf.dtd2 * cf.scale};
+}
+template <bool CONS, bool FAST = false>
 __device__ __forceinline__ CnMode cn_mode(const PlainCoef& cf, double mq2)
 {
     const double t = mq2 - cf.q0sq;
     double al = cf.eps - cf.alpha * t * t;
     if constexpr (CONS) al = mq2 * al;
-    return CnMode{1.0 + cf.dtd2 * al, 1.0 / (1.0 - cf.dtd2 * al), cf.dtd2 * cf.scale};
+    const double c2 = 1.0 - cf.dtd2 * al;
+    double ic2;
+    if constexpr (FAST) {
+        ic2 = (double)__builtin_amdgcn_rcpf((float)c2);
+        ic2 = ic2 * (2.0 - c2 * ic2);
+    } else {
+        ic2 = 1.0 / c2;
+    }
+    return CnMode{1.0 + cf.dtd2 * al, ic2, cf.dtd2 * cf.scale};
 }
 
 // x pass: Nq = REDFT10_x(a_in) (times |q|^2 in the conserved variant);
@@ -570,31 +597,46 @@ __device__ __forceinline__ void plain_xh_body(const T* a_in, T* w_out, const T* 
     const int j = blockIdx.x / nct;
     const int k0 = (blockIdx.x % nct) * NL;
     const double qy = j * cf.qfy;
-    auto point = [&](int p, int c, T hv, T& hn, T& ph) {
-        const double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
-        const double mq2 = qx * qx + qy * qy + qz * qz;
-        const CnMode m = cn_mode<CONS>(cf, mq2);
+    // t = |q|^2 - q0^2 from qx^2 and the column's qy^2 + qz^2 - q0^2
+    auto point = [&](int p, int c, double t, T hv, T& hn, T& ph) {
+        const CnMode m = cn_mode_t<CONS, sizeof(T) == 4>(cf, t);
         double nq = (double)lds[c * LS + PR(p)];
-        if constexpr (CONS) nq *= mq2;
+        if constexpr (CONS) nq *= t + cf.q0sq;
         const double r = ((double)hv + 3.0 * m.c * nq) * m.ic2;
         hn = T(m.c1 * r - m.c * nq);
         ph = T(r);
         lds[c * LS + PR(p)] = ph;
     };
     if (g.ncol == NL) {
-        rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
+        // a thread's 16-byte group keeps its columns c0..c0+W-1 on every
+        // row it visits (WG is a multiple of the groups per row), so the
+        // columns' share of |q|^2 is formed once
+        constexpr int GPR = NL / W, PSTEP = WG / GPR;
+        static_assert(WG % GPR == 0, "x pass rows");
+        const int c0 = (threadIdx.x % GPR) * W;
+        double acol[W];
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            const double qz = (k0 + c0 + t) * cf.qfz;
+            acol[t] = qy * qy + qz * qz - cf.q0sq;
+        }
+#pragma unroll 4
+        for (int p = threadIdx.x / GPR; p < N; p += PSTEP) {
+            const long gi = g.base + (long)p * g.stride + c0;
+            const double qx = p * cf.qfx, qx2 = qx * qx;
             const V16<T> hv = *reinterpret_cast<const V16<T>*>(h_in + gi);
             V16<T> hn, ph;
 #pragma unroll
-            for (int t = 0; t < W; ++t) point(p, c0 + t, hv.v[t], hn.v[t], ph.v[t]);
+            for (int t = 0; t < W; ++t) point(p, c0 + t, qx2 + acol[t], hv.v[t], hn.v[t], ph.v[t]);
             *reinterpret_cast<V16<T>*>(h_out + gi) = hn;
             if constexpr (TAPE) *reinterpret_cast<V16<T>*>(ph_tape + gi) = ph;
-        });
+        }
     } else {
         tile_for<N, NL>(g, 0, [&](int p, int c, long gi, bool ok) {
             if (ok) {
                 T hn, ph;
-                point(p, c, h_in[gi], hn, ph);
+                const double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
+                point(p, c, qx * qx + qy * qy + qz * qz - cf.q0sq, h_in[gi], hn, ph);
                 h_out[gi] = hn;
                 if constexpr (TAPE) ph_tape[gi] = ph;
             } else {
@@ -1962,22 +2004,23 @@ __global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__
     // recurrence in FP64, as the forward update computes them (an FP32
     // C1/C2 is off by a fixed relative 1e-8..1e-7 per mode, which the
     // recurrence compounds over K steps); 1/C2 from the FP32 reciprocal and
-    // one Newton step (relative error ~1e-15) instead of an FP64 division
-    auto point_f = [&](int p, int c, float& lb, float phv) {
-        const double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
-        const double t = qx * qx + qy * qy + qz * qz - cf.q0sq;
+    // one Newton step (relative error ~1e-14) instead of an FP64 division.
+    // The gradient terms are formed in FP32 and summed per thread in FP32
+    // (at most 4 N / 64 terms) before the FP64 block sums.
+    // t = |q|^2 - q0^2; wgt = w of the mode (1/2 per axis at index 0)
+    float fs0 = 0.0f, fs1 = 0.0f;
+    auto point_f = [&](int p, int c, double t, float wgt, float& lb, float phv) {
         const double al = cf.eps - cf.alpha * t * t;
         const double c2 = 1.0 - cf.dtd2 * al;
-        double inv = (double)(1.0f / (float)c2);
+        double inv = (double)__builtin_amdgcn_rcpf((float)c2);
         inv = inv * (2.0 - c2 * inv);
         const double h = cf.dtd2 * inv;
-        const double a = (1.0 + cf.dtd2 * al) * inv;
+        const double a = (2.0 - c2) * inv;
         const double lold = (double)lb;
         const double lnew = a * lold + (double)lds[c * LS + PR(p)];
-        const float wgt = (p == 0 ? 0.5f : 1.0f) * (j == 0 ? 0.5f : 1.0f) * (k0 + c == 0 ? 0.5f : 1.0f);
         const float term = float(wgt * h) * phv * float(mode == 2 ? lold : lnew + lold);
-        sums[0] += (double)term;
-        sums[1] -= (double)(float(t * t) * term);
+        fs0 += term;
+        fs1 -= float(t * t) * term;
         if (mode == 2) {
             lds[c * LS + PR(p)] = float(cf.scale * lnew);
             return;
@@ -1985,21 +2028,19 @@ __global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__
         lb = float(lnew);
         lds[c * LS + PR(p)] = float(cf.scale * h * (cm * lnew - lold));
     };
-    auto point = [&](int p, int c, T& lb, T phv) {
+    auto point = [&](int p, int c, double t, T& lb, T phv) {
+        const float wgt = (p == 0 ? 0.5f : 1.0f) * (j == 0 ? 0.5f : 1.0f) * (k0 + c == 0 ? 0.5f : 1.0f);
         if constexpr (sizeof(T) == 4) {
-            point_f(p, c, lb, phv);
+            point_f(p, c, t, wgt, lb, phv);
             return;
         }
-        double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
-        double t = qx * qx + qy * qy + qz * qz - cf.q0sq;
         double al = cf.eps - cf.alpha * t * t;
         const double inv = 1.0 / (1.0 - cf.dtd2 * al);  // 1 / C2
         double h = cf.dtd2 * inv;                        // dt/2 / C2
         double a = (1.0 + cf.dtd2 * al) * inv;           // C1 / C2
         double lold = (double)lb;
         double lnew = a * lold + (double)lds[c * LS + PR(p)];
-        double wgt = (p == 0 ? 0.5 : 1.0) * (j == 0 ? 0.5 : 1.0) * (k0 + c == 0 ? 0.5 : 1.0);
-        double term = wgt * h * (double)phv * (mode == 2 ? lold : lnew + lold);
+        double term = (double)wgt * h * (double)phv * (mode == 2 ? lold : lnew + lold);
         sums[0] += term;
         sums[1] -= t * t * term;
         if (mode == 2) {
@@ -2011,23 +2052,44 @@ __global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__
         lds[c * LS + PR(p)] = T(cc * (cm * lnew - lold));
     };
     if (g.ncol == NL) {
-        rows_vec<T, N, NL, 1>(g, [&](int p, int c0, long gi) {
+        // a thread keeps its columns c0..c0+W-1 on every row it visits, so
+        // their share of |q|^2 is formed once (as in the forward x pass)
+        constexpr int W = V16<T>::W, GPR = NL / W, PSTEP = WG / GPR;
+        static_assert(WG % GPR == 0, "x pass rows");
+        const int c0 = (threadIdx.x % GPR) * W;
+        double acol[W];
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            const double qz = (k0 + c0 + t) * cf.qfz;
+            acol[t] = qy * qy + qz * qz - cf.q0sq;
+        }
+// not unrolled: its FP64 arithmetic interleaved over rows takes 126 (two
+        // rows) to 152 (four) VGPRs in FP32
+#pragma unroll 1
+        for (int p = threadIdx.x / GPR; p < N; p += PSTEP) {
+            const long gi = g.base + (long)p * g.stride + c0;
+            const double qx = p * cf.qfx, qx2 = qx * qx;
             V16<T> lb = *reinterpret_cast<const V16<T>*>(lbuf + gi);
             V16<T> pv = *reinterpret_cast<const V16<T>*>(ph + gi);
 #pragma unroll
-            for (int t = 0; t < V16<T>::W; ++t) point(p, c0 + t, lb.v[t], pv.v[t]);
+            for (int t = 0; t < W; ++t) point(p, c0 + t, qx2 + acol[t], lb.v[t], pv.v[t]);
             if (mode != 2) *reinterpret_cast<V16<T>*>(lbuf + gi) = lb;
-        });
+        }
     } else {
         tile_for<N, NL>(g, 0, [&](int p, int c, long gi, bool ok) {
             if (ok) {
                 T lb = lbuf[gi];
-                point(p, c, lb, ph[gi]);
+                const double qx = p * cf.qfx, qz = (k0 + c) * cf.qfz;
+                point(p, c, qx * qx + qy * qy + qz * qz - cf.q0sq, lb, ph[gi]);
                 if (mode != 2) lbuf[gi] = lb;
             } else {
                 lds[c * LS + PR(p)] = T(0);
             }
         });
+    }
+    if constexpr (sizeof(T) == 4) {
+        sums[0] = (double)fs0;
+        sums[1] = (double)fs1;
     }
     block_accumulate<2>(sums, acc);
     __syncthreads();
@@ -2046,9 +2108,10 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restr
     constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
-    // In FP32 the weight and the gradient terms are formed in FP32 (each term
-    // is converted to FP64 before it is summed): forming them in FP64 kept
-    // the FP64 copies of all the thread's values in flight (169 VGPRs).
+    // In FP32 the weight and the gradient terms are formed in FP32 and the
+    // thread's terms (at most 2 N / TPL of them) summed in FP32 before the
+    // FP64 block sums: forming them in FP64 kept the FP64 copies of all the
+    // thread's values in flight (169 VGPRs).
     constexpr bool FUSED = OpFused<N, TPL>::value && N <= 512;  // two tiles must fit LDS
     __shared__ T lds[NL * LS];
     __shared__ T tl[FUSED ? NL * LS : 1];  // psi tape at Makhoul positions
@@ -2074,19 +2137,20 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restr
         // r = REDFT01_z(w) in registers, g = (3 beta psi^2 - 5 gamma psi^4) r
         // with psi from the tape, then REDFT10_z(g)
         const cpx<T>* tz = reinterpret_cast<const cpx<T>*>(tl + l * LS);
+        float fs0 = 0.0f, fs1 = 0.0f;  // FP32: the thread's terms summed in FP32
         auto op = [&](cpx<T>& v, int m) {
             const cpx<T> x = tz[PC(m)];
             if constexpr (sizeof(T) == 4) {
                 const float b3f = float(b3), g5f = float(g5);
                 float r = v.re, x2 = x.re * x.re, x3 = x2 * x.re;
-                sums[0] += (double)(r * x3);
-                sums[1] -= (double)(r * x3 * x2);
+                fs0 += r * x3;
+                fs1 -= r * x3 * x2;
                 v.re = (b3f * x2 - g5f * x2 * x2) * r;
                 r = v.im;
                 x2 = x.im * x.im;
                 x3 = x2 * x.im;
-                sums[0] += (double)(r * x3);
-                sums[1] -= (double)(r * x3 * x2);
+                fs0 += r * x3;
+                fs1 -= r * x3 * x2;
                 v.im = (b3f * x2 - g5f * x2 * x2) * r;
                 return;
             }
@@ -2102,6 +2166,10 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adj_z(T* w, const T* __restr
             v.im = T((b3 * x2 - g5 * x2 * x2) * r);
         };
         dct3_map_dct2<T, N, TPL, true, true>(lds + l * LS, u, ct, op, false, [] {});
+        if constexpr (sizeof(T) == 4) {
+            sums[0] = (double)fs0;
+            sums[1] = (double)fs1;
+        }
     } else {
         dct3_line<T, N, TPL, true>(lds + l * LS, u, ct);
         zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {

@@ -119,6 +119,23 @@ __device__ __forceinline__ cpx<T> cis96(int j)
     return cpx<T>{T(kCosPi96[j]), -T(kCosPi96[j <= 48 ? 48 - j : j - 48])};
 }
 
+// v e^{-i pi j/96}, 0 <= j <= 96, for a j that is a compile-time constant
+// once loops unroll: the quarter turns as swaps and negations, the eighth
+// turns e^{-i pi/4} = c (1 - i) and e^{-3 i pi/4} = -c (1 + i) as two sums and
+// two real products (a general product would keep the products by the exact
+// zeros and ones, which the compiler may not drop).
+template <typename T>
+__device__ __forceinline__ cpx<T> mul96(cpx<T> v, int j)
+{
+    const T c = T(0.70710678118654752440);
+    if (j == 0) return v;
+    if (j == 48) return cpx<T>{v.im, -v.re};
+    if (j == 96) return cpx<T>{-v.re, -v.im};
+    if (j == 24) return cpx<T>{c * (v.re + v.im), c * (v.im - v.re)};
+    if (j == 72) return cpx<T>{c * (v.im - v.re), -c * (v.re + v.im)};
+    return mul(v, cis96<T>(j));
+}
+
 // Copy the quarter-wave table into LDS (caller synchronises).
 template <typename T, int N>
 __device__ __forceinline__ void load_cosq(T* dst, const Tw<T>& tw)
@@ -510,21 +527,22 @@ __device__ __forceinline__ int mk_pos(int p)
 //   v0 = conj(h_k)(X_k - i X_{N-k}),  v1 = conj(h_{k+M})(X_{M+k} - i X_{M-k}),
 //   z_k = (v0 + v1) + i (v0 - v1) conj(s_k),
 // with h = h_k = omega^k, s = s_k = omega^{4k} given, h_{k+M} = h_k e^{-i pi/4}.
+// With A = X_k - i X_{N-k} and B = e^{i pi/4} (X_{M+k} - i X_{M-k}) this is
+//   z_k = conj(h) ((A + B) + i conj(s) (A - B)).
 template <typename T, int N>
 __device__ __forceinline__ cpx<T> dct3_pre(const T* lr, int k, cpx<T> h, cpx<T> s)
 {
     constexpr int M = N / 2;
+    const T c = T(0.70710678118654752440);
     T xk = lr[PR(k)];
     T xnk = (k == 0) ? T(0) : lr[PR(N - k)];
     T xpk = lr[PR(M + k)];
     T xmk = lr[PR(M - k)];
-    const cpx<T> hc = conj(h);
-    const cpx<T> hm = mul(hc, conj(cis96<T>(24)));
-    cpx<T> v0 = mul(hc, mk(xk, -xnk));
-    cpx<T> v1 = mul(hm, mk(xpk, -xmk));
-    cpx<T> E = add(v0, v1);
-    cpx<T> O = mul(sub(v0, v1), conj(s));
-    return mk(E.re - O.im, E.im + O.re);  // E + i O
+    const cpx<T> A = mk(xk, -xnk);
+    const cpx<T> B = mk(c * (xpk + xmk), c * (xpk - xmk));
+    const cpx<T> P = add(A, B);
+    const cpx<T> Q = mul(sub(A, B), conj(s));
+    return mul(conj(h), mk(P.re - Q.im, P.im + Q.re));  // conj(h) (P + i Q)
 }
 
 // DCT-II real split + post-twiddle of output pair (Z_k, Z_{M-k}):
@@ -532,21 +550,24 @@ __device__ __forceinline__ cpx<T> dct3_pre(const T* lr, int k, cpx<T> h, cpx<T> 
 //   V_k = A/2 - i s_k B/2,  Y_k = 2 Re(h_k V_k),  Y_{N-k} = -2 Im(h_k V_k),
 // with h = h_k, s = s_k given.  For k = 0 the second value is
 // Y_M = 2 Re(h_M (conj(A)/2 - i conj(s_0 B)/2)), h_M = e^{-i pi/4}.
+// The factors 1/2 and 2 cancel (exactly, being powers of two):
+//   Y_k = Re(h_k W_k), Y_{N-k} = -Im(h_k W_k),  W_k = A - i s_k B,
+//   Y_M = Re(e^{-i pi/4} (conj(A) - i conj(s_0 B))).
 template <typename T, int N>
 __device__ __forceinline__ void dct2_post(cpx<T> zk, cpx<T> zm, bool k0, cpx<T> h, cpx<T> s, T& y0, T& y1)
 {
     cpx<T> A = mk(zk.re + zm.re, zk.im - zm.im);
     cpx<T> B = mk(zk.re - zm.re, zk.im + zm.im);
     cpx<T> sB = mul(s, B);
-    cpx<T> vk = mk(T(0.5) * (A.re + sB.im), T(0.5) * (A.im - sB.re));
+    cpx<T> wk = mk(A.re + sB.im, A.im - sB.re);
     if (!k0) {
-        cpx<T> a = mul(h, vk);
-        y0 = T(2) * a.re;
-        y1 = T(-2) * a.im;
+        cpx<T> a = mul(h, wk);
+        y0 = a.re;
+        y1 = -a.im;
     } else {
-        cpx<T> bm = mul(cis96<T>(24), mk(T(0.5) * (A.re - sB.im), T(-0.5) * (A.im + sB.re)));
-        y0 = T(2) * vk.re;
-        y1 = T(2) * bm.re;
+        cpx<T> bm = mul96(mk(A.re - sB.im, -(A.im + sB.re)), 24);
+        y0 = wk.re;
+        y1 = bm.re;
     }
 }
 
@@ -568,8 +589,7 @@ __device__ __forceinline__ void dct3_first_stage(T* lr, int u, const T* __restri
             for (int r = 0; r < R; ++r) {
                 const int k = b + r * B::NB;
                 if constexpr (24 % R == 0)
-                    v[p][r] = dct3_pre<T, N>(lr, k, mul(hb, cis96<T>(r * (24 / R))),
-                                             mul(sb, cis96<T>(r * (96 / R))));
+                    v[p][r] = dct3_pre<T, N>(lr, k, mul96(hb, r * (24 / R)), mul96(sb, r * (96 / R)));
                 else
                     v[p][r] = dct3_pre<T, N>(lr, k, omega<T, N>(ct, k), omega<T, N>(ct, 4 * k));
             }
@@ -619,8 +639,7 @@ __device__ __forceinline__ void dct2_last_split(T* lr, int u, const BfRegs<T, N,
                 zm = (p == 0) ? v[0][(R - r) % R] : v[pp][R - 1 - r];
             }
             T y0, y1;
-            dct2_post<T, N>(v[p][r], zm, k == 0, mul(hb, cis96<T>(r * (24 / R))),
-                            mul(sb, cis96<T>(r * (96 / R))), y0, y1);
+            dct2_post<T, N>(v[p][r], zm, k == 0, mul96(hb, r * (24 / R)), mul96(sb, r * (96 / R)), y0, y1);
             lr[PR(k)] = y0;
             lr[PR(k == 0 ? N / 2 : N - k)] = y1;
         }
