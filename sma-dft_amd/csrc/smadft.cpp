@@ -15,6 +15,8 @@
 #include "../../include/smadft/smadft.h"
 #include "kernels.hpp"
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -901,6 +903,12 @@ struct AdvEngine : EngineBase {
     // of phin)
     void front(bool keep_prev, Prof* prof, const T* phin = nullptr)
     {
+        front_grad(keep_prev, prof, phin);
+        front_vel(prof);
+    }
+    // psi, grad psi and Sp2 of psihat (or of phin)
+    void front_grad(bool keep_prev, Prof* prof, const T* phin = nullptr)
+    {
         if (!phin) phin = ph.p;
         auto mk = [&](int c) { if (prof) prof->mark(c); };
         // psi, grad psi and Sp2 = REDFT01^3(-aLin psihat) (:847-991) with
@@ -919,6 +927,11 @@ struct AdvEngine : EngineBase {
         Launch<T>::axis(s, sp.p, sp.p, d, 0, REDFT01, T(1), twx.tw);
         mk(1);
         if (keep_prev) HIPCHK(hipMemcpyAsync(psi_prev.p, psi.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    }
+    // the force S = mu grad psi and the Stokes velocity u from psi, grad psi, Sp2
+    void front_vel(Prof* prof)
+    {
+        auto mk = [&](int c) { if (prof) prof->mark(c); };
         // S = mu grad psi formed in its forward z passes (:996-1027)
         Launch<T>::adv_force_z(s, psi.p, sp.p, gx.p, gy.p, gz.p, sx.p, sy.p, sz.p, d, ac, twz.tw);
         mk(2);
@@ -1084,12 +1097,28 @@ struct AdvEngine : EngineBase {
 
     // ------------------------------------------------------------ adjoint --
     // The forward run tapes psi_hat of every step (K+1 fields) and the
-    // initial psi; the backward sweep of step n recomputes the step's front
-    // half from its psi_hat and transposes the step (the kernels k_aa_*).
+    // initial psi, and, for the last steps that the tape budget holds, the
+    // velocity u (written by the step straight into its tape slots); the
+    // backward sweep of step n recomputes psi, grad psi and Sp2 from its
+    // psi_hat (and u, where it is not taped) and transposes the step (the
+    // kernels k_aa_*).
     std::vector<std::unique_ptr<DevBuf<T>>> tape;
+    std::vector<std::array<std::unique_ptr<DevBuf<T>>, 3>> utape;
+    int u_first = 0;  // steps >= u_first have u taped
+    // bytes for the velocity tape; < 0 (the default): a quarter of the device
+    // memory that is free (or already held by the velocity tape)
+    double tape_budget = -1.0;
     DevBuf<T> ic, end_nh, lam0, lam1, lam2, mub, ubx, uby, ubz, gbx, gby, gbz;
     DevBuf<double> acc_l, acc_b, gsum;
     int tape_K = 0;
+
+    void set_tape_budget(double bytes) override { tape_budget = bytes > 0 ? bytes : 0.0; }
+    void swap_u(int m)
+    {
+        std::swap(ux.p, utape[m][0]->p);
+        std::swap(uy.p, utape[m][1]->p);
+        std::swap(uz.p, utape[m][2]->p);
+    }
 
     void copy_dd(T* dst, const T* src)
     {
@@ -1115,13 +1144,36 @@ struct AdvEngine : EngineBase {
             acc_b.alloc(2 * (size_t)nb);
             gsum.alloc(4);
         }
+        double budget = tape_budget;
+        double held = 0.0;
+        for (const auto& a : utape)
+            if (a[0]) held += 3.0 * n * sizeof(T);
+        if (budget < 0.0) {
+            size_t fr = 0, tot = 0;
+            HIPCHK(hipMemGetInfo(&fr, &tot));
+            budget = 0.25 * ((double)fr + held);
+        }
+        const int ntu = (int)std::min((double)K, std::floor(budget / (3.0 * n * sizeof(T))));
+        u_first = K - ntu;
+        if ((int)utape.size() < K) utape.resize(K);
+        for (int m = u_first; m < K; ++m)
+            for (auto& b : utape[m])
+                if (!b) {
+                    b.reset(new DevBuf<T>);
+                    b->alloc(n);
+                }
         copy_dd(ic.p, psi.p);  // the initial psi (the pre-loop history's argument)
         EvPair ev;
         HIPCHK(hipEventRecord(ev.a, s));
         for (int m = 0; m < K; ++m) {
             copy_dd(tape[m]->p, ph.p);
+            const bool tu = m >= u_first;
+            if (tu) swap_u(m);  // the step writes u into slot m
             step_one(m == K - 1, nullptr);
+            if (tu) swap_u(m);
         }
+        if (u_first < K)
+            for (int a = 0; a < 3; ++a) copy_dd((a == 0 ? ux : a == 1 ? uy : uz).p, utape[K - 1][a]->p);
         copy_dd(tape[K]->p, ph.p);
         copy_dd(end_nh.p, nhat.p);
         LAUNCHCHK();
@@ -1156,10 +1208,16 @@ struct AdvEngine : EngineBase {
         T* L1 = lam1.p;
         T* L2 = lam2.p;
         for (int m = K - 1; m >= 0; --m) {
-            front(false, nullptr, tape[m]->p);  // psi, grad psi, Sp2, S_hat, u of step m
+            // psi, grad psi, Sp2 (and u) of step m
+            const bool tu = m >= u_first;
+            front_grad(false, nullptr, tape[m]->p);
+            if (!tu) front_vel(nullptr);
+            const T* vx = tu ? utape[m][0]->p : ux.p;
+            const T* vy = tu ? utape[m][1]->p : uy.p;
+            const T* vz = tu ? utape[m][2]->p : uz.p;
             Launch<T>::aa_w(s, L1, L2, tape[m]->p, tape[m + 1]->p, nr.p, d, cf, 0, acc_l.p);
             tr3(nr.p, nr.p, REDFT01, REDFT01, REDFT01);  // r = CT^T w
-            Launch<T>::aa_p1(s, nr.p, gx.p, gy.p, gz.p, ux.p, uy.p, uz.p, ubx.p, uby.p, ubz.p, gbx.p, gby.p, gbz.p,
+            Launch<T>::aa_p1(s, nr.p, gx.p, gy.p, gz.p, vx, vy, vz, ubx.p, uby.p, ubz.p, gbx.p, gby.p, gbz.p,
                              (long)n);
             ist3T_fwd(ubx.p, 0);
             ist3T_fwd(uby.p, 1);
