@@ -1184,14 +1184,17 @@ __device__ __forceinline__ double mq2_at(const PlainCoef& cf, int i, int j, int 
 
 // after the switch (:1369-1384): divv = -((rho - rho_old)/dt + u . grad rho) / (rho + rho_old)/2,
 // rf = rho f.  Before it: lap = rho * lap.  Then rho_old = rho.  rho and
-// rho_old hold rho - rho0: the difference is taken of the stored values.
+// rho_old hold rho - rho0: the difference is taken of the stored values, or
+// read from drho where it is formed apart (the FP32 storage form,
+// k_q_drho_z).
 template <typename T>
 __global__ __launch_bounds__(WG) void k_q_divv(const T* __restrict__ rho, T* __restrict__ rho_old,
                                                const T* __restrict__ ux, const T* __restrict__ uy,
                                                const T* __restrict__ uz, const T* __restrict__ grx,
                                                const T* __restrict__ gry, const T* __restrict__ grz,
                                                const T* __restrict__ f, T* __restrict__ divv,
-                                               T* __restrict__ lap, long n, QuasiCoef qc)
+                                               T* __restrict__ lap, const T* __restrict__ drho, long n,
+                                               QuasiCoef qc)
 {
     for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
         const double dr = (double)rho[id], dro = (double)rho_old[id], r = qc.rho0 + dr;
@@ -1200,7 +1203,8 @@ __global__ __launch_bounds__(WG) void k_q_divv(const T* __restrict__ rho, T* __r
         } else {
             double adv = (double)ux[id] * (double)grx[id] + (double)uy[id] * (double)gry[id] +
                          (double)uz[id] * (double)grz[id];
-            divv[id] = T(-((dr - dro) / qc.dt + adv) / (qc.rho0 + 0.5 * (dr + dro)));
+            const double delta = drho ? (double)drho[id] : dr - dro;
+            divv[id] = T(-(delta / qc.dt + adv) / (qc.rho0 + 0.5 * (dr + dro)));
             lap[id] = T(r * (double)f[id]);
         }
         rho_old[id] = rho[id];
@@ -1359,6 +1363,73 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_rho_z(const T* __restrict_
             // scales with kp |psi| instead of rho0); the filter keeps the
             // mean mode, so rho0 is added back exactly where rho is used
             lds[c * LS + PR(mk_pos<N>(p + t))] = T(qc.kp * sqrt(qc.pc.q0sq * x * x + g1 * g1 + g2 * g2 + g3 * g3));
+        }
+    });
+    lsync<true>();
+    dct2_line<T, N, TPL, true>(lds + l * LS, u, ct);
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+        V16<T> v;
+#pragma unroll
+        for (int t = 0; t < W; ++t) v.v[t] = lds[c * LS + PR(p + t)];
+        *reinterpret_cast<V16<T>*>(out + gi) = v;
+    });
+}
+
+// The density's change over the last step without the difference of two
+// rounded densities (FP32 storage form).  rho - rho0 = kp F(amp) with F the
+// linear Gaussian filter and amp = sqrt(A), A = q0^2 psi^2 + |grad psi|^2,
+// so rho^n - rho^{n-1} = kp F(amp^n - amp^{n-1}) and, with dpsi = psi^n -
+// psi^{n-1} and dgrad its gradient,
+//   amp^n - amp^{n-1} = (A^n - A^{n-1}) / (amp^n + amp^{n-1}),
+//   A^n - A^{n-1} = q0^2 dpsi (2 psi - dpsi) + dgrad . (2 grad psi - dgrad).
+// dpsi comes from the transforms of psi_hat^n - psi_hat^{n-1} (formed
+// exactly from the two stored states), so its rounding is relative to the
+// change, not to psi: (rho^n - rho^{n-1})/dt in divv no longer carries the
+// FP32 transforms' rounding of rho divided by dt.  This kernel is
+// REDFT10_z of kp (amp^n - amp^{n-1}), formed in the loads.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_q_dhat(const T* __restrict__ ph, const T* __restrict__ pho,
+                                               T* __restrict__ out, long n)
+{
+    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG)
+        out[id] = T((double)ph[id] - (double)pho[id]);
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(LinesZ<T>::WGZ) void k_q_drho_z(const T* __restrict__ psi, const T* __restrict__ gx,
+                                                             const T* __restrict__ gy, const T* __restrict__ gz,
+                                                             const T* __restrict__ dp, const T* __restrict__ dx,
+                                                             const T* __restrict__ dy, const T* __restrict__ dz,
+                                                             T* __restrict__ out, Dims d, QuasiCoef qc, Tw<T> tw)
+{
+    constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
+    constexpr int WGZ = LinesZ<T>::WGZ;
+    constexpr int W = V16<T>::W;
+    __shared__ T lds[NL * LS];
+    __shared__ T ct[N + 1];
+    load_cosq<T, N>(ct, tw);
+    __syncthreads();
+    const int l = threadIdx.x / TPL, u = threadIdx.x % TPL;
+    const double q0sq = qc.pc.q0sq;
+    zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
+        const V16<T> a = *reinterpret_cast<const V16<T>*>(psi + gi);
+        const V16<T> hx = *reinterpret_cast<const V16<T>*>(gx + gi);
+        const V16<T> hy = *reinterpret_cast<const V16<T>*>(gy + gi);
+        const V16<T> hz = *reinterpret_cast<const V16<T>*>(gz + gi);
+        const V16<T> b = *reinterpret_cast<const V16<T>*>(dp + gi);
+        const V16<T> ex = *reinterpret_cast<const V16<T>*>(dx + gi);
+        const V16<T> ey = *reinterpret_cast<const V16<T>*>(dy + gi);
+        const V16<T> ez = *reinterpret_cast<const V16<T>*>(dz + gi);
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            const double x = (double)a.v[t], g1 = (double)hx.v[t], g2 = (double)hy.v[t], g3 = (double)hz.v[t];
+            const double y = (double)b.v[t], e1 = (double)ex.v[t], e2 = (double)ey.v[t], e3 = (double)ez.v[t];
+            const double an = sqrt(q0sq * x * x + g1 * g1 + g2 * g2 + g3 * g3);
+            const double xo = x - y, o1 = g1 - e1, o2 = g2 - e2, o3 = g3 - e3;
+            const double ao = sqrt(q0sq * xo * xo + o1 * o1 + o2 * o2 + o3 * o3);
+            const double dA = q0sq * y * (x + xo) + e1 * (g1 + o1) + e2 * (g2 + o2) + e3 * (g3 + o3);
+            const double den = an + ao;
+            lds[c * LS + PR(mk_pos<N>(p + t))] = T(den > 0.0 ? qc.kp * dA / den : 0.0);
         }
     });
     lsync<true>();
@@ -2562,7 +2633,11 @@ struct Launch {
     static void sum(hipStream_t s, const T* a, long n, double* part, int nparts, double* out);
     static void q_divv(hipStream_t s, const T* rho, T* rho_old, const T* ux, const T* uy,
                        const T* uz, const T* grx, const T* gry, const T* grz, const T* f, T* divv,
-                       T* lap, Dims d, const QuasiCoef& qc);
+                       T* lap, const T* drho, Dims d, const QuasiCoef& qc);
+    static void q_dhat(hipStream_t s, const T* ph, const T* pho, T* out, Dims d);
+    static void q_drho_z(hipStream_t s, const T* psi, const T* gx, const T* gy, const T* gz, const T* dp,
+                         const T* dx, const T* dy, const T* dz, T* out, Dims d, const QuasiCoef& qc,
+                         const Tw<T>& tw);
     static void q_force(hipStream_t s, const T* psi, const T* f, const T* lap, const T* rho,
                         const T* gpx, const T* gpy, const T* gpz, const T* grx, const T* gry,
                         const T* grz, T* e, T* mu, T* sx, T* sy, T* sz, Dims d, const QuasiCoef& qc);

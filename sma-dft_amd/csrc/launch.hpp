@@ -420,9 +420,24 @@ void Launch<T>::sum(hipStream_t s, const T* a, long n, double* part, int nparts,
 template <typename T>
 void Launch<T>::q_divv(hipStream_t s, const T* rho, T* rho_old, const T* ux, const T* uy,
                        const T* uz, const T* grx, const T* gry, const T* grz, const T* f, T* divv,
-                       T* lap, Dims d, const QuasiCoef& qc)
+                       T* lap, const T* drho, Dims d, const QuasiCoef& qc)
 {
-    SMAD_FLAT(k_q_divv, rho, rho_old, ux, uy, uz, grx, gry, grz, f, divv, lap, d.total(), qc);
+    SMAD_FLAT(k_q_divv, rho, rho_old, ux, uy, uz, grx, gry, grz, f, divv, lap, drho, d.total(), qc);
+}
+
+template <typename T>
+void Launch<T>::q_dhat(hipStream_t s, const T* ph, const T* pho, T* out, Dims d)
+{
+    SMAD_FLAT(k_q_dhat, ph, pho, out, d.total());
+}
+
+template <typename T>
+void Launch<T>::q_drho_z(hipStream_t s, const T* psi, const T* gx, const T* gy, const T* gz, const T* dp,
+                         const T* dx, const T* dy, const T* dz, T* out, Dims d, const QuasiCoef& qc,
+                         const Tw<T>& tw)
+{
+    dim3 grid(num_tiles(d, 2, linesz_nl<T>(d.nz)));
+    SMAD_DISPATCH_Z(d.nz, k_q_drho_z, grid, psi, gx, gy, gz, dp, dx, dy, dz, out, d, qc, tw);
 }
 
 template <typename T>

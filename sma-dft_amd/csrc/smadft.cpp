@@ -1289,6 +1289,9 @@ struct QuasiEngine : EngineBase {
     // grx2/gry2/grz2: the next step's, computed with this step's rho
     DevBuf<T> ph, pho, nl, nlo, rhoq, rho, rho_old, divv, ux, uy, uz, psi, psi_prev, f, gpx, gpy,
         gpz, grx, gry, grz, grx2, gry2, grz2, lap, e, mu, sx, sy, sz;
+    // FP32 storage: the density's change over the step, formed apart
+    // (k_q_drho_z): psi_hat^n - psi_hat^{n-1} and its psi and grad psi
+    DevBuf<T> dph, dpsi, dgx, dgy, dgz;
     DevBuf<double> pr, part, red;
     int nparts = 1024;
     int nloop = 0;
@@ -1328,6 +1331,8 @@ struct QuasiEngine : EngineBase {
                              &psi_prev, &f, &gpx, &gpy, &gpz, &grx, &gry, &grz, &grx2, &gry2, &grz2, &lap,
                              &e, &mu, &sx, &sy, &sz})
             b->alloc(n);
+        if (sizeof(T) == 4)
+            for (DevBuf<T>* b : {&dph, &dpsi, &dgx, &dgy, &dgz}) b->alloc(n);
         pr.alloc(n);
         part.alloc(3 * nparts);
         red.alloc(8);
@@ -1428,8 +1433,30 @@ struct QuasiEngine : EngineBase {
         mk(0);
         if (p.load && nloop == 1)
             HIPCHK(hipMemcpyAsync(rho_old.p, rho.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        // FP32 storage: rho^n - rho^{n-1} = kp F(amp^n - amp^{n-1}) from the
+        // transforms of psi_hat^n - psi_hat^{n-1} (k_q_drho_z), instead of
+        // the difference of two FP32 densities, which (rho - rho_old)/dt
+        // would divide by dt together with the transforms' rounding
+        const T* drho = nullptr;
+        if (sizeof(T) == 4 && !qc.pre_switch) {
+            Launch<T>::q_dhat(s, ph.p, pho.p, dph.p, d);
+            Launch<T>::zpass(s, ZS_GRAD, dph.p, pass2(dpsi.p, REDFT01, OP_ID, dgz.p, RODFT01, OP_SHIFT), d, sc,
+                             twz.tw);
+            Launch<T>::spass(s, dpsi.p, pass2(dgy.p, RODFT01, OP_SHIFT, dpsi.p, REDFT01, OP_ID), d, 1, sc, twy.tw);
+            Launch<T>::axis(s, dgz.p, dgz.p, d, 1, REDFT01, T(1), twy.tw);
+            Launch<T>::spass(s, dpsi.p, pass2(dgx.p, RODFT01, OP_SHIFT, dpsi.p, REDFT01, OP_ID), d, 0, sc, twx.tw);
+            Launch<T>::axis(s, dgy.p, dgy.p, d, 0, REDFT01, T(1), twx.tw);
+            Launch<T>::axis(s, dgz.p, dgz.p, d, 0, REDFT01, T(1), twx.tw);
+            Launch<T>::q_drho_z(s, psi.p, gpx.p, gpy.p, gpz.p, dpsi.p, dgx.p, dgy.p, dgz.p, dph.p, d, qc, twz.tw);
+            Launch<T>::axis(s, dph.p, dph.p, d, 1, REDFT10, T(1), twy.tw);
+            Launch<T>::axis(s, dph.p, dph.p, d, 0, REDFT10, T(1), twx.tw);
+            Launch<T>::zpass(s, ZS_FILT, dph.p, pass1(dph.p, REDFT01, OP_FILT), d, sc, twz.tw);
+            Launch<T>::axis(s, dph.p, dph.p, d, 1, REDFT01, T(1), twy.tw);
+            Launch<T>::axis(s, dph.p, dph.p, d, 0, REDFT01, T(1), twx.tw);
+            drho = dph.p;
+        }
         Launch<T>::q_divv(s, rho.p, rho_old.p, ux.p, uy.p, uz.p, grx.p, gry.p, grz.p, f.p, divv.p,
-                          lap.p, d, qc);
+                          lap.p, drho, d, qc);
         mk(1);
         if (!qc.pre_switch) {
             // lap = REDFT01^3(-s |q|^2 REDFT10^3(rho f)) (:1386-1404)

@@ -60,3 +60,26 @@ def test_quasi_fp32_mixed_error(smadft, oracle, shape):
     # the pressure comes from S = mu grad psi + e grad rho, i.e. up to fourth
     # derivatives of FP32 psi; its FP32 error is reported, bounded loosely
     assert rel_l2(pr, q.p) < 1e-3
+
+
+def test_quasi_fp32_velocity_and_pressure_after_the_switch(smadft, oracle):
+    """In FP32 storage the density's change over a step is formed from the
+    transforms of psi_hat^n - psi_hat^{n-1} (k_q_drho_z), so (rho - rho_old)/dt
+    in divv (dt = 1e-3) no longer divides the FP32 transforms' rounding of rho
+    by dt: the velocity and the pressure follow the FP64 oracle closely over
+    hundreds of steps after the switch."""
+    p = smadft.Params.config(3, 32)
+    p.precision = 32
+    p.div_switch = 50
+    psi0 = smadft.config_ic(3, p)
+    q = oracle.QuasiOracle(p, psi0)
+    q.step(300)
+    with smadft.Stepper(p, psi0) as s:
+        s.step(300)
+        got = [s.fetch(f) for f in (smadft.FIELD_VEL_X, smadft.FIELD_VEL_Y, smadft.FIELD_VEL_Z,
+                                    smadft.FIELD_PRESSURE, smadft.FIELD_PSI)]
+    errs = [rel_l2(g, r) for g, r in zip(got, list(q.vel) + [q.p, q.psi])]
+    print(f"quasi FP32, config 3 32^3, 300 steps (switch at 50): ux uy uz p psi {errs}", flush=True)
+    # measured: u 1.2e-5 - 5.4e-5, p 1.1e-5, psi 1.0e-6 (the difference of two
+    # FP32 densities gave u and p errors near 1e-3 at this horizon)
+    assert max(errs[:3]) < 2e-4 and errs[3] < 1e-4 and errs[4] < 1e-5, errs
