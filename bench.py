@@ -314,9 +314,10 @@ def main():
                     sps = 1e3 * args.model_steps / ms
                     entry[f"fp{prec}_steps_per_s"] = sps
                     # algorithmic bytes per step: 103F (advective), 202F (quasi;
-                    # the FP32-mixed form adds 18 FP32-sized fields of FP64 work)
+                    # the FP32-mixed form adds 18 FP32-sized fields of FP64 work
+                    # and 37F for the density's change formed apart)
                     fb = grid ** 3 * (8 if prec == 64 else 4)
-                    nf = 103 if model == smadft.MODEL_ADV else (202 if prec == 64 else 220)
+                    nf = 103 if model == smadft.MODEL_ADV else (202 if prec == 64 else 257)
                     entry[f"fp{prec}_alg_GBps"] = nf * fb * sps / 1e9
                     log(f"{label} {grid}^3 fp{prec}: {sps:.1f} steps/s ({nf * fb * sps / 1e12:.2f} TB/s algorithmic)")
                 if nacc > 0:
