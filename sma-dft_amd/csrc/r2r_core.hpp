@@ -364,26 +364,34 @@ __device__ __forceinline__ void bf_load(const cpx<T>* line, int u, BfRegs<T, N, 
     }
 }
 
-// w[r] = omega^{e r}, r = 1..R-1, from at most three table reads; the other
-// powers are products of two looked-up ones.
-template <typename T, int N, int R>
-__device__ __forceinline__ void tw_powers(const T* __restrict__ ct, int e, cpx<T> (&w)[R])
+// v[r] *= omega^{e r} (conjugated for the inverse), r = 1..R-1, each twiddle
+// formed just before its product so that few are live at once (a radix-8
+// butterfly with all seven twiddles live held 28 more VGPRs in FP64): three
+// table reads for R = 8 (the other powers as products of two), two for
+// R = 4, one per element otherwise.
+template <typename T, int N, int R, bool INV>
+__device__ __forceinline__ void tw_apply(const T* __restrict__ ct, int e, cpx<T> (&v)[R])
 {
+    auto cj = [](cpx<T> w) { return INV ? conj(w) : w; };
     if constexpr (R == 8) {
-        w[1] = omega<T, N>(ct, e);
-        w[2] = omega<T, N>(ct, 2 * e);
-        w[4] = omega<T, N>(ct, 4 * e);
-        w[3] = mul(w[1], w[2]);
-        w[5] = mul(w[1], w[4]);
-        w[6] = mul(w[2], w[4]);
-        w[7] = mul(w[3], w[4]);
+        const cpx<T> w1 = omega<T, N>(ct, e), w2 = omega<T, N>(ct, 2 * e);
+        v[1] = mul(v[1], cj(w1));
+        v[2] = mul(v[2], cj(w2));
+        const cpx<T> w3 = mul(w1, w2);
+        v[3] = mul(v[3], cj(w3));
+        const cpx<T> w4 = omega<T, N>(ct, 4 * e);
+        v[4] = mul(v[4], cj(w4));
+        v[5] = mul(v[5], cj(mul(w1, w4)));
+        v[6] = mul(v[6], cj(mul(w2, w4)));
+        v[7] = mul(v[7], cj(mul(w3, w4)));
     } else if constexpr (R == 4) {
-        w[1] = omega<T, N>(ct, e);
-        w[2] = omega<T, N>(ct, 2 * e);
-        w[3] = mul(w[1], w[2]);
+        const cpx<T> w1 = omega<T, N>(ct, e), w2 = omega<T, N>(ct, 2 * e);
+        v[1] = mul(v[1], cj(w1));
+        v[2] = mul(v[2], cj(w2));
+        v[3] = mul(v[3], cj(mul(w1, w2)));
     } else {
 #pragma unroll
-        for (int r = 1; r < R; ++r) w[r] = omega<T, N>(ct, r * e);
+        for (int r = 1; r < R; ++r) v[r] = mul(v[r], cj(omega<T, N>(ct, r * e)));
     }
 }
 
@@ -398,13 +406,7 @@ __device__ __forceinline__ void bf_compute(BfRegs<T, N, R, TPL>& v, int u, int N
         if (B::live(b)) {
             if constexpr (!FIRST) {
                 const int e = 8 * (b % Ns) * (B::M / (Ns * R));
-                cpx<T> w[R];
-                tw_powers<T, N, R>(ct, e, w);
-#pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    if (INV) w[r].im = -w[r].im;
-                    v[p][r] = mul(v[p][r], w[r]);
-                }
+                tw_apply<T, N, R, INV>(ct, e, v[p]);
             }
             dftR<T, R, INV>(v[p]);
         }
