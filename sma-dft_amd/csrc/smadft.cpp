@@ -278,6 +278,9 @@ struct PlainEngine : EngineBase {
     int nparts = 1024;
     bool psi_valid = false, prev_valid = false;
     bool fresh = true;  // the state is the pre-loop's (psihat, nhat exact)
+    // the history in H is the nonlinearity of psi_prev with the current
+    // beta and gamma (true after steps, until new parameters or fields)
+    bool hist_prev = false;
 
     void coefs()
     {
@@ -310,7 +313,7 @@ struct PlainEngine : EngineBase {
         coefs();
         // the carried history follows the new coefficients (same psihat and
         // Nq_past, the latter formed with the old beta and gamma)
-        if (fresh || prev_valid) {
+        if (fresh || hist_prev) {
             const PlainCoef now = cf;
             cf = old;
             cur_history();
@@ -321,6 +324,7 @@ struct PlainEngine : EngineBase {
             Launch<T>::plain_hist(s, psihat.p, hh.p, nullptr, d, old, cf, 2);
         }
         LAUNCHCHK();
+        hist_prev = false;
     }
 
     PlainEngine(const smad_params& pp, hipStream_t st) : p(pp), s(st)
@@ -365,7 +369,7 @@ struct PlainEngine : EngineBase {
     void cur_history()
     {
         if (fresh) return;
-        if (prev_valid) {
+        if (hist_prev) {
             Launch<T>::plain_z(s, psi_prev.p, nhat.p, nullptr, d, cf, 1, twz.tw);
             Launch<T>::axis(s, nhat.p, nhat.p, d, 1, REDFT10, T(1), twy.tw);
             Launch<T>::axis(s, nhat.p, nhat.p, d, 0, REDFT10, T(1), twx.tw);
@@ -392,6 +396,7 @@ struct PlainEngine : EngineBase {
         LAUNCHCHK();
         psi_valid = true;
         prev_valid = false;
+        hist_prev = false;
         fresh = true;
     }
 
@@ -437,6 +442,7 @@ struct PlainEngine : EngineBase {
         if (nsteps > 0) {
             psi_valid = false;
             prev_valid = true;
+            hist_prev = true;
             fresh = false;
         }
     }
@@ -581,6 +587,7 @@ struct PlainEngine : EngineBase {
         tape_nseg = nseg;
         psi_valid = false;
         prev_valid = false;
+        hist_prev = false;
         fresh = false;
     }
 
@@ -797,6 +804,7 @@ struct PlainEngine : EngineBase {
         }
         psi_valid = false;
         prev_valid = false;
+        hist_prev = false;
         fresh = false;
     }
 

@@ -163,18 +163,24 @@ def test_set_params_mid_run_keeps_psi_and_history(smadft, oracle, prec):
     for the new coefficients: the run continues from the same psi and AB2
     history, as the oracle does when its parameters change between steps."""
     pa = params(smadft, (32, 32, 16), prec=prec)
-    pb = params(smadft, (32, 32, 16), prec=prec, eps=-0.65, alpha=1.1, dt=0.05)
+    pb = params(smadft, (32, 32, 16), prec=prec, eps=-0.65, alpha=1.1, beta=2.1, dt=0.05)
     psi0 = smadft.ic_curved(pa, 1.328, 1e-3, seed=5)
     o = oracle.PlainOracle(pa, psi0)
     o.step(30)
+    nq_a = o.nq.copy()
     o.op = oracle.oparams(pb)
     o.step(30)
     with smadft.Stepper(pa, psi0) as s:
         s.step(30)
         s.set_params(pb)
+        # the history is still the nonlinearity formed with the old beta and
+        # gamma (recovered from H, no longer from psi_prev)
+        nq = s.fetch(smadft.FIELD_N_HAT)
         s.step(30)
         psi = s.fetch()
     assert rel_l2(psi, o.psi) < (1e-10 if prec == 64 else 1e-5)
+    if prec == 64:
+        assert rel_l2(nq, nq_a) < 1e-8
 
 
 def test_set_psi_hat_and_n_hat_restart(smadft):
