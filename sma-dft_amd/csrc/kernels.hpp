@@ -405,16 +405,13 @@ __device__ __forceinline__ double cn_ab2(const PlainCoef& cf, double mq2, double
 }
 
 // x pass: Nq = REDFT10_x(w); CN/AB2 update of psihat with the history nhat
-// (in place: nhat holds Nq_past on entry, Nq on exit; with HOOP the history
-// is read from nh_in instead and nhat only written); w = REDFT01_x(psihat').
+// (in place: nhat holds Nq_past on entry, Nq on exit); w = REDFT01_x(psihat').
 // psihat is the pre-scaled spectral field.
-template <typename T, int N, bool CONS, bool HOOP = false>
+template <typename T, int N, bool CONS>
 __device__ __forceinline__ void plain_x_body(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
                                              T* __restrict__ nhat, const T* __restrict__ sub, const Dims& d,
-                                             const PlainCoef& cf, int inverse, const Tw<T>& tw,
-                                             const T* __restrict__ nh_in = nullptr)
+                                             const PlainCoef& cf, int inverse, const Tw<T>& tw)
 {
-    const T* __restrict__ nh_src = HOOP ? nh_in : nhat;
     constexpr int NL = Lines<T>::NL, TPL = Lines<T>::TPL, LS = LineLS<N>::value;
     constexpr int W = V16<T>::W;
     __shared__ T lds[NL * LS];
@@ -440,7 +437,7 @@ __device__ __forceinline__ void plain_x_body(const T* a_in, T* w_out, const T* p
     if (g.ncol == NL) {
         rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
             V16<T> ph = *reinterpret_cast<const V16<T>*>(ph_in + gi);
-            V16<T> np = *reinterpret_cast<const V16<T>*>(nh_src + gi);
+            V16<T> np = *reinterpret_cast<const V16<T>*>(nhat + gi);
             V16<T> sb{};
             if constexpr (CONS)
                 if (sub) sb = *reinterpret_cast<const V16<T>*>(sub + gi);
@@ -467,7 +464,7 @@ __device__ __forceinline__ void plain_x_body(const T* a_in, T* w_out, const T* p
                 double mq2 = qx * qx + qy * qy + qz * qz;
                 T nqt = lds[c * LS + PR(p)];
                 if constexpr (CONS) nqt = T(mq2 * (double)nqt - (sub ? (double)sub[gi] : 0.0));
-                double r = cn_ab2<CONS>(cf, mq2, (double)ph_in[gi], (double)nqt, (double)nh_src[gi]);
+                double r = cn_ab2<CONS>(cf, mq2, (double)ph_in[gi], (double)nqt, (double)nhat[gi]);
                 ph_out[gi] = T(r);
                 nhat[gi] = nqt;
                 lds[c * LS + PR(p)] = T(r);
@@ -492,15 +489,6 @@ __global__ __launch_bounds__(WG) void k_plain_x(const T* a_in, T* w_out, const T
 }
 // conserved variants: aLin times |q|^2 and Nq = |q|^2 REDFT10_x(a_in) - sub
 // (sub = the advection term's spectrum, or null)
-// the first step of a checkpointed segment: the history comes from the
-// checkpoint nh_in, which stays untouched
-template <typename T, int N>
-__global__ __launch_bounds__(WG) void k_plain_x_hoop(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
-                                                     T* __restrict__ nhat, const T* __restrict__ nh_in, Dims d,
-                                                     PlainCoef cf, int inverse, Tw<T> tw)
-{
-    plain_x_body<T, N, false, true>(a_in, w_out, ph_in, ph_out, nhat, nullptr, d, cf, inverse, tw, nh_in);
-}
 template <typename T, int N>
 __global__ __launch_bounds__(WG) void k_plain_x_cons(const T* a_in, T* w_out, const T* ph_in, T* ph_out,
                                                      T* __restrict__ nhat, const T* __restrict__ sub, Dims d,
@@ -730,7 +718,7 @@ enum PassOp : int {
 // launchers check the kinds the caller passes against it.  With the kinds
 // known, the outputs' transforms would share their twiddle arithmetic in
 // registers (FP64 384-point z pass: 158 VGPRs), so each output's transform
-// starts from an opaque_zero() at every size (102).
+// starts from an opaque zero (102) at every size but 578 (opaque_zero_multi).
 constexpr int op_kind(int op) { return op == OP_SHIFT || op == OP_FILT_SHIFT ? RODFT01 : REDFT01; }
 
 // Op sets of the z passes with several outputs.
@@ -820,7 +808,7 @@ __device__ __forceinline__ void zmulti_one(const T* in, T* out, const Dims& d, c
     constexpr int NL = LinesZ<T, N>::NL, TPL = LinesZ<T, N>::TPL, LS = LineLS<N>::value;
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
-    const int e0 = opaque_zero();
+    const int e0 = opaque_zero_multi<N>();
     zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
         int i = 0, j = 0;
@@ -883,7 +871,7 @@ __device__ __forceinline__ void multi_one(const T* in, T* out, int axis, const T
         else { i = outer; j = p; }
     };
     __syncthreads();
-    const int e0 = opaque_zero();
+    const int e0 = opaque_zero_multi<N>();
     if (g.ncol == NL) {
         rows_vec<T, N, NL>(g, [&](int p, int c0, long gi) {
             V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
@@ -1084,7 +1072,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_stokes_z(const T* __rest
         }
     };
     auto component = [&](int comp, T* __restrict__ out, int kind) {
-        const int e0 = opaque_zero();  // 384 points: 149 -> 90 VGPRs
+        const int e0 = opaque_zero_multi<N>();  // 384 points: 149 -> 90 VGPRs
         InMap<T, N> im{kind};
         zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
             const long L = gi / N;
@@ -2593,7 +2581,7 @@ struct Launch {
                         const PlainCoef& cf, int from_real, const Tw<T>& tw);
     static void plain_x(hipStream_t s, const T* a_in, T* w_out, const T* ph_in, T* ph_out,
                         T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse = 1,
-                        const T* sub = nullptr, const T* nh_in = nullptr);
+                        const T* sub = nullptr);
     // the plain step's x pass in combined-history form (k_plain_xh); ph_tape
     // non-null: psihat' to the adjoint's tape as well
     static void plain_xh(hipStream_t s, const T* a_in, T* w_out, const T* h_in, T* h_out, T* ph_tape, Dims d,

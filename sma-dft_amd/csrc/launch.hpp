@@ -89,15 +89,11 @@ void Launch<T>::plain_z(hipStream_t s, const T* in, T* out, T* psi_out, Dims d,
 
 template <typename T>
 void Launch<T>::plain_x(hipStream_t s, const T* a_in, T* w_out, const T* ph_in, T* ph_out,
-                        T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse, const T* sub,
-                        const T* nh_in)
+                        T* nhat, Dims d, const PlainCoef& cf, const Tw<T>& tw, int inverse, const T* sub)
 {
     dim3 grid(num_tiles(d, 0, Lines<T>::NL));
     if (cf.conserved) {
-        if (nh_in) throw std::logic_error("plain_x: no separate history in the conserved variant");
         SMAD_DISPATCH(d.nx, k_plain_x_cons, grid, a_in, w_out, ph_in, ph_out, nhat, sub, d, cf, inverse, tw);
-    } else if (nh_in) {
-        SMAD_DISPATCH(d.nx, k_plain_x_hoop, grid, a_in, w_out, ph_in, ph_out, nhat, nh_in, d, cf, inverse, tw);
     } else {
         SMAD_DISPATCH(d.nx, k_plain_x, grid, a_in, w_out, ph_in, ph_out, nhat, d, cf, inverse, tw);
     }

@@ -169,6 +169,18 @@ __device__ __forceinline__ int opaque_zero_if()
     else return 0;
 }
 
+// opaque_zero() for the passes that run several transforms in a row, except
+// at 578 points (radix-17 lines), where recomputing the index arithmetic of
+// each transform spills instead: FP64 578-point multi-output passes 256 VGPRs
+// and 100-132 bytes of scratch per lane with it, 143-175 and none without.
+// Their LDS tile caps them at one workgroup per CU either way.
+template <int N>
+__device__ __forceinline__ int opaque_zero_multi()
+{
+    if constexpr (N == 578) return 0;
+    else return opaque_zero();
+}
+
 // Synchronisation of the threads that share a line.  Lines never straddle a
 // wavefront (TPL divides 64), so inside the transforms a wavefront-scope
 // barrier with LDS fences suffices; WAVE = false falls back to the
