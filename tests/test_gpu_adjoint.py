@@ -110,7 +110,8 @@ def test_autograd_rollout_matches_oracle_and_fd(smadft, oracle):
     for i, name in enumerate(["eps", "alpha", "beta", "gamma"]):
         v = getattr(p, name)
         fd = (J(**{name: v + h}) - J(**{name: v - h})) / (2 * h)
-        assert theta.grad[i].item() == pytest.approx(fd, rel=1e-6), name
+        print(f"autograd dJ/d{name}: {theta.grad[i].item():.9e}, FD {fd:.9e}", flush=True)
+        assert theta.grad[i].item() == pytest.approx(fd, rel=1e-6, abs=0), name
 
 
 def test_adjoint_reports_nonfinite_gradients(smadft):

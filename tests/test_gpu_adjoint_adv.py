@@ -55,10 +55,12 @@ def test_adv_adjoint_central_differences_on_gpu(smadft):
     for i, name in enumerate(("eps", "alpha", "beta", "gamma")):
         v = getattr(p, name)
         fd = (J(dataclasses.replace(p, **{name: v + h}), psi0) - J(dataclasses.replace(p, **{name: v - h}), psi0)) / (2 * h)
-        assert dp[i] == pytest.approx(fd, rel=1e-6), name
+        print(f"adv dJ/d{name}: adjoint {dp[i]:.9e}, FD {fd:.9e}", flush=True)
+        assert dp[i] == pytest.approx(fd, rel=1e-6, abs=0), name
     u = d0 / np.linalg.norm(d0)
     fd = (J(p, psi0 + h * u) - J(p, psi0 - h * u)) / (2 * h)
-    assert float(np.sum(d0 * u)) == pytest.approx(fd, rel=1e-6)
+    print(f"adv dJ/dpsi0 . u: adjoint {float(np.sum(d0 * u)):.9e}, FD {fd:.9e}", flush=True)
+    assert float(np.sum(d0 * u)) == pytest.approx(fd, rel=1e-6, abs=0)
 
 
 def test_adv_adjoint_fp32_close(smadft, oracle):
