@@ -114,6 +114,22 @@ struct alignas(16) V16 {
     T v[W];
 };
 
+// Element-wise kernels take the fields in 16-byte groups: a thread reads and
+// writes the W = 16 / sizeof(T) consecutive elements of a group with one
+// 16-byte access per field (the fields are separate 256-byte aligned
+// allocations), and the last n % W elements one per thread.
+template <typename T>
+__device__ __forceinline__ V16<T> ldg16(const T* __restrict__ p, long g)
+{
+    return reinterpret_cast<const V16<T>*>(p)[g];
+}
+template <typename T>
+__device__ __forceinline__ void stg16(T* __restrict__ p, long g, const V16<T>& v)
+{
+    reinterpret_cast<V16<T>*>(p)[g] = v;
+}
+
+
 // Iterate a strided tile (axis x or y) in 16-byte groups: f(p, c0, gi) for
 // columns c0 .. c0+W-1 of row p.  A wavefront covers 64 groups, i.e. whole
 // 128 B row segments.  Requires a full tile (ncol == NL).  e0 is zero (an
@@ -1184,22 +1200,68 @@ __global__ __launch_bounds__(WG) void k_q_divv(const T* __restrict__ rho, T* __r
                                                T* __restrict__ lap, const T* __restrict__ drho, long n,
                                                QuasiCoef qc)
 {
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        const double dr = (double)rho[id], dro = (double)rho_old[id], r = qc.rho0 + dr;
+    constexpr int W = V16<T>::W;
+    const long ng = n / W, stride = (long)gridDim.x * WG, g0 = (long)blockIdx.x * WG + threadIdx.x;
+    auto dv = [&](T dr, T dro, T dlt, T vx, T vy, T vz, T gx, T gy, T gz) {
+        const double adv = (double)vx * (double)gx + (double)vy * (double)gy + (double)vz * (double)gz;
+        const double delta = drho ? (double)dlt : (double)dr - (double)dro;
+        return T(-(delta / qc.dt + adv) / (qc.rho0 + 0.5 * ((double)dr + (double)dro)));
+    };
+    if (qc.pre_switch) {
+        for (long g = g0; g < ng; g += stride) {
+            const V16<T> r = ldg16(rho, g);
+            V16<T> l = ldg16(lap, g);
+#pragma unroll
+            for (int t = 0; t < W; ++t) l.v[t] = T((qc.rho0 + (double)r.v[t]) * (double)l.v[t]);
+            stg16(lap, g, l);
+            stg16(rho_old, g, r);
+        }
+    } else {
+        for (long g = g0; g < ng; g += stride) {
+            const V16<T> r = ldg16(rho, g), ro = ldg16(rho_old, g), fv = ldg16(f, g);
+            const V16<T> vx = ldg16(ux, g), vy = ldg16(uy, g), vz = ldg16(uz, g);
+            const V16<T> gx = ldg16(grx, g), gy = ldg16(gry, g), gz = ldg16(grz, g);
+            const V16<T> dl = drho ? ldg16(drho, g) : r;
+            V16<T> od, ol;
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                od.v[t] = dv(r.v[t], ro.v[t], dl.v[t], vx.v[t], vy.v[t], vz.v[t], gx.v[t], gy.v[t], gz.v[t]);
+                ol.v[t] = T((qc.rho0 + (double)r.v[t]) * (double)fv.v[t]);
+            }
+            stg16(divv, g, od);
+            stg16(lap, g, ol);
+            stg16(rho_old, g, r);
+        }
+    }
+    for (long id = ng * W + g0; id < n; id += stride) {
+        const double r = qc.rho0 + (double)rho[id];
         if (qc.pre_switch) {
             lap[id] = T(r * (double)lap[id]);
         } else {
-            double adv = (double)ux[id] * (double)grx[id] + (double)uy[id] * (double)gry[id] +
-                         (double)uz[id] * (double)grz[id];
-            const double delta = drho ? (double)drho[id] : dr - dro;
-            divv[id] = T(-(delta / qc.dt + adv) / (qc.rho0 + 0.5 * (dr + dro)));
+            divv[id] = dv(rho[id], rho_old[id], drho ? drho[id] : T(0), ux[id], uy[id], uz[id], grx[id], gry[id],
+                          grz[id]);
             lap[id] = T(r * (double)f[id]);
         }
         rho_old[id] = rho[id];
     }
 }
 
-// e, mu and S = mu grad psi + e grad rho (:1449-1488)
+// e, mu and S = mu grad psi + e grad rho at one point (:1449-1488)
+template <typename T>
+__device__ __forceinline__ void q_force_pt(const QuasiCoef& qc, T psi, T f, T lap, T rho, T gpx, T gpy, T gpz,
+                                           T grx, T gry, T grz, T& e, T& mu, T& sx, T& sy, T& sz)
+{
+    const PlainCoef& cf = qc.pc;
+    double x = (double)psi, x2 = x * x, x3 = x2 * x, fv = (double)f;
+    double ev = -0.5 * cf.eps * x2 + 0.5 * fv * fv - 0.25 * cf.beta * x2 * x2 + (cf.gamma / 6) * x3 * x3;
+    double mv = (double)lap + (qc.rho0 + (double)rho) * (fv - cf.eps * x - cf.beta * x3 + cf.gamma * x3 * x2);
+    e = T(ev);
+    mu = T(mv);
+    sx = T(mv * (double)gpx + ev * (double)grx);
+    sy = T(mv * (double)gpy + ev * (double)gry);
+    sz = T(mv * (double)gpz + ev * (double)grz);
+}
+
 template <typename T>
 __global__ __launch_bounds__(WG) void k_q_force(const T* __restrict__ psi, const T* __restrict__ f,
                                                 const T* __restrict__ lap, const T* __restrict__ rho,
@@ -1210,17 +1272,26 @@ __global__ __launch_bounds__(WG) void k_q_force(const T* __restrict__ psi, const
                                                 T* __restrict__ sx, T* __restrict__ sy,
                                                 T* __restrict__ sz, long n, QuasiCoef qc)
 {
-    const PlainCoef& cf = qc.pc;
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        double x = (double)psi[id], x2 = x * x, x3 = x2 * x, fv = (double)f[id];
-        double ev = -0.5 * cf.eps * x2 + 0.5 * fv * fv - 0.25 * cf.beta * x2 * x2 + (cf.gamma / 6) * x3 * x3;
-        double mv = (double)lap[id] + (qc.rho0 + (double)rho[id]) * (fv - cf.eps * x - cf.beta * x3 + cf.gamma * x3 * x2);
-        e[id] = T(ev);
-        mu[id] = T(mv);
-        sx[id] = T(mv * (double)gpx[id] + ev * (double)grx[id]);
-        sy[id] = T(mv * (double)gpy[id] + ev * (double)gry[id]);
-        sz[id] = T(mv * (double)gpz[id] + ev * (double)grz[id]);
+    constexpr int W = V16<T>::W;
+    const long ng = n / W, stride = (long)gridDim.x * WG;
+    for (long g = (long)blockIdx.x * WG + threadIdx.x; g < ng; g += stride) {
+        const V16<T> a = ldg16(psi, g), b = ldg16(f, g), c = ldg16(lap, g), r = ldg16(rho, g);
+        const V16<T> px = ldg16(gpx, g), py = ldg16(gpy, g), pz = ldg16(gpz, g);
+        const V16<T> rx = ldg16(grx, g), ry = ldg16(gry, g), rz = ldg16(grz, g);
+        V16<T> oe, om, ox, oy, oz;
+#pragma unroll
+        for (int t = 0; t < W; ++t)
+            q_force_pt(qc, a.v[t], b.v[t], c.v[t], r.v[t], px.v[t], py.v[t], pz.v[t], rx.v[t], ry.v[t], rz.v[t],
+                       oe.v[t], om.v[t], ox.v[t], oy.v[t], oz.v[t]);
+        stg16(e, g, oe);
+        stg16(mu, g, om);
+        stg16(sx, g, ox);
+        stg16(sy, g, oy);
+        stg16(sz, g, oz);
     }
+    for (long id = ng * W + (long)blockIdx.x * WG + threadIdx.x; id < n; id += stride)
+        q_force_pt(qc, psi[id], f[id], lap[id], rho[id], gpx[id], gpy[id], gpz[id], grx[id], gry[id], grz[id], e[id],
+                   mu[id], sx[id], sy[id], sz[id]);
 }
 
 // pressure (:1510-1592) and velocity (:1599-2109), FP64 arithmetic:
@@ -1379,8 +1450,16 @@ template <typename T>
 __global__ __launch_bounds__(WG) void k_q_dhat(const T* __restrict__ ph, const T* __restrict__ pho,
                                                T* __restrict__ out, long n)
 {
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG)
-        out[id] = T((double)ph[id] - (double)pho[id]);
+    constexpr int W = V16<T>::W;
+    const long ng = n / W, stride = (long)gridDim.x * WG, g0 = (long)blockIdx.x * WG + threadIdx.x;
+    for (long g = g0; g < ng; g += stride) {
+        const V16<T> a = ldg16(ph, g), b = ldg16(pho, g);
+        V16<T> o;
+#pragma unroll
+        for (int t = 0; t < W; ++t) o.v[t] = T((double)a.v[t] - (double)b.v[t]);
+        stg16(out, g, o);
+    }
+    for (long id = ng * W + g0; id < n; id += stride) out[id] = T((double)ph[id] - (double)pho[id]);
 }
 
 template <typename T, int N>
@@ -1832,22 +1911,52 @@ __global__ __launch_bounds__(WG) void k_c_force(const T* __restrict__ psi, const
                                                 T* __restrict__ fvz, long n, CompCoef cc)
 {
     const PlainCoef& cf = cc.pc;
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        const double x = (double)psi[id], x2 = x * x, x3 = x2 * x, fv = (double)f[id], r = (double)rho[id];
-        const double a = (double)amp[id], lp = (double)lap[id], e0 = 0.5 * fv * fv;
+    // inputs in[0..10] = psi f lap rho amp gpx gpy gpz grx gry grz,
+    // outputs o[0..8] = fg E mu sx sy sz fvx fvy fvz
+    auto pt = [&](const T* in, T* o) {
+        const double x = (double)in[0], x2 = x * x, x3 = x2 * x, fv = (double)in[1], r = (double)in[3];
+        const double a = (double)in[4], lp = (double)in[2], e0 = 0.5 * fv * fv;
         const double req = cc.rho0 + cc.kp * a;
-        fg[id] = T(0.5 * cc.zeta * (r * r - req * req));
+        o[0] = T(0.5 * cc.zeta * (r * r - req * req));
         const double mt = lp + r * fv;
-        const double hx = (double)gpx[id], hy = (double)gpy[id], hz = (double)gpz[id];
-        sx[id] = T(mt * hx + e0 * (double)grx[id]);
-        sy[id] = T(mt * hy + e0 * (double)gry[id]);
-        sz[id] = T(mt * hz + e0 * (double)grz[id]);
-        mu[id] = T(lp + r * (fv - cf.eps * x - cf.beta * x3 + cf.gamma * x3 * x2));
-        E[id] = T(r * e0);
+        const double hx = (double)in[5], hy = (double)in[6], hz = (double)in[7];
+        o[3] = T(mt * hx + e0 * (double)in[8]);
+        o[4] = T(mt * hy + e0 * (double)in[9]);
+        o[5] = T(mt * hz + e0 * (double)in[10]);
+        o[2] = T(lp + r * (fv - cf.eps * x - cf.beta * x3 + cf.gamma * x3 * x2));
+        o[1] = T(r * e0);
         const double pen = cc.kp * cc.zeta * (r - cc.rho0 - cc.kp * a) / (a + 0.001);
-        fvx[id] = T(pen * hx);
-        fvy[id] = T(pen * hy);
-        fvz[id] = T(pen * hz);
+        o[6] = T(pen * hx);
+        o[7] = T(pen * hy);
+        o[8] = T(pen * hz);
+    };
+    const T* const ins[11] = {psi, f, lap, rho, amp, gpx, gpy, gpz, grx, gry, grz};
+    T* const outs[9] = {fg, E, mu, sx, sy, sz, fvx, fvy, fvz};
+    constexpr int W = V16<T>::W;
+    const long ng = n / W, stride = (long)gridDim.x * WG, g0 = (long)blockIdx.x * WG + threadIdx.x;
+    for (long g = g0; g < ng; g += stride) {
+        V16<T> vi[11], vo[9];
+#pragma unroll
+        for (int q = 0; q < 11; ++q) vi[q] = ldg16(ins[q], g);
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            T in[11], o[9];
+#pragma unroll
+            for (int q = 0; q < 11; ++q) in[q] = vi[q].v[t];
+            pt(in, o);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) vo[q].v[t] = o[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) stg16(outs[q], g, vo[q]);
+    }
+    for (long id = ng * W + g0; id < n; id += stride) {
+        T in[11], o[9];
+#pragma unroll
+        for (int q = 0; q < 11; ++q) in[q] = ins[q][id];
+        pt(in, o);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) outs[q][id] = o[q];
     }
 }
 
@@ -1949,15 +2058,45 @@ __global__ __launch_bounds__(WG) void k_c_update(const T* __restrict__ ux, const
                                                  T* __restrict__ rho, T* __restrict__ frho, T* __restrict__ nl,
                                                  long n, CompCoef cc)
 {
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        const double vx = (double)ux[id], vy = (double)uy[id], vz = (double)uz[id], r = (double)rho[id];
-        const double R = -vx * (double)grx[id] - vy * (double)gry[id] - vz * (double)grz[id] - r * (double)divv[id];
-        const double rn = r + cc.dt * (1.5 * R - 0.5 * (double)frho[id]);
-        rho[id] = T(rn);
-        frho[id] = T(R);
-        const double a = (double)amp[id];
-        nl[id] = T(-(double)mu[id] + (cc.kp * cc.zeta / (a + 0.001)) * (rn - cc.rho0 - cc.kp * a) * (double)psi[id] -
-                   (double)adg[id] - vx * (double)gpx[id] - vy * (double)gpy[id] - vz * (double)gpz[id]);
+    // inputs in[0..15] = ux uy uz grx gry grz divv mu amp psi adg gpx gpy gpz rho frho,
+    // outputs o[0..2] = rho frho nl
+    auto pt = [&](const T* in, T* o) {
+        const double vx = (double)in[0], vy = (double)in[1], vz = (double)in[2], r = (double)in[14];
+        const double R = -vx * (double)in[3] - vy * (double)in[4] - vz * (double)in[5] - r * (double)in[6];
+        const double rn = r + cc.dt * (1.5 * R - 0.5 * (double)in[15]);
+        o[0] = T(rn);
+        o[1] = T(R);
+        const double a = (double)in[8];
+        o[2] = T(-(double)in[7] + (cc.kp * cc.zeta / (a + 0.001)) * (rn - cc.rho0 - cc.kp * a) * (double)in[9] -
+                 (double)in[10] - vx * (double)in[11] - vy * (double)in[12] - vz * (double)in[13]);
+    };
+    const T* const ins[16] = {ux, uy, uz, grx, gry, grz, divv, mu, amp, psi, adg, gpx, gpy, gpz, rho, frho};
+    T* const outs[3] = {rho, frho, nl};
+    constexpr int W = V16<T>::W;
+    const long ng = n / W, stride = (long)gridDim.x * WG, g0 = (long)blockIdx.x * WG + threadIdx.x;
+    for (long g = g0; g < ng; g += stride) {
+        V16<T> vi[16], vo[3];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) vi[q] = ldg16(ins[q], g);
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            T in[16], o[3];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) in[q] = vi[q].v[t];
+            pt(in, o);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) vo[q].v[t] = o[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) stg16(outs[q], g, vo[q]);
+    }
+    for (long id = ng * W + g0; id < n; id += stride) {
+        T in[16], o[3];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) in[q] = ins[q][id];
+        pt(in, o);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) outs[q][id] = o[q];
     }
 }
 
