@@ -140,12 +140,16 @@ __device__ __forceinline__ void rows_vec(const TileGeo& g, F&& f, int e0 = 0)
 {
     constexpr int W = V16<T>::W;
     constexpr int GPR = NL / W;
-    constexpr int TOT = N * GPR;
+    // a thread keeps its columns on every row it visits and steps PSTEP rows
+    static_assert(WG % GPR == 0, "rows_vec: the workgroup covers whole rows");
+    constexpr int PSTEP = WG / GPR;
+    const unsigned e = threadIdx.x + (unsigned)e0;
+    const int c0 = (int)(e % GPR) * W;
+    int p = (int)(e / GPR);
+    long gi = g.base + (long)p * g.stride + c0;
+    const long gstep = (long)PSTEP * g.stride;
 #pragma unroll UNROLL
-    for (int e = threadIdx.x + e0; e < TOT; e += WG) {
-        int p = e / GPR, c0 = (e - p * GPR) * W;
-        f(p, c0, g.base + (long)p * g.stride + c0);
-    }
+    for (; p < N; p += PSTEP, gi += gstep) f(p, c0, gi);
 }
 
 // Scalar iteration of any tile (partial tiles, small grids).
@@ -220,9 +224,9 @@ __device__ __forceinline__ void zlines_vec(const Dims& d, F&& f, int e0 = 0)
     const long nlines = (long)d.nx * d.ny;
     const long l0 = (long)blockIdx.x * NL + wv * LPW;
 #pragma unroll 4
-    for (int e = lane + e0; e < TOT; e += 64) {
-        int lc = (e * W) / N;
-        int p = e * W - lc * N;
+    for (unsigned e = lane + (unsigned)e0; e < (unsigned)TOT; e += 64) {
+        const int lc = (int)((e * W) / N);
+        const int p = (int)(e * W) - lc * N;
         if (l0 + lc < nlines) f(p, wv * LPW + lc, (l0 + lc) * N + p);
     }
 }
