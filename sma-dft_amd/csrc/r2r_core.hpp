@@ -71,8 +71,9 @@ struct Tw {
 template <typename T, int N>
 __device__ __forceinline__ cpx<T> omega(const T* __restrict__ c, int e)
 {
-    const int q = e / N;
-    const int r = e - q * N;
+    // e >= 0: unsigned, the quotient is a shift or a multiply-high
+    const unsigned q = (unsigned)e / N;
+    const int r = (int)((unsigned)e - q * N);
     const T a = c[r], b = c[N - r];
     switch (q & 3) {
     case 0: return cpx<T>{a, -b};
@@ -417,7 +418,7 @@ __device__ __forceinline__ void bf_compute(BfRegs<T, N, R, TPL>& v, int u, int N
         int b = u + p * TPL;
         if (B::live(b)) {
             if constexpr (!FIRST) {
-                const int e = 8 * (b % Ns) * (B::M / (Ns * R));
+                const int e = 8 * (int)((unsigned)b % (unsigned)Ns) * (B::M / (Ns * R));
                 tw_apply<T, N, R, INV>(ct, e, v[p]);
             }
             dftR<T, R, INV>(v[p]);
@@ -433,7 +434,8 @@ __device__ __forceinline__ void bf_store(cpx<T>* line, int u, int Ns, const BfRe
     for (int p = 0; p < B::PER; ++p) {
         int b = u + p * TPL;
         if (B::live(b)) {
-            int base = (b / Ns) * Ns * R + b % Ns;
+            const unsigned ub = (unsigned)b, uNs = (unsigned)Ns;  // b >= 0
+            const int base = (int)((ub / uNs) * uNs * R + ub % uNs);
 #pragma unroll
             for (int r = 0; r < R; ++r) line[PC(base + r * Ns)] = v[p][r];
         }
