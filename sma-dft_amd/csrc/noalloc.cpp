@@ -188,13 +188,18 @@ int smad_r2r_3d_noalloc(int nx, int ny, int nz, const int kinds[3], int precisio
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     const Dims d{nx, ny, nz};
-    if (precision == SMAD_FP64) {
-        transform<double>(s, in_dev, out_dev, d, kinds, slot.ptr);
-    } else {
-        float* stage = reinterpret_cast<float*>(slot.ptr + STAGE_OFF);
-        hipLaunchKernelGGL(k_to_f32, conv_grid(n), dim3(WG), 0, s, in_dev, stage, n);
-        transform<float>(s, stage, stage, d, kinds, slot.ptr + TW64_BYTES);
-        hipLaunchKernelGGL(k_to_f64, conv_grid(n), dim3(WG), 0, s, stage, out_dev, n);
+    try {
+        if (precision == SMAD_FP64) {
+            transform<double>(s, in_dev, out_dev, d, kinds, slot.ptr);
+        } else {
+            float* stage = reinterpret_cast<float*>(slot.ptr + STAGE_OFF);
+            hipLaunchKernelGGL(k_to_f32, conv_grid(n), dim3(WG), 0, s, in_dev, stage, n);
+            transform<float>(s, stage, stage, d, kinds, slot.ptr + TW64_BYTES);
+            hipLaunchKernelGGL(k_to_f64, conv_grid(n), dim3(WG), 0, s, stage, out_dev, n);
+        }
+    } catch (const std::exception& e) {  // no C++ exception crosses the C ABI
+        g_noalloc_error = std::string("smad_r2r_3d_noalloc: ") + e.what();
+        return SMAD_EINVAL;
     }
     e = hipGetLastError();
     if (e != hipSuccess) {

@@ -59,6 +59,8 @@ struct DevBuf {
     size_t n = 0;
     void alloc(size_t count)
     {
+        if (p) (void)hipFree(p);
+        p = nullptr;
         n = count;
         hipError_t e = hipMalloc(&p, count * sizeof(T));
         if (e != hipSuccess) {
@@ -66,6 +68,13 @@ struct DevBuf {
             throw ApiError{SMAD_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
                                             " bytes failed: " + hipGetErrorString(e)};
         }
+    }
+    // Lazily allocated buffers test each buffer, so that one whose
+    // allocation failed (SMAD_ENOMEM) is allocated again on the next call
+    // instead of reaching a kernel as a null pointer.
+    void ensure(size_t count)
+    {
+        if (!p) alloc(count);
     }
     ~DevBuf()
     {
@@ -135,7 +144,11 @@ struct EvPair {
     EvPair()
     {
         HIPCHK(hipEventCreate(&a));
-        HIPCHK(hipEventCreate(&b));
+        const hipError_t e = hipEventCreate(&b);
+        if (e != hipSuccess) {  // the destructor does not run for a throwing constructor
+            (void)hipEventDestroy(a);
+            HIPCHK(e);
+        }
     }
     ~EvPair()
     {
@@ -233,9 +246,9 @@ struct Prof {
     {
         hipEvent_t e;
         HIPCHK(hipEventCreate(&e));
-        HIPCHK(hipEventRecord(e, s));
-        ev.push_back(e);
+        ev.push_back(e);  // owned (and destroyed) from here on
         cls.push_back(c);
+        HIPCHK(hipEventRecord(e, s));
     }
     // per-class summed milliseconds and launch counts
     void collect(std::vector<double>& ms, std::vector<int>& cnt)
@@ -491,7 +504,7 @@ struct PlainEngine : EngineBase {
             if (t)
                 for (const auto* v : {&t->psi, &t->ph})
                     for (const auto& e : *v)
-                        if (e) b += (double)n * sizeof(T);
+                        if (e && e->p) b += (double)n * sizeof(T);
         return b;
     }
 
@@ -503,11 +516,10 @@ struct PlainEngine : EngineBase {
     static void ensure(std::vector<std::unique_ptr<DevBuf<T>>>& v, int count, size_t n)
     {
         if ((int)v.size() < count) v.resize(count);
-        for (int i = 0; i < count; ++i)
-            if (!v[i]) {
-                v[i].reset(new DevBuf<T>);
-                v[i]->alloc(n);
-            }
+        for (int i = 0; i < count; ++i) {
+            if (!v[i]) v[i].reset(new DevBuf<T>);
+            v[i]->ensure(n);
+        }
     }
     void ensure_tape(std::unique_ptr<Tape>& t, int ck)
     {
@@ -550,18 +562,14 @@ struct PlainEngine : EngineBase {
         for (int g = first; g < nseg; ++g) ensure_tape(tapes[g], ck);
         if (first > 0) {
             ensure_tape(scratch[0], ck);
-            if (!rw.p) {
-                rw.alloc(n);
-                rh.alloc(n);
-            }
+            rw.ensure(n);
+            rh.ensure(n);
         }
         if (first > 1) ensure_tape(scratch[1], ck);
-        if (!ph0.p) {
-            for (DevBuf<T>* b : {&ph0, &lbuf, &gbuf}) b->alloc(n);
-            acc_x.alloc(2 * (size_t)num_tiles(d, 0, Lines<T>::NL));
-            acc_z.alloc(2 * (size_t)num_tiles(d, 2, linesz_nl<T>(d.nz)));
-            gsum.alloc(4);
-        }
+        for (DevBuf<T>* b : {&ph0, &lbuf, &gbuf}) b->ensure(n);
+        acc_x.ensure(2 * (size_t)num_tiles(d, 0, Lines<T>::NL));
+        acc_z.ensure(2 * (size_t)num_tiles(d, 2, linesz_nl<T>(d.nz)));
+        gsum.ensure(4);
         tape_ck = ck;
         tape_first = first;
         copy_dd(ph0.p, psihat.p);  // psihat^0 for the adjoint's last pass
@@ -721,10 +729,8 @@ struct PlainEngine : EngineBase {
     {
         if (cf.conserved) throw ApiError{SMAD_EINVAL, "surface diagnostics are defined for the plain model"};
         const size_t cols = (size_t)p.nx * p.ny;
-        if (!sf_nq.p) {
-            for (DevBuf<T>* b : {&sf_dxx, &sf_dyy, &sf_dzz, &sf_dt, &sf_nq}) b->alloc(n);
-            sf_out.alloc(4 * cols);
-        }
+        for (DevBuf<T>* b : {&sf_dxx, &sf_dyy, &sf_dzz, &sf_dt, &sf_nq}) b->ensure(n);
+        sf_out.ensure(4 * cols);
         materialize();
         // nq = REDFT10^3(beta psi^3 - gamma psi^5) of this psi
         Launch<T>::plain_z(s, psi.p, sf_nq.p, nullptr, d, cf, 1, twz.tw);
@@ -1080,10 +1086,8 @@ struct AdvEngine : EngineBase {
         if (ac.pc.conserved)
             throw ApiError{SMAD_EINVAL, "surface diagnostics are defined for the plain and advective models"};
         const size_t cols = (size_t)p.nx * p.ny;
-        if (!sf_nq.p) {
-            for (DevBuf<T>* b : {&sf_dxx, &sf_dyy, &sf_dzz, &sf_dt, &sf_nq}) b->alloc(n);
-            sf_out.alloc(4 * cols);
-        }
+        for (DevBuf<T>* b : {&sf_dxx, &sf_dyy, &sf_dzz, &sf_dt, &sf_nq}) b->ensure(n);
+        sf_out.ensure(4 * cols);
         front(false, nullptr);
         psi_valid = true;  // psi now holds the current psi
         Launch<T>::adv_nl_z(s, psi.p, ux.p, uy.p, uz.p, gx.p, gy.p, gz.p, sf_nq.p, d, ac, twz.tw);
@@ -1139,23 +1143,21 @@ struct AdvEngine : EngineBase {
         if (!fresh) throw ApiError{SMAD_ESTATE, "forward_tape must start from the pre-loop state (after init or set_field(PSI))"};
         if (K < 1) throw ApiError{SMAD_EINVAL, "K must be >= 1"};
         if ((int)tape.size() < K + 1) tape.resize(K + 1);
-        for (int m = 0; m <= K; ++m)
-            if (!tape[m]) {
-                tape[m].reset(new DevBuf<T>);
-                tape[m]->alloc(n);
-            }
-        if (!ic.p) {
-            for (DevBuf<T>* b : {&ic, &end_nh, &lam0, &lam1, &lam2, &mub, &ubx, &uby, &ubz, &gbx, &gby, &gbz})
-                b->alloc(n);
-            const int nb = Launch<T>::flat_blocks((long)n);
-            acc_l.alloc(2 * (size_t)nb);
-            acc_b.alloc(2 * (size_t)nb);
-            gsum.alloc(4);
+        for (int m = 0; m <= K; ++m) {
+            if (!tape[m]) tape[m].reset(new DevBuf<T>);
+            tape[m]->ensure(n);
         }
+        for (DevBuf<T>* b : {&ic, &end_nh, &lam0, &lam1, &lam2, &mub, &ubx, &uby, &ubz, &gbx, &gby, &gbz})
+            b->ensure(n);
+        const int nb = Launch<T>::flat_blocks((long)n);
+        acc_l.ensure(2 * (size_t)nb);
+        acc_b.ensure(2 * (size_t)nb);
+        gsum.ensure(4);
         double budget = tape_budget;
         double held = 0.0;
         for (const auto& a : utape)
-            if (a[0]) held += 3.0 * n * sizeof(T);
+            for (const auto& b : a)
+                if (b && b->p) held += (double)n * sizeof(T);
         if (budget < 0.0) {
             size_t fr = 0, tot = 0;
             HIPCHK(hipMemGetInfo(&fr, &tot));
@@ -1165,11 +1167,10 @@ struct AdvEngine : EngineBase {
         u_first = K - ntu;
         if ((int)utape.size() < K) utape.resize(K);
         for (int m = u_first; m < K; ++m)
-            for (auto& b : utape[m])
-                if (!b) {
-                    b.reset(new DevBuf<T>);
-                    b->alloc(n);
-                }
+            for (auto& b : utape[m]) {
+                if (!b) b.reset(new DevBuf<T>);
+                b->ensure(n);
+            }
         copy_dd(ic.p, psi.p);  // the initial psi (the pre-loop history's argument)
         EvPair ev;
         HIPCHK(hipEventRecord(ev.a, s));
@@ -1562,7 +1563,7 @@ struct QuasiEngine : EngineBase {
     int save_output(int field, double* out)
     {
         if (!prev_valid) throw ApiError{SMAD_ESTATE, "save-block outputs need at least one step"};
-        if (!outbuf.p) outbuf.alloc(n);
+        outbuf.ensure(n);
         static const int ist_kind[3][3] = {{RODFT01, REDFT01, REDFT01}, {REDFT01, RODFT01, REDFT01},
                                            {REDFT01, REDFT01, RODFT01}};
         const T* S[3] = {sx.p, sy.p, sz.p};
@@ -2286,24 +2287,23 @@ int smad_time_steps(smad_ctx* c, int nsteps, double* ms)
 {
     return guarded(c, [&] {
         if (nsteps < 1 || !ms) throw ApiError{SMAD_EINVAL, "nsteps < 1 or null output"};
-        hipEvent_t e0, e1;
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        struct G {
-            hipEvent_t a, b;
+        struct G {  // destroys whichever events were created
+            hipEvent_t a = nullptr, b = nullptr;
             ~G()
             {
-                (void)hipEventDestroy(a);
-                (void)hipEventDestroy(b);
+                if (a) (void)hipEventDestroy(a);
+                if (b) (void)hipEventDestroy(b);
             }
-        } g{e0, e1};
+        } g;
+        HIPCHK(hipEventCreate(&g.a));
+        HIPCHK(hipEventCreate(&g.b));
         HIPCHK(hipStreamSynchronize(c->stream));
-        HIPCHK(hipEventRecord(e0, c->stream));
+        HIPCHK(hipEventRecord(g.a, c->stream));
         c->eng->step(nsteps, nullptr);
-        HIPCHK(hipEventRecord(e1, c->stream));
-        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventRecord(g.b, c->stream));
+        HIPCHK(hipEventSynchronize(g.b));
         float t = 0.f;
-        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        HIPCHK(hipEventElapsedTime(&t, g.a, g.b));
         *ms = t;
         c->steps += nsteps;
         return SMAD_OK;
@@ -2423,6 +2423,9 @@ int smad_r2r_3d(int nx, int ny, int nz, const int kinds[3], int precision, const
     } catch (const ApiError& e) {
         g_init_error = e.msg;
         return e.code;
+    } catch (const std::exception& e) {  // no C++ exception crosses the C ABI
+        g_init_error = e.what();
+        return SMAD_EINVAL;
     }
     return SMAD_OK;
 }
@@ -2441,6 +2444,9 @@ int smad_stream_copy(void* dst, const void* src, size_t bytes, void* stream)
     } catch (const HipError& e) {
         g_init_error = e.msg;
         return SMAD_EHIP;
+    } catch (const std::exception& e) {
+        g_init_error = e.what();
+        return SMAD_EINVAL;
     }
     return SMAD_OK;
 }
