@@ -231,6 +231,32 @@ __device__ __forceinline__ void zlines_vec(const Dims& d, F&& f, int e0 = 0)
     }
 }
 
+// (i, j) of the lines of a z tile (as zlines_vec numbers them): the LPW lines
+// of a wavefront are consecutive, so tile line c is (i0, j0 + c - wv LPW),
+// wrapping into row i0 + 1 at most once (LPW <= 4 < ny).  One 32-bit
+// division per thread instead of one 64-bit division per 16-byte group.
+template <int NL, int WGK>
+struct ZLineIJ {
+    static constexpr int LPW = NL / (WGK / 64);
+    int i0, j0, cw;
+    __device__ __forceinline__ explicit ZLineIJ(const Dims& d)
+    {
+        cw = (int)(threadIdx.x >> 6) * LPW;
+        const unsigned l = blockIdx.x * (unsigned)NL + (unsigned)cw;
+        i0 = (int)(l / (unsigned)d.ny);
+        j0 = (int)(l - (unsigned)i0 * (unsigned)d.ny);
+    }
+    __device__ __forceinline__ void at(int c, const Dims& d, int& i, int& j) const
+    {
+        i = i0;
+        j = j0 + (c - cw);
+        while (j >= d.ny) {
+            j -= d.ny;
+            ++i;
+        }
+    }
+};
+
 // Makhoul input position for a forward kind / natural position otherwise,
 // with the DST sign and reversal folded in (see in_pos / in_neg).
 template <typename T, int N>
@@ -532,10 +558,14 @@ __global__ __launch_bounds__(WG) void k_plain_x_cons(const T* a_in, T* w_out, co
 // (i, j, k) of a flat field index
 __device__ __forceinline__ void unflat(long id, const Dims& d, int& i, int& j, int& k)
 {
-    k = (int)(id % d.nz);
-    long r = id / d.nz;
-    j = (int)(r % d.ny);
-    i = (int)(r / d.ny);
+    // id < nx ny nz <= 2^30: 32-bit unsigned divisions (a 64-bit division by a
+    // runtime value costs several times as many instructions)
+    const unsigned u = (unsigned)id, nz = (unsigned)d.nz, ny = (unsigned)d.ny;
+    const unsigned r = u / nz;
+    k = (int)(u - r * nz);
+    const unsigned q = r / ny;
+    j = (int)(r - q * ny);
+    i = (int)q;
 }
 
 // Per-mode coefficients of the update (FP64): C1, 1/C2 and c.  FAST: 1/C2
@@ -545,6 +575,19 @@ __device__ __forceinline__ void unflat(long id, const Dims& d, int& i, int& j, i
 struct CnMode {
     double c1, ic2, c;
 };
+// 1/x (x > 0) in FP64: the IEEE division, or with FAST (the FP32 storage
+// form) the FP32 hardware reciprocal refined by one Newton step
+// r' = r (2 - x r), which squares its relative error (~1e-7 -> ~1e-14).
+template <bool FAST>
+__device__ __forceinline__ double recip64(double x)
+{
+    if constexpr (FAST) {
+        const double r = (double)__builtin_amdgcn_rcpf((float)x);
+        return r * (2.0 - x * r);
+    } else {
+        return 1.0 / x;
+    }
+}
 // t = |q|^2 - q0^2 given
 template <bool CONS, bool FAST = false>
 __device__ __forceinline__ CnMode cn_mode_t(const PlainCoef& cf, double t)
@@ -552,13 +595,7 @@ __device__ __forceinline__ CnMode cn_mode_t(const PlainCoef& cf, double t)
     double al = cf.eps - cf.alpha * t * t;
     if constexpr (CONS) al = (t + cf.q0sq) * al;
     const double c2 = 1.0 - cf.dtd2 * al;
-    double ic2;
-    if constexpr (FAST) {
-        ic2 = (double)__builtin_amdgcn_rcpf((float)c2);
-        ic2 = ic2 * (2.0 - c2 * ic2);
-    } else {
-        ic2 = 1.0 / c2;
-    }
+    const double ic2 = recip64<FAST>(c2);
     return CnMode{2.0 - c2, ic2, cf.dtd2 * cf.scale};
 }
 template <bool CONS, bool FAST = false>
@@ -568,13 +605,7 @@ __device__ __forceinline__ CnMode cn_mode(const PlainCoef& cf, double mq2)
     double al = cf.eps - cf.alpha * t * t;
     if constexpr (CONS) al = mq2 * al;
     const double c2 = 1.0 - cf.dtd2 * al;
-    double ic2;
-    if constexpr (FAST) {
-        ic2 = (double)__builtin_amdgcn_rcpf((float)c2);
-        ic2 = ic2 * (2.0 - c2 * ic2);
-    } else {
-        ic2 = 1.0 / c2;
-    }
+    const double ic2 = recip64<FAST>(c2);
     return CnMode{1.0 + cf.dtd2 * al, ic2, cf.dtd2 * cf.scale};
 }
 
@@ -829,14 +860,11 @@ __device__ __forceinline__ void zmulti_one(const T* in, T* out, const Dims& d, c
     constexpr int WGZ = LinesZ<T>::WGZ;
     constexpr int W = V16<T>::W;
     const int e0 = opaque_zero_multi<N>();
+    const ZLineIJ<NL, WGZ> lij(d);
     zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
         V16<T> v = *reinterpret_cast<const V16<T>*>(in + gi);
         int i = 0, j = 0;
-        if constexpr (needs_ijk<OP>()) {
-            const long L = gi / N;
-            i = (int)(L / d.ny);
-            j = (int)(L - (long)i * d.ny);
-        }
+        if constexpr (needs_ijk<OP>()) lij.at(c, d, i, j);
 #pragma unroll
         for (int t = 0; t < W; ++t) {
             int q;
@@ -1044,7 +1072,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_stokes_z(const T* __rest
 #pragma unroll
             for (int t = 0; t < W; ++t) {
                 const double qz = (p + t) * cf.qfz;
-                const double inv = 1.0 / (base + qz * qz);
+                const double inv = recip64<sizeof(T) == 4>(base + qz * qz);
                 const double szv = (double)(t == 0 ? cprev : cz.v[t - 1]);
                 const double dot = qsx * (double)a.v[t] + qy * (double)b.v[t] + qz * szv;
                 v[t] = cs * inv * ((double)a.v[t] - qsx * inv * dot);
@@ -1066,7 +1094,7 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_stokes_z(const T* __rest
 #pragma unroll
             for (int t = 0; t < W; ++t) {
                 const double qz = (p + t) * cf.qfz;
-                const double inv = 1.0 / (base + qz * qz);
+                const double inv = recip64<sizeof(T) == 4>(base + qz * qz);
                 const double szv = (double)(t == 0 ? cprev : cz.v[t - 1]);
                 const double dot = qx * (double)b.v[t] + qsy * (double)a.v[t] + qz * szv;
                 v[t] = p + t < d.nz - 1 ? cs * inv * ((double)a.v[t] - qsy * inv * dot) : 0.0;
@@ -1084,19 +1112,20 @@ __global__ __launch_bounds__(LinesZ<T>::WGZ) void k_adv_stokes_z(const T* __rest
 #pragma unroll
         for (int t = 0; t < W; ++t) {
             const double qsz = (p + t + 1) * cf.qfz;
-            const double inv = 1.0 / (base + qsz * qsz);
+            const double inv = recip64<sizeof(T) == 4>(base + qsz * qsz);
             const double sxv = (double)(t < W - 1 ? bx.v[t + 1] : bnext);
             const double syv = (double)(t < W - 1 ? cy.v[t + 1] : cnext);
             const double dot = qx * sxv + qy * syv + qsz * (double)a.v[t];
             v[t] = p + t < d.nz - 1 ? cs * inv * ((double)a.v[t] - qsz * inv * dot) : 0.0;
         }
     };
+    const ZLineIJ<NL, WGZ> lij(d);
     auto component = [&](int comp, T* __restrict__ out, int kind) {
         const int e0 = opaque_zero_multi<N>();  // 384 points: 149 -> 90 VGPRs
         InMap<T, N> im{kind};
         zlines_vec<T, N, NL, WGZ>(d, [&](int p, int c, long gi) {
-            const long L = gi / N;
-            const int i = (int)(L / d.ny), j = (int)(L - (long)i * d.ny);
+            int i, j;
+            lij.at(c, d, i, j);
             double v[W];
             vel(comp, i, j, p, gi, v);
 #pragma unroll
@@ -1310,7 +1339,11 @@ __global__ __launch_bounds__(WG) void k_q_pvel(const T* __restrict__ Sx, const T
 {
     const PlainCoef& cf = qc.pc;
     const long n = d.total(), sx = (long)d.ny * d.nz, sy = d.nz;
-    const double nu = qc.nu, sc = cf.scale;
+    const double nu = qc.nu, sc = cf.scale, scn = sc / nu;
+    // one reciprocal of |q|^2 per component, the coefficients as its products
+    // (FP64 arithmetic; the FP32 storage form takes the refined FP32
+    // reciprocal instead of the IEEE division)
+    constexpr bool FAST = sizeof(T) == 4;
     for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
         int i, j, k;
         unflat(id, d, i, j, k);
@@ -1320,31 +1353,28 @@ __global__ __launch_bounds__(WG) void k_q_pvel(const T* __restrict__ Sx, const T
         double syv = j > 0 ? (double)Sy[id - sy] : 0.0;
         double szv = k > 0 ? (double)Sz[id - 1] : 0.0;
         ph[id] = (i + j + k != 0)
-                     ? sc * (-(qx * sxv + qy * syv + qz * szv) / (qx * qx + qy * qy + qz * qz) +
+                     ? sc * (-(qx * sxv + qy * syv + qz * szv) * recip64<FAST>(qx * qx + qy * qy + qz * qz) +
                              (qc.lambda + 2 * nu) * (double)dv[id])
                      : 0.0;
         double vx = 0.0, vy = 0.0, vz = 0.0;
         if (i < d.nx - 1) {
             double s2 = j != 0 ? (double)Sy[id + sx - sy] : 0.0;
             double s3 = k != 0 ? (double)Sz[id + sx - 1] : 0.0;
-            double mq2 = qsx * qsx + qy * qy + qz * qz;
-            double cm1 = sc / (nu * mq2);
+            double inv = recip64<FAST>(qsx * qsx + qy * qy + qz * qz);
             double dot = qsx * (double)Sx[id] + qy * s2 + qz * s3;
-            vx = cm1 * ((double)Sx[id] - qsx / mq2 * dot) + cm1 * (nu * qsx * (double)dv[id + sx]);
+            vx = scn * inv * ((double)Sx[id] - qsx * inv * dot + nu * qsx * (double)dv[id + sx]);
         }
         if (j < d.ny - 1 && k < d.nz - 1) {
             double s1 = i != 0 ? (double)Sx[id - sx + sy] : 0.0;
             double s3 = k != 0 ? (double)Sz[id + sy - 1] : 0.0;
-            double mq2 = qx * qx + qsy * qsy + qz * qz;
-            double cm1 = sc / (nu * mq2);
+            double inv = recip64<FAST>(qx * qx + qsy * qsy + qz * qz);
             double dot = qx * s1 + qsy * (double)Sy[id] + qz * s3;
-            vy = cm1 * ((double)Sy[id] - qsy / mq2 * dot) + cm1 * (nu * qsy * (double)dv[id + sy]);
+            vy = scn * inv * ((double)Sy[id] - qsy * inv * dot + nu * qsy * (double)dv[id + sy]);
             s1 = i != 0 ? (double)Sx[id - sx + 1] : 0.0;
             double s2 = j != 0 ? (double)Sy[id - sy + 1] : 0.0;
-            mq2 = qx * qx + qy * qy + qsz * qsz;
-            cm1 = sc / (nu * mq2);
+            inv = recip64<FAST>(qx * qx + qy * qy + qsz * qsz);
             dot = qx * s1 + qy * s2 + qsz * (double)Sz[id];
-            vz = cm1 * ((double)Sz[id] - qsz / mq2 * dot) + cm1 * (nu * qsz * (double)dv[id + 1]);
+            vz = scn * inv * ((double)Sz[id] - qsz * inv * dot + nu * qsz * (double)dv[id + 1]);
         }
         ux[id] = T(vx);
         uy[id] = T(vy);
@@ -1710,7 +1740,7 @@ __global__ __launch_bounds__(WG) void k_surf_track(const T* __restrict__ psi, co
 {
     const long col = (long)blockIdx.x * WG + threadIdx.x;
     if (col >= (long)d.nx * d.ny) return;
-    const int j = (int)(col % d.ny);
+    const int j = (int)((unsigned)col % (unsigned)d.ny);
     const long base = col * d.nz;
     double o_s = -1.0, o_v = 0.0, o_h = 0.0, o_k = 0.0;
     long idx = -1;  // the point the derivatives are taken at
@@ -1877,7 +1907,7 @@ __global__ __launch_bounds__(WG) void k_c_relax(const T* __restrict__ amp, T* __
 {
     const long n = d.total();
     for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        const int k = (int)(id % d.nz);
+        const int k = (int)((unsigned)id % (unsigned)d.nz);
         const double r = cc.rho0 + cc.kp * (double)amp[id];
         double l = r * (double)lap[id], rn = r;
         if (cc.rho_up > 0) {
@@ -1983,6 +2013,11 @@ __global__ __launch_bounds__(WG) void k_c_vel(const T* __restrict__ Sx, const T*
     const PlainCoef& cf = cc.pc;
     const long n = d.total(), sx = (long)d.ny * d.nz, sy = d.nz;
     const double sc = cf.scale, c2 = -cc.sig2 * 2.0, l2n = cc.lambda + 2 * cc.nu;
+    // one reciprocal of |q|^2 per component, the coefficients as its products
+    // (FP64 arithmetic; the FP32 storage form takes the refined FP32
+    // reciprocal instead of the IEEE division)
+    constexpr bool FAST = sizeof(T) == 4;
+    const double snu = sc / cc.nu, sl2n = sc / l2n;
     for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
         int i, j, k;
         unflat(id, d, i, j, k);
@@ -1992,28 +2027,28 @@ __global__ __launch_bounds__(WG) void k_c_vel(const T* __restrict__ Sx, const T*
         if (i < d.nx - 1) {
             const double s2 = j != 0 ? (double)Sy[id + sx - sy] : 0.0;
             const double s3 = k != 0 ? (double)Sz[id + sx - 1] : 0.0;
-            const double mq2 = qsx * qsx + qy * qy + qz * qz, cm1 = sc / mq2, cm2 = qsx / mq2;
+            const double mq2 = qsx * qsx + qy * qy + qz * qz, inv = recip64<FAST>(mq2), cm2 = qsx * inv;
             const double dot = qsx * (double)Sx[id] + qy * s2 + qz * s3, g = exp(c2 * mq2);
             const double G = (double)fg[id + sx] + (double)E[id + sx] * g;
-            vs[0] = (cm1 / cc.nu) * ((double)Sx[id] - cm2 * dot);
-            vr[0] = (cm1 / l2n) * (qsx * G + cm2 * dot * g);
+            vs[0] = snu * inv * ((double)Sx[id] - cm2 * dot);
+            vr[0] = sl2n * inv * (qsx * G + cm2 * dot * g);
         }
         if (j < d.ny - 1 && k < d.nz - 1) {
             const double s1 = i != 0 ? (double)Sx[id - sx + sy] : 0.0;
             const double s3 = k != 0 ? (double)Sz[id + sy - 1] : 0.0;
-            const double mq2y = qx * qx + qsy * qsy + qz * qz, cm1 = sc / mq2y, cm2 = qsy / mq2y;
+            const double mq2y = qx * qx + qsy * qsy + qz * qz, inv = recip64<FAST>(mq2y), cm2 = qsy * inv;
             const double dot = qx * s1 + qsy * (double)Sy[id] + qz * s3, g = exp(c2 * mq2y);
             const double G = (double)fg[id + sy] + (double)E[id + sy] * g;
-            vs[1] = (cm1 / cc.nu) * ((double)Sy[id] - cm2 * dot);
-            vr[1] = (cm1 / l2n) * (qsy * G + cm2 * dot * g);
+            vs[1] = snu * inv * ((double)Sy[id] - cm2 * dot);
+            vr[1] = sl2n * inv * (qsy * G + cm2 * dot * g);
             const double t1 = i != 0 ? (double)Sx[id - sx + 1] : 0.0;
             const double t2 = j != 0 ? (double)Sy[id - sy + 1] : 0.0;
-            const double mq2 = qx * qx + qy * qy + qsz * qsz, cz1 = sc / mq2, cz2 = qsz / mq2;
+            const double mq2 = qx * qx + qy * qy + qsz * qsz, invz = recip64<FAST>(mq2), cz2 = qsz * invz;
             const double dz = qx * t1 + qy * t2 + qsz * (double)Sz[id], gz = exp(c2 * mq2);
             const double gd = (i >= 1 && j >= 1 && k == 0) ? g : gz;
             const double Gz = (double)fg[id + 1] + (double)E[id + 1] * gz;
-            vs[2] = (cz1 / cc.nu) * ((double)Sz[id] - cz2 * dz);
-            vr[2] = (cz1 / l2n) * (qsz * Gz + cz2 * dz * gd);
+            vs[2] = snu * invz * ((double)Sz[id] - cz2 * dz);
+            vr[2] = sl2n * invz * (qsz * Gz + cz2 * dz * gd);
         }
         ux[id] = T(vs[0] + vr[0]);
         uy[id] = T(vs[1] + vr[1]);
@@ -2214,8 +2249,7 @@ __global__ __launch_bounds__(WG) void k_adj_x(T* __restrict__ w, T* __restrict__
     auto point_f = [&](int p, int c, double t, float wgt, float& lb, float phv) {
         const double al = cf.eps - cf.alpha * t * t;
         const double c2 = 1.0 - cf.dtd2 * al;
-        double inv = (double)__builtin_amdgcn_rcpf((float)c2);
-        inv = inv * (2.0 - c2 * inv);
+        const double inv = recip64<true>(c2);
         const double h = cf.dtd2 * inv;
         const double a = (2.0 - c2) * inv;
         const double lold = (double)lb;
