@@ -1339,46 +1339,94 @@ __global__ __launch_bounds__(WG) void k_q_pvel(const T* __restrict__ Sx, const T
 {
     const PlainCoef& cf = qc.pc;
     const long n = d.total(), sx = (long)d.ny * d.nz, sy = d.nz;
-    const double nu = qc.nu, sc = cf.scale, scn = sc / nu;
+    const double nu = qc.nu, sc = cf.scale, scn = sc / nu, l2n = qc.lambda + 2 * nu;
     // one reciprocal of |q|^2 per component, the coefficients as its products
     // (FP64 arithmetic; the FP32 storage form takes the refined FP32
     // reciprocal instead of the IEEE division)
     constexpr bool FAST = sizeof(T) == 4;
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        int i, j, k;
-        unflat(id, d, i, j, k);
-        double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
-        double qsx = (i + 1) * cf.qfx, qsy = (j + 1) * cf.qfy, qsz = (k + 1) * cf.qfz;
-        double sxv = i > 0 ? (double)Sx[id - sx] : 0.0;
-        double syv = j > 0 ? (double)Sy[id - sy] : 0.0;
-        double szv = k > 0 ? (double)Sz[id - 1] : 0.0;
-        ph[id] = (i + j + k != 0)
-                     ? sc * (-(qx * sxv + qy * syv + qz * szv) * recip64<FAST>(qx * qx + qy * qy + qz * qz) +
-                             (qc.lambda + 2 * nu) * (double)dv[id])
-                     : 0.0;
-        double vx = 0.0, vy = 0.0, vz = 0.0;
-        if (i < d.nx - 1) {
-            double s2 = j != 0 ? (double)Sy[id + sx - sy] : 0.0;
-            double s3 = k != 0 ? (double)Sz[id + sx - 1] : 0.0;
-            double inv = recip64<FAST>(qsx * qsx + qy * qy + qz * qz);
-            double dot = qsx * (double)Sx[id] + qy * s2 + qz * s3;
-            vx = scn * inv * ((double)Sx[id] - qsx * inv * dot + nu * qsx * (double)dv[id + sx]);
+    // A thread takes W consecutive k of one line (nz is a multiple of W, which
+    // the launcher checks): the own and same-k neighbour values are 16-byte
+    // loads, the k-1 and k+1 neighbours the loaded group shifted by one
+    // element plus one scalar read across the group edge.
+    constexpr int W = V16<T>::W;
+    const long gx = sx / W, gy = sy / W, ng = n / W;
+    V16<T> zero;
+#pragma unroll
+    for (int t = 0; t < W; ++t) zero.v[t] = T(0);
+    for (long g = (long)blockIdx.x * WG + threadIdx.x; g < ng; g += (long)gridDim.x * WG) {
+        const long id0 = g * W;
+        int i, j, k0;
+        unflat(id0, d, i, j, k0);
+        const double qx = i * cf.qfx, qy = j * cf.qfy, qsx = (i + 1) * cf.qfx, qsy = (j + 1) * cf.qfy;
+        const bool xin = i < d.nx - 1, yin = j < d.ny - 1, more = k0 + W < d.nz;
+        const V16<T> ax = ldg16(Sx, g), ay = ldg16(Sy, g), az = ldg16(Sz, g), dvo = ldg16(dv, g);
+        const T azm = k0 > 0 ? Sz[id0 - 1] : T(0);
+        const V16<T> xm = i > 0 ? ldg16(Sx, g - gx) : zero;  // Sx[i-1]
+        const V16<T> ym = j > 0 ? ldg16(Sy, g - gy) : zero;  // Sy[j-1]
+        // pressure (:1510-1592)
+        double pv[W];
+#pragma unroll
+        for (int t = 0; t < W; ++t) {
+            const int k = k0 + t;
+            const double qz = k * cf.qfz;
+            const double szv = (double)(t == 0 ? azm : az.v[t - 1]);
+            pv[t] = (i + j + k != 0) ? sc * (-(qx * (double)xm.v[t] + qy * (double)ym.v[t] + qz * szv) *
+                                                 recip64<FAST>(qx * qx + qy * qy + qz * qz) +
+                                             l2n * (double)dvo.v[t])
+                                      : 0.0;
         }
-        if (j < d.ny - 1 && k < d.nz - 1) {
-            double s1 = i != 0 ? (double)Sx[id - sx + sy] : 0.0;
-            double s3 = k != 0 ? (double)Sz[id + sy - 1] : 0.0;
-            double inv = recip64<FAST>(qx * qx + qsy * qsy + qz * qz);
-            double dot = qx * s1 + qsy * (double)Sy[id] + qz * s3;
-            vy = scn * inv * ((double)Sy[id] - qsy * inv * dot + nu * qsy * (double)dv[id + sy]);
-            s1 = i != 0 ? (double)Sx[id - sx + 1] : 0.0;
-            double s2 = j != 0 ? (double)Sy[id - sy + 1] : 0.0;
-            inv = recip64<FAST>(qx * qx + qy * qy + qsz * qsz);
-            dot = qx * s1 + qy * s2 + qsz * (double)Sz[id];
-            vz = scn * inv * ((double)Sz[id] - qsz * inv * dot + nu * qsz * (double)dv[id + 1]);
+#pragma unroll
+        for (int h = 0; h < W / 2; ++h) {
+            V16<double> o;
+            o.v[0] = pv[2 * h];
+            o.v[1] = pv[2 * h + 1];
+            stg16(ph, g * (W / 2) + h, o);
         }
-        ux[id] = T(vx);
-        uy[id] = T(vy);
-        uz[id] = T(vz);
+        // velocity (:1599-2109)
+        V16<T> ox = zero, oy = zero, oz = zero;
+        if (xin) {
+            const V16<T> b = j != 0 ? ldg16(Sy, g + gx - gy) : zero;  // Sy[i+1, j-1]
+            const V16<T> c = ldg16(Sz, g + gx);                      // Sz[i+1, k-1], shifted
+            const T cm = k0 > 0 ? Sz[id0 + sx - 1] : T(0);
+            const V16<T> dx = ldg16(dv, g + gx);                     // divv[i+1]
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                const double qz = (k0 + t) * cf.qfz;
+                const double inv = recip64<FAST>(qsx * qsx + qy * qy + qz * qz);
+                const double a = (double)ax.v[t];
+                const double dot = qsx * a + qy * (double)b.v[t] + qz * (double)(t == 0 ? cm : c.v[t - 1]);
+                ox.v[t] = T(scn * inv * (a - qsx * inv * dot + nu * qsx * (double)dx.v[t]));
+            }
+        }
+        if (yin) {
+            const V16<T> s1 = i != 0 ? ldg16(Sx, g - gx + gy) : zero;  // Sx[i-1, j+1]
+            const V16<T> c = ldg16(Sz, g + gy);                        // Sz[j+1, k-1], shifted
+            const T cm = k0 > 0 ? Sz[id0 + sy - 1] : T(0);
+            const V16<T> dyv = ldg16(dv, g + gy);                      // divv[j+1]
+            // k+1 neighbours: Sx[i-1, k+1], Sy[j-1, k+1], divv[k+1]
+            const T xn = i != 0 && more ? Sx[id0 - sx + W] : T(0);
+            const T yn = j != 0 && more ? Sy[id0 - sy + W] : T(0);
+            const T dn = more ? dv[id0 + W] : T(0);
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                const int k = k0 + t;
+                if (k >= d.nz - 1) continue;
+                const double qz = k * cf.qfz, qsz = (k + 1) * cf.qfz;
+                const double a = (double)ay.v[t];
+                const double iy = recip64<FAST>(qx * qx + qsy * qsy + qz * qz);
+                const double doty = qx * (double)s1.v[t] + qsy * a + qz * (double)(t == 0 ? cm : c.v[t - 1]);
+                oy.v[t] = T(scn * iy * (a - qsy * iy * doty + nu * qsy * (double)dyv.v[t]));
+                const double t1 = (double)(t < W - 1 ? xm.v[t + 1] : xn);
+                const double t2 = (double)(t < W - 1 ? ym.v[t + 1] : yn);
+                const double zv = (double)az.v[t];
+                const double iz = recip64<FAST>(qx * qx + qy * qy + qsz * qsz);
+                const double dotz = qx * t1 + qy * t2 + qsz * zv;
+                oz.v[t] = T(scn * iz * (zv - qsz * iz * dotz + nu * qsz * (double)(t < W - 1 ? dvo.v[t + 1] : dn)));
+            }
+        }
+        stg16(ux, g, ox);
+        stg16(uy, g, oy);
+        stg16(uz, g, oz);
     }
 }
 

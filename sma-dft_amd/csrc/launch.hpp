@@ -449,6 +449,7 @@ template <typename T>
 void Launch<T>::q_pvel(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, const T* dv,
                        double* ph, T* ux, T* uy, T* uz, Dims d, const QuasiCoef& qc)
 {
+    if (d.nz % V16<T>::W) throw std::logic_error("q_pvel: nz must be a multiple of the 16-byte group");
     SMAD_FLAT(k_q_pvel, Sx, Sy, Sz, dv, ph, ux, uy, uz, d, qc);
 }
 
