@@ -2066,44 +2066,85 @@ __global__ __launch_bounds__(WG) void k_c_vel(const T* __restrict__ Sx, const T*
     // reciprocal instead of the IEEE division)
     constexpr bool FAST = sizeof(T) == 4;
     const double snu = sc / cc.nu, sl2n = sc / l2n;
-    for (long id = (long)blockIdx.x * WG + threadIdx.x; id < n; id += (long)gridDim.x * WG) {
-        int i, j, k;
-        unflat(id, d, i, j, k);
-        const double qx = i * cf.qfx, qy = j * cf.qfy, qz = k * cf.qfz;
-        const double qsx = (i + 1) * cf.qfx, qsy = (j + 1) * cf.qfy, qsz = (k + 1) * cf.qfz;
-        double vs[3] = {0.0, 0.0, 0.0}, vr[3] = {0.0, 0.0, 0.0};
+    // A thread takes W consecutive k of one line (nz is a multiple of W, which
+    // the launcher checks): the own and same-k neighbour values are 16-byte
+    // loads, the k-1 and k+1 neighbours the loaded group shifted by one
+    // element plus one scalar read across the group edge.
+    constexpr int W = V16<T>::W;
+    const long gx = sx / W, gy = sy / W, ng = n / W;
+    V16<T> zero;
+#pragma unroll
+    for (int t = 0; t < W; ++t) zero.v[t] = T(0);
+    for (long g = (long)blockIdx.x * WG + threadIdx.x; g < ng; g += (long)gridDim.x * WG) {
+        const long id0 = g * W;
+        int i, j, k0;
+        unflat(id0, d, i, j, k0);
+        const double qx = i * cf.qfx, qy = j * cf.qfy, qsx = (i + 1) * cf.qfx, qsy = (j + 1) * cf.qfy;
+        const bool more = k0 + W < d.nz;
+        V16<T> oux = zero, ouy = zero, ouz = zero, orx = zero, ory = zero, orz = zero;
         if (i < d.nx - 1) {
-            const double s2 = j != 0 ? (double)Sy[id + sx - sy] : 0.0;
-            const double s3 = k != 0 ? (double)Sz[id + sx - 1] : 0.0;
-            const double mq2 = qsx * qsx + qy * qy + qz * qz, inv = recip64<FAST>(mq2), cm2 = qsx * inv;
-            const double dot = qsx * (double)Sx[id] + qy * s2 + qz * s3, g = exp(c2 * mq2);
-            const double G = (double)fg[id + sx] + (double)E[id + sx] * g;
-            vs[0] = snu * inv * ((double)Sx[id] - cm2 * dot);
-            vr[0] = sl2n * inv * (qsx * G + cm2 * dot * g);
+            const V16<T> a = ldg16(Sx, g);
+            const V16<T> b = j != 0 ? ldg16(Sy, g + gx - gy) : zero;  // Sy[i+1, j-1]
+            const V16<T> c = ldg16(Sz, g + gx);                      // Sz[i+1, k-1], shifted
+            const T cm = k0 > 0 ? Sz[id0 + sx - 1] : T(0);
+            const V16<T> F = ldg16(fg, g + gx), Ev = ldg16(E, g + gx);
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                const double qz = (k0 + t) * cf.qfz;
+                const double mq2 = qsx * qsx + qy * qy + qz * qz, inv = recip64<FAST>(mq2), cm2 = qsx * inv;
+                const double av = (double)a.v[t];
+                const double dot = qsx * av + qy * (double)b.v[t] + qz * (double)(t == 0 ? cm : c.v[t - 1]);
+                const double ge = exp(c2 * mq2);
+                const double G = (double)F.v[t] + (double)Ev.v[t] * ge;
+                const double vs = snu * inv * (av - cm2 * dot);
+                const double vr = sl2n * inv * (qsx * G + cm2 * dot * ge);
+                oux.v[t] = T(vs + vr);
+                orx.v[t] = T(vr);
+            }
         }
-        if (j < d.ny - 1 && k < d.nz - 1) {
-            const double s1 = i != 0 ? (double)Sx[id - sx + sy] : 0.0;
-            const double s3 = k != 0 ? (double)Sz[id + sy - 1] : 0.0;
-            const double mq2y = qx * qx + qsy * qsy + qz * qz, inv = recip64<FAST>(mq2y), cm2 = qsy * inv;
-            const double dot = qx * s1 + qsy * (double)Sy[id] + qz * s3, g = exp(c2 * mq2y);
-            const double G = (double)fg[id + sy] + (double)E[id + sy] * g;
-            vs[1] = snu * inv * ((double)Sy[id] - cm2 * dot);
-            vr[1] = sl2n * inv * (qsy * G + cm2 * dot * g);
-            const double t1 = i != 0 ? (double)Sx[id - sx + 1] : 0.0;
-            const double t2 = j != 0 ? (double)Sy[id - sy + 1] : 0.0;
-            const double mq2 = qx * qx + qy * qy + qsz * qsz, invz = recip64<FAST>(mq2), cz2 = qsz * invz;
-            const double dz = qx * t1 + qy * t2 + qsz * (double)Sz[id], gz = exp(c2 * mq2);
-            const double gd = (i >= 1 && j >= 1 && k == 0) ? g : gz;
-            const double Gz = (double)fg[id + 1] + (double)E[id + 1] * gz;
-            vs[2] = snu * invz * ((double)Sz[id] - cz2 * dz);
-            vr[2] = sl2n * invz * (qsz * Gz + cz2 * dz * gd);
+        if (j < d.ny - 1) {
+            const V16<T> ay = ldg16(Sy, g), az = ldg16(Sz, g);
+            const V16<T> s1 = i != 0 ? ldg16(Sx, g - gx + gy) : zero;  // Sx[i-1, j+1]
+            const V16<T> c = ldg16(Sz, g + gy);                        // Sz[j+1, k-1], shifted
+            const T cm = k0 > 0 ? Sz[id0 + sy - 1] : T(0);
+            const V16<T> F = ldg16(fg, g + gy), Ev = ldg16(E, g + gy);
+            // k+1 neighbours: Sx[i-1, k+1], Sy[j-1, k+1], fg[k+1], E[k+1]
+            const V16<T> xm = i != 0 ? ldg16(Sx, g - gx) : zero, ym = j != 0 ? ldg16(Sy, g - gy) : zero;
+            const V16<T> Fo = ldg16(fg, g), Eo = ldg16(E, g);
+            const T xn = i != 0 && more ? Sx[id0 - sx + W] : T(0);
+            const T yn = j != 0 && more ? Sy[id0 - sy + W] : T(0);
+            const T fn = more ? fg[id0 + W] : T(0), en = more ? E[id0 + W] : T(0);
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+                const int k = k0 + t;
+                if (k >= d.nz - 1) continue;
+                const double qz = k * cf.qfz, qsz = (k + 1) * cf.qfz;
+                const double mq2y = qx * qx + qsy * qsy + qz * qz, inv = recip64<FAST>(mq2y), cm2 = qsy * inv;
+                const double yv = (double)ay.v[t];
+                const double dot = qx * (double)s1.v[t] + qsy * yv + qz * (double)(t == 0 ? cm : c.v[t - 1]);
+                const double ge = exp(c2 * mq2y);
+                const double G = (double)F.v[t] + (double)Ev.v[t] * ge;
+                const double vry1 = sl2n * inv * (qsy * G + cm2 * dot * ge);
+                ouy.v[t] = T(snu * inv * (yv - cm2 * dot) + vry1);
+                ory.v[t] = T(vry1);
+                const double t1 = (double)(t < W - 1 ? xm.v[t + 1] : xn);
+                const double t2 = (double)(t < W - 1 ? ym.v[t + 1] : yn);
+                const double zv = (double)az.v[t];
+                const double mq2 = qx * qx + qy * qy + qsz * qsz, invz = recip64<FAST>(mq2), cz2 = qsz * invz;
+                const double dz = qx * t1 + qy * t2 + qsz * zv, gz = exp(c2 * mq2);
+                const double gd = (i >= 1 && j >= 1 && k == 0) ? ge : gz;
+                const double Gz = (double)(t < W - 1 ? Fo.v[t + 1] : fn) + (double)(t < W - 1 ? Eo.v[t + 1] : en) * gz;
+                const double vrz1 = sl2n * invz * (qsz * Gz + cz2 * dz * gd);
+                ouz.v[t] = T(snu * invz * (zv - cz2 * dz) + vrz1);
+                orz.v[t] = T(vrz1);
+            }
         }
-        ux[id] = T(vs[0] + vr[0]);
-        uy[id] = T(vs[1] + vr[1]);
-        uz[id] = T(vs[2] + vr[2]);
-        vrx[id] = T(vr[0]);
-        vry[id] = T(vr[1]);
-        vrz[id] = T(vr[2]);
+        stg16(ux, g, oux);
+        stg16(uy, g, ouy);
+        stg16(uz, g, ouz);
+        stg16(vrx, g, orx);
+        stg16(vry, g, ory);
+        stg16(vrz, g, orz);
     }
 }
 

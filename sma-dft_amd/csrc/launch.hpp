@@ -379,6 +379,7 @@ template <typename T>
 void Launch<T>::c_vel(hipStream_t s, const T* Sx, const T* Sy, const T* Sz, const T* fg, const T* E, T* ux, T* uy,
                       T* uz, T* vrx, T* vry, T* vrz, Dims d, const CompCoef& cc)
 {
+    if (d.nz % V16<T>::W) throw std::logic_error("c_vel: nz must be a multiple of the 16-byte group");
     SMAD_FLAT(k_c_vel, Sx, Sy, Sz, fg, E, ux, uy, uz, vrx, vry, vrz, d, cc);
 }
 
